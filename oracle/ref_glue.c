@@ -1,0 +1,142 @@
+/*
+ * ref_glue.c -- caller-side glue for the reference build in oracle/_ref.
+ * TEST INFRASTRUCTURE ONLY: linked with the reference's own sources (compiled
+ * from /root/reference/src by oracle/Makefile) to produce golden vectors and to
+ * cross-check the C restatement.  It plays the part of the reference's main
+ * programs: it owns the global config[] (as src/hydrocode_2D/hydrocode.c:123
+ * does) and hands flat arrays to the reference's entry points.
+ *
+ * Interface-side vectors use 15 doubles in the order
+ *   gamma, lambda_u, lambda_v, rho, u, v, p, d_rho, d_u, d_v, d_p, t_rho, t_u, t_v, t_p
+ * (the same order as orc_ifv in grp_oracle.h).
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdbool.h>
+
+#include "var_struc.h"
+#include "riemann_solver.h"
+#include "flux_calc.h"
+#include "finite_volume.h"
+
+double config[N_CONF];
+
+void ref_config_reset(void)
+{
+    for (int k = 1; k < N_CONF; ++k) config[k] = INFINITY;
+    config[0] = 2.0;
+}
+void ref_config_set(int idx, double val) { config[idx] = val; }
+double ref_config_get(int idx) { return config[idx]; }
+
+static void to_ifv(struct i_f_var *f, const double *s)
+{
+    memset(f, 0, sizeof *f);
+    f->gamma = s[0]; f->lambda_u = s[1]; f->lambda_v = s[2];
+    f->RHO = s[3]; f->U = s[4]; f->V = s[5]; f->P = s[6];
+    f->d_rho = s[7]; f->d_u = s[8]; f->d_v = s[9]; f->d_p = s[10];
+    f->t_rho = s[11]; f->t_u = s[12]; f->t_v = s[13]; f->t_p = s[14];
+}
+
+/* in: gammaL gammaR u_L u_R p_L p_R c_L c_R eps tol ; N
+ * out: U_star P_star CRW0 CRW1 gap */
+void ref_riemann_exact(const double *in, int N, double *out)
+{
+    _Bool CRW[2] = {false, false};
+    double us = 0, ps = 0;
+    double gap = Riemann_solver_exact(&us, &ps, in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7],
+                                      CRW, in[8], in[9], N);
+    out[0] = us; out[1] = ps; out[2] = CRW[0]; out[3] = CRW[1]; out[4] = gap;
+}
+
+/* in: gamma u_L u_R p_L p_R c_L c_R eps tol ; N ; out as above */
+void ref_riemann_exact_ben(const double *in, int N, double *out)
+{
+    _Bool CRW[2] = {false, false};
+    double us = 0, ps = 0;
+    double gap = Riemann_solver_exact_Ben(&us, &ps, in[0], in[1], in[2], in[3], in[4], in[5], in[6],
+                                          CRW, in[7], in[8], N);
+    out[0] = us; out[1] = ps; out[2] = CRW[0]; out[3] = CRW[1]; out[4] = gap;
+}
+
+/* out: wave_speed[2], D[0..3], U[0..3], U_star[6]  (16 doubles) */
+void ref_q1d(const double *Ls, const double *Rs, double eps, double atc, double *out)
+{
+    struct i_f_var L, R;
+    to_ifv(&L, Ls); to_ifv(&R, Rs);
+    double ws[2], D[6], U[6], Us[6];
+    linear_GRP_solver_Edir_Q1D(ws, D, U, Us, &L, &R, eps, atc);
+    out[0] = ws[0]; out[1] = ws[1];
+    for (int q = 0; q < 4; ++q) { out[2 + q] = D[q]; out[6 + q] = U[q]; }
+    for (int q = 0; q < 6; ++q) out[10 + q] = Us[q];
+}
+
+/* GRP_2D_flux; config[4] (eps) must be set.  out: F_rho F_u F_v F_e RHO_int U_int V_int P_int */
+int ref_grp_2d_flux(const double *Ls, const double *Rs, double n_x, double n_y, double tau, double *out)
+{
+    struct i_f_var L, R;
+    to_ifv(&L, Ls); to_ifv(&R, Rs);
+    L.n_x = n_x; L.n_y = n_y; R.n_x = n_x; R.n_y = n_y;
+    int rc = GRP_2D_flux(&L, &R, tau);
+    out[0] = L.F_rho; out[1] = L.F_u; out[2] = L.F_v; out[3] = L.F_e;
+    out[4] = L.RHO_int; out[5] = L.U_int; out[6] = L.V_int; out[7] = L.P_int;
+    return rc;
+}
+
+/* out: D[3], U[3] */
+void ref_grp_edir(const double *Ls, const double *Rs, double eps, double atc, double *out)
+{
+    struct i_f_var L, R;
+    to_ifv(&L, Ls); to_ifv(&R, Rs);
+    double D[3], U[3];
+    linear_GRP_solver_Edir(D, U, &L, &R, eps, atc);
+    for (int q = 0; q < 3; ++q) { out[q] = D[q]; out[3 + q] = U[q]; }
+}
+
+/* Runs GRP_solver_2D_split_EUL_source.  fields: 5*N_T blocks of m*n doubles,
+ * order rho u v p e, frame-major ([f][k][j*n+i]); frame 0 = initial data.
+ * config[] must be prepared by the caller (ref_config_*). */
+void ref_split2d(int m, int n, double *fields, int N_T, int *N_plot, double *time_plot)
+{
+    struct cell_var_stru *CV = calloc((size_t)N_T, sizeof *CV);
+    double **rows = malloc((size_t)5 * N_T * m * sizeof(double *));
+    const size_t mn = (size_t)m * n;
+    for (int f = 0; f < 5; ++f)
+        for (int k = 0; k < N_T; ++k) {
+            double **r = rows + ((size_t)f * N_T + k) * m;
+            for (int j = 0; j < m; ++j) r[j] = fields + ((size_t)f * N_T + k) * mn + (size_t)j * n;
+            switch (f) {
+            case 0: CV[k].RHO = r; break;
+            case 1: CV[k].U = r; break;
+            case 2: CV[k].V = r; break;
+            case 3: CV[k].P = r; break;
+            default: CV[k].E = r; break;
+            }
+        }
+    double **X = malloc((size_t)(m + 1) * sizeof(double *)), **Y = malloc((size_t)(m + 1) * sizeof(double *));
+    for (int j = 0; j <= m; ++j) {
+        X[j] = malloc((size_t)(n + 1) * sizeof(double));
+        Y[j] = malloc((size_t)(n + 1) * sizeof(double));
+        for (int i = 0; i <= n; ++i) { X[j][i] = j * config[10]; Y[j][i] = i * config[11]; }
+    }
+    double *cpu_time = calloc((size_t)N_T, sizeof(double));
+    GRP_solver_2D_split_EUL_source(m, n, CV, X, Y, cpu_time, "glue", N_T, N_plot, time_plot);
+    for (int j = 0; j <= m; ++j) { free(X[j]); free(Y[j]); }
+    free(X); free(Y); free(cpu_time); free(rows); free(CV);
+}
+
+/* Runs GRP_solver_EUL_source (1D).  fields: 4*N_T blocks of m doubles (rho u p e). */
+void ref_grp1d(int m, double *fields, int N_T, int *N_plot, double *time_plot)
+{
+    struct cell_var_stru CV;
+    memset(&CV, 0, sizeof CV);
+    double **rows = malloc((size_t)4 * N_T * sizeof(double *));
+    for (int f = 0; f < 4; ++f)
+        for (int k = 0; k < N_T; ++k) rows[f * N_T + k] = fields + ((size_t)f * N_T + k) * m;
+    CV.RHO = rows; CV.U = rows + N_T; CV.P = rows + 2 * N_T; CV.E = rows + 3 * N_T;
+    double *cpu_time = calloc((size_t)N_T, sizeof(double));
+    *N_plot = N_T;
+    GRP_solver_EUL_source(m, CV, cpu_time, N_plot, time_plot);
+    free(cpu_time); free(rows);
+}

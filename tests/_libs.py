@@ -1,0 +1,251 @@
+"""ctypes bindings used by the tests.
+
+* ``Oracle``  -- oracle/liboracle.so, the CPU restatement of the reference
+  (test infrastructure: the checker, never the thing measured).
+* ``RefLib``  -- oracle/_ref/libhydroref.so, the reference's own sources
+  compiled in this container (absent on machines without /root/reference).
+* ``hydrogrp`` -- the product library (HIP), imported from the package.
+"""
+import ctypes as C
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref.so")
+PKG_DIR = os.path.join(ROOT, "hydrocode-fusion_amd")
+if PKG_DIR not in sys.path:
+    sys.path.insert(0, PKG_DIR)
+
+D = C.c_double
+DP = C.POINTER(C.c_double)
+IP = C.POINTER(C.c_int)
+
+
+def dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(DP)
+
+
+class OrcIfv(C.Structure):
+    _fields_ = [(n, D) for n in (
+        "gamma", "lambda_u", "lambda_v", "rho", "u", "v", "p",
+        "d_rho", "d_u", "d_v", "d_p", "t_rho", "t_u", "t_v", "t_p")]
+
+
+def ifv_from(vec):
+    f = OrcIfv()
+    for k, (name, _) in enumerate(OrcIfv._fields_):
+        setattr(f, name, float(vec[k]))
+    return f
+
+
+N_CONF = 400
+
+
+def make_config(**kw):
+    """config[] with the reference's defaults for the keys this path reads."""
+    cfg = np.full(N_CONF, np.inf)
+    cfg[0] = 2.0
+    named = dict(t_all=1, eps=4, N=5, gamma=6, CFL=7, h_x=10, h_y=11, n_x=13, n_y=14,
+                 tau=16, bc_x=17, bc_y=18, split=33, alpha=41)
+    defaults = dict(eps=1e-9, gamma=1.4, CFL=0.45, alpha=1.9)
+    defaults.update(kw)
+    for k, v in defaults.items():
+        cfg[named[k]] = v
+    if np.isfinite(cfg[1]) and not np.isfinite(cfg[5]):
+        cfg[5] = float(2**31 - 1)
+    return cfg
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} missing: run `make -C oracle`")
+        L = self.lib = C.CDLL(path)
+        L.orc_riemann_exact.restype = D
+        L.orc_riemann_exact.argtypes = [DP, DP, D, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP]
+        L.orc_riemann_exact_ben.restype = D
+        L.orc_riemann_exact_ben.argtypes = [DP, DP, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP]
+        L.orc_grp_q1d.restype = None
+        L.orc_grp_q1d.argtypes = [DP, DP, DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), D, D]
+        L.orc_grp_2d_flux.restype = C.c_int
+        L.orc_grp_2d_flux.argtypes = [OrcIfv, OrcIfv, D, D, D, D, DP, DP]
+        L.orc_grp_edir.restype = None
+        L.orc_grp_edir.argtypes = [DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), D, D]
+        PP = C.POINTER(DP)
+        L.orc_split2d.restype = C.c_int
+        L.orc_split2d.argtypes = [C.c_int, C.c_int, PP, PP, PP, PP, PP, C.c_int, IP, DP, DP, DP]
+        L.orc_grp1d.restype = C.c_int
+        L.orc_grp1d.argtypes = [C.c_int, PP, PP, PP, PP, IP, DP, DP, DP]
+
+    def riemann(self, gL, gR, uL, uR, pL, pR, cL, cR, eps, tol, N):
+        us, ps = D(), D()
+        crw = (C.c_int * 2)()
+        nit = C.c_int()
+        gap = self.lib.orc_riemann_exact(C.byref(us), C.byref(ps), gL, gR, uL, uR, pL, pR, cL, cR,
+                                         crw, eps, tol, N, C.byref(nit))
+        return us.value, ps.value, crw[0], crw[1], gap, nit.value
+
+    def riemann_ben(self, g, uL, uR, pL, pR, cL, cR, eps, tol, N):
+        us, ps = D(), D()
+        crw = (C.c_int * 2)()
+        nit = C.c_int()
+        gap = self.lib.orc_riemann_exact_ben(C.byref(us), C.byref(ps), g, uL, uR, pL, pR, cL, cR,
+                                             crw, eps, tol, N, C.byref(nit))
+        return us.value, ps.value, crw[0], crw[1], gap, nit.value
+
+    def q1d(self, Lv, Rv, eps, atc):
+        ws, Dd, U, Us = (D * 2)(), (D * 4)(), (D * 4)(), (D * 6)()
+        self.lib.orc_grp_q1d(ws, Dd, U, Us, C.byref(ifv_from(Lv)), C.byref(ifv_from(Rv)), eps, atc)
+        return np.array(list(ws) + list(Dd) + list(U) + list(Us))
+
+    def flux2d(self, Lv, Rv, tau, eps, nx, ny):
+        F, I = (D * 4)(), (D * 4)()
+        rc = self.lib.orc_grp_2d_flux(ifv_from(Lv), ifv_from(Rv), tau, eps, nx, ny, F, I)
+        return rc, np.array(list(F) + list(I))
+
+    def edir(self, Lv, Rv, eps, atc):
+        Dd, U = (D * 3)(), (D * 3)()
+        self.lib.orc_grp_edir(Dd, U, C.byref(ifv_from(Lv)), C.byref(ifv_from(Rv)), eps, atc)
+        return np.array(list(Dd) + list(U))
+
+    def split2d(self, m, n, init, config, time_plot=None, N_T=None):
+        """init: dict rho,u,v,p (m x n arrays, [j, i]); returns (frames, N_plot, time_plot, config, status)."""
+        if time_plot is None:
+            time_plot = np.array([0.0, config[1]])
+        time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
+        N_plot = len(time_plot)
+        if N_T is None:
+            N_T = min(5, N_plot)
+        gamma = config[6]
+        frames = np.zeros((5, N_T, m, n))
+        r, u, v, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "v", "p"))
+        frames[0, 0], frames[1, 0], frames[2, 0], frames[3, 0] = r, u, v, p
+        frames[4, 0] = initial_energy(r, u, v, p, gamma)
+        ptrs = [(DP * N_T)(*[frames[f, k].ctypes.data_as(DP) for k in range(N_T)]) for f in range(5)]
+        cfg = np.array(config, dtype=np.float64)
+        npl = C.c_int(N_plot)
+        cpu = np.zeros(N_T)
+        st = self.lib.orc_split2d(m, n, *ptrs, N_T, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
+        return frames, npl.value, time_plot, cfg, st
+
+    def grp1d(self, m, init, config, N_T=2, time_plot=None):
+        if time_plot is None:
+            time_plot = np.array([0.0, config[1]])
+        time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
+        gamma = config[6]
+        frames = np.zeros((4, N_T, m))
+        r, u, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "p"))
+        frames[0, 0], frames[1, 0], frames[2, 0] = r, u, p
+        frames[3, 0] = 0.5 * u * u + p / (gamma - 1.0) / r
+        ptrs = [(DP * N_T)(*[frames[f, k].ctypes.data_as(DP) for k in range(N_T)]) for f in range(4)]
+        cfg = np.array(config, dtype=np.float64)
+        npl = C.c_int(N_T)
+        cpu = np.zeros(N_T)
+        st = self.lib.orc_grp1d(m, *ptrs, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
+        return frames, npl.value, time_plot, cfg, st
+
+
+def initial_energy(r, u, v, p, gamma):
+    """E exactly as src/hydrocode_2D/hydrocode.c:251-252 forms it."""
+    e = 0.5 * u * u + p / (gamma - 1.0) / r
+    e = e + 0.5 * v * v
+    return e
+
+
+class RefLib:
+    """The reference's own sources built in oracle/_ref (checker only)."""
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = C.CDLL(path)
+        L.ref_config_reset.restype = None
+        L.ref_config_set.argtypes = [C.c_int, D]
+        L.ref_config_get.argtypes = [C.c_int]
+        L.ref_config_get.restype = D
+        L.ref_riemann_exact.argtypes = [DP, C.c_int, DP]
+        L.ref_riemann_exact_ben.argtypes = [DP, C.c_int, DP]
+        L.ref_q1d.argtypes = [DP, DP, D, D, DP]
+        L.ref_grp_2d_flux.argtypes = [DP, DP, D, D, D, DP]
+        L.ref_grp_2d_flux.restype = C.c_int
+        L.ref_grp_edir.argtypes = [DP, DP, D, D, DP]
+        L.ref_split2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
+        L.ref_grp1d.argtypes = [C.c_int, DP, C.c_int, IP, DP]
+
+    def set_config(self, cfg):
+        self.lib.ref_config_reset()
+        for k, v in enumerate(cfg):
+            self.lib.ref_config_set(k, float(v))
+
+    def riemann(self, inp, N):
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        out = np.zeros(5)
+        self.lib.ref_riemann_exact(dptr(inp), N, dptr(out))
+        return out
+
+    def riemann_ben(self, inp, N):
+        inp = np.ascontiguousarray(inp, dtype=np.float64)
+        out = np.zeros(5)
+        self.lib.ref_riemann_exact_ben(dptr(inp), N, dptr(out))
+        return out
+
+    def q1d(self, Lv, Rv, eps, atc):
+        Lv = np.ascontiguousarray(Lv, dtype=np.float64)
+        Rv = np.ascontiguousarray(Rv, dtype=np.float64)
+        out = np.zeros(16)
+        self.lib.ref_q1d(dptr(Lv), dptr(Rv), eps, atc, dptr(out))
+        return out
+
+    def flux2d(self, Lv, Rv, tau, eps, nx, ny):
+        self.lib.ref_config_set(4, eps)
+        Lv = np.ascontiguousarray(Lv, dtype=np.float64)
+        Rv = np.ascontiguousarray(Rv, dtype=np.float64)
+        out = np.zeros(8)
+        rc = self.lib.ref_grp_2d_flux(dptr(Lv), dptr(Rv), nx, ny, tau, dptr(out))
+        return rc, out
+
+    def edir(self, Lv, Rv, eps, atc):
+        Lv = np.ascontiguousarray(Lv, dtype=np.float64)
+        Rv = np.ascontiguousarray(Rv, dtype=np.float64)
+        out = np.zeros(6)
+        self.lib.ref_grp_edir(dptr(Lv), dptr(Rv), eps, atc, dptr(out))
+        return out
+
+    def split2d(self, m, n, init, config, time_plot=None, N_T=None):
+        if time_plot is None:
+            time_plot = np.array([0.0, config[1]])
+        time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
+        N_plot = len(time_plot)
+        if N_T is None:
+            N_T = min(5, N_plot)
+        gamma = config[6]
+        frames = np.zeros((5, N_T, m, n))
+        r, u, v, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "v", "p"))
+        frames[0, 0], frames[1, 0], frames[2, 0], frames[3, 0] = r, u, v, p
+        frames[4, 0] = initial_energy(r, u, v, p, gamma)
+        self.set_config(config)
+        npl = C.c_int(N_plot)
+        self.lib.ref_split2d(m, n, dptr(frames), N_T, C.byref(npl), dptr(time_plot))
+        cfg = np.array([self.lib.ref_config_get(k) for k in range(N_CONF)])
+        return frames, npl.value, time_plot, cfg
+
+    def grp1d(self, m, init, config, N_T=2, time_plot=None):
+        if time_plot is None:
+            time_plot = np.array([0.0, config[1]])
+        time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
+        gamma = config[6]
+        frames = np.zeros((4, N_T, m))
+        r, u, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "p"))
+        frames[0, 0], frames[1, 0], frames[2, 0] = r, u, p
+        frames[3, 0] = 0.5 * u * u + p / (gamma - 1.0) / r
+        cfg = np.array(config, dtype=np.float64)
+        cfg[0] = 1.0
+        self.set_config(cfg)
+        npl = C.c_int(N_T)
+        self.lib.ref_grp1d(m, dptr(frames), N_T, C.byref(npl), dptr(time_plot))
+        cfg = np.array([self.lib.ref_config_get(k) for k in range(N_CONF)])
+        return frames, npl.value, time_plot, cfg

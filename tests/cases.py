@@ -1,0 +1,109 @@
+"""Synthetic inputs shared by the tests, bench.py and the fixture generator.
+
+Every generator is deterministic for a given seed.  Numbers follow the
+reference's case definitions:
+* four_quadrant: the 2D Riemann problem of data_in/two-dim/RP2D_Positive/Config3
+  (same states and split at (0.5, 0.5); value_start.m:1-4,27-48), any n_x x n_y.
+* sod_1d: Toro's Sod tube, (1, 0, 1) | (0.125, 0, 0.1), split at x = 0.5.
+"""
+import numpy as np
+
+# (rho, u, v, p) of the four quadrants
+FQ_NE = (1.5, 0.0, 0.0, 1.5)
+FQ_NW = (0.5323, 1.206, 0.0, 0.3)
+FQ_SW = (0.138, 1.206, 1.206, 0.029)
+FQ_SE = (0.5323, 0.0, 1.206, 0.3)
+
+
+def four_quadrant(m, n):
+    """Cell (j, i) = (x index, y index); x < 0.5 <=> j < m/2, y < 0.5 <=> i < n/2."""
+    out = {k: np.empty((m, n)) for k in ("rho", "u", "v", "p")}
+    west = (np.arange(m) + 0.5) / m < 0.5
+    south = (np.arange(n) + 0.5) / n < 0.5
+    for wj in (True, False):
+        for si in (True, False):
+            st = {(True, True): FQ_SW, (True, False): FQ_NW,
+                  (False, True): FQ_SE, (False, False): FQ_NE}[(wj, si)]
+            mask = np.outer(west == wj, south == si)
+            for k, val in zip(("rho", "u", "v", "p"), st):
+                out[k][mask] = val
+    return out
+
+
+def four_quadrant_config(m, n, t_all=0.3, **kw):
+    from _libs import make_config
+    return make_config(t_all=t_all, h_x=1.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-4, split=1, **kw)
+
+
+def smooth_random(m, n, seed, amp=0.2):
+    """Smooth positive random data (sums of a few sine modes) for BC tests."""
+    rng = np.random.default_rng(seed)
+    x = (np.arange(m) + 0.5) / m
+    y = (np.arange(n) + 0.5) / n
+    X, Y = np.meshgrid(x, y, indexing="ij")
+    out = {}
+    for k, base in (("rho", 1.0), ("u", 0.0), ("v", 0.0), ("p", 1.0)):
+        f = np.full((m, n), base)
+        for _ in range(3):
+            a, b = rng.uniform(-amp, amp, 2)
+            kx, ky = rng.integers(1, 4, 2)
+            ph = rng.uniform(0, 2 * np.pi)
+            f = f + a * np.sin(2 * np.pi * kx * X + ph) * np.cos(2 * np.pi * ky * Y + b)
+        out[k] = f
+    return out
+
+
+def sod_1d(m):
+    x = (np.arange(m) + 0.5) / m
+    left = x < 0.5
+    return {"rho": np.where(left, 1.0, 0.125), "u": np.zeros(m), "p": np.where(left, 1.0, 0.1)}
+
+
+def interface_pairs(count, seed, gamma=1.4):
+    """Random left/right interface states covering every branch of the GRP solvers.
+
+    Returns (L, R) arrays of shape (count, 15) in orc_ifv order: gamma, lambda_u,
+    lambda_v, rho, u, v, p, d_rho, d_u, d_v, d_p, t_rho, t_u, t_v, t_p.
+    Mixes: strong jumps (density ratio 1e-3..1e3, pressure ratio 1e-4..1e4,
+    velocity jump +-5 c_L), supersonic drifts (trivial / sonic-fan branches),
+    near-identical states (acoustic branch), identical states, and weak jumps.
+    """
+    rng = np.random.default_rng(seed)
+    L = np.zeros((count, 15))
+    R = np.zeros((count, 15))
+    L[:, 0] = R[:, 0] = gamma
+    kind = rng.integers(0, 6, count)
+    rhoL = 10 ** rng.uniform(-1, 1, count)
+    pL = 10 ** rng.uniform(-1, 1, count)
+    cL = np.sqrt(gamma * pL / rhoL)
+    rratio = 10 ** rng.uniform(-3, 3, count)
+    pratio = 10 ** rng.uniform(-4, 4, count)
+    du = rng.uniform(-5, 5, count) * cL
+    drift = rng.uniform(-3, 3, count) * cL
+    weak = 10 ** rng.uniform(-8, -2, count)
+    tiny = 10 ** rng.uniform(-13, -9.5, count)
+    rhoR = np.where(kind <= 1, rhoL / rratio, rhoL * (1 + weak * rng.uniform(-1, 1, count)))
+    pR = np.where(kind <= 1, pL / pratio, pL * (1 + weak * rng.uniform(-1, 1, count)))
+    uL = np.where(kind == 1, drift, rng.uniform(-0.5, 0.5, count) * cL)
+    uR = np.where(kind <= 1, uL + du, uL + weak * cL * rng.uniform(-1, 1, count))
+    # kind 3: acoustic (dist < 1e-9); kind 4: identical; kind 5: weak + drift
+    near = kind == 3
+    rhoR = np.where(near, rhoL + tiny, rhoR)
+    pR = np.where(near, pL - tiny, pR)
+    uR = np.where(near, uL + tiny, uR)
+    same = kind == 4
+    rhoR = np.where(same, rhoL, rhoR)
+    pR = np.where(same, pL, pR)
+    uR = np.where(same, uL, uR)
+    uL = np.where(kind == 5, uL + drift, uL)
+    uR = np.where(kind == 5, uR + drift, uR)
+    L[:, 3], L[:, 4], L[:, 6] = rhoL, uL, pL
+    R[:, 3], R[:, 4], R[:, 6] = rhoR, uR, pR
+    L[:, 5] = rng.uniform(-1, 1, count)
+    R[:, 5] = rng.uniform(-1, 1, count)
+    # slopes: normal derivatives always, tangential ones on a third of the cases
+    for A in (L, R):
+        A[:, 7:11] = rng.normal(0, 1, (count, 4)) * np.where(rng.random((count, 1)) < 0.8, 1.0, 0.0)
+        tmask = rng.random((count, 1)) < 0.33
+        A[:, 11:15] = rng.normal(0, 1, (count, 4)) * tmask
+    return L, R
