@@ -1,0 +1,460 @@
+// grp2d_api.hip -- host side of the 2D split GRP solver: device buffers, the
+// reference's time loop (schedule, frozen tau, plot frames, stop/abort rules)
+// and the batched interface entry points.  C ABI in include/hydrocode_grp.h.
+//
+// The time loop restates GRP_solver_2D_split_EUL_source
+// (src/finite_volume/grp_solver_2D_split_EUL_source.c:147-341) on the host side
+// of a GPU pipeline: sweeps are enqueued without waiting; each sweep records
+// its failures in a status slot and a sweep whose predecessor failed does
+// nothing, so the host can check every few dozen sweeps and still end exactly
+// where the reference ends.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <chrono>
+#include <vector>
+
+#include "../../include/hydrocode_grp.h"
+#include "grp_device.hpp"
+#include "sweep2d.hpp"
+
+namespace hgrp {
+hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st);
+hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st);
+hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
+                      int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
+}
+
+using namespace hgrp;
+
+#define HCHECK(x)                                                     \
+    do {                                                              \
+        hipError_t e_ = (x);                                          \
+        if (e_ != hipSuccess) {                                       \
+            fprintf(stderr, "hydrogrp: %s failed: %s (%s:%d)\n", #x,  \
+                    hipGetErrorString(e_), __FILE__, __LINE__);       \
+            return -(int)e_ - 1;                                      \
+        }                                                             \
+    } while (0)
+
+static constexpr int STATUS_SLOTS = 96;    // sweeps between host checks
+static constexpr int ARG_ERROR = -1000;
+
+struct hgrp2d {
+    int m, n, ld;
+    double cfg[HGRP_N_CONF];
+    hipStream_t stream;
+    double *block;              // one device allocation for everything below
+    double *st[2][5];           // state ping-pong: rho u v p e
+    double *sx[2][4], *sy[2][4];
+    double *bnd[4];             // BC -1 ghosts from frame 0: x_lo, x_hi (n*4), y_lo, y_hi (m*4)
+    SweepStatus *status;        // STATUS_SLOTS slots
+    unsigned long long *cfl_bits;
+    int cs, cx, cy;             // current buffer of state / x-slopes / y-slopes
+    int found_x, found_y;       // the reference's find_bound_x / find_bound_y
+    double h_S_max;
+    bool uploaded;
+    hipEvent_t ev[2];
+};
+
+static size_t pitch_of(int n) { return (size_t)((n + 31) / 32) * 32; }
+
+extern "C" int hgrp2d_create(hgrp2d **out, int m, int n, const double *config)
+{
+    if (!out || m < 1 || n < 1 || !config) return ARG_ERROR - 1;
+    *out = nullptr;
+    hgrp2d *c = (hgrp2d *)calloc(1, sizeof(hgrp2d));
+    if (!c) return ARG_ERROR - 5;
+    c->m = m; c->n = n; c->ld = (int)pitch_of(n);
+    memcpy(c->cfg, config, sizeof c->cfg);
+    const size_t cells = (size_t)m * c->ld;
+    const size_t nb = (size_t)(2 * 5 + 2 * 4 + 2 * 4) * cells + 2 * (size_t)n * 4 + 2 * (size_t)m * 4;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->block, nb * sizeof(double));
+    if (e == hipSuccess) e = hipMalloc((void **)&c->status, STATUS_SLOTS * sizeof(SweepStatus) + 64);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev[0]);
+    if (e == hipSuccess) e = hipEventCreate(&c->ev[1]);
+    if (e != hipSuccess) {
+        fprintf(stderr, "hydrogrp: device allocation of %.1f MiB failed: %s\n",
+                nb * 8.0 / 1048576.0, hipGetErrorString(e));
+        if (c->block) (void)hipFree(c->block);
+        if (c->status) (void)hipFree(c->status);
+        if (c->stream) (void)hipStreamDestroy(c->stream);
+        free(c);
+        return -(int)e - 1;
+    }
+    c->cfl_bits = (unsigned long long *)((char *)c->status + STATUS_SLOTS * sizeof(SweepStatus));
+    double *q = c->block;
+    for (int b = 0; b < 2; ++b)
+        for (int f = 0; f < 5; ++f) { c->st[b][f] = q; q += cells; }
+    for (int b = 0; b < 2; ++b)
+        for (int f = 0; f < 4; ++f) { c->sx[b][f] = q; q += cells; }
+    for (int b = 0; b < 2; ++b)
+        for (int f = 0; f < 4; ++f) { c->sy[b][f] = q; q += cells; }
+    c->bnd[0] = q; q += (size_t)n * 4;
+    c->bnd[1] = q; q += (size_t)n * 4;
+    c->bnd[2] = q; q += (size_t)m * 4;
+    c->bnd[3] = q;
+    // slopes start at zero (only read after the first sweep of a direction has written them)
+    if (hipMemsetAsync(c->block, 0, nb * sizeof(double), c->stream) != hipSuccess ||
+        hipStreamSynchronize(c->stream) != hipSuccess) {
+        hgrp2d_destroy(c);
+        return -1;
+    }
+    *out = c;
+    return 0;
+}
+
+extern "C" int hgrp2d_destroy(hgrp2d *c)
+{
+    if (!c) return 0;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->block) (void)hipFree(c->block);
+    if (c->status) (void)hipFree(c->status);
+    if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
+    if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    free(c);
+    return 0;
+}
+
+static int copy_rows_in(hgrp2d *c, double *dst, const double *src)
+{
+    HCHECK(hipMemcpy2DAsync(dst, (size_t)c->ld * 8, src, (size_t)c->n * 8, (size_t)c->n * 8, (size_t)c->m,
+                            hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+extern "C" int hgrp2d_upload(hgrp2d *c, const double *rho, const double *u, const double *v,
+                             const double *p, const double *e)
+{
+    if (!c || !rho || !u || !v || !p) return ARG_ERROR - 1;
+    const int m = c->m, n = c->n;
+    const double gamma = c->cfg[6];
+    const size_t mn = (size_t)m * n;
+    std::vector<double> E;
+    if (!e) {
+        E.resize(mn);
+        for (size_t k = 0; k < mn; ++k) {
+            double t = 0.5 * u[k] * u[k] + p[k] / (gamma - 1.0) / rho[k];
+            t += 0.5 * v[k] * v[k];
+            E[k] = t;
+        }
+        e = E.data();
+    }
+    const double *src[5] = {rho, u, v, p, e};
+    for (int f = 0; f < 5; ++f) {
+        int rc = copy_rows_in(c, c->st[0][f], src[f]);
+        if (rc) return rc;
+    }
+    // BC -1 ghosts come from the initial data (bound_cond_slope_limiter_x.c:37-46)
+    std::vector<double> b((size_t)2 * n * 4 + (size_t)2 * m * 4);
+    double *xl = b.data(), *xh = xl + (size_t)n * 4, *yl = xh + (size_t)n * 4, *yh = yl + (size_t)m * 4;
+    const double *q4[4] = {rho, u, v, p};
+    for (int i = 0; i < n; ++i)
+        for (int q = 0; q < 4; ++q) {
+            xl[(size_t)i * 4 + q] = q4[q][i];
+            xh[(size_t)i * 4 + q] = q4[q][(size_t)(m - 1) * n + i];
+        }
+    for (int j = 0; j < m; ++j)
+        for (int q = 0; q < 4; ++q) {
+            yl[(size_t)j * 4 + q] = q4[q][(size_t)j * n];
+            yh[(size_t)j * 4 + q] = q4[q][(size_t)j * n + n - 1];
+        }
+    HCHECK(hipMemcpyAsync(c->bnd[0], b.data(), b.size() * 8, hipMemcpyHostToDevice, c->stream));
+    // frozen CFL reduction over the initial data (grp_solver_2D_split_EUL_source.c:183-191)
+    const unsigned long long inf_bits = 0x7ff0000000000000ull;
+    HCHECK(hipMemcpyAsync(c->cfl_bits, &inf_bits, 8, hipMemcpyHostToDevice, c->stream));
+    HCHECK(launch_cfl(c->st[0][0], c->st[0][1], c->st[0][2], c->st[0][3], m, n, c->ld, gamma,
+                      fmin(c->cfg[10], c->cfg[11]), c->cfl_bits, c->stream));
+    unsigned long long bits = 0;
+    HCHECK(hipMemcpyAsync(&bits, c->cfl_bits, 8, hipMemcpyDeviceToHost, c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    memcpy(&c->h_S_max, &bits, 8);
+    c->cs = c->cx = c->cy = 0;
+    c->found_x = c->found_y = 0;
+    c->uploaded = true;
+    return 0;
+}
+
+extern "C" int hgrp2d_download(hgrp2d *c, double *rho, double *u, double *v, double *p, double *e)
+{
+    if (!c) return ARG_ERROR - 1;
+    double *dst[5] = {rho, u, v, p, e};
+    for (int f = 0; f < 5; ++f) {
+        if (!dst[f]) continue;
+        HCHECK(hipMemcpy2DAsync(dst[f], (size_t)c->n * 8, c->st[c->cs][f], (size_t)c->ld * 8, (size_t)c->n * 8,
+                                (size_t)c->m, hipMemcpyDeviceToHost, c->stream));
+    }
+    HCHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hgrp2d_h_over_smax(hgrp2d *c, double *h)
+{
+    if (!c || !h || !c->uploaded) return ARG_ERROR - 1;
+    *h = c->h_S_max;
+    return 0;
+}
+
+// One enqueued sweep and the host loop state it belongs to.
+struct SweepRec {
+    int dir;              // 0 x, 1 y
+    int k, DS;            // loop counter and pass kind when it was enqueued
+    double time_c;        // time_c when it was enqueued
+    int in_s, in_d;       // state buffer and slope buffer read
+};
+
+static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double nu, int slot, int prev_slot)
+{
+    const int in = c->cs, out = 1 - c->cs;
+    a.rho = c->st[in][0]; a.u = c->st[in][1]; a.v = c->st[in][2]; a.p = c->st[in][3]; a.e = c->st[in][4];
+    a.rho_o = c->st[out][0]; a.u_o = c->st[out][1]; a.v_o = c->st[out][2]; a.p_o = c->st[out][3]; a.e_o = c->st[out][4];
+    const int sin = dir == 0 ? c->cx : c->cy;
+    for (int q = 0; q < 4; ++q) {
+        a.s[q] = dir == 0 ? c->sx[sin][q] : c->sy[sin][q];
+        a.s_o[q] = dir == 0 ? c->sx[1 - sin][q] : c->sy[1 - sin][q];
+    }
+    a.bnd_lo = c->bnd[dir == 0 ? 0 : 2];
+    a.bnd_hi = c->bnd[dir == 0 ? 1 : 3];
+    a.m = c->m; a.n = c->n; a.ld = c->ld;
+    a.bc = (int)c->cfg[dir == 0 ? 17 : 18];
+    a.limiter3 = dir == 0 ? c->found_x : c->found_y;
+    a.h = c->cfg[dir == 0 ? 10 : 11];
+    a.hh = 0.5 * a.h;
+    a.tau = tau_flux;
+    a.nu = nu;
+    a.gamma = c->cfg[6];
+    a.eps = c->cfg[4];
+    a.alpha = c->cfg[41];
+    a.status = c->status + slot;
+    a.prev = prev_slot >= 0 ? c->status + prev_slot : nullptr;
+}
+
+static int enqueue_sweep(hgrp2d *c, int dir, double tau_flux, double nu, int slot, int prev_slot)
+{
+    SweepArgs a;
+    fill_args(c, a, dir, tau_flux, nu, slot, prev_slot);
+    HCHECK(dir == 0 ? launch_sweep_x(a, c->stream) : launch_sweep_y(a, c->stream));
+    c->cs = 1 - c->cs;
+    if (dir == 0) { c->cx = 1 - c->cx; c->found_x = 1; }
+    else          { c->cy = 1 - c->cy; c->found_y = 1; }
+    return 0;
+}
+
+static bool valid_bc(int b) { return b == -1 || b == -2 || b == -4 || b == -7 || b == -24; }
+
+extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x, double *ms_y)
+{
+    if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
+    if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
+    const double tau = c->cfg[7] * c->h_S_max;
+    const double half_tau = tau * 0.5, half_nu = half_tau / c->cfg[10], mu = tau / c->cfg[11];
+    HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
+    // per-sweep events for the average launch duration of each sweep kind
+    std::vector<hipEvent_t> evs((size_t)6 * steps);
+    for (auto &e : evs) HCHECK(hipEventCreate(&e));
+    HCHECK(hipEventRecord(c->ev[0], c->stream));
+    int slot = 0;
+    for (int s = 0; s < steps; ++s) {
+        const int dirs[3] = {0, 1, 0};
+        for (int w = 0; w < 3; ++w) {
+            const int d = dirs[w];
+            HCHECK(hipEventRecord(evs[(size_t)6 * s + 2 * w], c->stream));
+            // status slots are not chained here: the benchmark must run every sweep
+            int rc = enqueue_sweep(c, d, d == 0 ? half_tau : tau, d == 0 ? half_nu : mu, slot % STATUS_SLOTS, -1);
+            if (rc) return rc;
+            HCHECK(hipEventRecord(evs[(size_t)6 * s + 2 * w + 1], c->stream));
+            ++slot;
+        }
+    }
+    HCHECK(hipEventRecord(c->ev[1], c->stream));
+    HCHECK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (ms_total) *ms_total = ms;
+    double sx = 0, sy = 0;
+    for (int s = 0; s < steps; ++s)
+        for (int w = 0; w < 3; ++w) {
+            float t = 0.f;
+            HCHECK(hipEventElapsedTime(&t, evs[(size_t)6 * s + 2 * w], evs[(size_t)6 * s + 2 * w + 1]));
+            if (w == 1) sy += t; else sx += t;
+        }
+    if (ms_x) *ms_x = sx / (2.0 * steps);
+    if (ms_y) *ms_y = sy / steps;
+    for (auto &e : evs) (void)hipEventDestroy(e);
+    return 0;
+}
+
+// ---- the reference's time loop -------------------------------------------------
+namespace {
+struct Loop {
+    hgrp2d *c;
+    std::vector<SweepRec> recs;      // sweeps since the last check; recs[r] uses status slot r
+    int sweeps = 0;
+
+    // reads the status of the pending sweeps; returns the index of the first
+    // failed one, -1 if none, or a negative HIP error - 2
+    int check(unsigned long long *key, unsigned *flags)
+    {
+        if (recs.empty()) return -1;
+        std::vector<SweepStatus> st(recs.size());
+        if (hipMemcpyAsync(st.data(), c->status, st.size() * sizeof(SweepStatus), hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return -2;
+        for (size_t r = 0; r < st.size(); ++r)
+            if (st[r].fail_key || (st[r].flags & 1u)) {
+                *key = st[r].fail_key;
+                *flags = st[r].flags;
+                return (int)r;
+            }
+        return -1;
+    }
+};
+}  // namespace
+
+extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int N_T,
+                            hgrp2d_frame_fn fn, void *user, hgrp2d_result *res)
+{
+    if (!c || !c->uploaded || !time_plot || N_plot < 2 || N_T < 1 || !res) return ARG_ERROR - 1;
+    const double t_all = c->cfg[1], eps = c->cfg[4], CFL = c->cfg[7];
+    const int N = (int)c->cfg[5];
+    const double h_x = c->cfg[10], h_y = c->cfg[11];
+    double tau = c->cfg[16];
+    const int bc_x = (int)c->cfg[17], bc_y = (int)c->cfg[18];
+    const auto t_start = std::chrono::steady_clock::now();
+
+    Loop L;
+    L.c = c;
+    int k = 0, nt = 0, nt_plot = 0, DS = 1, status = 0;
+    double time_c = 0.0, half_tau = 0.0, half_nu = 0.0, mu = 0.0;
+    bool stop_t = false;
+    HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
+
+    // finish at a sweep that failed on the device: restore the loop state it
+    // ran in and apply the reference's rule for that failure
+    auto settle = [&](int r, unsigned long long key, unsigned flags) {
+        const SweepRec &R = L.recs[(size_t)r];
+        const int code = key ? (int)((key - 1ull) & 3ull) : 0;
+        k = R.k; DS = R.DS; time_c = R.time_c;
+        L.sweeps += r + 1 - (int)L.recs.size();
+        if (code == 1) {                       // ifvar_check failed: goto return_NULL, no update
+            printf("<0.0 or non-finite reconstruction in the %c sweep of step %d\n", R.dir ? 'y' : 'x', k);
+            c->cs = R.in_s;
+            status = 1;
+        } else {                               // GRP or positivity failure: update kept, stop
+            if (code == 2) printf("GRP star/derivative error in the %c sweep of step %d\n", R.dir ? 'y' : 'x', k);
+            if (flags & 1u) printf("<0.0 error in the update of the %c sweep of step %d\n", R.dir ? 'y' : 'x', k);
+            c->cs = 1 - R.in_s;
+            if (R.dir == 1) time_c += tau;
+            status = 2;
+        }
+        (void)flags;
+    };
+    auto flush = [&]() -> int {               // returns 1 if a failure ended the run
+        unsigned long long key = 0;
+        unsigned flags = 0;
+        const int r = L.check(&key, &flags);
+        if (r <= -2) return -2;
+        if (r >= 0) { settle(r, key, flags); return 1; }
+        L.recs.clear();
+        return 0;
+    };
+    // after a clean check every status slot is still zero, so slots restart at 0
+    auto sweep = [&](int dir, double tf, double nu) -> int {
+        if ((int)L.recs.size() + 1 > STATUS_SLOTS) {
+            const int f = flush();
+            if (f) return f;
+        }
+        const int slot = (int)L.recs.size();
+        const int prev = L.recs.empty() ? -1 : slot - 1;
+        L.recs.push_back({dir, k, DS, time_c, c->cs, dir == 0 ? c->cx : c->cy});
+        ++L.sweeps;
+        int rc = enqueue_sweep(c, dir, tf, nu, slot, prev);
+        return rc ? rc : 0;
+    };
+
+    bool ended = false;
+    for (k = 1; k <= N; DS ? k : ++k) {
+        if (time_c >= time_plot[nt_plot] && nt_plot < (N_plot - 1)) {          // plot frame
+            const int f = flush();
+            if (f < 0) return -2;
+            if (f) { ended = true; break; }
+            if (fn && fn(user, nt, nt_plot, time_c)) { status = 1; ended = true; break; }
+            nt_plot++;
+            if (nt < (N_T - 1)) nt++;
+        }
+        if (DS) {                                                             // time step
+            const double h_S_max = c->h_S_max;
+            if (isfinite(t_all) || !isfinite(c->cfg[16]) || c->cfg[16] <= 0.0) {
+                tau = CFL * h_S_max;
+                if (tau < eps) {
+                    printf("\nThe length of the time step is so small on [%d, %g, %g] (t_n, time_c, tau)\n", k, time_c, tau);
+                    stop_t = true;
+                } else if ((time_c + tau) > (t_all - eps)) {
+                    tau = t_all - time_c;
+                } else if (!isfinite(tau)) {
+                    printf("NAN or INFinite error on [%d, %g, %g] (t_n, time_c, tau) - CFL\n", k, time_c, tau);
+                    const int f = flush();
+                    if (f < 0) return -2;
+                    status = 1;
+                    ended = true;
+                    break;
+                }
+            }
+            half_tau = tau * 0.5;
+            half_nu = half_tau / h_x;
+            mu = tau / h_y;
+        }
+        if (!valid_bc(bc_x)) {
+            printf("No suitable boundary coditions in x direction!\n");
+            const int f = flush();
+            if (f < 0) return -2;
+            if (!f) status = 1;
+            ended = true;
+            break;
+        }
+        int rc = sweep(0, half_tau, half_nu);
+        if (rc < 0) return rc;
+        if (rc > 0) { ended = true; break; }
+        if (stop_t) break;
+        if (DS) {
+            if (!valid_bc(bc_y)) {
+                printf("No suitable boundary coditions in y direction!\n");
+                const int f = flush();
+                if (f < 0) return -2;
+                if (!f) status = 1;
+                ended = true;
+                break;
+            }
+            rc = sweep(1, tau, mu);
+            if (rc < 0) return rc;
+            if (rc > 0) { ended = true; break; }
+            time_c += tau;
+        }
+        if (time_c > (t_all - eps) || !isfinite(time_c)) break;
+        DS = DS ? 0 : 1;
+    }
+    if (!ended) {
+        const int f = flush();
+        if (f < 0) return -2;
+        if (stop_t && status == 0) status = 2;
+    }
+    HCHECK(hipStreamSynchronize(c->stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (status != 1) {
+        printf("\nTime is up at time step %d.\n", k);
+        printf("The cost of CPU time for 2D-GRP Eulerian scheme with dimension splitting for this problem is %g seconds.\n", secs);
+    }
+    res->k = k;
+    res->nt = nt;
+    res->status = status;
+    res->sweeps = L.sweeps;
+    res->time_c = time_c;
+    res->tau = tau;
+    res->seconds = secs;
+    return 0;
+}

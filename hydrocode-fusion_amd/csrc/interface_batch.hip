@@ -1,0 +1,136 @@
+// interface_batch.hip -- batched per-interface entry points (one thread per
+// interface) used to check the device GRP solver against the reference's
+// per-function vectors.  C ABI in include/hydrocode_grp.h.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <string.h>
+
+#include "../../include/hydrocode_grp.h"
+#include "grp_device.hpp"
+
+using namespace hgrp;
+
+__global__ void flux_batch_kernel(const double *L, const double *R, int count, double tau, double eps,
+                                  int dir, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *l = L + (size_t)k * 15, *r = R + (size_t)k * 15;
+    Side SL = {l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]};
+    Side SR = {r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+    double F[4], I[4];
+    const int code = dir == 0 ? grp_2d_flux<0>(SL, SR, l[0], tau, eps, F, I)
+                              : grp_2d_flux<1>(SL, SR, l[0], tau, eps, F, I);
+    double *o = out + (size_t)k * 9;
+    o[0] = code;
+    for (int q = 0; q < 4; ++q) { o[1 + q] = F[q]; o[5 + q] = I[q]; }
+}
+
+__global__ void riemann_batch_kernel(const double *in, int count, int N, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *a = in + (size_t)k * 10;
+    const StarResult r = riemann_exact(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], N);
+    double *o = out + (size_t)k * 5;
+    o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters;
+}
+
+__global__ void pow_batch_kernel(const double *x, const double *y, int count, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) out[k] = gpow(x[k], y[k]);
+}
+
+namespace {
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+int run_batch(const void *const *hin, const size_t *in_bytes, int nin, void *hout, size_t out_bytes,
+              void (*launch)(void *const *din, void *dout, void *ctx), void *ctx)
+{
+    DevBuf bufs[4], ob;
+    void *din[4] = {nullptr, nullptr, nullptr, nullptr};
+    for (int q = 0; q < nin; ++q) {
+        if (hipMalloc(&bufs[q].p, in_bytes[q] ? in_bytes[q] : 8) != hipSuccess) return -2;
+        if (in_bytes[q] && hipMemcpy(bufs[q].p, hin[q], in_bytes[q], hipMemcpyHostToDevice) != hipSuccess) return -3;
+        din[q] = bufs[q].p;
+    }
+    if (hipMalloc(&ob.p, out_bytes ? out_bytes : 8) != hipSuccess) return -2;
+    launch(din, ob.p, ctx);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) {
+        fprintf(stderr, "hydrogrp: batch kernel failed: %s\n", hipGetErrorString(e));
+        return -(int)e - 1;
+    }
+    if (out_bytes && hipMemcpy(hout, ob.p, out_bytes, hipMemcpyDeviceToHost) != hipSuccess) return -3;
+    return 0;
+}
+
+struct FluxCtx { int count; double tau, eps; int dir; };
+struct RiemCtx { int count, N; };
+}  // namespace
+
+extern "C" int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double tau,
+                                     double eps, int dir, double *out)
+{
+    if (count < 0 || (count && (!L || !R || !out)) || (dir != 0 && dir != 1)) return -1001;
+    if (count == 0) return 0;
+    const void *in[2] = {L, R};
+    const size_t nb[2] = {(size_t)count * 15 * 8, (size_t)count * 15 * 8};
+    FluxCtx fc = {count, tau, eps, dir};
+    return run_batch(in, nb, 2, out, (size_t)count * 9 * 8,
+                     [](void *const *d, void *o, void *cx) {
+                         const FluxCtx *f = (const FluxCtx *)cx;
+                         hipLaunchKernelGGL(flux_batch_kernel, dim3((f->count + 127) / 128), dim3(128), 0, 0,
+                                            (const double *)d[0], (const double *)d[1], f->count, f->tau, f->eps,
+                                            f->dir, (double *)o);
+                     },
+                     &fc);
+}
+
+extern "C" int hgrp_riemann_batch(const double *in, int count, int N, double *out)
+{
+    if (count < 0 || (count && (!in || !out))) return -1001;
+    if (count == 0) return 0;
+    const void *hin[1] = {in};
+    const size_t nb[1] = {(size_t)count * 10 * 8};
+    RiemCtx rc = {count, N};
+    return run_batch(hin, nb, 1, out, (size_t)count * 5 * 8,
+                     [](void *const *d, void *o, void *cx) {
+                         const RiemCtx *r = (const RiemCtx *)cx;
+                         hipLaunchKernelGGL(riemann_batch_kernel, dim3((r->count + 127) / 128), dim3(128), 0, 0,
+                                            (const double *)d[0], r->count, r->N, (double *)o);
+                     },
+                     &rc);
+}
+
+extern "C" int hgrp_pow_batch(const double *x, const double *y, int count, double *out)
+{
+    if (count < 0 || (count && (!x || !y || !out))) return -1001;
+    if (count == 0) return 0;
+    const void *hin[2] = {x, y};
+    const size_t nb[2] = {(size_t)count * 8, (size_t)count * 8};
+    int cnt = count;
+    return run_batch(hin, nb, 2, out, (size_t)count * 8,
+                     [](void *const *d, void *o, void *cx) {
+                         const int c = *(const int *)cx;
+                         hipLaunchKernelGGL(pow_batch_kernel, dim3((c + 255) / 256), dim3(256), 0, 0,
+                                            (const double *)d[0], (const double *)d[1], c, (double *)o);
+                     },
+                     &cnt);
+}
+
+extern "C" int hgrp_device_name(char *buf, int len)
+{
+    if (!buf || len < 1) return -1001;
+    buf[0] = '\0';
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
+    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    return 0;
+}

@@ -1,0 +1,311 @@
+// sweep2d.hip -- gfx950 kernels of the dimension-split 2D GRP step.
+//
+//   sweep_x_kernel   one X(tau_s) sweep: threads walk down x-lines (i fixed) with a
+//                    sliding register window; a wave covers 64 consecutive i, so
+//                    every load/store is one coalesced 512-byte row segment.
+//   sweep_y_kernel   one Y(tau) sweep: a wave owns 64 consecutive cells of a y-line
+//                    (contiguous in memory) and exchanges neighbour values,
+//                    slopes and fluxes with lane shuffles; 63 cells per wave.
+//   cfl_kernel       h/S_max over the frozen initial data (order-independent min).
+//
+// Semantics follow src/finite_volume/grp_solver_2D_split_EUL_source.c:182-303,
+// src/inter_process/bound_cond_slope_limiter_{x,y}.c:27-129,
+// src/inter_process/slope_limiter{,_2D_x}.c and src/flux_calc/flux_generator_{x,y}.c:31-165.
+#include <hip/hip_runtime.h>
+#include "grp_device.hpp"
+#include "sweep2d.hpp"
+
+namespace hgrp {
+
+__device__ __forceinline__ bool sweep_halted(const SweepArgs &a)
+{
+    if (a.prev == nullptr) return false;
+    const SweepStatus pv = *a.prev;
+    if (pv.fail_key != 0ull || pv.flags != 0u) {
+        if (threadIdx.x == 0) atomicOr(&a.status->flags, 2u);
+        return true;
+    }
+    return false;
+}
+
+// One grid line along the sweep direction.  DIR 0: x-line i (cells j = 0..m-1,
+// stride ld); DIR 1: y-line j (cells i = 0..n-1, stride 1).  Index -1 and cnt
+// are the ghost cells.
+template <int DIR>
+struct Line {
+    const SweepArgs &a;
+    size_t base, stride;
+    int cnt, line;
+    static constexpr int NV = DIR == 0 ? 1 : 2;   // normal velocity: u in x, v in y
+
+    __device__ Line(const SweepArgs &args, int ln) : a(args), line(ln)
+    {
+        if (DIR == 0) { base = (size_t)ln; stride = (size_t)args.ld; cnt = args.m; }
+        else          { base = (size_t)ln * (size_t)args.ld; stride = 1; cnt = args.n; }
+    }
+    __device__ size_t at(int k) const { return base + (size_t)k * stride; }
+
+    __device__ void cell(int k, double w[4]) const
+    {
+        const size_t o = at(k);
+        w[0] = a.rho[o]; w[1] = a.u[o]; w[2] = a.v[o]; w[3] = a.p[o];
+    }
+    // value of cell k, ghost values at k == -1 / k == cnt (bound_cond_slope_limiter_x.c:34-82)
+    __device__ void value(int k, double w[4]) const
+    {
+        if (k >= 0 && k < cnt) { cell(k, w); return; }
+        const bool lo = k < 0;
+        switch (a.bc) {
+        case BC_INITIAL: {
+            const double *b = (lo ? a.bnd_lo : a.bnd_hi) + (size_t)line * 4;
+            w[0] = b[0]; w[1] = b[1]; w[2] = b[2]; w[3] = b[3];
+            return;
+        }
+        case BC_PERIODIC:
+            cell(lo ? cnt - 1 : 0, w);
+            return;
+        case BC_REFLECT:
+            cell(lo ? 0 : cnt - 1, w);
+            w[NV] = -w[NV];
+            return;
+        case BC_REFLECT_FREE:
+            cell(lo ? 0 : cnt - 1, w);
+            if (lo) w[NV] = -w[NV];
+            return;
+        default:   // BC_FREE
+            cell(lo ? 0 : cnt - 1, w);
+            return;
+        }
+    }
+    // limited slope of cell k from its neighbours and its previous slope (slope_limiter_2D_x.c:177-211)
+    __device__ void limited(int k, const double vm[4], const double v0[4], const double vp[4], double s[4]) const
+    {
+        const size_t o = at(k);
+        const double h = a.h;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double s_L = (v0[q] - vm[q]) / h;
+            const double s_R = (vp[q] - v0[q]) / h;
+            s[q] = a.limiter3 ? minmod3(a.alpha * s_L, a.alpha * s_R, a.s[q][o]) : minmod2(s_L, s_R);
+        }
+    }
+    __device__ void limited_at(int k, double s[4]) const
+    {
+        double vm[4], v0[4], vp[4];
+        value(k - 1, vm); value(k, v0); value(k + 1, vp);
+        limited(k, vm, v0, vp, s);
+    }
+    // ghost slopes (bound_cond_slope_limiter_x.c:94-109); zero where the reference leaves them unset
+    __device__ void ghost_slope(bool lo, double s[4]) const
+    {
+        s[0] = s[1] = s[2] = s[3] = 0.0;
+        double t[4];
+        switch (a.bc) {
+        case BC_REFLECT:
+            limited_at(lo ? 0 : cnt - 1, t);
+            s[NV] = t[NV];
+            break;
+        case BC_PERIODIC:
+            limited_at(lo ? cnt - 1 : 0, s);
+            break;
+        case BC_REFLECT_FREE:
+            if (lo) { limited_at(0, t); s[NV] = t[NV]; }
+            break;
+        default:
+            break;
+        }
+    }
+    // conservative update of cell k and slope refresh (grp_solver_2D_split_EUL_source.c:236-254)
+    __device__ void update(int k, const double w[4], const double Fl[4], const double Fr[4],
+                           const double Il[4], const double Ir[4]) const
+    {
+        const size_t o = at(k);
+        const double nu = a.nu;
+        const double e = a.e[o];
+        const double mom_x = w[0] * w[1] - nu * (Fr[1] - Fl[1]);
+        const double mom_y = w[0] * w[2] - nu * (Fr[2] - Fl[2]);
+        const double ene = w[0] * e - nu * (Fr[3] - Fl[3]);
+        const double rho = w[0] + -nu * (Fr[0] - Fl[0]);
+        const double u = mom_x / rho;
+        const double v = mom_y / rho;
+        const double p = (ene - 0.5 * mom_x * u - 0.5 * mom_y * v) * (a.gamma - 1.0);
+        a.rho_o[o] = rho; a.u_o[o] = u; a.v_o[o] = v; a.e_o[o] = ene / rho; a.p_o[o] = p;
+        if (p < a.eps || rho < a.eps) atomicOr(&a.status->flags, 1u);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) a.s_o[q][o] = (Ir[q] - Il[q]) / a.h;
+    }
+};
+
+__device__ __forceinline__ Side recon(const double w[4], const double s[4], double hh, bool left)
+{
+    Side f;
+    if (left) {
+        f.rho = w[0] + hh * s[0]; f.u = w[1] + hh * s[1]; f.v = w[2] + hh * s[2]; f.p = w[3] + hh * s[3];
+    } else {
+        f.rho = w[0] - hh * s[0]; f.u = w[1] - hh * s[1]; f.v = w[2] - hh * s[2]; f.p = w[3] - hh * s[3];
+    }
+    f.d_rho = s[0]; f.d_u = s[1]; f.d_v = s[2]; f.d_p = s[3];
+    return f;
+}
+
+template <int DIR>
+__device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &SL, const Side &SR,
+                                                unsigned long long loop_index, double F[4], double I[4])
+{
+    int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
+    if (grp_2d_flux<DIR>(SL, SR, a.gamma, a.tau, a.eps, F, I)) code = 2;
+    if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
+}
+
+// X sweep: thread = x-line i, segment of TJ cells in j.
+template <int TJ>
+__global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
+{
+    if (sweep_halted(a)) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.n) return;
+    const int m = a.m;
+    const int j0 = blockIdx.y * TJ;
+    const int j1 = min(j0 + TJ, m);
+    const Line<0> L(a, i);
+    const double hh = a.hh;
+
+    double vm[4], v0[4], vp[4], sm[4], s0[4], Fp[4], Ip[4];
+    L.value(j0 - 1, vm);
+    L.cell(j0, v0);
+    if (j0 == 0) {
+        L.ghost_slope(true, sm);
+    } else {
+        double vmm[4];
+        L.value(j0 - 2, vmm);
+        L.limited(j0 - 1, vmm, vm, v0, sm);
+    }
+    for (int jf = j0; jf <= j1; ++jf) {
+        if (jf < m) {
+            L.value(jf + 1, vp);
+            L.limited(jf, vm, v0, vp, s0);
+        } else {
+            L.ghost_slope(false, s0);
+        }
+        const Side SL = recon(vm, sm, hh, true);
+        const Side SR = recon(v0, s0, hh, false);
+        double F[4], I[4];
+        solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)jf, F, I);
+        if (jf > j0) L.update(jf - 1, vm, Fp, F, Ip, I);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            vm[q] = v0[q]; v0[q] = vp[q]; sm[q] = s0[q]; Fp[q] = F[q]; Ip[q] = I[q];
+        }
+    }
+}
+
+// Y sweep: a wave owns cells c0-1 .. c0+62 of y-line j (lane t: cell c0-1+t),
+// computes interfaces c0 .. c0+63 (lane t: interface c0+t) and updates cells
+// c0 .. c0+62 (lane t >= 1).
+__global__ void __launch_bounds__(256) sweep_y_kernel(SweepArgs a, int chunks)
+{
+    if (sweep_halted(a)) return;
+    const int lane = threadIdx.x & 63;
+    const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+    const int j = (int)(wave / chunks);
+    if (j >= a.m) return;                              // uniform per wave
+    const int c0 = (int)(wave % chunks) * 63;
+    const int n = a.n;
+    const int c = c0 - 1 + lane;
+    const Line<1> L(a, j);
+    const double hh = a.hh;
+
+    double w[4] = {1.0, 0.0, 0.0, 1.0};
+    if (c <= n) L.value(c, w);
+    double wl[4], wr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
+    if (lane == 0 && c >= 0) L.value(c - 1, wl);
+    if (lane == 63 && c + 1 <= n) L.value(c + 1, wr);
+
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (c >= 0 && c < n) L.limited(c, wl, w, wr, s);
+    else if (c == -1) L.ghost_slope(true, s);
+    else if (c == n) L.ghost_slope(false, s);
+
+    double sr[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) sr[q] = __shfl_down(s[q], 1);
+    if (lane == 63) {
+        if (c + 1 < n) {
+            double wrr[4];
+            L.value(c + 2, wrr);
+            L.limited(c + 1, w, wr, wrr, sr);
+        } else if (c + 1 == n) {
+            L.ghost_slope(false, sr);
+        }
+    }
+
+    double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
+    const int f = c + 1;
+    if (f <= n) {
+        const Side SL = recon(w, s, hh, true);
+        const Side SR = recon(wr, sr, hh, false);
+        solve_interface<1>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I);
+    }
+    double Fl[4], Il[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
+    if (lane >= 1 && c >= 0 && c < n) L.update(c, w, Fl, F, Il, I);
+}
+
+// min over cells of min(h_x, h_y) / (|c| + |u| + |v|)  (grp_solver_2D_split_EUL_source.c:185-191).
+// fmin is exact, so the order of the reduction does not matter; positive
+// doubles order like their bit patterns, so a 64-bit atomicMin reduces them.
+__global__ void __launch_bounds__(256) cfl_kernel(const double *rho, const double *u, const double *v,
+                                                  const double *p, int m, int n, int ld, double gamma,
+                                                  double hmin, unsigned long long *out)
+{
+    double best = INFINITY;
+    const long total = (long)m * n;
+    for (long k = (long)blockIdx.x * blockDim.x + threadIdx.x; k < total; k += (long)gridDim.x * blockDim.x) {
+        const long j = k / n, i = k % n;
+        const size_t o = (size_t)j * ld + i;
+        const double c = sqrt(gamma * p[o] / rho[o]);
+        const double sigma = fabs(c) + fabs(u[o]) + fabs(v[o]);
+        best = fmin(best, hmin / sigma);
+    }
+    for (int off = 32; off > 0; off >>= 1) best = fmin(best, __shfl_xor(best, off));
+    if ((threadIdx.x & 63) == 0) {
+        unsigned long long bits = __double_as_longlong(best);
+        if (!(best >= 0.0)) bits = 0x7ff0000000000000ull;   // NaN never wins a fmin
+        atomicMin(out, bits);
+    }
+}
+
+// ---- launchers -----------------------------------------------------------------
+static constexpr int TJ_X = 32;
+
+hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
+{
+    const int bx = a.n >= 256 ? 256 : (a.n >= 128 ? 128 : 64);
+    dim3 grid((a.n + bx - 1) / bx, (a.m + TJ_X - 1) / TJ_X);
+    hipLaunchKernelGGL(sweep_x_kernel<TJ_X>, grid, dim3(bx), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st)
+{
+    const int chunks = (a.n + 62) / 63;
+    const long waves = (long)a.m * chunks;
+    const long blocks = (waves + 3) / 4;
+    hipLaunchKernelGGL(sweep_y_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+    return hipGetLastError();
+}
+
+hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
+                      int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st)
+{
+    const long total = (long)m * n;
+    long blocks = (total + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(cfl_kernel, dim3((unsigned)blocks), dim3(256), 0, st, rho, u, v, p, m, n, ld, gamma, hmin, out);
+    return hipGetLastError();
+}
+
+}  // namespace hgrp

@@ -1,0 +1,43 @@
+// sweep2d.hpp -- fused directional sweeps of the dimension-split 2D GRP step.
+//
+// One launch performs, for one direction, everything the reference spreads over
+// bound_cond_slope_limiter_{x,y} + flux_generator_{x,y} + the update loop of
+// GRP_solver_2D_split_EUL_source (src/finite_volume/grp_solver_2D_split_EUL_source.c:214-303):
+// ghost values, minmod slope limiting, interface reconstruction, the GRP solve,
+// the conservative update and the slope refresh from the t+tau interface
+// values.  State is read from one buffer set and written to another (ping-pong),
+// so neighbouring tiles never see half-updated cells and an aborted sweep
+// leaves its input intact.
+//
+// Grid layout in HBM: SoA, cell (j, i) at j*ld + i, i (y index) contiguous,
+// ld >= n a multiple of 32 doubles.  Fields: rho u v p e, x-slopes sx[4],
+// y-slopes sy[4] (rho u v p).
+#pragma once
+#include <stdint.h>
+
+namespace hgrp {
+
+// boundary condition codes (config[17], config[18]; doc/config.csv)
+enum { BC_INITIAL = -1, BC_REFLECT = -2, BC_FREE = -4, BC_PERIODIC = -7, BC_REFLECT_FREE = -24 };
+
+struct SweepStatus {
+    unsigned long long fail_key;   // max over failing interfaces of ((loop index << 2) | code) + 1
+    unsigned int flags;            // bit 0: positivity failure in the update, bit 1: skipped (halted)
+    unsigned int pad;
+};
+
+struct SweepArgs {
+    const double *rho, *u, *v, *p, *e;        // state in
+    double *rho_o, *u_o, *v_o, *p_o, *e_o;    // state out
+    const double *s[4];                       // slopes of this direction (rho u v p), in
+    double *s_o[4];                           // refreshed slopes, out
+    const double *bnd_lo, *bnd_hi;            // BC -1 ghost values from the initial data: [line*4 + q]
+    int m, n, ld;                             // cells in x, in y, row pitch
+    int bc;                                   // boundary condition along the sweep
+    int limiter3;                             // 0: minmod2 (first sweep of this direction), 1: minmod3
+    double h, hh, tau, nu, gamma, eps, alpha; // h, h/2, flux time, update factor, ...
+    SweepStatus *status;                      // this sweep
+    const SweepStatus *prev;                  // the sweep before (nullptr for the first)
+};
+
+}  // namespace hgrp

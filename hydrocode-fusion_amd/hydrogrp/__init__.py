@@ -1,0 +1,170 @@
+"""Python access to libhydrogrp.so (the gfx950 Eulerian GRP library) via ctypes.
+
+The library's C ABI is declared in include/hydrocode_grp.h and
+include/hydrocode_dropin.h; this module only marshals numpy arrays.  There is
+no CPU fallback: if the shared library is missing, importing ``lib()`` raises.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libhydrogrp.so")
+N_CONF = 400
+
+_D, _DP, _IP = C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)
+
+
+class Result(C.Structure):
+    _fields_ = [("k", C.c_int), ("nt", C.c_int), ("status", C.c_int), ("sweeps", C.c_int),
+                ("time_c", _D), ("tau", _D), ("seconds", _D)]
+
+
+FRAME_FN = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.c_int, _D)
+
+_lib = None
+
+
+def lib():
+    """Load libhydrogrp.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `make -C hydrocode-fusion_amd` "
+                           "(or __graft_entry__.build()); there is no CPU fallback")
+    L = C.CDLL(LIB_PATH)
+    L.hgrp2d_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, C.c_int, _DP]
+    L.hgrp2d_destroy.argtypes = [C.c_void_p]
+    L.hgrp2d_upload.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
+    L.hgrp2d_download.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
+    L.hgrp2d_solve.argtypes = [C.c_void_p, _DP, C.c_int, C.c_int, FRAME_FN, C.c_void_p, C.POINTER(Result)]
+    L.hgrp2d_bench.argtypes = [C.c_void_p, C.c_int, _DP, _DP, _DP]
+    L.hgrp2d_h_over_smax.argtypes = [C.c_void_p, _DP]
+    L.hgrp_grp2d_flux_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _DP]
+    L.hgrp_riemann_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
+    L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
+    L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
+    _lib = L
+    return L
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_DP)
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed with code {rc}")
+
+
+def device_name():
+    buf = C.create_string_buffer(256)
+    lib().hgrp_device_name(buf, 256)
+    return buf.value.decode()
+
+
+class Solver2D:
+    """Dimension-split 2D GRP solver resident on the GPU (hgrp2d_* entry points)."""
+
+    def __init__(self, m, n, config):
+        self.m, self.n = m, n
+        self.config = np.ascontiguousarray(config, dtype=np.float64)
+        assert self.config.size == N_CONF
+        self._h = C.c_void_p()
+        _check(lib().hgrp2d_create(C.byref(self._h), m, n, _p(self.config)), "hgrp2d_create")
+
+    def close(self):
+        if self._h:
+            lib().hgrp2d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, rho, u, v, p, e=None):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).reshape(self.m, self.n) for a in (rho, u, v, p)]
+        ee = None if e is None else np.ascontiguousarray(e, dtype=np.float64).reshape(self.m, self.n)
+        _check(lib().hgrp2d_upload(self._h, *[_p(a) for a in arrs], _p(ee)), "hgrp2d_upload")
+
+    def download(self):
+        out = np.empty((5, self.m, self.n))
+        _check(lib().hgrp2d_download(self._h, *[_p(out[f]) for f in range(5)]), "hgrp2d_download")
+        return out
+
+    def h_over_smax(self):
+        h = _D()
+        _check(lib().hgrp2d_h_over_smax(self._h, C.byref(h)), "hgrp2d_h_over_smax")
+        return h.value
+
+    def solve(self, time_plot, N_T=None, on_frame=None):
+        """Run the reference's time loop; on_frame(nt, nt_plot, time_c) sees each plot frame."""
+        tp = np.ascontiguousarray(time_plot, dtype=np.float64)
+        if N_T is None:
+            N_T = min(5, len(tp))
+        res = Result()
+        cb = FRAME_FN(lambda user, nt, ntp, tc: int(bool(on_frame(nt, ntp, tc))) if on_frame else 0)
+        _check(lib().hgrp2d_solve(self._h, _p(tp), len(tp), N_T, cb, None, C.byref(res)), "hgrp2d_solve")
+        return res
+
+    def bench(self, steps):
+        t, x, y = _D(), _D(), _D()
+        _check(lib().hgrp2d_bench(self._h, steps, C.byref(t), C.byref(x), C.byref(y)), "hgrp2d_bench")
+        return t.value, x.value, y.value
+
+
+def run_split2d(m, n, init, config, time_plot=None, N_T=None):
+    """Whole run like the oracle's split2d(): returns (frames, N_plot, time_plot, config, status)."""
+    config = np.array(config, dtype=np.float64)
+    if time_plot is None:
+        time_plot = np.array([0.0, config[1]])
+    time_plot = np.array(time_plot, dtype=np.float64)
+    if N_T is None:
+        N_T = min(5, len(time_plot))
+    s = Solver2D(m, n, config)
+    try:
+        s.upload(init["rho"], init["u"], init["v"], init["p"], init.get("e"))
+        frames = np.zeros((5, N_T, m, n))
+        frames[:, 0] = s.download()
+
+        def on_frame(nt, ntp, tc):
+            frames[:, nt] = s.download()
+            return 0
+
+        res = s.solve(time_plot, N_T, on_frame)
+        frames[:, res.nt] = s.download()
+    finally:
+        s.close()
+    config[5] = res.k
+    time_plot[res.nt] = res.time_c
+    return frames, res.nt + 1, time_plot, config, res
+
+
+def flux_batch(L, R, tau, eps, direction):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    out = np.zeros((len(L), 9))
+    _check(lib().hgrp_grp2d_flux_batch(_p(L), _p(R), len(L), tau, eps, direction, _p(out)), "flux batch")
+    return out
+
+
+def riemann_batch(inp, N):
+    inp = np.ascontiguousarray(inp, dtype=np.float64)
+    out = np.zeros((len(inp), 5))
+    _check(lib().hgrp_riemann_batch(_p(inp), len(inp), N, _p(out)), "riemann batch")
+    return out
+
+
+def pow_batch(x, y):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.zeros(len(x))
+    _check(lib().hgrp_pow_batch(_p(x), _p(y), len(x), _p(out)), "pow batch")
+    return out

@@ -1,0 +1,108 @@
+/*
+ * hydrocode_grp.h -- C ABI of the MI355X (gfx950) Eulerian GRP library.
+ *
+ * Plain C types only (pointers, sizes, doubles).  Two groups of entry points:
+ *
+ *  1. Drop-in replacements with the reference's names and signatures; see
+ *     hydrocode_dropin.h (they need the reference's struct layouts).
+ *
+ *  2. The device-resident solver and batched interface solves that the
+ *     drop-ins are built on.  They have no single counterpart in the reference:
+ *     each names the reference routine whose semantics it implements.
+ *
+ * Every function returns 0 on success, a negative value for a HIP/runtime
+ * error (-(hipError_t) or -1000 - code for argument errors), and never exits
+ * the process.  All calls on one context are synchronous on return unless
+ * stated otherwise; contexts are not thread-safe.
+ */
+#ifndef HYDROCODE_GRP_H
+#define HYDROCODE_GRP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HGRP_N_CONF 400
+
+/* ------------------------------------------------------------------------- */
+/* 2D dimension-split Eulerian GRP solver resident on one GPU.                */
+/* Implements GRP_solver_2D_split_EUL_source                                  */
+/* (src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379) on device      */
+/* buffers.  Host arrays handed in/out are contiguous m*n doubles with cell   */
+/* (j, i) at j*n + i, the reference's CV[k].X[j][i] with its rows packed.     */
+/* ------------------------------------------------------------------------- */
+typedef struct hgrp2d hgrp2d;
+
+typedef struct hgrp2d_result {
+    int    k;          /* the reference's step counter k on exit (config[5])          */
+    int    nt;         /* index of the frame holding the final state (N_plot - 1)    */
+    int    status;     /* 0 normal end, 1 aborted (goto return_NULL), 2 stop flag set  */
+    int    sweeps;     /* directional sweeps executed                               */
+    double time_c;     /* simulated time reached                                     */
+    double tau;        /* last time step                                             */
+    double seconds;    /* wall time of the step loop (excludes uploads)              */
+} hgrp2d_result;
+
+/* Called when the reference would write a plot frame (grp_solver_2D_split_EUL_source.c:157-176):
+ * the device state is the state at that moment; the callback may call
+ * hgrp2d_download().  nt: frame index the state belongs to; nt_plot: the
+ * time_plot[] entry that triggered it.  Return non-zero to abort the run. */
+typedef int (*hgrp2d_frame_fn)(void *user, int nt, int nt_plot, double time_c);
+
+/* config: the reference's config[] (HGRP_N_CONF doubles, src/include/var_struc.h:44);
+ * indices read: 1 t_all, 4 eps, 5 max steps, 6 gamma, 7 CFL, 10/11 h_x/h_y,
+ * 16 fixed tau, 17/18 boundary conditions, 41 minmod alpha. */
+int hgrp2d_create(hgrp2d **ctx, int m, int n, const double *config);
+int hgrp2d_destroy(hgrp2d *ctx);
+
+/* Initial data (frame 0).  e may be NULL: then E is formed as the reference's
+ * main program does (src/hydrocode_2D/hydrocode.c:251-252). */
+int hgrp2d_upload(hgrp2d *ctx, const double *rho, const double *u, const double *v,
+                  const double *p, const double *e);
+/* Current state (rho, u, v, p, E); any pointer may be NULL. */
+int hgrp2d_download(hgrp2d *ctx, double *rho, double *u, double *v, double *p, double *e);
+
+/* The reference's whole time loop: frozen-CFL tau, X(tau/2) Y(tau) X(tau/2)
+ * schedule, plot frames at time_plot[0..N_plot-1] (N_T frames kept), stop and
+ * abort rules.  fn may be NULL. */
+int hgrp2d_solve(hgrp2d *ctx, const double *time_plot, int N_plot, int N_T,
+                 hgrp2d_frame_fn fn, void *user, hgrp2d_result *res);
+
+/* Benchmark form: `steps` full steps X(tau/2) Y(tau) X(tau/2) with the frozen
+ * tau = CFL * h/S_max, no time clipping, plot frames or host syncs inside.
+ * ms_total: device time of all steps (HIP events on the solver's stream);
+ * ms_x, ms_y (may be NULL): average device time of one X / one Y sweep launch. */
+int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double *ms_y);
+
+/* h/S_max of the uploaded initial data (the frozen CFL reduction). */
+int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
+
+/* ------------------------------------------------------------------------- */
+/* Batched interface solves (host arrays in/out; for unit parity).            */
+/* Interface sides are 15 doubles: gamma, lambda_u, lambda_v, rho, u, v, p,   */
+/* d_rho, d_u, d_v, d_p, t_rho, t_u, t_v, t_p.                                */
+/* ------------------------------------------------------------------------- */
+
+/* GRP_2D_flux (src/flux_calc/flux_solver.c:158-237) for the split path
+ * (lambda = 0, tangential derivatives taken as zero): dir 0 = x, 1 = y.
+ * out: count x 9 = check code, F_rho F_u F_v F_e, RHO_int U_int V_int P_int. */
+int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double tau,
+                          double eps, int dir, double *out);
+
+/* Riemann_solver_exact (src/riemann_solver/riemann_solver_exact_Ben.c:30-211).
+ * in: count x 10 = gammaL gammaR u_L u_R p_L p_R c_L c_R eps tol;
+ * out: count x 5 = U_star P_star CRW0 CRW1 Newton-iterations. */
+int hgrp_riemann_batch(const double *in, int count, int N, double *out);
+
+/* pow(x[k], y[k]) as evaluated on the device (precision check of the GPU's FP64 pow). */
+int hgrp_pow_batch(const double *x, const double *y, int count, double *out);
+
+/* Name of the device the library runs on (empty if none). */
+int hgrp_device_name(char *buf, int len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

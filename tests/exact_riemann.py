@@ -1,0 +1,111 @@
+"""Exact solution of the Riemann problem for the 1D Euler equations (gamma law).
+
+Textbook construction (Toro, "Riemann Solvers and Numerical Methods for Fluid
+Dynamics", ch. 4): the star pressure solves f_L(p) + f_R(p) + (u_R - u_L) = 0
+with the shock (Rankine-Hugoniot) or rarefaction (isentrope + Riemann
+invariant) branch on each side; the solution is sampled at x/t.  Written
+independently of the reference's iteration so that it can pin it.  Pressure is
+solved by safeguarded Newton to machine precision.
+"""
+import math
+
+import numpy as np
+
+
+def _f(p, rho, pk, c, g):
+    """Velocity jump across one wave and its derivative in p."""
+    if p > pk:                                     # shock
+        A = 2.0 / ((g + 1.0) * rho)
+        B = (g - 1.0) / (g + 1.0) * pk
+        s = math.sqrt(A / (p + B))
+        return (p - pk) * s, s * (1.0 - 0.5 * (p - pk) / (p + B))
+    # rarefaction
+    r = (p / pk) ** ((g - 1.0) / (2.0 * g))
+    return 2.0 * c / (g - 1.0) * (r - 1.0), 1.0 / (rho * c) * (p / pk) ** (-(g + 1.0) / (2.0 * g))
+
+
+def star_state(rhoL, uL, pL, rhoR, uR, pR, g=1.4):
+    cL, cR = math.sqrt(g * pL / rhoL), math.sqrt(g * pR / rhoR)
+    du = uR - uL
+    if 2.0 * (cL + cR) / (g - 1.0) <= du:
+        raise ValueError("vacuum")
+    lo, hi = 0.0, max(pL, pR)
+    while _f(hi, rhoL, pL, cL, g)[0] + _f(hi, rhoR, pR, cR, g)[0] + du < 0:
+        hi *= 2.0
+    p = 0.5 * (pL + pR)
+    for _ in range(200):
+        fl, dl = _f(p, rhoL, pL, cL, g)
+        fr, dr = _f(p, rhoR, pR, cR, g)
+        F = fl + fr + du
+        if F > 0:
+            hi = min(hi, p)
+        else:
+            lo = max(lo, p)
+        pn = p - F / (dl + dr)
+        if not (lo < pn < hi):
+            pn = 0.5 * (lo + hi)
+        if abs(pn - p) <= 1e-16 * p:
+            p = pn
+            break
+        p = pn
+    fl = _f(p, rhoL, pL, cL, g)[0]
+    fr = _f(p, rhoR, pR, cR, g)[0]
+    u = 0.5 * (uL + uR) + 0.5 * (fr - fl)
+    return p, u
+
+
+def sample(rhoL, uL, pL, rhoR, uR, pR, xi, g=1.4):
+    """(rho, u, p) at similarity coordinates xi = x / t (array)."""
+    ps, us = star_state(rhoL, uL, pL, rhoR, uR, pR, g)
+    cL, cR = math.sqrt(g * pL / rhoL), math.sqrt(g * pR / rhoR)
+    xi = np.asarray(xi, dtype=np.float64)
+    rho = np.empty_like(xi)
+    u = np.empty_like(xi)
+    p = np.empty_like(xi)
+    gm, gp = g - 1.0, g + 1.0
+    for k, s in enumerate(xi.flat):
+        if s <= us:                                            # left of contact
+            if ps > pL:
+                SL = uL - cL * math.sqrt(gp / (2 * g) * ps / pL + gm / (2 * g))
+                if s <= SL:
+                    st = (rhoL, uL, pL)
+                else:
+                    st = (rhoL * (ps / pL + gm / gp) / (gm / gp * ps / pL + 1.0), us, ps)
+            else:
+                cs = cL * (ps / pL) ** (gm / (2 * g))
+                if s <= uL - cL:
+                    st = (rhoL, uL, pL)
+                elif s >= us - cs:
+                    st = (rhoL * (ps / pL) ** (1.0 / g), us, ps)
+                else:
+                    c = 2.0 / gp * (cL + gm / 2.0 * (uL - s))
+                    uu = 2.0 / gp * (cL + gm / 2.0 * uL + s)
+                    st = (rhoL * (c / cL) ** (2.0 / gm), uu, pL * (c / cL) ** (2.0 * g / gm))
+        else:
+            if ps > pR:
+                SR = uR + cR * math.sqrt(gp / (2 * g) * ps / pR + gm / (2 * g))
+                if s >= SR:
+                    st = (rhoR, uR, pR)
+                else:
+                    st = (rhoR * (ps / pR + gm / gp) / (gm / gp * ps / pR + 1.0), us, ps)
+            else:
+                cs = cR * (ps / pR) ** (gm / (2 * g))
+                if s >= uR + cR:
+                    st = (rhoR, uR, pR)
+                elif s <= us + cs:
+                    st = (rhoR * (ps / pR) ** (1.0 / g), us, ps)
+                else:
+                    c = 2.0 / gp * (cR - gm / 2.0 * (uR - s))
+                    uu = 2.0 / gp * (-cR + gm / 2.0 * uR + s)
+                    st = (rhoR * (c / cR) ** (2.0 / gm), uu, pR * (c / cR) ** (2.0 * g / gm))
+        rho.flat[k], u.flat[k], p.flat[k] = st
+    return rho, u, p
+
+
+def sod_l1(rho, u, p, t=0.2, x0=0.5):
+    """Cell-average-free L1 errors (midpoint sampling) of a Sod-1000-type state."""
+    m = len(rho)
+    x = (np.arange(m) + 0.5) / m
+    er, eu, ep = sample(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, (x - x0) / t)
+    h = 1.0 / m
+    return np.array([np.sum(np.abs(rho - er)) * h, np.sum(np.abs(u - eu)) * h, np.sum(np.abs(p - ep)) * h])

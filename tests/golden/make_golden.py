@@ -1,0 +1,146 @@
+"""Generate the golden fixtures in tests/golden/ from the reference itself.
+
+Run in a container that has /root/reference:
+
+    make -C oracle ref && python tests/golden/make_golden.py
+
+It drives the reference's own C sources (compiled into oracle/_ref by
+oracle/Makefile, reached through oracle/ref_glue.c) on seeded inputs and writes
+inputs and outputs as compressed .npz files (plain float64 arrays, no pickles).
+The fixtures are data only; the reference cannot travel to the GPU machine,
+these vectors do.
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+from _libs import RefLib, make_config  # noqa: E402
+import cases  # noqa: E402
+
+EPS = 1e-9
+
+
+def interface_vectors(ref):
+    L, R = cases.interface_pairs(2000, seed=20240607)
+    cL = np.sqrt(L[:, 0] * L[:, 6] / L[:, 3])
+    cR = np.sqrt(R[:, 0] * R[:, 6] / R[:, 3])
+    riem = np.zeros((len(L), 5))
+    riem12 = np.zeros((len(L), 5))
+    ben = np.zeros((len(L), 5))
+    q1d = np.zeros((len(L), 16))
+    fx = np.zeros((len(L), 9))
+    fy = np.zeros((len(L), 9))
+    edir = np.zeros((len(L), 6))
+    tau = 2.5e-3
+    ref.set_config(make_config())
+    for k in range(len(L)):
+        riem[k] = ref.riemann([L[k, 0], R[k, 0], L[k, 4], R[k, 4], L[k, 6], R[k, 6], cL[k], cR[k], EPS, EPS], 500)
+        riem12[k] = ref.riemann([L[k, 0], R[k, 0], L[k, 4], R[k, 4], L[k, 6], R[k, 6], cL[k], cR[k], 1e-12, 1e-12], 500)
+        ben[k] = ref.riemann_ben([L[k, 0], L[k, 4], R[k, 4], L[k, 6], R[k, 6], cL[k], cR[k], EPS, EPS], 50)
+        q1d[k] = ref.q1d(L[k], R[k], EPS, EPS)
+        rc, out = ref.flux2d(L[k], R[k], tau, EPS, 1.0, 0.0)
+        fx[k, 0], fx[k, 1:] = rc, out
+        rc, out = ref.flux2d(L[k], R[k], tau, EPS, 0.0, 1.0)
+        fy[k, 0], fy[k, 1:] = rc, out
+        edir[k] = ref.edir(L[k], R[k], EPS, EPS)
+    np.savez_compressed(os.path.join(HERE, "interface_vectors.npz"), L=L, R=R, tau=tau, eps=EPS,
+                        riemann=riem, riemann_tol12=riem12, riemann_ben=ben, q1d=q1d,
+                        flux_x=fx, flux_y=fy, edir=edir)
+
+
+def split2d_cases(ref):
+    out = {}
+    # four-quadrant 64^2 to t = 0.3 (127 steps), full final state
+    m = n = 64
+    cfg = cases.four_quadrant_config(m, n)
+    fr, npl, tp, c = ref.split2d(m, n, cases.four_quadrant(m, n), cfg)
+    out["fq64_final"] = fr[:, npl - 1]
+    out["fq64_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    # four-quadrant 32 x 48 with 5 output frames
+    m, n = 32, 48
+    cfg = cases.four_quadrant_config(m, n, t_all=0.2)
+    tpin = np.array([0.0, 0.05, 0.1, 0.15, 0.2])
+    fr, npl, tp, c = ref.split2d(m, n, cases.four_quadrant(m, n), cfg, time_plot=tpin)
+    out["frames_all"] = fr
+    out["frames_meta"] = np.concatenate([[c[5], npl], tp])
+    # every boundary-condition pair on a 12 x 10 smooth field, t = 0.05
+    bcs = (-1, -2, -4, -7, -24)
+    init = cases.smooth_random(12, 10, 5)
+    out["bc_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    finals, metas = [], []
+    for bx in bcs:
+        for by in bcs:
+            cfg = make_config(t_all=0.05, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=1)
+            fr, npl, tp, c = ref.split2d(12, 10, init, cfg)
+            finals.append(fr[:, npl - 1])
+            metas.append([bx, by, c[5], npl, tp[npl - 1]])
+    out["bc_final"] = np.stack(finals)
+    out["bc_meta"] = np.array(metas, dtype=np.float64)
+    # 2D Sod tube 100 x 4, t = 0.2 (the survey's frozen-dt check: 53 steps)
+    m, n = 100, 4
+    x = (np.arange(m) + 0.5) / m
+    init = {"rho": np.repeat(np.where(x < 0.5, 1.0, 0.125)[:, None], n, 1),
+            "u": np.zeros((m, n)), "v": np.zeros((m, n)),
+            "p": np.repeat(np.where(x < 0.5, 1.0, 0.1)[:, None], n, 1)}
+    cfg = make_config(t_all=0.2, h_x=0.01, h_y=0.01, bc_x=-4, bc_y=-4, split=1)
+    fr, npl, tp, c = ref.split2d(m, n, init, cfg)
+    out["sod2d_final"] = fr[:, npl - 1]
+    out["sod2d_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    # fixed-tau mode (t_all infinite, config[16] > 0, 7 steps): step counter quirk
+    cfg = make_config(N=7, tau=1e-3, h_x=1 / 16, h_y=1 / 16, bc_x=-4, bc_y=-7, split=1)
+    cfg[1] = np.inf
+    init = cases.smooth_random(16, 16, 9)
+    out["fixtau_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    fr, npl, tp, c = ref.split2d(16, 16, init, cfg, time_plot=np.array([0.0, np.inf]))
+    out["fixtau_final"] = fr[:, npl - 1]
+    out["fixtau_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    # four-quadrant 256^2 to t = 0.3: too big to store whole; per-field sums,
+    # L1 norms and probe cells
+    m = n = 256
+    cfg = cases.four_quadrant_config(m, n)
+    fr, npl, tp, c = ref.split2d(m, n, cases.four_quadrant(m, n), cfg)
+    fin = fr[:, npl - 1]
+    probes = np.array([[3, 5], [100, 200], [128, 128], [200, 31], [255, 255], [77, 160], [160, 77]])
+    out["fq256_sum"] = fin.reshape(5, -1).sum(axis=1)
+    out["fq256_l1"] = np.abs(fin).reshape(5, -1).sum(axis=1)
+    out["fq256_probes"] = probes.astype(np.float64)
+    out["fq256_probe_vals"] = fin[:, probes[:, 0], probes[:, 1]]
+    out["fq256_rows"] = fin[:, ::37, :]       # every 37th x-line in full
+    out["fq256_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    np.savez_compressed(os.path.join(HERE, "split2d_cases.npz"), **out)
+
+
+def grp1d_cases(ref):
+    out = {}
+    cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
+    fr, npl, tp, c = ref.grp1d(1000, cases.sod_1d(1000), cfg)
+    out["sod1000_final"] = fr[:, npl - 1]
+    out["sod1000_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    init = {k: v[:, 0] for k, v in cases.smooth_random(200, 1, 3).items() if k != "v"}
+    out["bc_init"] = np.stack([init[k] for k in ("rho", "u", "p")])
+    finals, metas = [], []
+    for bc in (-1, -2, -4, -7, -24):
+        cfg = make_config(t_all=0.1, h_x=1 / 200, bc_x=bc, CFL=0.45)
+        fr, npl, tp, c = ref.grp1d(200, init, cfg)
+        finals.append(fr[:, npl - 1])
+        metas.append([bc, c[5], npl, tp[npl - 1]])
+    out["bc_final"] = np.stack(finals)
+    out["bc_meta"] = np.array(metas)
+    np.savez_compressed(os.path.join(HERE, "grp1d_cases.npz"), **out)
+
+
+def main():
+    ref = RefLib()
+    interface_vectors(ref)
+    split2d_cases(ref)
+    grp1d_cases(ref)
+    for f in sorted(os.listdir(HERE)):
+        if f.endswith(".npz"):
+            print(f, os.path.getsize(os.path.join(HERE, f)))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,150 @@
+"""The oracle (oracle/liboracle.so) against the reference's own outputs.
+
+Fixtures in tests/golden/ were produced by tests/golden/make_golden.py from the
+reference sources compiled in oracle/_ref.  Equality is bit-for-bit: the
+oracle restates the reference operation for operation.
+"""
+import numpy as np
+import pytest
+
+import cases
+from _libs import make_config
+
+EPS = 1e-9
+
+
+def _eq(a, b):
+    return np.array_equal(np.asarray(a), np.asarray(b), equal_nan=True)
+
+
+def test_interface_vectors_bitexact(oracle, golden_iface):
+    g = golden_iface
+    L, R, tau = g["L"], g["R"], float(g["tau"])
+    bad = []
+    for k in range(len(L)):
+        cL = np.sqrt(L[k, 0] * L[k, 6] / L[k, 3])
+        cR = np.sqrt(R[k, 0] * R[k, 6] / R[k, 3])
+        r = oracle.riemann(L[k, 0], R[k, 0], L[k, 4], R[k, 4], L[k, 6], R[k, 6], cL, cR, EPS, EPS, 500)
+        if not _eq(r[:5], g["riemann"][k]):
+            bad.append(("riemann", k))
+        r = oracle.riemann(L[k, 0], R[k, 0], L[k, 4], R[k, 4], L[k, 6], R[k, 6], cL, cR, 1e-12, 1e-12, 500)
+        if not _eq(r[:5], g["riemann_tol12"][k]):
+            bad.append(("riemann_tol12", k))
+        r = oracle.riemann_ben(L[k, 0], L[k, 4], R[k, 4], L[k, 6], R[k, 6], cL, cR, EPS, EPS, 50)
+        if not _eq(r[:5], g["riemann_ben"][k]):
+            bad.append(("riemann_ben", k))
+        if not _eq(oracle.q1d(L[k], R[k], EPS, EPS), g["q1d"][k]):
+            bad.append(("q1d", k))
+        for key, (nx, ny) in (("flux_x", (1.0, 0.0)), ("flux_y", (0.0, 1.0))):
+            rc, out = oracle.flux2d(L[k], R[k], tau, EPS, nx, ny)
+            if rc != g[key][k, 0] or (rc == 0 and not _eq(out, g[key][k, 1:])):
+                bad.append((key, k))
+        if not _eq(oracle.edir(L[k], R[k], EPS, EPS), g["edir"][k]):
+            bad.append(("edir", k))
+    assert not bad, bad[:10]
+
+
+def test_fixture_covers_every_branch(golden_iface):
+    """The vectors exercise shocks and rarefactions on both sides, the acoustic
+    shortcut, equal states and supersonic (trivial) interfaces."""
+    g = golden_iface
+    crw = g["riemann"][:, 2:4]
+    assert set(map(tuple, crw.astype(int))) >= {(0, 0), (0, 1), (1, 0), (1, 1)}
+    L, R = g["L"], g["R"]
+    dist = np.sqrt((L[:, 3] - R[:, 3]) ** 2 + (L[:, 4] - R[:, 4]) ** 2 + (L[:, 6] - R[:, 6]) ** 2)
+    assert (dist < EPS).sum() > 50 and (dist == 0).sum() > 50
+    ws = g["q1d"][:, :2]
+    assert (ws[:, 0] > 0).sum() > 20 and (ws[:, 1] < 0).sum() > 20
+    assert (g["flux_x"][:, 0] != 0).sum() >= 0   # failure codes recorded as data
+
+
+def test_split2d_four_quadrant_64(oracle, golden_2d):
+    g = golden_2d
+    fr, npl, tp, cfg, st = oracle.split2d(64, 64, cases.four_quadrant(64, 64), cases.four_quadrant_config(64, 64))
+    assert st == 0
+    assert (cfg[5], npl, tp[npl - 1]) == tuple(g["fq64_meta"])
+    assert int(cfg[5]) == 127                     # frozen-dt step count of the survey's run
+    assert _eq(fr[:, npl - 1], g["fq64_final"])
+
+
+def test_split2d_frames(oracle, golden_2d):
+    g = golden_2d
+    tpin = np.array([0.0, 0.05, 0.1, 0.15, 0.2])
+    fr, npl, tp, cfg, st = oracle.split2d(32, 48, cases.four_quadrant(32, 48),
+                                          cases.four_quadrant_config(32, 48, t_all=0.2), time_plot=tpin)
+    meta = g["frames_meta"]
+    assert (cfg[5], npl) == (meta[0], meta[1])
+    assert _eq(tp, meta[2:])
+    assert _eq(fr, g["frames_all"])
+
+
+def test_split2d_all_boundary_conditions(oracle, golden_2d):
+    g = golden_2d
+    init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))
+    for k, (bx, by, steps, npl, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.05, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=1)
+        fr, n_plot, tp, c, st = oracle.split2d(12, 10, init, cfg)
+        assert (c[5], n_plot, tp[n_plot - 1]) == (steps, npl, t_end), (bx, by)
+        assert _eq(fr[:, n_plot - 1], g["bc_final"][k]), (bx, by)
+
+
+def test_split2d_sod_and_fixed_tau(oracle, golden_2d):
+    g = golden_2d
+    m, n = 100, 4
+    x = (np.arange(m) + 0.5) / m
+    init = {"rho": np.repeat(np.where(x < 0.5, 1.0, 0.125)[:, None], n, 1), "u": np.zeros((m, n)),
+            "v": np.zeros((m, n)), "p": np.repeat(np.where(x < 0.5, 1.0, 0.1)[:, None], n, 1)}
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, make_config(t_all=0.2, h_x=0.01, h_y=0.01, bc_x=-4, bc_y=-4, split=1))
+    assert int(c[5]) == 53 == int(g["sod2d_meta"][0])
+    assert _eq(fr[:, npl - 1], g["sod2d_final"])
+
+    cfg = make_config(N=7, tau=1e-3, h_x=1 / 16, h_y=1 / 16, bc_x=-4, bc_y=-7, split=1)
+    cfg[1] = np.inf
+    init = dict(zip(("rho", "u", "v", "p"), g["fixtau_init"]))
+    fr, npl, tp, c, st = oracle.split2d(16, 16, init, cfg, time_plot=np.array([0.0, np.inf]))
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["fixtau_meta"])
+    assert int(c[5]) == 8                         # N = 7 steps leave k = N + 1
+    assert _eq(fr[:, npl - 1], g["fixtau_final"])
+
+
+def test_split2d_four_quadrant_256(oracle, golden_2d):
+    g = golden_2d
+    fr, npl, tp, c, st = oracle.split2d(256, 256, cases.four_quadrant(256, 256), cases.four_quadrant_config(256, 256))
+    fin = fr[:, npl - 1]
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["fq256_meta"])
+    probes = g["fq256_probes"].astype(int)
+    assert _eq(fin[:, probes[:, 0], probes[:, 1]], g["fq256_probe_vals"])
+    assert _eq(fin[:, ::37, :], g["fq256_rows"])
+    assert _eq(fin.reshape(5, -1).sum(axis=1), g["fq256_sum"])
+
+
+def test_grp1d_sod_and_boundaries(oracle, golden_1d):
+    g = golden_1d
+    cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
+    fr, npl, tp, c, st = oracle.grp1d(1000, cases.sod_1d(1000), cfg)
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["sod1000_meta"])
+    assert int(c[5]) == 975
+    assert _eq(fr[:, npl - 1], g["sod1000_final"])
+    init = dict(zip(("rho", "u", "p"), g["bc_init"]))
+    for k, (bc, steps, npl0, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.1, h_x=1 / 200, bc_x=bc, CFL=0.45)
+        fr, npl, tp, c, st = oracle.grp1d(200, init, cfg)
+        assert (c[5], npl, tp[npl - 1]) == (steps, npl0, t_end), bc
+        assert _eq(fr[:, npl - 1], g["bc_final"][k]), bc
+
+
+def test_against_live_reference(oracle, reflib):
+    """Fresh random vectors and runs (not in the fixtures) against the reference build."""
+    L, R = cases.interface_pairs(3000, seed=99)
+    for k in range(len(L)):
+        assert _eq(oracle.q1d(L[k], R[k], EPS, EPS), reflib.q1d(L[k], R[k], EPS, EPS)), k
+        rc, o = oracle.flux2d(L[k], R[k], 1e-3, EPS, 0.0, 1.0)
+        rr, r = reflib.flux2d(L[k], R[k], 1e-3, EPS, 0.0, 1.0)
+        assert rc == rr and (rc != 0 or _eq(o, r)), k
+        assert _eq(oracle.edir(L[k], R[k], EPS, EPS), reflib.edir(L[k], R[k], EPS, EPS)), k
+    for (m, n, seed) in ((7, 9, 1), (1, 13, 2), (13, 1, 3), (33, 31, 4)):
+        cfg = make_config(t_all=0.03, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-2, split=1)
+        init = cases.smooth_random(m, n, seed)
+        fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg)
+        fr, npr, tpr, cr = reflib.split2d(m, n, init, cfg)
+        assert co[5] == cr[5] and _eq(fo[:, :npo], fr[:, :npr]), (m, n)

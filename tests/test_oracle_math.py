@@ -1,0 +1,277 @@
+"""Pins of the oracle that do not depend on the reference's own outputs:
+closed forms (Sod star state, Rankine-Hugoniot / isentrope relations), limits
+(acoustic switch continuity), the Euler equations on the trivial branches,
+conservation, symmetry/invariance and the scheme's convergence order.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import exact_riemann as ex
+from _libs import make_config
+
+
+def sound(g, p, rho):
+    return math.sqrt(g * p / rho)
+
+
+def test_sod_star_state_closed_form(oracle):
+    g = 1.4
+    cL, cR = sound(g, 1.0, 1.0), sound(g, 0.1, 0.125)
+    us, ps, crw0, crw1, gap, nit = oracle.riemann(g, g, 0.0, 0.0, 1.0, 0.1, cL, cR, 1e-14, 1e-14, 500)
+    p_ex, u_ex = ex.star_state(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, g)
+    assert abs(p_ex - 0.30313) < 5e-6 and abs(u_ex - 0.92745) < 5e-6       # Toro, table 4.2
+    assert abs(ps - p_ex) <= 1e-13 * p_ex and abs(us - u_ex) <= 1e-13
+    assert (crw0, crw1) == (1, 0)                                           # left fan, right shock
+    # default tolerance (eps = 1e-9 on the velocity gap)
+    us, ps, *_ = oracle.riemann(g, g, 0.0, 0.0, 1.0, 0.1, cL, cR, 1e-9, 1e-9, 500)
+    assert abs(ps - p_ex) < 1e-8 and abs(us - u_ex) < 1e-8
+
+
+def test_star_state_against_independent_solver(oracle):
+    L, R = cases.interface_pairs(1500, seed=5)
+    g = 1.4
+    checked = 0
+    for k in range(len(L)):
+        rl, ul, pl = L[k, 3], L[k, 4], L[k, 6]
+        rr, ur, pr = R[k, 3], R[k, 4], R[k, 6]
+        cl, cr = sound(g, pl, rl), sound(g, pr, rr)
+        if ur - ul >= 2 * (cl + cr) / (g - 1) * 0.999 or (ul == ur and pl == pr):
+            continue
+        us, ps, crw0, crw1, gap, nit = oracle.riemann(g, g, ul, ur, pl, pr, cl, cr, 1e-13, 1e-13, 500)
+        p_ex, u_ex = ex.star_state(rl, ul, pl, rr, ur, pr, g)
+        scale_u = max(cl, cr, abs(ul), abs(ur))
+        assert abs(ps - p_ex) <= 1e-9 * p_ex, k
+        assert abs(us - u_ex) <= 1e-9 * scale_u, k
+        # wave types from the star pressure (outside the eps margin)
+        if abs(p_ex - pl) > 1e-6 * pl:
+            assert crw0 == (p_ex < pl), k
+        if abs(p_ex - pr) > 1e-6 * pr:
+            assert crw1 == (p_ex < pr), k
+        checked += 1
+    assert checked > 1000
+
+
+def test_star_densities_rankine_hugoniot_and_isentrope(oracle):
+    L, R = cases.interface_pairs(800, seed=6)
+    g = 1.4
+    z = (g - 1) / (g + 1)
+    n = 0
+    for k in range(len(L)):
+        dist = math.sqrt((L[k, 3] - R[k, 3]) ** 2 + (L[k, 4] - R[k, 4]) ** 2 + (L[k, 6] - R[k, 6]) ** 2)
+        if dist < 1e-9:
+            continue
+        out = oracle.q1d(L[k], R[k], 1e-9, 1e-9)
+        rsL, us, rsR, ps, csL, csR = out[10:16]
+        for rho, p, rs, cs in ((L[k, 3], L[k, 6], rsL, csL), (R[k, 3], R[k, 6], rsR, csR)):
+            if ps > p * (1 + 1e-6):       # shock: Hugoniot density ratio
+                expect = rho * (ps / p + z) / (z * ps / p + 1.0)
+            elif ps < p * (1 - 1e-6):     # rarefaction: isentrope
+                expect = rho * (ps / p) ** (1.0 / g)
+            else:
+                continue
+            assert abs(rs - expect) <= 1e-12 * expect, k
+            assert abs(cs - math.sqrt(g * ps / rs)) <= 1e-12 * cs, k
+            n += 1
+    assert n > 500
+
+
+def test_equal_and_symmetric_states(oracle):
+    g = 1.4
+    for (r, u, p) in ((1.0, 0.3, 2.0), (0.1, -3.0, 0.01)):
+        c = sound(g, p, r)
+        us, ps, c0, c1, gap, nit = oracle.riemann(g, g, u, u, p, p, c, c, 1e-9, 1e-9, 500)
+        assert (us, ps, nit) == (u, p, -1)
+    for (r, u, p) in ((1.0, 0.7, 1.0), (2.0, -0.4, 0.5)):
+        c = sound(g, p, r)
+        us, ps, *_ = oracle.riemann(g, g, u, -u, p, p, c, c, 1e-12, 1e-12, 500)
+        assert abs(us) <= 1e-14 * c
+        assert (ps > p) == (u > 0)
+
+
+def _ifv(rho, u, v, p, d, t=(0, 0, 0, 0), g=1.4):
+    return np.array([g, 0.0, 0.0, rho, u, v, p, *d, *t], dtype=np.float64)
+
+
+def test_trivial_branches_are_the_euler_equations(oracle):
+    """Outside the fan the GRP derivative is the primitive-form Euler system
+    rho_t = -u rho_x - v rho_y - rho (u_x + v_y),  u_t = -u u_x - v u_y - p_x / rho,
+    v_t = -u v_x - v v_y - p_y / rho,  p_t = -u p_x - v p_y - rho c^2 (u_x + v_y)."""
+    rng = np.random.default_rng(11)
+    for _ in range(200):
+        rho, p, v = rng.uniform(0.5, 2), rng.uniform(0.5, 2), rng.uniform(-1, 1)
+        c = sound(1.4, p, rho)
+        u = c * rng.uniform(1.2, 3.0)                     # supersonic to the right
+        d, t = rng.normal(size=4), rng.normal(size=4)
+        Lv = _ifv(rho, u, v, p, d, t)
+        Rv = _ifv(rho * 1.1, u * 1.05, v, p * 0.9, rng.normal(size=4), rng.normal(size=4))
+        out = oracle.q1d(Lv, Rv, 1e-9, 1e-9)
+        D, U = out[2:6], out[6:10]
+        rx, ux, vx, px = d
+        ry, uy, vy, py = t
+        expect = [-u * rx - v * ry - rho * (ux + vy), -u * ux - v * uy - px / rho,
+                  -u * vx - v * vy - py / rho, -u * px - v * py - rho * c * c * (ux + vy)]
+        assert np.allclose(D, expect, rtol=1e-13, atol=1e-13)
+        assert np.array_equal(U, [rho, u, v, p])
+
+
+def test_grp_derivative_continuous_across_acoustic_switch(oracle):
+    """Weak jumps just above eps take the nonlinear (Riemann) branches, jumps
+    below eps the acoustic one; the time derivatives must agree in the limit."""
+    rng = np.random.default_rng(12)
+    n = 0
+    for _ in range(400):
+        rho, p = rng.uniform(0.5, 2), rng.uniform(0.5, 2)
+        c = sound(1.4, p, rho)
+        u = c * rng.uniform(-0.8, 0.8)
+        v = rng.uniform(-1, 1)
+        dL, dR = rng.normal(size=4), rng.normal(size=4)
+        direction = rng.normal(size=3)
+        direction /= np.linalg.norm(direction)
+        outs = []
+        for jump in (3e-7, 3e-11):
+            dr, du, dp = direction * jump
+            Lv = _ifv(rho, u, v, p, dL)
+            Rv = _ifv(rho + dr, u + du, v, p + dp, dR)
+            outs.append(oracle.q1d(Lv, Rv, 1e-9, 1e-9))
+        Dn, Da = outs[0][2:6], outs[1][2:6]
+        scale = np.max(np.abs(Da)) + 1.0
+        assert np.max(np.abs(Dn - Da)) <= 1e-4 * scale
+        n += 1
+    assert n == 400
+
+
+def _fields(fr, npl):
+    return fr[:, npl - 1]
+
+
+def test_uniform_state_and_line_invariance(oracle):
+    m, n = 24, 17
+    init = {"rho": np.full((m, n), 1.3), "u": np.full((m, n), 0.4), "v": np.full((m, n), -0.2),
+            "p": np.full((m, n), 0.8)}
+    cfg = make_config(t_all=0.05, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-4, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    fin = _fields(fr, npl)
+    for f, val in ((0, 1.3), (1, 0.4), (2, -0.2), (3, 0.8)):
+        assert np.max(np.abs(fin[f] - val)) <= 1e-14 * max(1.0, abs(val))
+    # data that only varies in x: every y-column of the result is the same, bit for bit
+    base = cases.smooth_random(m, 1, 4)
+    init = {k: np.repeat(v, n, axis=1) for k, v in base.items()}
+    cfg = make_config(t_all=0.05, h_x=1 / m, h_y=1 / n, bc_x=-2, bc_y=-7, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    fin = _fields(fr, npl)
+    assert all(np.array_equal(fin[:, :, i], fin[:, :, 0]) for i in range(n))
+    # and the transposed case: only y-variation -> every x-line identical
+    base = cases.smooth_random(1, n, 4)
+    init = {k: np.repeat(v, m, axis=0) for k, v in base.items()}
+    cfg = make_config(t_all=0.05, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-24, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    fin = _fields(fr, npl)
+    assert all(np.array_equal(fin[:, j, :], fin[:, 0, :]) for j in range(m))
+
+
+def test_conservation_with_periodic_boundaries(oracle):
+    m, n = 40, 36
+    init = cases.smooth_random(m, n, 8, amp=0.25)
+    cfg = make_config(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-7, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    def totals(f):
+        rho, u, v, p, e = f
+        return np.array([rho.sum(), (rho * u).sum(), (rho * v).sum(), (rho * e).sum()])
+    t0, t1 = totals(fr[:, 0]), totals(fr[:, npl - 1])
+    assert int(c[5]) > 5
+    assert np.all(np.abs(t1 - t0) <= 1e-12 * (np.abs(t0) + m * n))
+
+
+def _advect_1d(oracle, m):
+    x = (np.arange(m) + 0.5) / m
+    init = {"rho": 1.0 + 0.2 * np.sin(2 * np.pi * x), "u": np.ones(m), "p": np.ones(m)}
+    cfg = make_config(t_all=0.2, h_x=1.0 / m, bc_x=-7, CFL=0.45)
+    fr, npl, tp, c, st = oracle.grp1d(m, init, cfg)
+    exact = 1.0 + 0.2 * np.sin(2 * np.pi * (x - tp[npl - 1]))
+    return np.sum(np.abs(fr[0, npl - 1] - exact)) / m
+
+
+def _advect_2d(oracle, m):
+    x = (np.arange(m) + 0.5) / m
+    X, Y = np.meshgrid(x, x, indexing="ij")
+    init = {"rho": 1.0 + 0.2 * np.sin(2 * np.pi * (X + Y)), "u": np.ones((m, m)), "v": np.ones((m, m)),
+            "p": np.ones((m, m))}
+    cfg = make_config(t_all=0.1, h_x=1.0 / m, h_y=1.0 / m, bc_x=-7, bc_y=-7, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, m, init, cfg)
+    t = tp[npl - 1]
+    # the final step has no trailing X(tau/2) (grp_solver_2D_split_EUL_source.c:312-313),
+    # so x-advection has run for t - tau_last/2 while y-advection has run for t
+    h_S_max = np.min((1.0 / m) / (np.sqrt(1.4 * init["p"] / init["rho"]) + 2.0))
+    tau0 = 0.45 * h_S_max
+    tau_last = t - (int(c[5]) - 1) * tau0
+    exact = 1.0 + 0.2 * np.sin(2 * np.pi * (X - (t - 0.5 * tau_last) + Y - t))
+    return np.sum(np.abs(fr[0, npl - 1] - exact)) / (m * m)
+
+
+def test_second_order_convergence_1d(oracle):
+    errs = [_advect_1d(oracle, m) for m in (50, 100, 200, 400)]
+    orders = [math.log2(errs[k] / errs[k + 1]) for k in range(3)]
+    assert min(orders[1:]) > 1.7, (errs, orders)
+
+
+def test_second_order_convergence_2d(oracle):
+    errs = [_advect_2d(oracle, m) for m in (24, 48, 96)]
+    orders = [math.log2(errs[k] / errs[k + 1]) for k in range(2)]
+    assert orders[-1] > 1.7, (errs, orders)
+
+
+def test_sod_1d_l1_error(oracle, golden_1d):
+    """L1 error of the 1000-cell Sod run against the exact solution; identical
+    for the oracle and for the reference's own final state."""
+    cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
+    fr, npl, tp, c, st = oracle.grp1d(1000, cases.sod_1d(1000), cfg)
+    l1 = ex.sod_l1(fr[0, npl - 1], fr[1, npl - 1], fr[2, npl - 1], t=tp[npl - 1])
+    ref = golden_1d["sod1000_final"]
+    l1_ref = ex.sod_l1(ref[0], ref[1], ref[2], t=0.2)
+    assert np.array_equal(l1, l1_ref)
+    assert np.all(l1 < [4e-3, 6e-3, 3e-3]), l1
+
+
+def test_error_paths(oracle):
+    """Fail-stop rules: negative pressure aborts before the update; a time step
+    below eps stops after one x half-sweep; an unknown boundary aborts."""
+    m, n = 8, 6
+    # p below eps but positive: only the reconstruction check fails -> abort, no update
+    init = cases.smooth_random(m, n, 2)
+    init["p"][3, 2] = 1e-12
+    cfg = make_config(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-4, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    assert st == 1 and int(c[5]) == 1 and tp[npl - 1] == 0.0
+    assert np.array_equal(fr[:, npl - 1], fr[:, 0])           # update not applied
+    # p < 0: the GRP solve fails too and its later error code wins -> stop after the update
+    init = cases.smooth_random(m, n, 2)
+    init["p"][3, 2] = -1.0
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    assert st == 2 and int(c[5]) == 1 and tp[npl - 1] == 0.0
+    init = cases.smooth_random(m, n, 2)
+    cfg = make_config(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-4, split=1, CFL=1e-12)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    assert st == 2 and int(c[5]) == 1 and tp[npl - 1] == 0.0
+    assert not np.array_equal(fr[:, npl - 1], fr[:, 0])       # the x half-sweep ran
+    cfg = make_config(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-3, bc_y=-4, split=1)
+    fr, npl, tp, c, st = oracle.split2d(m, n, init, cfg)
+    assert st == 1 and int(c[5]) == 1
+
+
+def test_error_paths_match_reference(oracle, reflib):
+    m, n = 8, 6
+    for mutate, kw in ((lambda d: d["p"].__setitem__((3, 2), -1.0), {}),
+                       (lambda d: d["p"].__setitem__((3, 2), 1e-12), {}),
+                       (lambda d: None, {"CFL": 1e-12}),
+                       (lambda d: d["rho"].__setitem__((5, 1), 1e-12), {})):
+        init = cases.smooth_random(m, n, 2)
+        mutate(init)
+        cfg = make_config(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-4, split=1, **kw)
+        fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg)
+        fr, npr, tpr, cr = reflib.split2d(m, n, init, cfg)
+        assert co[5] == cr[5] and npo == npr and tpo[npo - 1] == tpr[npr - 1]
+        if st != 2:   # a GRP failure stores a schedule-dependent flux in the reference
+            assert np.array_equal(fo[:, npo - 1], fr[:, npr - 1], equal_nan=True)
