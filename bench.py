@@ -1,0 +1,120 @@
+"""Benchmark: 2D dimension-split Eulerian GRP, FP64, cell-updates per second.
+
+Workload: the 2D four-quadrant Riemann problem (data_in/two-dim/RP2D_Positive/Config3
+states) at 1024 x 1024, gamma 1.4, CFL 0.45, free boundaries.  One step is one
+full X(tau/2) Y(tau) X(tau/2) step over the whole grid with the frozen
+tau = CFL * h / S_max of the reference; a cell-update is one cell advanced by
+one step.  The state is resident in HBM before the timed region; device time
+comes from HIP events on the solver's stream (hgrp2d_bench).
+
+Prints one JSON line (rank 0).  `roofline` reports the dominant kernel (the
+x-sweep, two launches per step) against HBM peak with the algorithmic bytes of
+a sweep (128 B per cell: read and write rho u v p and their four slopes);
+`cpu_baseline` times the oracle (the C99 restatement of the reference, OpenMP)
+on a bounded number of steps of the same workload on the host's cores.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "hydrocode-fusion_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+SWEEP_BYTES_PER_CELL = 128.0   # 384 B per cell-update / 3 sweeps
+M = N = 1024
+WORKLOAD = "2D four-quadrant Riemann problem 1024x1024, dimension-split GRP (X/2 Y X/2), FP64"
+
+
+def gpu_leg(steps, warmup):
+    import numpy as np
+    import hydrogrp
+    import cases
+    cfg = cases.four_quadrant_config(M, N)
+    s = hydrogrp.Solver2D(M, N, cfg)
+    init = cases.four_quadrant(M, N)
+    s.upload(init["rho"], init["u"], init["v"], init["p"])
+    if warmup:
+        s.bench(warmup)
+    ms_total, ms_x, ms_y = s.bench(steps)
+    state = s.download()
+    s.close()
+    assert np.all(np.isfinite(state)), "non-finite state after the benchmark"
+    return ms_total, ms_x, ms_y, hydrogrp.device_name()
+
+
+def cpu_leg(target_seconds=12.0):
+    """The oracle on the host cores, fixed-tau mode (exactly the frozen tau), bounded sample."""
+    import numpy as np
+    import cases
+    from _libs import Oracle, make_config
+    orc = Oracle()
+    init = cases.four_quadrant(M, N)
+    h = 1.0 / M
+    c = np.sqrt(1.4 * init["p"] / init["rho"])
+    tau = 0.45 * np.min(h / (c + np.abs(init["u"]) + np.abs(init["v"])))
+
+    def run(k):
+        cfg = make_config(N=k, tau=tau, h_x=h, h_y=h, bc_x=-4, bc_y=-4, split=1)
+        cfg[1] = np.inf
+        t0 = time.perf_counter()
+        orc.split2d(M, N, init, cfg, time_plot=np.array([0.0, np.inf]))
+        return time.perf_counter() - t0
+
+    t1 = run(1)
+    k = max(2, int(target_seconds / max(t1, 1e-3)))
+    t = run(k)
+    threads = orc.lib.orc_threads()
+    return {"value": M * N * k / t, "unit": "cell-updates/s", "cores": int(threads), "kind": "port",
+            "sample": f"{k} steps of the same 1024x1024 four-quadrant workload ({t:.1f} s), "
+                      f"oracle/liboracle.so (C99 restatement, gcc -O2 -fopenmp, {threads} threads)"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.perf_counter()
+    ms_total, ms_x, ms_y, device = gpu_leg(args.steps, args.warmup)
+    wall = time.perf_counter() - t0
+    value = M * N * args.steps / (ms_total / 1000.0)
+    achieved = SWEEP_BYTES_PER_CELL * M * N / (ms_x / 1000.0) / 1e9
+    line = {
+        "metric": "2D GRP FP64 cell-updates/s",
+        "value": value,
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_total / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (four-quadrant initial data generated in-process)",
+        "config": {"workload": WORKLOAD, "n_x": M, "n_y": N, "gamma": 1.4, "cfl": 0.45,
+                   "boundary": "free (-4, -4)", "device": device},
+        "roofline": {"bound": "hbm", "kernel": "sweep_x_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}},
+        "wall_s": wall,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_leg()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,52 @@
+/*
+ * hydrocode_dropin.h -- entry points with the reference's names and
+ * signatures, implemented by this project (lib/libhydrocode2d.so, sources in
+ * hydrocode-fusion_amd/host/).
+ *
+ * The solver is a drop-in for the reference's dimension-split 2D Eulerian GRP
+ * path: a program that calls it as src/hydrocode_2D/hydrocode.c:264-265 does
+ * gets the same CV frames, config[5], N_plot and time_plot back, computed on
+ * the GPU through the C ABI of include/hydrocode_grp.h.  The I/O routines keep
+ * the reference's data_in / data_out formats.
+ */
+#ifndef HYDROCODE_DROPIN_H
+#define HYDROCODE_DROPIN_H
+
+#include <stdio.h>
+#include "var_struc.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* replaces GRP_solver_2D_split_EUL_source, src/include/finite_volume.h:48-49
+ * (src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379) */
+void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
+                                    double *cpu_time, const char *problem, int N_T, int *N_plot,
+                                    double time_plot[]);
+
+/* replaces the 2D program's file I/O, src/include/file_io.h:13-54 */
+void example_io(const char *example, char *add_mkdir, const int i_or_o);        /* io_control.c:40-115   */
+int flu_var_count(FILE *fp, const char *add);                                    /* io_control.c:124-153  */
+int flu_var_count_line(FILE *fp, const char *add, int *n_x);                     /* io_control.c:163-210  */
+int flu_var_read(FILE *fp, double *U, const int num);                            /* io_control.c:221-261  */
+int time_plot_read(const char *add_in, const int N_max, int *N_plot, double **time_plot); /* io_control.c:288-335 */
+void arg_preprocess(const int argc_least, const int argc, char *argv[], char *scheme);     /* terminal_io.c:26-98 */
+void configurate(const char *name);                                              /* config_handle.c:208-243 */
+void config_write(const char *add_out, const double *cpu_time, const char *name);/* config_handle.c:253-300 */
+struct flu_var initialize_2D(const char *name, int *N, int *N_plot, double **time_plot); /* file_2D_in.c:119-179 */
+void file_2D_write(const int n_x, const int n_y, const int N, const struct cell_var_stru CV[],
+                   double **X, double **Y, const double *cpu_time, const char *problem,
+                   const double time_plot[]);                                    /* file_2D_out.c:55-88   */
+void file_2D_write_POINT_TEC(const int n_x, const int n_y, const int N, const struct cell_var_stru CV[],
+                             double **X, double **Y, const double *cpu_time, const char *problem,
+                             const double time_plot[]);                          /* file_2D_out.c:103-151 */
+
+/* replaces src/tools/sys_pro.c:37-100 */
+void DispPro(const double pro, const int step);
+int CreateDir(const char *pPath);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -104,6 +104,9 @@ int orc_grp1d(int m, double *const *rho, double *const *u, double *const *p,
               double *const *e, int *N_plot, double *time_plot, double *cpu_time,
               double *config);
 
+/* OpenMP threads the oracle's loops use (for reporting CPU baselines). */
+int orc_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

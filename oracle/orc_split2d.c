@@ -480,3 +480,13 @@ done:
     free(g->gL); free(g->gR); free(g->gD); free(g->gU);
     return status;
 }
+
+/* threads the OpenMP loops of this library will use (for reporting) */
+int orc_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

@@ -249,3 +249,49 @@ class RefLib:
         self.lib.ref_grp1d(m, dptr(frames), N_T, C.byref(npl), dptr(time_plot))
         cfg = np.array([self.lib.ref_config_get(k) for k in range(N_CONF)])
         return frames, npl.value, time_plot, cfg
+
+
+# ---- the reference's struct layouts through ctypes (include/var_struc.h) -----------
+DPP = C.POINTER(DP)
+
+
+class CellVarStru(C.Structure):
+    _fields_ = ([(n, DPP) for n in ("E", "RHO", "U", "V", "P")]
+                + [(n, DP) for n in ("d_rho", "d_u", "d_v", "d_p")]
+                + [(n, DPP) for n in ("s_rho", "s_u", "s_v", "s_p", "t_rho", "t_u", "t_v", "t_p",
+                                      "rhoIx", "uIx", "vIx", "pIx", "rhoIy", "uIy", "vIy", "pIy",
+                                      "F_rho", "F_e", "F_u", "F_v", "G_rho", "G_e", "G_u", "G_v")])
+
+
+class FluVar(C.Structure):
+    _fields_ = [(n, DP) for n in ("RHO", "U", "V", "P")]
+
+
+class CVFrames:
+    """N_T frames of rho u v p e as row-pointer arrays (CV[k].X[j][i]) backed by numpy."""
+
+    ORDER = ("RHO", "U", "V", "P", "E")
+
+    def __init__(self, m, n, N_T, h_x=1.0, h_y=1.0):
+        self.m, self.n, self.N_T = m, n, N_T
+        self.data = np.zeros((5, N_T, m, n))
+        self.CV = (CellVarStru * N_T)()
+        self._rows = []
+        for k in range(N_T):
+            for f, name in enumerate(self.ORDER):
+                rows = (DP * m)(*[self.data[f, k, j].ctypes.data_as(DP) for j in range(m)])
+                self._rows.append(rows)
+                setattr(self.CV[k], name, C.cast(rows, DPP))
+        self.Xa = np.array([[j * h_x for _ in range(n + 1)] for j in range(m + 1)])
+        self.Ya = np.array([[i * h_y for i in range(n + 1)] for _ in range(m + 1)])
+        self.X = (DP * (m + 1))(*[self.Xa[j].ctypes.data_as(DP) for j in range(m + 1)])
+        self.Y = (DP * (m + 1))(*[self.Ya[j].ctypes.data_as(DP) for j in range(m + 1)])
+
+    def set_initial(self, init, gamma):
+        r, u, v, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "v", "p"))
+        self.data[0, 0], self.data[1, 0], self.data[2, 0], self.data[3, 0] = r, u, v, p
+        self.data[4, 0] = initial_energy(r, u, v, p, gamma)
+
+    def frames(self):
+        """(5, N_T, m, n) in the oracle's field order rho u v p e."""
+        return self.data

@@ -1,0 +1,136 @@
+"""GPU: the drop-in entry point GRP_solver_2D_split_EUL_source (called with the
+reference's CV row-pointer frames through ctypes) and the whole 2D program
+(bin/hydrocode_2D.out) against the oracle, and against the reference's own
+program where it has been built (oracle/_ref)."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import cases
+from _libs import CVFrames, ROOT
+from conftest import rel_linf
+
+pytestmark = pytest.mark.gpu
+HOST_SO = os.path.join(ROOT, "hydrocode-fusion_amd", "lib", "libhydrocode2d.so")
+PROGRAM = os.path.join(ROOT, "hydrocode-fusion_amd", "bin", "hydrocode_2D.out")
+REF_PROGRAM = os.path.join(ROOT, "oracle", "_ref", "hydrocode_2D.out")
+
+
+@pytest.fixture(scope="module")
+def host():
+    L = C.CDLL(HOST_SO)
+    L.GRP_solver_2D_split_EUL_source.restype = None
+    L.GRP_solver_2D_split_EUL_source.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                                                 C.POINTER(C.c_double), C.c_char_p, C.c_int,
+                                                 C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.cfg = (C.c_double * 400).in_dll(L, "config")
+    return L
+
+
+def test_dropin_entry_point(host, oracle, tmp_path):
+    m, n = 40, 36
+    init = cases.four_quadrant(m, n)
+    cfg = cases.four_quadrant_config(m, n, t_all=0.2)
+    cfg[8], cfg[9] = 0.0, 2.0
+    time_plot = [0.0, 0.05, 0.12, 0.2]
+    N_T = 4
+    for k in range(400):
+        host.cfg[k] = float(cfg[k])
+    fr = CVFrames(m, n, N_T, h_x=1.0 / m, h_y=1.0 / n)
+    fr.set_initial(init, 1.4)
+    N_plot = C.c_int(len(time_plot))
+    tp = (C.c_double * len(time_plot))(*time_plot)
+    cpu = (C.c_double * N_T)()
+    run = tmp_path / "run" / "here"
+    run.mkdir(parents=True)
+    cwd = os.getcwd()
+    os.chdir(run)
+    try:
+        host.GRP_solver_2D_split_EUL_source(m, n, C.cast(fr.CV, C.c_void_p), C.cast(fr.X, C.c_void_p),
+                                            C.cast(fr.Y, C.c_void_p), cpu, b"dropin", N_T, C.byref(N_plot), tp)
+    finally:
+        os.chdir(cwd)
+    fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg, time_plot=np.array(time_plot), N_T=N_T)
+    assert (host.cfg[5], N_plot.value) == (co[5], npo)
+    assert np.array_equal(np.array(tp[:]), tpo)
+    for k in range(npo):
+        d = rel_linf(fr.frames()[:, k], fo[:, k]).max()
+        assert d <= 1e-10, (k, d)
+    # the reference leaves CV[0]'s scratch members NULL on return
+    for name in ("s_rho", "t_p", "rhoIx", "F_e", "G_v"):
+        assert not getattr(fr.CV[0], name)
+    tec = sorted(p.name for p in (tmp_path / "data_out" / "two-dim" / "EUL_2_order" / "dropin").iterdir())
+    want = sorted("FLU_VAR_%.8g.tec" % (t + 1e-9) for t in time_plot[:-1])   # frames written at plot times
+    assert tec == want, tec
+
+
+def _read_frames(path, n_x, n_y):
+    frames, cur = [], []
+    for line in open(path):
+        s = line.strip()
+        if s:
+            cur.append([float(x) for x in s.split("\t")])
+        elif cur:
+            frames.append(np.array(cur))
+            cur = []
+    if cur:
+        frames.append(np.array(cur))
+    for f in frames:
+        assert f.shape == (n_y, n_x)
+    return np.array(frames)
+
+
+def _run_program(exe, tmp_path, tag, m, n, init, time_plot):
+    import hydrogrp.datain as datain
+    base = tmp_path / tag
+    datain.write_case_2d(str(base / "data_in" / "two-dim" / "fqcase"), init,
+                         [(1, 0.15), (4, 1e-9), (6, 1.4), (10, 1.0 / m), (11, 1.0 / n), (17, -4), (18, -4)],
+                         time_plot=time_plot)
+    run = base / "run" / "here"
+    run.mkdir(parents=True)
+    print(f"running {os.path.basename(exe)} ({tag})", flush=True)
+    p = subprocess.run([exe, "fqcase", "fqout", "2_GRP", "EUL", "33=1"], cwd=run, capture_output=True,
+                       text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    return base / "data_out" / "two-dim" / "EUL_2_order" / "fqout"
+
+
+def test_program_end_to_end(oracle, tmp_path):
+    m, n = 48, 40
+    init = cases.four_quadrant(m, n)
+    out = _run_program(PROGRAM, tmp_path, "gpu", m, n, init, [0.05, 0.1])
+    cfg = cases.four_quadrant_config(m, n, t_all=0.15)
+    fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg, time_plot=np.array([0.0, 0.05, 0.1, 0.15]))
+    for f, name in enumerate(("RHO", "U", "V", "P", "E")):
+        got = _read_frames(out / f"{name}.dat", m, n)
+        assert got.shape[0] == npo
+        want = np.transpose(fo[f, :npo], (0, 2, 1))
+        scale = np.max(np.abs(want))
+        assert np.max(np.abs(got - want)) <= 1e-9 * scale + 1e-9 * np.max(np.abs(want)), name
+    assert [float(x) for x in open(out / "time_plot.dat")] == pytest.approx(list(tpo[:npo]), rel=1e-10)
+    assert f"A total of {int(co[5])} time steps are computed." in open(out / "log.dat").read()
+
+
+def test_program_matches_reference_program(tmp_path):
+    if not os.path.exists(REF_PROGRAM):
+        pytest.skip("reference program not built here (oracle/_ref)")
+    m, n = 32, 24
+    init = cases.four_quadrant(m, n)
+    ours = _run_program(PROGRAM, tmp_path, "gpu", m, n, init, [0.07])
+    ref = _run_program(REF_PROGRAM, tmp_path, "ref", m, n, init, [0.07])
+    names = sorted(p.name for p in ours.iterdir())
+    assert names == sorted(p.name for p in ref.iterdir())
+    for name in names:
+        a, b = open(ours / name).read().split("\n"), open(ref / name).read().split("\n")
+        assert len(a) == len(b), name
+        for la, lb in zip(a, b):
+            ta, tb = la.split("\t"), lb.split("\t")
+            assert len(ta) == len(tb), (name, la, lb)
+            for x, y in zip(ta, tb):
+                if x == y:
+                    continue
+                fx, fy = float(x), float(y)          # numbers printed with %.10g
+                assert abs(fx - fy) <= 1e-9 * max(abs(fy), 1e-300) or abs(fx - fy) <= 1e-12, (name, x, y)
