@@ -14,7 +14,6 @@ a sweep (128 B per cell: read and write rho u v p and their four slopes);
 on a bounded number of steps of the same workload on the host's cores.
 """
 import argparse
-import ctypes
 import json
 import os
 import sys
@@ -40,11 +39,15 @@ def gpu_leg(steps, warmup):
     s.upload(init["rho"], init["u"], init["v"], init["p"])
     if warmup:
         s.bench(warmup)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])    # timed run starts from t = 0
     ms_total, ms_x, ms_y = s.bench(steps)
     state = s.download()
     s.close()
     assert np.all(np.isfinite(state)), "non-finite state after the benchmark"
-    return ms_total, ms_x, ms_y, hydrogrp.device_name()
+    # a plain copy of 1 GiB: the measured HBM ceiling for streaming kernels
+    nbytes = 1 << 30
+    copy_gbs = 2.0 * nbytes / (hydrogrp.copy_probe(nbytes, 10) / 1000.0) / 1e9
+    return ms_total, ms_x, ms_y, hydrogrp.device_name(), copy_gbs
 
 
 def cpu_leg(target_seconds=12.0):
@@ -77,7 +80,8 @@ def cpu_leg(target_seconds=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    # 2017 steps = the whole run to t = 0.3 at the frozen tau (tau = 0.45 h / 2.9544)
+    ap.add_argument("--steps", type=int, default=2017)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     args = ap.parse_args()
@@ -87,7 +91,7 @@ def main():
         sys.exit(2)
 
     t0 = time.perf_counter()
-    ms_total, ms_x, ms_y, device = gpu_leg(args.steps, args.warmup)
+    ms_total, ms_x, ms_y, device, copy_gbs = gpu_leg(args.steps, args.warmup)
     wall = time.perf_counter() - t0
     value = M * N * args.steps / (ms_total / 1000.0)
     achieved = SWEEP_BYTES_PER_CELL * M * N / (ms_x / 1000.0) / 1e9
@@ -108,6 +112,7 @@ def main():
                    "boundary": "free (-4, -4)", "device": device},
         "roofline": {"bound": "hbm", "kernel": "sweep_x_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "copy_kernel_gbs": copy_gbs,
                      "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}},
         "wall_s": wall,
     }

@@ -36,6 +36,14 @@ __global__ void riemann_batch_kernel(const double *in, int count, int N, double 
     o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters;
 }
 
+// plain streaming copy: the calibration kernel for HBM byte counters
+__global__ void __launch_bounds__(256) copy_probe_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                         size_t n2)
+{
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (size_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
+}
+
 __global__ void pow_batch_kernel(const double *x, const double *y, int count, double *out)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
@@ -124,6 +132,31 @@ extern "C" int hgrp_pow_batch(const double *x, const double *y, int count, doubl
                      &cnt);
 }
 
+extern "C" int hgrp_copy_probe(size_t bytes, int reps, double *ms_per_copy)
+{
+    if (bytes < 16 || reps < 1) return -1001;
+    const size_t n2 = bytes / 16;
+    DevBuf a, b;
+    if (hipMalloc(&a.p, n2 * 16) != hipSuccess || hipMalloc(&b.p, n2 * 16) != hipSuccess) return -2;
+    if (hipMemset(a.p, 0, n2 * 16) != hipSuccess) return -3;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) return -4;
+    const unsigned blocks = 256 * 8;
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, (const double2 *)a.p, (double2 *)b.p, n2);
+    (void)hipEventRecord(e0, 0);
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(copy_probe_kernel, dim3(blocks), dim3(256), 0, 0, (const double2 *)a.p, (double2 *)b.p, n2);
+    (void)hipEventRecord(e1, 0);
+    hipError_t e = hipEventSynchronize(e1);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return -(int)e - 1;
+    if (ms_per_copy) *ms_per_copy = ms / reps;
+    return 0;
+}
+
 extern "C" int hgrp_device_name(char *buf, int len)
 {
     if (!buf || len < 1) return -1001;
@@ -131,6 +164,7 @@ extern "C" int hgrp_device_name(char *buf, int len)
     int dev = 0;
     hipDeviceProp_t prop;
     if (hipGetDevice(&dev) != hipSuccess || hipGetDeviceProperties(&prop, dev) != hipSuccess) return -1;
-    snprintf(buf, (size_t)len, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+    snprintf(buf, (size_t)len, "%s%s%s, %d CUs", prop.name, prop.name[0] ? " " : "", prop.gcnArchName,
+             prop.multiProcessorCount);
     return 0;
 }

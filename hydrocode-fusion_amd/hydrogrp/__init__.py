@@ -46,6 +46,7 @@ def lib():
     L.hgrp_riemann_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
     L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
     L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
+    L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
     _lib = L
     return L
 
@@ -60,6 +61,13 @@ def _p(a):
 def _check(rc, what):
     if rc != 0:
         raise RuntimeError(f"{what} failed with code {rc}")
+
+
+def copy_probe(nbytes, reps=20):
+    """Device time (ms) of one plain copy of nbytes (read nbytes + write nbytes)."""
+    ms = _D()
+    _check(lib().hgrp_copy_probe(nbytes, reps, C.byref(ms)), "copy probe")
+    return ms.value
 
 
 def device_name():

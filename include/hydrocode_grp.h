@@ -99,6 +99,11 @@ int hgrp_riemann_batch(const double *in, int count, int N, double *out);
 /* pow(x[k], y[k]) as evaluated on the device (precision check of the GPU's FP64 pow). */
 int hgrp_pow_batch(const double *x, const double *y, int count, double *out);
 
+/* Streams `bytes` from one device buffer to another `reps` times (a plain copy
+ * kernel, copy_probe_kernel); the calibration point for HBM byte counters and
+ * the measured copy bandwidth.  ms_per_copy: average device time of one copy. */
+int hgrp_copy_probe(size_t bytes, int reps, double *ms_per_copy);
+
 /* Name of the device the library runs on (empty if none). */
 int hgrp_device_name(char *buf, int len);
 
