@@ -26,7 +26,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 SWEEP_BYTES_PER_CELL = 128.0   # 384 B per cell-update / 3 sweeps
 M = N = 1024
-WORKLOAD = "2D four-quadrant Riemann problem 1024x1024, dimension-split GRP (X/2 Y X/2), FP64"
+WORKLOAD = "2D four-quadrant Riemann problem {m}x{n}, dimension-split GRP (X/2 Y X/2), FP64"
 
 
 def gpu_leg(steps, warmup):
@@ -73,7 +73,7 @@ def cpu_leg(target_seconds=12.0):
     t = run(k)
     threads = orc.lib.orc_threads()
     return {"value": M * N * k / t, "unit": "cell-updates/s", "cores": int(threads), "kind": "port",
-            "sample": f"{k} steps of the same 1024x1024 four-quadrant workload ({t:.1f} s), "
+            "sample": f"{k} steps of the same {M}x{N} four-quadrant workload ({t:.1f} s), "
                       f"oracle/liboracle.so (C99 restatement, gcc -O2 -fopenmp, {threads} threads)"}
 
 
@@ -84,7 +84,10 @@ def main():
     ap.add_argument("--steps", type=int, default=2017)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--size", type=int, default=1024, help="grid cells per side (default: the 1024^2 workload)")
     args = ap.parse_args()
+    global M, N
+    M = N = args.size
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
@@ -108,7 +111,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (four-quadrant initial data generated in-process)",
-        "config": {"workload": WORKLOAD, "n_x": M, "n_y": N, "gamma": 1.4, "cfl": 0.45,
+        "config": {"workload": WORKLOAD.format(m=M, n=N), "n_x": M, "n_y": N, "gamma": 1.4, "cfl": 0.45,
                    "boundary": "free (-4, -4)", "device": device},
         "roofline": {"bound": "hbm", "kernel": "sweep_x_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -12,6 +12,7 @@
 // src/inter_process/bound_cond_slope_limiter_{x,y}.c:27-129,
 // src/inter_process/slope_limiter{,_2D_x}.c and src/flux_calc/flux_generator_{x,y}.c:31-165.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include "grp_device.hpp"
 #include "sweep2d.hpp"
 
@@ -199,6 +200,82 @@ __global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
     }
 }
 
+// X sweep, row-parallel form: a block of W waves covers 64 consecutive i and
+// the cell rows j0-1 .. j0+W-2 (wave w: row c = j0-1+w, lane: column i).
+// Neighbours along x live in the next wave, so values, slopes and then fluxes
+// are exchanged through LDS.  Wave w solves interface c+1 (between rows c and
+// c+1); waves 1..W-1 update rows j0 .. j0+W-2.  Every interface row is one
+// wave, so the GRP work is spread over all rows of the grid at once.
+template <int W>
+__global__ void __launch_bounds__(64 * W) sweep_x_rows_kernel(SweepArgs a)
+{
+    __shared__ double xa[W][4][64];   // values, then fluxes
+    __shared__ double xb[W][4][64];   // slopes, then interface values
+    if (sweep_halted(a)) return;
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int i = blockIdx.x * 64 + lane;
+    const bool col = i < a.n;
+    const int m = a.m;
+    const int j0 = blockIdx.y * (W - 1);
+    const int c = j0 - 1 + w;                     // owned row; -1 and m are ghost rows
+    const Line<0> L(a, col ? i : 0);
+    const double hh = a.hh;
+
+    double v[4] = {1.0, 0.0, 0.0, 1.0}, vl[4], vr[4];
+    if (col && c <= m) L.value(c, v);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
+    if (col && w == 0 && c >= 0) L.value(c - 1, vl);
+    if (col && w == W - 1 && c + 1 <= m) L.value(c + 1, vr);
+    __syncthreads();
+    if (w > 0)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vl[q] = xa[w - 1][q][lane];
+    if (w < W - 1)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vr[q] = xa[w + 1][q][lane];
+
+    double s[4] = {0.0, 0.0, 0.0, 0.0}, sr[4] = {0.0, 0.0, 0.0, 0.0};
+    if (col) {
+        if (c >= 0 && c < m) L.limited(c, vl, v, vr, s);
+        else if (c == -1) L.ghost_slope(true, s);
+        else if (c == m) L.ghost_slope(false, s);
+        if (w == W - 1) {
+            if (c + 1 < m) {
+                double vrr[4];
+                L.value(c + 2, vrr);
+                L.limited(c + 1, v, vr, vrr, sr);
+            } else if (c + 1 == m) {
+                L.ghost_slope(false, sr);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) xb[w][q][lane] = s[q];
+    __syncthreads();
+    if (w < W - 1)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sr[q] = xb[w + 1][q][lane];
+
+    double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
+    const int f = c + 1;
+    if (col && f <= m) {
+        const Side SL = recon(v, s, hh, true);
+        const Side SR = recon(vr, sr, hh, false);
+        solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f, F, I);
+    }
+    __syncthreads();                               // everyone is done reading values/slopes
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { xa[w][q][lane] = F[q]; xb[w][q][lane] = I[q]; }
+    __syncthreads();
+    if (col && w >= 1 && c >= 0 && c < m) {
+        double Fl[4], Il[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { Fl[q] = xa[w - 1][q][lane]; Il[q] = xb[w - 1][q][lane]; }
+        L.update(c, v, Fl, F, Il, I);
+    }
+}
+
 // Y sweep: a wave owns cells c0-1 .. c0+62 of y-line j (lane t: cell c0-1+t),
 // computes interfaces c0 .. c0+63 (lane t: interface c0+t) and updates cells
 // c0 .. c0+62 (lane t >= 1).
@@ -281,11 +358,40 @@ __global__ void __launch_bounds__(256) cfl_kernel(const double *rho, const doubl
 // ---- launchers -----------------------------------------------------------------
 static constexpr int TJ_X = 32;
 
+// x-sweep variant: 0 = row-parallel (LDS exchange, 8 rows per block),
+// 1 = row-parallel with 16 rows per block, 2 = sliding register window.
+// HGRP_SWEEP_X overrides the default; read once.
+static int sweep_x_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("HGRP_SWEEP_X");
+        v = e ? atoi(e) : 0;
+        if (v < 0 || v > 2) v = 0;
+    }
+    return v;
+}
+
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
 {
-    const int bx = a.n >= 256 ? 256 : (a.n >= 128 ? 128 : 64);
-    dim3 grid((a.n + bx - 1) / bx, (a.m + TJ_X - 1) / TJ_X);
-    hipLaunchKernelGGL(sweep_x_kernel<TJ_X>, grid, dim3(bx), 0, st, a);
+    switch (sweep_x_variant()) {
+    case 2: {
+        const int bx = a.n >= 256 ? 256 : (a.n >= 128 ? 128 : 64);
+        dim3 grid((a.n + bx - 1) / bx, (a.m + TJ_X - 1) / TJ_X);
+        hipLaunchKernelGGL(sweep_x_kernel<TJ_X>, grid, dim3(bx), 0, st, a);
+        break;
+    }
+    case 1: {
+        dim3 grid((a.n + 63) / 64, (a.m + 14) / 15);
+        hipLaunchKernelGGL(sweep_x_rows_kernel<16>, grid, dim3(64 * 16), 0, st, a);
+        break;
+    }
+    default: {
+        dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
+        hipLaunchKernelGGL(sweep_x_rows_kernel<8>, grid, dim3(64 * 8), 0, st, a);
+        break;
+    }
+    }
     return hipGetLastError();
 }
 
