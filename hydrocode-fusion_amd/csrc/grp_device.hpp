@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "fastpow.hpp"
+
 namespace hgrp {
 
 // Interface side state: value and normal derivative of rho, u, v, p.
@@ -46,14 +48,16 @@ __device__ __forceinline__ double minmod3(double s_L, double s_R, double s_m)
     return s_L;
 }
 
-// x^y for the exponents of the gamma-law wave curves (x > 0 in every
-// well-posed call).  OCML's pow: faithfully rounded FP64.
-__device__ __forceinline__ double gpow(double x, double y) { return pow(x, y); }
+// x^y, correctly rounded in all but ~1e-5 of cases (fastpow.hpp); powers of
+// one base share its logarithm through PowBase.
+__device__ __forceinline__ double gpow(double x, double y) { return fast_pow(x, y); }
 
-// velocity on the 1-wave (sign -1) or 3-wave (sign +1) curve through (u_s, p_s) at pressure p
+// velocity on the 1-wave (sign -1) or 3-wave (sign +1) curve through (u_s, p_s)
+// at pressure p; on the rarefaction branch *base receives log(p / p_s) for the
+// Newton slope at the same p
 template <int SIGN>
 __device__ __forceinline__ double wave_curve(double p, double u_s, double p_s, double c_s, double g,
-                                             double mu, double nu)
+                                             double mu, double nu, PowBase *base)
 {
     double w;
     if (p > p_s) {
@@ -61,7 +65,8 @@ __device__ __forceinline__ double wave_curve(double p, double u_s, double p_s, d
         const double s = sqrt(1.0 + nu * dp / p_s);
         w = dp * c_s / g / p_s / s;
     } else {
-        w = gpow(p / p_s, mu) - 1.0;
+        *base = pow_base(p / p_s);
+        w = pow_with(*base, mu) - 1.0;
         w = 2.0 * c_s * w / (g - 1.0);
     }
     return SIGN < 0 ? u_s - w : u_s + w;
@@ -73,9 +78,8 @@ struct StarResult {
     int iters;     // Newton iterations taken (-1: equal-state early exit)
 };
 
-// Riemann_solver_exact.  p_guess > 0 replaces the reference's halved
-// acoustic-tangent guess (used by the learned-guess path); pass 0 for the
-// reference behaviour.
+// Riemann_solver_exact: wave types, the halved acoustic-tangent guess, Newton
+// on the velocity gap, u* from the last tangents.
 __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double u_L, double u_R,
                                                     double p_L, double p_R, double c_L, double c_R,
                                                     double eps, double tol, int N)
@@ -83,8 +87,9 @@ __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double
     const double muL = (gL - 1.0) / (2.0 * gL), nuL = (gL + 1.0) / (2.0 * gL);
     const double muR = (gR - 1.0) / (2.0 * gR), nuR = (gR + 1.0) / (2.0 * gR);
     StarResult r;
-    const double u_LR = wave_curve<-1>(p_R, u_L, p_L, c_L, gL, muL, nuL);
-    const double u_RL = wave_curve<+1>(p_L, u_R, p_R, c_R, gR, muR, nuR);
+    PowBase bL, bR;
+    const double u_LR = wave_curve<-1>(p_R, u_L, p_L, c_L, gL, muL, nuL, &bL);
+    const double u_RL = wave_curve<+1>(p_L, u_R, p_R, c_R, gR, muR, nuR, &bR);
     r.crw1 = (u_LR > u_R + eps) ? 0 : 1;
     r.crw0 = (u_RL > u_L - eps) ? 1 : 0;
 
@@ -101,8 +106,8 @@ __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double
         r.iters = -1;
         return r;
     }
-    double v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, gL, muL, nuL);
-    double v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, gR, muR, nuR);
+    double v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, gL, muL, nuL, &bL);
+    double v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, gR, muR, nuR, &bR);
     double gap = fabs(v_L - v_R);
 
     int n = 0;
@@ -113,10 +118,10 @@ __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double
             t1 = 1.0 / sqrt(1.0 + nuL * dp / p_L);
             t2 = c_L / gL / p_L;
             t3 = 0.5 * t2 * nuL / p_L;
-            k1 = t3 * dp * gpow(t1, 3.0) - t2 * t1;
+            k1 = t3 * dp * cube(t1) - t2 * t1;
         } else {
             t2 = c_L / gL / p_L;
-            t1 = 1.0 / gpow(p_INT / p_L, nuL);
+            t1 = 1.0 / pow_with(bL, nuL);          // bL = log(p_INT / p_L) from the curve value
             k1 = -t1 * t2;
         }
         if (p_INT > p_R) {
@@ -124,17 +129,17 @@ __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double
             t1 = 1.0 / sqrt(1.0 + nuR * dp / p_R);
             t2 = c_R / gR / p_R;
             t3 = 0.5 * t2 * nuR / p_R;
-            k3 = t2 * t1 - t3 * dp * gpow(t1, 3.0);
+            k3 = t2 * t1 - t3 * dp * cube(t1);
         } else {
             t2 = c_R / gR / p_R;
-            t1 = 1.0 / gpow(p_INT / p_R, nuR);
+            t1 = 1.0 / pow_with(bR, nuR);
             k3 = t1 * t2;
         }
         const double p_new = p_INT + (v_R - v_L) / (k1 - k3);
         p_INT = (p_new < 0.0) ? 0.5 * p_INT : p_new;
         ++n;
-        v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, gL, muL, nuL);
-        v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, gR, muR, nuR);
+        v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, gL, muL, nuL, &bL);
+        v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, gR, muR, nuR, &bR);
         gap = fabs(v_L - v_R);
     }
     r.p_star = p_INT;
@@ -190,8 +195,9 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         const StarResult st = riemann_exact(gL, gR, u_L, u_R, p_L, p_R, c_L, c_R, eps, eps, 500);
         u_star = st.u_star; p_star = st.p_star; crw0 = st.crw0; crw1 = st.crw1;
         if (crw0) {
-            rs_L = rho_L * gpow(p_star / p_L, 1.0 / gL);
-            cs_L = c_L * gpow(p_star / p_L, 0.5 * (gL - 1.0) / gL);
+            const PowBase b = pow_base(p_star / p_L);
+            rs_L = rho_L * pow_with(b, 1.0 / gL);
+            cs_L = c_L * pow_with(b, 0.5 * (gL - 1.0) / gL);
             speed_L = u_L - c_L;
         } else {
             rs_L = rho_L * (p_star + zL * p_L) / (p_L + zL * p_star);
@@ -199,8 +205,9 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
             speed_L = u_L - c_L * sqrt(0.5 * ((gL + 1.0) * (p_star / p_L) + (gL - 1.0)) / gL);
         }
         if (crw1) {
-            rs_R = rho_R * gpow(p_star / p_R, 1.0 / gR);
-            cs_R = c_R * gpow(p_star / p_R, 0.5 * (gR - 1.0) / gR);
+            const PowBase b = pow_base(p_star / p_R);
+            rs_R = rho_R * pow_with(b, 1.0 / gR);
+            cs_R = c_R * pow_with(b, 0.5 * (gR - 1.0) / gR);
             speed_R = u_R + c_R;
         } else {
             rs_R = rho_R * (p_star + zR * p_R) / (p_R + zR * p_star);
@@ -251,13 +258,13 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
     if (crw0 && ((u_star - cs_L) > lu)) {                 // t-axis inside the 1-CRW
         U[1] = zL * (u_L + 2.0 * (c_L + lu) / (gL - 1.0));
         C = U[1] - lu;
-        U[3] = gpow(C / c_L, 2.0 * gL / (gL - 1.0)) * p_L;
+        const PowBase b = pow_base(C / c_L);             // c_frac = C / c_L below
+        U[3] = pow_with(b, 2.0 * gL / (gL - 1.0)) * p_L;
         U[0] = gL * U[3] / C / C;
         U[2] = v_L;
-        const double c_frac = C / c_L;
         const double TdS = (d_p_L - d_rho_L * c_L * c_L) / (gL - 1.0) / rho_L;
         const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / (gL - 1.0) / rho_L;
-        const double pw_a = gpow(c_frac, 0.5 / zL), pw_b = gpow(c_frac, (1.0 + zL) / zL);
+        const double pw_a = pow_with(b, 0.5 / zL), pw_b = pow_with(b, (1.0 + zL) / zL);
         D[1] = ((1.0 + zL) * pw_a + zL * pw_b);
         D[1] = D[1] / (1.0 + 2.0 * zL) * TdS;
         D[1] = D[1] - c_L * pw_a * d_Psi;
@@ -268,13 +275,13 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
     } else if (crw1 && ((u_star + cs_R) < lu)) {          // t-axis inside the 3-CRW
         U[1] = zR * (u_R - 2.0 * (c_R - lu) / (gR - 1.0));
         C = lu - U[1];
-        U[3] = gpow(C / c_R, 2.0 * gR / (gR - 1.0)) * p_R;
+        const PowBase b = pow_base(C / c_R);
+        U[3] = pow_with(b, 2.0 * gR / (gR - 1.0)) * p_R;
         U[0] = gR * U[3] / C / C;
         U[2] = v_R;
-        const double c_frac = C / c_R;
         const double TdS = (d_p_R - d_rho_R * c_R * c_R) / (gR - 1.0) / rho_R;
         const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / (gR - 1.0) / rho_R;
-        const double pw_a = gpow(c_frac, 0.5 / zR), pw_b = gpow(c_frac, (1.0 + zR) / zR);
+        const double pw_a = pow_with(b, 0.5 / zR), pw_b = pow_with(b, (1.0 + zR) / zR);
         D[1] = ((1.0 + zR) * pw_a + zR * pw_b);
         D[1] = D[1] / (1.0 + 2.0 * zR) * TdS;
         D[1] = D[1] + c_R * pw_a * d_Phi;
@@ -294,11 +301,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         if (crw0) {
             a_L = 1.0;
             b_L = 1.0 / rs_L / cs_L;
-            const double c_frac = cs_L / c_L;
+            const PowBase b = pow_base(cs_L / c_L);
             const double TdS = (d_p_L - d_rho_L * c_L * c_L) / (gL - 1.0) / rho_L;
             const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / (gL - 1.0) / rho_L;
-            const double pw_a = gpow(c_frac, 0.5 / zL);
-            pwL_b = gpow(c_frac, (1.0 + zL) / zL);
+            const double pw_a = pow_with(b, 0.5 / zL);
+            pwL_b = pow_with(b, (1.0 + zL) / zL);
             d_L = ((1.0 + zL) * pw_a + zL * pwL_b);
             d_L = d_L / (1.0 + 2.0 * zL) * TdS;
             d_L = d_L - c_L * pw_a * d_Psi;
@@ -319,11 +326,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         if (crw1) {
             a_R = 1.0;
             b_R = -1.0 / rs_R / cs_R;
-            const double c_frac = cs_R / c_R;
+            const PowBase b = pow_base(cs_R / c_R);
             const double TdS = (d_p_R - d_rho_R * c_R * c_R) / (gR - 1.0) / rho_R;
             const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / (gR - 1.0) / rho_R;
-            const double pw_a = gpow(c_frac, 0.5 / zR);
-            pwR_b = gpow(c_frac, (1.0 + zR) / zR);
+            const double pw_a = pow_with(b, 0.5 / zR);
+            pwR_b = pow_with(b, (1.0 + zR) / zR);
             d_R = ((1.0 + zR) * pw_a + zR * pwR_b);
             d_R = d_R / (1.0 + 2.0 * zR) * TdS;
             d_R = d_R + c_R * pw_a * d_Phi;

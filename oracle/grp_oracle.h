@@ -104,6 +104,9 @@ int orc_grp1d(int m, double *const *rho, double *const *u, double *const *p,
               double *const *e, int *N_plot, double *time_plot, double *cpu_time,
               double *config);
 
+/* pow(x[k], y[k]) from the C library the oracle is linked with (glibc). */
+void orc_pow_many(const double *x, const double *y, int n, double *out);
+
 /* OpenMP threads the oracle's loops use (for reporting CPU baselines). */
 int orc_threads(void);
 

@@ -120,3 +120,11 @@ double orc_riemann_exact_ben(double *U_star, double *P_star, double gamma,
     return riemann_core(U_star, P_star, gamma, gamma, mu, nu, mu, nu,
                         u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter);
 }
+
+/* libm's pow over arrays: the pow the reference's arithmetic uses (glibc), for
+ * checking the device pow */
+void orc_pow_many(const double *x, const double *y, int n, double *out)
+{
+    for (int k = 0; k < n; ++k)
+        out[k] = pow(x[k], y[k]);
+}

@@ -80,6 +80,15 @@ class Oracle:
         L.orc_split2d.argtypes = [C.c_int, C.c_int, PP, PP, PP, PP, PP, C.c_int, IP, DP, DP, DP]
         L.orc_grp1d.restype = C.c_int
         L.orc_grp1d.argtypes = [C.c_int, PP, PP, PP, PP, IP, DP, DP, DP]
+        L.orc_pow_many.argtypes = [DP, DP, C.c_int, DP]
+
+    def libm_pow(self, x, y):
+        """glibc's pow elementwise (numpy's power is a different implementation)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.ascontiguousarray(y, dtype=np.float64)
+        out = np.empty_like(x)
+        self.lib.orc_pow_many(dptr(x), dptr(y), len(x), dptr(out))
+        return out
 
     def riemann(self, gL, gR, uL, uR, pL, pR, cL, cR, eps, tol, N):
         us, ps = D(), D()
