@@ -226,7 +226,7 @@ static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double 
     a.hh = 0.5 * a.h;
     a.tau = tau_flux;
     a.nu = nu;
-    a.gamma = c->cfg[6];
+    a.gas = make_gas(c->cfg[6]);
     a.eps = c->cfg[4];
     a.alpha = c->cfg[41];
     a.status = c->status + slot;

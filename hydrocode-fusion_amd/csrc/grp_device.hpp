@@ -30,6 +30,44 @@ struct Side {
     double d_rho, d_u, d_v, d_p;
 };
 
+// Constants of one gamma-law gas, formed once (on the host) with exactly the
+// expressions the reference writes inline, so every use sees the same double.
+// Passed by value in kernel arguments, they stay in scalar registers.
+struct Gas {
+    double g, gm1, gp1;            // gamma, gamma - 1, gamma + 1
+    double mu, nu;                 // (g-1)/(2g), (g+1)/(2g)       wave curves
+    double z, opz, op2z, omz;      // zeta = (g-1)/(g+1), 1+zeta, 1+2 zeta, 1-zeta
+    double tpz, zz, omzz, zzm1;    // 2+zeta, zeta^2, 1-zeta^2, zeta^2-1
+    double inv_g, e_cs;            // 1/g, 0.5 (g-1)/g            star density / sound speed
+    double e_sonic, e_a, e_b;      // 2g/(g-1), 0.5/zeta, (1+zeta)/zeta
+    double g_gm1;                  // g/(g-1)                      energy flux
+};
+
+__host__ __device__ inline Gas make_gas(double g)
+{
+    Gas s;
+    s.g = g;
+    s.gm1 = g - 1.0;
+    s.gp1 = g + 1.0;
+    s.mu = (g - 1.0) / (2.0 * g);
+    s.nu = (g + 1.0) / (2.0 * g);
+    s.z = (g - 1.0) / (g + 1.0);
+    s.opz = 1.0 + s.z;
+    s.op2z = 1.0 + 2.0 * s.z;
+    s.omz = 1 - s.z;
+    s.tpz = 2.0 + s.z;
+    s.zz = s.z * s.z;
+    s.omzz = 1.0 - s.z * s.z;
+    s.zzm1 = s.z * s.z - 1.0;
+    s.inv_g = 1.0 / g;
+    s.e_cs = 0.5 * (g - 1.0) / g;
+    s.e_sonic = 2.0 * g / (g - 1.0);
+    s.e_a = 0.5 / s.z;
+    s.e_b = (1.0 + s.z) / s.z;
+    s.g_gm1 = g / (g - 1.0);
+    return s;
+}
+
 __device__ __forceinline__ double minmod2(double s_L, double s_R)
 {
     if (s_L * s_R <= 0.0) return 0.0;
@@ -56,20 +94,31 @@ __device__ __forceinline__ double gpow(double x, double y) { return fast_pow(x, 
 // at pressure p; on the rarefaction branch *base receives log(p / p_s) for the
 // Newton slope at the same p
 template <int SIGN>
-__device__ __forceinline__ double wave_curve(double p, double u_s, double p_s, double c_s, double g,
-                                             double mu, double nu, PowBase *base)
+__device__ __forceinline__ double wave_curve(double p, double u_s, double p_s, double c_s, const Gas &G,
+                                             DD *logx)
 {
     double w;
     if (p > p_s) {
         const double dp = p - p_s;
-        const double s = sqrt(1.0 + nu * dp / p_s);
-        w = dp * c_s / g / p_s / s;
+        const double s = sqrt(1.0 + G.nu * dp / p_s);
+        w = dp * c_s / G.g / p_s / s;
     } else {
-        *base = pow_base(p / p_s);
-        w = pow_with(*base, mu) - 1.0;
-        w = 2.0 * c_s * w / (g - 1.0);
+        const PowBase b = pow_base(p / p_s);
+        *logx = b.L;
+        w = pow_with(b, G.mu) - 1.0;
+        w = 2.0 * c_s * w / G.gm1;
     }
     return SIGN < 0 ? u_s - w : u_s + w;
+}
+
+// x^y for x = p/p_s on the rarefaction branch, given log(x) from wave_curve
+__device__ __forceinline__ double pow_from_log(double x, DD L, double y)
+{
+    PowBase b;
+    b.x = x;
+    b.ok = pow_base_ok(x);
+    b.L = L;
+    return pow_with(b, y);
 }
 
 struct StarResult {
@@ -80,16 +129,15 @@ struct StarResult {
 
 // Riemann_solver_exact: wave types, the halved acoustic-tangent guess, Newton
 // on the velocity gap, u* from the last tangents.
-__device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double u_L, double u_R,
+__device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR, double u_L, double u_R,
                                                     double p_L, double p_R, double c_L, double c_R,
                                                     double eps, double tol, int N)
 {
-    const double muL = (gL - 1.0) / (2.0 * gL), nuL = (gL + 1.0) / (2.0 * gL);
-    const double muR = (gR - 1.0) / (2.0 * gR), nuR = (gR + 1.0) / (2.0 * gR);
+    const double gL = GL.g, gR = GR.g;
     StarResult r;
-    PowBase bL, bR;
-    const double u_LR = wave_curve<-1>(p_R, u_L, p_L, c_L, gL, muL, nuL, &bL);
-    const double u_RL = wave_curve<+1>(p_L, u_R, p_R, c_R, gR, muR, nuR, &bR);
+    DD bL = {0.0, 0.0}, bR = {0.0, 0.0};
+    const double u_LR = wave_curve<-1>(p_R, u_L, p_L, c_L, GL, &bL);
+    const double u_RL = wave_curve<+1>(p_L, u_R, p_R, c_R, GR, &bR);
     r.crw1 = (u_LR > u_R + eps) ? 0 : 1;
     r.crw0 = (u_RL > u_L - eps) ? 1 : 0;
 
@@ -106,8 +154,8 @@ __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double
         r.iters = -1;
         return r;
     }
-    double v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, gL, muL, nuL, &bL);
-    double v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, gR, muR, nuR, &bR);
+    double v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
+    double v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
     double gap = fabs(v_L - v_R);
 
     int n = 0;
@@ -115,31 +163,31 @@ __device__ __forceinline__ StarResult riemann_exact(double gL, double gR, double
         double t1, t2, t3, dp;
         if (p_INT > p_L) {
             dp = p_INT - p_L;
-            t1 = 1.0 / sqrt(1.0 + nuL * dp / p_L);
+            t1 = 1.0 / sqrt(1.0 + GL.nu * dp / p_L);
             t2 = c_L / gL / p_L;
-            t3 = 0.5 * t2 * nuL / p_L;
+            t3 = 0.5 * t2 * GL.nu / p_L;
             k1 = t3 * dp * cube(t1) - t2 * t1;
         } else {
             t2 = c_L / gL / p_L;
-            t1 = 1.0 / pow_with(bL, nuL);          // bL = log(p_INT / p_L) from the curve value
+            t1 = 1.0 / pow_from_log(p_INT / p_L, bL, GL.nu);   // bL = log(p_INT / p_L) from the curve value
             k1 = -t1 * t2;
         }
         if (p_INT > p_R) {
             dp = p_INT - p_R;
-            t1 = 1.0 / sqrt(1.0 + nuR * dp / p_R);
+            t1 = 1.0 / sqrt(1.0 + GR.nu * dp / p_R);
             t2 = c_R / gR / p_R;
-            t3 = 0.5 * t2 * nuR / p_R;
+            t3 = 0.5 * t2 * GR.nu / p_R;
             k3 = t2 * t1 - t3 * dp * cube(t1);
         } else {
             t2 = c_R / gR / p_R;
-            t1 = 1.0 / pow_with(bR, nuR);
+            t1 = 1.0 / pow_from_log(p_INT / p_R, bR, GR.nu);
             k3 = t1 * t2;
         }
         const double p_new = p_INT + (v_R - v_L) / (k1 - k3);
         p_INT = (p_new < 0.0) ? 0.5 * p_INT : p_new;
         ++n;
-        v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, gL, muL, nuL, &bL);
-        v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, gR, muR, nuR, &bR);
+        v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
+        v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
         gap = fabs(v_L - v_R);
     }
     r.p_star = p_INT;
@@ -165,8 +213,9 @@ __device__ __forceinline__ void side_trivial(double D[4], double U[4], double rh
 // TL/TR: tangential derivatives (rho, u, v, p) of the two sides.
 __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L, const Side &R,
                                         const double TL[4], const double TR[4],
-                                        double gL, double gR, double eps)
+                                        const Gas &GL, const Gas &GR, double eps)
 {
+    const double gL = GL.g, gR = GR.g;
     const double lu = 0.0, lv = 0.0;
     const double atc = eps;
     const double rho_L = L.rho, rho_R = R.rho, u_L = L.u, u_R = R.u;
@@ -175,8 +224,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
     const double d_v_L = L.d_v, d_v_R = R.d_v, d_p_L = L.d_p, d_p_R = R.d_p;
     const double t_rho_L = TL[0], t_u_L = TL[1], t_v_L = TL[2], t_p_L = TL[3];
     const double t_rho_R = TR[0], t_u_R = TR[1], t_v_R = TR[2], t_p_R = TR[3];
-    const double zL = (gL - 1.0) / (gL + 1.0);
-    const double zR = (gR - 1.0) / (gR + 1.0);
+    const double zL = GL.z, zR = GR.z;
 
     int crw0 = 0, crw1 = 0;
     double u_star, p_star, rs_L, rs_R, cs_L, cs_R, speed_L, speed_R, C;
@@ -192,27 +240,27 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         rs_L = rho_L; cs_L = c_L; speed_L = u_star - cs_L;
         rs_R = rho_R; cs_R = c_R; speed_R = u_star + cs_R;
     } else {
-        const StarResult st = riemann_exact(gL, gR, u_L, u_R, p_L, p_R, c_L, c_R, eps, eps, 500);
+        const StarResult st = riemann_exact(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, eps, 500);
         u_star = st.u_star; p_star = st.p_star; crw0 = st.crw0; crw1 = st.crw1;
         if (crw0) {
             const PowBase b = pow_base(p_star / p_L);
-            rs_L = rho_L * pow_with(b, 1.0 / gL);
-            cs_L = c_L * pow_with(b, 0.5 * (gL - 1.0) / gL);
+            rs_L = rho_L * pow_with(b, GL.inv_g);
+            cs_L = c_L * pow_with(b, GL.e_cs);
             speed_L = u_L - c_L;
         } else {
             rs_L = rho_L * (p_star + zL * p_L) / (p_L + zL * p_star);
             cs_L = sqrt(gL * p_star / rs_L);
-            speed_L = u_L - c_L * sqrt(0.5 * ((gL + 1.0) * (p_star / p_L) + (gL - 1.0)) / gL);
+            speed_L = u_L - c_L * sqrt(0.5 * (GL.gp1 * (p_star / p_L) + GL.gm1) / gL);
         }
         if (crw1) {
             const PowBase b = pow_base(p_star / p_R);
-            rs_R = rho_R * pow_with(b, 1.0 / gR);
-            cs_R = c_R * pow_with(b, 0.5 * (gR - 1.0) / gR);
+            rs_R = rho_R * pow_with(b, GR.inv_g);
+            cs_R = c_R * pow_with(b, GR.e_cs);
             speed_R = u_R + c_R;
         } else {
             rs_R = rho_R * (p_star + zR * p_R) / (p_R + zR * p_star);
             cs_R = sqrt(gR * p_star / rs_R);
-            speed_R = u_R + c_R * sqrt(0.5 * ((gR + 1.0) * (p_star / p_R) + (gR - 1.0)) / gR);
+            speed_R = u_R + c_R * sqrt(0.5 * (GR.gp1 * (p_star / p_R) + GR.gm1) / gR);
         }
     }
 
@@ -256,37 +304,37 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
     }
 
     if (crw0 && ((u_star - cs_L) > lu)) {                 // t-axis inside the 1-CRW
-        U[1] = zL * (u_L + 2.0 * (c_L + lu) / (gL - 1.0));
+        U[1] = zL * (u_L + 2.0 * (c_L + lu) / GL.gm1);
         C = U[1] - lu;
         const PowBase b = pow_base(C / c_L);             // c_frac = C / c_L below
-        U[3] = pow_with(b, 2.0 * gL / (gL - 1.0)) * p_L;
+        U[3] = pow_with(b, GL.e_sonic) * p_L;
         U[0] = gL * U[3] / C / C;
         U[2] = v_L;
-        const double TdS = (d_p_L - d_rho_L * c_L * c_L) / (gL - 1.0) / rho_L;
-        const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / (gL - 1.0) / rho_L;
-        const double pw_a = pow_with(b, 0.5 / zL), pw_b = pow_with(b, (1.0 + zL) / zL);
-        D[1] = ((1.0 + zL) * pw_a + zL * pw_b);
-        D[1] = D[1] / (1.0 + 2.0 * zL) * TdS;
+        const double TdS = (d_p_L - d_rho_L * c_L * c_L) / GL.gm1 / rho_L;
+        const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / GL.gm1 / rho_L;
+        const double pw_a = pow_with(b, GL.e_a), pw_b = pow_with(b, GL.e_b);
+        D[1] = (GL.opz * pw_a + zL * pw_b);
+        D[1] = D[1] / GL.op2z * TdS;
         D[1] = D[1] - c_L * pw_a * d_Psi;
         D[3] = U[0] * (U[1] - lu) * D[1];
-        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * (gL - 1.0);
+        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GL.gm1;
         D[0] = (D[0] + D[3]) / C / C;
         D[2] = -(U[1] - lu) * d_v_L * U[0] / rho_L;
     } else if (crw1 && ((u_star + cs_R) < lu)) {          // t-axis inside the 3-CRW
-        U[1] = zR * (u_R - 2.0 * (c_R - lu) / (gR - 1.0));
+        U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1);
         C = lu - U[1];
         const PowBase b = pow_base(C / c_R);
-        U[3] = pow_with(b, 2.0 * gR / (gR - 1.0)) * p_R;
+        U[3] = pow_with(b, GR.e_sonic) * p_R;
         U[0] = gR * U[3] / C / C;
         U[2] = v_R;
-        const double TdS = (d_p_R - d_rho_R * c_R * c_R) / (gR - 1.0) / rho_R;
-        const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / (gR - 1.0) / rho_R;
-        const double pw_a = pow_with(b, 0.5 / zR), pw_b = pow_with(b, (1.0 + zR) / zR);
-        D[1] = ((1.0 + zR) * pw_a + zR * pw_b);
-        D[1] = D[1] / (1.0 + 2.0 * zR) * TdS;
+        const double TdS = (d_p_R - d_rho_R * c_R * c_R) / GR.gm1 / rho_R;
+        const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / GR.gm1 / rho_R;
+        const double pw_a = pow_with(b, GR.e_a), pw_b = pow_with(b, GR.e_b);
+        D[1] = (GR.opz * pw_a + zR * pw_b);
+        D[1] = D[1] / GR.op2z * TdS;
         D[1] = D[1] + c_R * pw_a * d_Phi;
         D[3] = U[0] * (U[1] - lu) * D[1];
-        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * (gR - 1.0);
+        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GR.gm1;
         D[0] = (D[0] + D[3]) / C / C;
         D[2] = -(U[1] - lu) * d_v_R * U[0] / rho_R;
     } else {                                              // non-sonic
@@ -302,19 +350,19 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
             a_L = 1.0;
             b_L = 1.0 / rs_L / cs_L;
             const PowBase b = pow_base(cs_L / c_L);
-            const double TdS = (d_p_L - d_rho_L * c_L * c_L) / (gL - 1.0) / rho_L;
-            const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / (gL - 1.0) / rho_L;
-            const double pw_a = pow_with(b, 0.5 / zL);
-            pwL_b = pow_with(b, (1.0 + zL) / zL);
-            d_L = ((1.0 + zL) * pw_a + zL * pwL_b);
-            d_L = d_L / (1.0 + 2.0 * zL) * TdS;
+            const double TdS = (d_p_L - d_rho_L * c_L * c_L) / GL.gm1 / rho_L;
+            const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / GL.gm1 / rho_L;
+            const double pw_a = pow_with(b, GL.e_a);
+            pwL_b = pow_with(b, GL.e_b);
+            d_L = (GL.opz * pw_a + zL * pwL_b);
+            d_L = d_L / GL.op2z * TdS;
             d_L = d_L - c_L * pw_a * d_Psi;
         } else {
-            const double SmUs = -sqrt(0.5 * ((gL + 1.0) * p_L + (gL - 1.0) * p_star) / rs_L);
-            const double SmUL = -sqrt(0.5 * ((gL + 1.0) * p_star + (gL - 1.0) * p_L) / rho_L);
-            const double VAR = sqrt((1 - zL) / (rho_L * (p_star + zL * p_L)));
-            const double H1 = 0.5 * VAR * (p_star + (1.0 + 2.0 * zL) * p_L) / (p_star + zL * p_L);
-            const double H2 = -0.5 * VAR * ((2.0 + zL) * p_star + zL * p_L) / (p_star + zL * p_L);
+            const double SmUs = -sqrt(0.5 * (GL.gp1 * p_L + GL.gm1 * p_star) / rs_L);
+            const double SmUL = -sqrt(0.5 * (GL.gp1 * p_star + GL.gm1 * p_L) / rho_L);
+            const double VAR = sqrt(GL.omz / (rho_L * (p_star + zL * p_L)));
+            const double H1 = 0.5 * VAR * (p_star + GL.op2z * p_L) / (p_star + zL * p_L);
+            const double H2 = -0.5 * VAR * (GL.tpz * p_star + zL * p_L) / (p_star + zL * p_L);
             const double H3 = -0.5 * VAR * (p_star - p_L) / rho_L;
             const double L_p = -1.0 / rho_L - SmUL * H2;
             const double L_u = SmUL + rho_L * (c_L * c_L * H2 + H3);
@@ -327,19 +375,19 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
             a_R = 1.0;
             b_R = -1.0 / rs_R / cs_R;
             const PowBase b = pow_base(cs_R / c_R);
-            const double TdS = (d_p_R - d_rho_R * c_R * c_R) / (gR - 1.0) / rho_R;
-            const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / (gR - 1.0) / rho_R;
-            const double pw_a = pow_with(b, 0.5 / zR);
-            pwR_b = pow_with(b, (1.0 + zR) / zR);
-            d_R = ((1.0 + zR) * pw_a + zR * pwR_b);
-            d_R = d_R / (1.0 + 2.0 * zR) * TdS;
+            const double TdS = (d_p_R - d_rho_R * c_R * c_R) / GR.gm1 / rho_R;
+            const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / GR.gm1 / rho_R;
+            const double pw_a = pow_with(b, GR.e_a);
+            pwR_b = pow_with(b, GR.e_b);
+            d_R = (GR.opz * pw_a + zR * pwR_b);
+            d_R = d_R / GR.op2z * TdS;
             d_R = d_R + c_R * pw_a * d_Phi;
         } else {
-            const double SmUs = sqrt(0.5 * ((gR + 1.0) * p_R + (gR - 1.0) * p_star) / rs_R);
-            const double SmUR = sqrt(0.5 * ((gR + 1.0) * p_star + (gR - 1.0) * p_R) / rho_R);
-            const double VAR = sqrt((1.0 - zR) / (rho_R * (p_star + zR * p_R)));
-            const double H1 = 0.5 * VAR * (p_star + (1 + 2.0 * zR) * p_R) / (p_star + zR * p_R);
-            const double H2 = -0.5 * VAR * ((2.0 + zR) * p_star + zR * p_R) / (p_star + zR * p_R);
+            const double SmUs = sqrt(0.5 * (GR.gp1 * p_R + GR.gm1 * p_star) / rs_R);
+            const double SmUR = sqrt(0.5 * (GR.gp1 * p_star + GR.gm1 * p_R) / rho_R);
+            const double VAR = sqrt(GR.omz / (rho_R * (p_star + zR * p_R)));
+            const double H1 = 0.5 * VAR * (p_star + GR.op2z * p_R) / (p_star + zR * p_R);
+            const double H2 = -0.5 * VAR * (GR.tpz * p_star + zR * p_R) / (p_star + zR * p_R);
             const double H3 = -0.5 * VAR * (p_star - p_R) / rho_R;
             const double L_p = -1.0 / rho_R + SmUR * H2;
             const double L_u = SmUR - rho_R * (c_R * c_R * H2 + H3);
@@ -361,11 +409,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[2] = -U[1] * d_v_R * U[0] / rho_R;
                 D[2] = D[2] + lu * d_v_R;
             } else {
-                const double SmUs = sqrt(0.5 * ((gR + 1.0) * p_R + (gR - 1.0) * p_star) / rs_R);
-                const double SmUR = sqrt(0.5 * ((gR + 1.0) * p_star + (gR - 1.0) * p_R) / rho_R);
+                const double SmUs = sqrt(0.5 * (GR.gp1 * p_R + GR.gm1 * p_star) / rs_R);
+                const double SmUR = sqrt(0.5 * (GR.gp1 * p_star + GR.gm1 * p_R) / rho_R);
                 const double VAR = p_R + zR * p_star;
-                const double H1 = rho_R * p_R * (1.0 - zR * zR) / VAR / VAR;
-                const double H2 = rho_R * p_star * (zR * zR - 1.0) / VAR / VAR;
+                const double H1 = rho_R * p_R * GR.omzz / VAR / VAR;
+                const double H2 = rho_R * p_star * GR.zzm1 / VAR / VAR;
                 const double H3 = (p_star + zR * p_R) / VAR;
                 const double L_rho = SmUR * H3 * d_rho_R;
                 const double L_u = -rho_R * (H2 * c_R * c_R + H3) * d_u_R;
@@ -385,11 +433,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[2] = -U[1] * d_v_L * U[0] / rho_L;
                 D[2] = D[2] + lu * d_v_L;
             } else {
-                const double SmUs = -sqrt(0.5 * ((gL + 1.0) * p_L + (gL - 1.0) * p_star) / rs_L);
-                const double SmUL = -sqrt(0.5 * ((gL + 1.0) * p_star + (gL - 1.0) * p_L) / rho_L);
+                const double SmUs = -sqrt(0.5 * (GL.gp1 * p_L + GL.gm1 * p_star) / rs_L);
+                const double SmUL = -sqrt(0.5 * (GL.gp1 * p_star + GL.gm1 * p_L) / rho_L);
                 const double VAR = p_L + zL * p_star;
-                const double H1 = rho_L * p_L * (1.0 - zL * zL) / VAR / VAR;
-                const double H2 = rho_L * p_star * (zL * zL - 1.0) / VAR / VAR;
+                const double H1 = rho_L * p_L * GL.omzz / VAR / VAR;
+                const double H2 = rho_L * p_star * GL.zzm1 / VAR / VAR;
                 const double H3 = (p_star + zL * p_L) / VAR;
                 const double L_rho = SmUL * H3 * d_rho_L;
                 const double L_u = -rho_L * (H2 * c_L * c_L + H3) * d_u_L;
@@ -436,7 +484,7 @@ __device__ __forceinline__ int star_dire_bad(const double mid[4], const double d
 // On success F = (F_rho, F_u, F_v, F_e), I = (rho, u, v, p) at t + tau; on a
 // star/derivative failure F and I are zero and the check code is returned.
 template <int DIR>
-__device__ __forceinline__ int grp_2d_flux(Side L, Side R, double gamma, double tau, double eps,
+__device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double tau, double eps,
                                            double F[4], double I[4])
 {
     constexpr double n_x = DIR == 0 ? 1.0 : 0.0;
@@ -456,7 +504,7 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, double gamma, double 
     const double T[4] = {tz, t_u, t_v, tz};
 
     double dire[4], mid[4];
-    grp_q1d(dire, mid, L, R, T, T, gamma, gamma, eps);
+    grp_q1d(dire, mid, L, R, T, T, G, G, eps);
     const int bad = star_dire_bad(mid, dire, eps);
     if (bad) {
         F[0] = F[1] = F[2] = F[3] = 0.0;
@@ -470,7 +518,7 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, double gamma, double 
     F[0] = rho_mid * (u_mid * n_x + v_mid * n_y);
     F[1] = F[0] * u_mid + p_mid * n_x;
     F[2] = F[0] * v_mid + p_mid * n_y;
-    F[3] = (gamma / (gamma - 1.0)) * p_mid / rho_mid + 0.5 * (u_mid * u_mid + v_mid * v_mid);
+    F[3] = G.g_gm1 * p_mid / rho_mid + 0.5 * (u_mid * u_mid + v_mid * v_mid);
     F[3] = F[0] * F[3];
     I[1] = (mid[1] + tau * dire[1]) * n_x - (mid[2] + tau * dire[2]) * n_y;
     I[2] = (mid[1] + tau * dire[1]) * n_y + (mid[2] + tau * dire[2]) * n_x;

@@ -19,8 +19,9 @@ __global__ void flux_batch_kernel(const double *L, const double *R, int count, d
     Side SL = {l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]};
     Side SR = {r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
     double F[4], I[4];
-    const int code = dir == 0 ? grp_2d_flux<0>(SL, SR, l[0], tau, eps, F, I)
-                              : grp_2d_flux<1>(SL, SR, l[0], tau, eps, F, I);
+    const Gas G = make_gas(l[0]);
+    const int code = dir == 0 ? grp_2d_flux<0>(SL, SR, G, tau, eps, F, I)
+                              : grp_2d_flux<1>(SL, SR, G, tau, eps, F, I);
     double *o = out + (size_t)k * 9;
     o[0] = code;
     for (int q = 0; q < 4; ++q) { o[1 + q] = F[q]; o[5 + q] = I[q]; }
@@ -31,7 +32,8 @@ __global__ void riemann_batch_kernel(const double *in, int count, int N, double 
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
     const double *a = in + (size_t)k * 10;
-    const StarResult r = riemann_exact(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], N);
+    const StarResult r = riemann_exact(make_gas(a[0]), make_gas(a[1]), a[2], a[3], a[4], a[5], a[6], a[7], a[8],
+                                       a[9], N);
     double *o = out + (size_t)k * 5;
     o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters;
 }

@@ -1,8 +1,10 @@
 // sweep2d.hip -- gfx950 kernels of the dimension-split 2D GRP step.
 //
-//   sweep_x_kernel   one X(tau_s) sweep: threads walk down x-lines (i fixed) with a
-//                    sliding register window; a wave covers 64 consecutive i, so
-//                    every load/store is one coalesced 512-byte row segment.
+//   sweep_x_rows_kernel  one X(tau_s) sweep: a block of W waves covers 64
+//                    consecutive i (one coalesced 512-byte row segment per wave)
+//                    and W cell rows; x-neighbours are exchanged through LDS.
+//   sweep_x_kernel   the same sweep with threads walking down x-lines in a
+//                    sliding register window (kept as a tuning alternative).
 //   sweep_y_kernel   one Y(tau) sweep: a wave owns 64 consecutive cells of a y-line
 //                    (contiguous in memory) and exchanges neighbour values,
 //                    slopes and fluxes with lane shuffles; 63 cells per wave.
@@ -129,7 +131,7 @@ struct Line {
         const double rho = w[0] + -nu * (Fr[0] - Fl[0]);
         const double u = mom_x / rho;
         const double v = mom_y / rho;
-        const double p = (ene - 0.5 * mom_x * u - 0.5 * mom_y * v) * (a.gamma - 1.0);
+        const double p = (ene - 0.5 * mom_x * u - 0.5 * mom_y * v) * a.gas.gm1;
         a.rho_o[o] = rho; a.u_o[o] = u; a.v_o[o] = v; a.e_o[o] = ene / rho; a.p_o[o] = p;
         if (p < a.eps || rho < a.eps) atomicOr(&a.status->flags, 1u);
 #pragma unroll
@@ -154,7 +156,7 @@ __device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &
                                                 unsigned long long loop_index, double F[4], double I[4])
 {
     int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
-    if (grp_2d_flux<DIR>(SL, SR, a.gamma, a.tau, a.eps, F, I)) code = 2;
+    if (grp_2d_flux<DIR>(SL, SR, a.gas, a.tau, a.eps, F, I)) code = 2;
     if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
 }
 
@@ -206,8 +208,8 @@ __global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
 // are exchanged through LDS.  Wave w solves interface c+1 (between rows c and
 // c+1); waves 1..W-1 update rows j0 .. j0+W-2.  Every interface row is one
 // wave, so the GRP work is spread over all rows of the grid at once.
-template <int W>
-__global__ void __launch_bounds__(64 * W) sweep_x_rows_kernel(SweepArgs a)
+template <int W, int MINW>
+__global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 {
     __shared__ double xa[W][4][64];   // values, then fluxes
     __shared__ double xb[W][4][64];   // slopes, then interface values
@@ -279,7 +281,8 @@ __global__ void __launch_bounds__(64 * W) sweep_x_rows_kernel(SweepArgs a)
 // Y sweep: a wave owns cells c0-1 .. c0+62 of y-line j (lane t: cell c0-1+t),
 // computes interfaces c0 .. c0+63 (lane t: interface c0+t) and updates cells
 // c0 .. c0+62 (lane t >= 1).
-__global__ void __launch_bounds__(256) sweep_y_kernel(SweepArgs a, int chunks)
+template <int MINW>
+__global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chunks)
 {
     if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63;
@@ -358,37 +361,59 @@ __global__ void __launch_bounds__(256) cfl_kernel(const double *rho, const doubl
 // ---- launchers -----------------------------------------------------------------
 static constexpr int TJ_X = 32;
 
-// x-sweep variant: 0 = row-parallel (LDS exchange, 8 rows per block),
-// 1 = row-parallel with 16 rows per block, 2 = sliding register window.
-// HGRP_SWEEP_X overrides the default; read once.
-static int sweep_x_variant()
+// Launch variants.  The defaults were picked by timing full four-quadrant runs
+// at 1024^2 and 4096^2 on MI355X:
+//   X: 8 rows per block; with the register budget for 4 waves/SIMD (128 VGPRs,
+//      a few spills) up to 2^22 cells, where the grid is only ~9 blocks per CU
+//      and the extra resident block hides the tail; with the compiler's free
+//      budget (2 waves/SIMD, no spills) above that.
+//   Y: register budget for 4 waves/SIMD at every size.
+// HGRP_SWEEP_X / HGRP_SWEEP_Y override them (read once, for tuning runs):
+//   HGRP_SWEEP_X  1 = 8 rows, free budget;  2 = 16 rows, 4 waves/SIMD;
+//                 3 = sliding register window;  4 = 8 rows, 4 waves/SIMD
+//   HGRP_SWEEP_Y  1 = free budget;  2 = 3 waves/SIMD;  3 = 4 waves/SIMD
+static int env_variant(const char *name, int hi)
+{
+    const char *e = getenv(name);
+    int v = e ? atoi(e) : 0;
+    return (v < 0 || v > hi) ? 0 : v;
+}
+static int sweep_x_variant(const SweepArgs &a)
 {
     static int v = -1;
-    if (v < 0) {
-        const char *e = getenv("HGRP_SWEEP_X");
-        v = e ? atoi(e) : 0;
-        if (v < 0 || v > 2) v = 0;
-    }
-    return v;
+    if (v < 0) v = env_variant("HGRP_SWEEP_X", 4);
+    if (v != 0) return v;
+    return (long)a.m * a.n <= (1l << 22) ? 4 : 1;
+}
+static int sweep_y_variant()
+{
+    static int v = -1;
+    if (v < 0) v = env_variant("HGRP_SWEEP_Y", 3);
+    return v != 0 ? v : 3;
 }
 
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
 {
-    switch (sweep_x_variant()) {
-    case 2: {
+    switch (sweep_x_variant(a)) {
+    case 3: {
         const int bx = a.n >= 256 ? 256 : (a.n >= 128 ? 128 : 64);
         dim3 grid((a.n + bx - 1) / bx, (a.m + TJ_X - 1) / TJ_X);
         hipLaunchKernelGGL(sweep_x_kernel<TJ_X>, grid, dim3(bx), 0, st, a);
         break;
     }
-    case 1: {
+    case 2: {
         dim3 grid((a.n + 63) / 64, (a.m + 14) / 15);
-        hipLaunchKernelGGL(sweep_x_rows_kernel<16>, grid, dim3(64 * 16), 0, st, a);
+        hipLaunchKernelGGL((sweep_x_rows_kernel<16, 4>), grid, dim3(64 * 16), 0, st, a);
+        break;
+    }
+    case 4: {
+        dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
+        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4>), grid, dim3(64 * 8), 0, st, a);
         break;
     }
     default: {
         dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-        hipLaunchKernelGGL(sweep_x_rows_kernel<8>, grid, dim3(64 * 8), 0, st, a);
+        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1>), grid, dim3(64 * 8), 0, st, a);
         break;
     }
     }
@@ -400,7 +425,11 @@ hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st)
     const int chunks = (a.n + 62) / 63;
     const long waves = (long)a.m * chunks;
     const long blocks = (waves + 3) / 4;
-    hipLaunchKernelGGL(sweep_y_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+    switch (sweep_y_variant()) {
+    case 1: hipLaunchKernelGGL(sweep_y_kernel<1>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    case 2: hipLaunchKernelGGL(sweep_y_kernel<3>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    default: hipLaunchKernelGGL(sweep_y_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    }
     return hipGetLastError();
 }
 

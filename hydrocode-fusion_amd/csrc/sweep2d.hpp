@@ -15,6 +15,8 @@
 #pragma once
 #include <stdint.h>
 
+#include "grp_device.hpp"
+
 namespace hgrp {
 
 // boundary condition codes (config[17], config[18]; doc/config.csv)
@@ -35,7 +37,8 @@ struct SweepArgs {
     int m, n, ld;                             // cells in x, in y, row pitch
     int bc;                                   // boundary condition along the sweep
     int limiter3;                             // 0: minmod2 (first sweep of this direction), 1: minmod3
-    double h, hh, tau, nu, gamma, eps, alpha; // h, h/2, flux time, update factor, ...
+    double h, hh, tau, nu, eps, alpha;        // h, h/2, flux time, update factor, ...
+    Gas gas;                                  // gamma and its derived constants
     SweepStatus *status;                      // this sweep
     const SweepStatus *prev;                  // the sweep before (nullptr for the first)
 };
