@@ -29,6 +29,19 @@ M = N = 1024
 WORKLOAD = "2D four-quadrant Riemann problem {m}x{n}, dimension-split GRP (X/2 Y X/2), FP64"
 
 
+def measured_traffic(kernel, m, n):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/*_hbm_traffic.md, a line
+    `traffic_bytes: {...}`), or None where no such profile exists."""
+    path = os.path.join(ROOT, "profiles", f"sweep_kernels_fourquadrant{m}_hbm_traffic.md")
+    if m != n or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes:"):
+            return json.loads(line.split(":", 1)[1]).get(kernel)
+    return None
+
+
 def gpu_leg(steps, warmup):
     import numpy as np
     import hydrogrp
@@ -41,13 +54,14 @@ def gpu_leg(steps, warmup):
         s.bench(warmup)
         s.upload(init["rho"], init["u"], init["v"], init["p"])    # timed run starts from t = 0
     ms_total, ms_x, ms_y = s.bench(steps)
+    kernels = (s.sweep_kernel(0), s.sweep_kernel(1))
     state = s.download()
     s.close()
     assert np.all(np.isfinite(state)), "non-finite state after the benchmark"
     # a plain copy of 1 GiB: the measured HBM ceiling for streaming kernels
     nbytes = 1 << 30
     copy_gbs = 2.0 * nbytes / (hydrogrp.copy_probe(nbytes, 10) / 1000.0) / 1e9
-    return ms_total, ms_x, ms_y, hydrogrp.device_name(), copy_gbs
+    return ms_total, ms_x, ms_y, kernels, hydrogrp.device_name(), copy_gbs
 
 
 def cpu_leg(target_seconds=12.0):
@@ -94,7 +108,7 @@ def main():
         sys.exit(2)
 
     t0 = time.perf_counter()
-    ms_total, ms_x, ms_y, device, copy_gbs = gpu_leg(args.steps, args.warmup)
+    ms_total, ms_x, ms_y, kernels, device, copy_gbs = gpu_leg(args.steps, args.warmup)
     wall = time.perf_counter() - t0
     value = M * N * args.steps / (ms_total / 1000.0)
     achieved = SWEEP_BYTES_PER_CELL * M * N / (ms_x / 1000.0) / 1e9
@@ -113,10 +127,11 @@ def main():
         "data": "synthetic (four-quadrant initial data generated in-process)",
         "config": {"workload": WORKLOAD.format(m=M, n=N), "n_x": M, "n_y": N, "gamma": 1.4, "cfl": 0.45,
                    "boundary": "free (-4, -4)", "device": device},
-        "roofline": {"bound": "hbm", "kernel": "sweep_x_kernel", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "roofline": {"bound": "hbm", "kernel": kernels[0], "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(kernels[0], M, N),
                      "copy_kernel_gbs": copy_gbs,
-                     "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}},
+                     "ms_per_launch": {kernels[0]: ms_x, kernels[1]: ms_y}},
         "wall_s": wall,
     }
     if not args.no_cpu:

@@ -23,6 +23,7 @@
 namespace hgrp {
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st);
 hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st);
+const char *sweep_kernel_name(int dir, int m, int n);
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 }
@@ -197,6 +198,12 @@ extern "C" int hgrp2d_h_over_smax(hgrp2d *c, double *h)
     if (!c || !h || !c->uploaded) return ARG_ERROR - 1;
     *h = c->h_S_max;
     return 0;
+}
+
+extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
+{
+    if (!c || (dir != 0 && dir != 1)) return "";
+    return sweep_kernel_name(dir, c->m, c->n);
 }
 
 // One enqueued sweep and the host loop state it belongs to.

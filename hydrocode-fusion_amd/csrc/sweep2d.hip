@@ -433,6 +433,25 @@ hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
+const char *sweep_kernel_name(int dir, int m, int n)
+{
+    SweepArgs a{};
+    a.m = m; a.n = n;
+    if (dir == 1) {
+        switch (sweep_y_variant()) {
+        case 1: return "sweep_y_kernel<1>";
+        case 2: return "sweep_y_kernel<3>";
+        default: return "sweep_y_kernel<4>";
+        }
+    }
+    switch (sweep_x_variant(a)) {
+    case 3: return "sweep_x_kernel<32>";
+    case 2: return "sweep_x_rows_kernel<16, 4>";
+    case 4: return "sweep_x_rows_kernel<8, 4>";
+    default: return "sweep_x_rows_kernel<8, 1>";
+    }
+}
+
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st)
 {

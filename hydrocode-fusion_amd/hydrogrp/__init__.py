@@ -122,6 +122,12 @@ class Solver2D:
         _check(lib().hgrp2d_solve(self._h, _p(tp), len(tp), N_T, cb, None, C.byref(res)), "hgrp2d_solve")
         return res
 
+    def sweep_kernel(self, direction):
+        """Name of the kernel launched for an x (0) or y (1) sweep of this grid."""
+        f = lib().hgrp2d_sweep_kernel
+        f.restype = C.c_char_p
+        return f(self._h, direction).decode()
+
     def bench(self, steps):
         t, x, y = _D(), _D(), _D()
         _check(lib().hgrp2d_bench(self._h, steps, C.byref(t), C.byref(x), C.byref(y)), "hgrp2d_bench")

@@ -79,6 +79,10 @@ int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double 
 /* h/S_max of the uploaded initial data (the frozen CFL reduction). */
 int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
 
+/* Name of the kernel the context launches for an x (dir 0) or y (dir 1) sweep
+ * (it depends on the grid size); "" for a bad argument.  For profiling tools. */
+const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
+
 /* ------------------------------------------------------------------------- */
 /* Batched interface solves (host arrays in/out; for unit parity).            */
 /* Interface sides are 15 doubles: gamma, lambda_u, lambda_v, rho, u, v, p,   */
