@@ -23,7 +23,7 @@
 namespace hgrp {
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st);
 hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st);
-const char *sweep_kernel_name(int dir, int m, int n);
+const char *sweep_kernel_name(int dir, int m, int n, bool learned);
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 }
@@ -57,6 +57,9 @@ struct hgrp2d {
     int found_x, found_y;       // the reference's find_bound_x / find_bound_y
     double h_S_max;
     bool uploaded;
+    double newton_tol;          // <= 0: the reference's (eps)
+    MlpPacked *mlp;             // device copy of the learned-guess weights
+    bool use_mlp;
     hipEvent_t ev[2];
 };
 
@@ -114,6 +117,7 @@ extern "C" int hgrp2d_destroy(hgrp2d *c)
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->block) (void)hipFree(c->block);
     if (c->status) (void)hipFree(c->status);
+    if (c->mlp) (void)hipFree(c->mlp);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
     if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -203,7 +207,23 @@ extern "C" int hgrp2d_h_over_smax(hgrp2d *c, double *h)
 extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
 {
     if (!c || (dir != 0 && dir != 1)) return "";
-    return sweep_kernel_name(dir, c->m, c->n);
+    return sweep_kernel_name(dir, c->m, c->n, c->use_mlp);
+}
+
+extern "C" int hgrp2d_set_newton(hgrp2d *c, double tol, const float *params)
+{
+    if (!c || !(tol == tol)) return ARG_ERROR - 1;
+    HCHECK(hipStreamSynchronize(c->stream));
+    c->newton_tol = tol;
+    c->use_mlp = false;
+    if (params) {
+        MlpPacked h;
+        mlp_pack_host(params, &h);
+        if (!c->mlp) HCHECK(hipMalloc((void **)&c->mlp, sizeof(MlpPacked)));
+        HCHECK(hipMemcpy(c->mlp, &h, sizeof h, hipMemcpyHostToDevice));
+        c->use_mlp = true;
+    }
+    return 0;
 }
 
 // One enqueued sweep and the host loop state it belongs to.
@@ -236,6 +256,8 @@ static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double 
     a.gas = make_gas(c->cfg[6]);
     a.eps = c->cfg[4];
     a.alpha = c->cfg[41];
+    a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
+    a.mlp = c->use_mlp ? c->mlp : nullptr;
     a.status = c->status + slot;
     a.prev = prev_slot >= 0 ? c->status + prev_slot : nullptr;
 }

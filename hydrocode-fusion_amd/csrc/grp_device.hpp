@@ -121,17 +121,62 @@ __device__ __forceinline__ double pow_from_log(double x, DD L, double y)
     return pow_with(b, y);
 }
 
+// Newton steps allowed from a learned starting guess before the solve restarts
+// from the reference's guess.
+constexpr int GUESS_ITERS = 3;
+
 struct StarResult {
     double u_star, p_star;
     int crw0, crw1;
     int iters;     // Newton iterations taken (-1: equal-state early exit)
 };
 
+// One Newton step of Riemann_solver_exact (riemann_solver_exact_Ben.c:133-203):
+// tangents k1, k3 at p, the update (halved when negative), the curves at the
+// new p.
+__device__ __forceinline__ void newton_step(const Gas &GL, const Gas &GR, double u_L, double u_R, double p_L,
+                                            double p_R, double c_L, double c_R, double &p_INT, double &v_L,
+                                            double &v_R, double &k1, double &k3, DD &bL, DD &bR)
+{
+    const double gL = GL.g, gR = GR.g;
+    double t1, t2, t3, dp;
+    if (p_INT > p_L) {
+        dp = p_INT - p_L;
+        t1 = 1.0 / sqrt(1.0 + GL.nu * dp / p_L);
+        t2 = c_L / gL / p_L;
+        t3 = 0.5 * t2 * GL.nu / p_L;
+        k1 = t3 * dp * cube(t1) - t2 * t1;
+    } else {
+        t2 = c_L / gL / p_L;
+        t1 = 1.0 / pow_from_log(p_INT / p_L, bL, GL.nu);   // bL = log(p_INT / p_L) from the curve value
+        k1 = -t1 * t2;
+    }
+    if (p_INT > p_R) {
+        dp = p_INT - p_R;
+        t1 = 1.0 / sqrt(1.0 + GR.nu * dp / p_R);
+        t2 = c_R / gR / p_R;
+        t3 = 0.5 * t2 * GR.nu / p_R;
+        k3 = t2 * t1 - t3 * dp * cube(t1);
+    } else {
+        t2 = c_R / gR / p_R;
+        t1 = 1.0 / pow_from_log(p_INT / p_R, bR, GR.nu);
+        k3 = t1 * t2;
+    }
+    const double p_new = p_INT + (v_R - v_L) / (k1 - k3);
+    p_INT = (p_new < 0.0) ? 0.5 * p_INT : p_new;
+    v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
+    v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
+}
+
 // Riemann_solver_exact: wave types, the halved acoustic-tangent guess, Newton
 // on the velocity gap, u* from the last tangents.
+// p_guess > 0 (not in the reference): first up to GUESS_ITERS Newton steps
+// from p_guess; if they meet tol that is the answer, otherwise the solve
+// restarts from the reference's guess exactly as the reference does.  iters
+// counts both phases.
 __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR, double u_L, double u_R,
                                                     double p_L, double p_R, double c_L, double c_R,
-                                                    double eps, double tol, int N)
+                                                    double eps, double tol, int N, double p_guess = -1.0)
 {
     const double gL = GL.g, gR = GR.g;
     StarResult r;
@@ -141,8 +186,9 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
     r.crw1 = (u_LR > u_R + eps) ? 0 : 1;
     r.crw0 = (u_RL > u_L - eps) ? 1 : 0;
 
-    double k1 = -c_L / p_L / gL;
-    double k3 = c_R / p_R / gR;
+    const double k1_0 = -c_L / p_L / gL;
+    const double k3_0 = c_R / p_R / gR;
+    double k1 = k1_0, k3 = k3_0;
     double p_INT = (k1 * p_L - k3 * p_R - u_L + u_R) / (k1 - k3);
     if (p_INT < 0)
         p_INT = (p_L < p_R) ? p_L : p_R;
@@ -154,45 +200,39 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
         r.iters = -1;
         return r;
     }
+    int done = 0;
+    if (p_guess > 0.0) {
+        double p = p_guess;
+        double v_L = wave_curve<-1>(p, u_L, p_L, c_L, GL, &bL);
+        double v_R = wave_curve<+1>(p, u_R, p_R, c_R, GR, &bR);
+        double gap = fabs(v_L - v_R);
+        while ((gap > tol) && (done != GUESS_ITERS)) {
+            newton_step(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, p, v_L, v_R, k1, k3, bL, bR);
+            ++done;
+            gap = fabs(v_L - v_R);
+        }
+        if (gap <= tol) {
+            r.p_star = p;
+            r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
+            r.iters = done;
+            return r;
+        }
+        k1 = k1_0;
+        k3 = k3_0;
+    }
     double v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
     double v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
     double gap = fabs(v_L - v_R);
 
     int n = 0;
     while ((gap > tol) && (n != N)) {
-        double t1, t2, t3, dp;
-        if (p_INT > p_L) {
-            dp = p_INT - p_L;
-            t1 = 1.0 / sqrt(1.0 + GL.nu * dp / p_L);
-            t2 = c_L / gL / p_L;
-            t3 = 0.5 * t2 * GL.nu / p_L;
-            k1 = t3 * dp * cube(t1) - t2 * t1;
-        } else {
-            t2 = c_L / gL / p_L;
-            t1 = 1.0 / pow_from_log(p_INT / p_L, bL, GL.nu);   // bL = log(p_INT / p_L) from the curve value
-            k1 = -t1 * t2;
-        }
-        if (p_INT > p_R) {
-            dp = p_INT - p_R;
-            t1 = 1.0 / sqrt(1.0 + GR.nu * dp / p_R);
-            t2 = c_R / gR / p_R;
-            t3 = 0.5 * t2 * GR.nu / p_R;
-            k3 = t2 * t1 - t3 * dp * cube(t1);
-        } else {
-            t2 = c_R / gR / p_R;
-            t1 = 1.0 / pow_from_log(p_INT / p_R, bR, GR.nu);
-            k3 = t1 * t2;
-        }
-        const double p_new = p_INT + (v_R - v_L) / (k1 - k3);
-        p_INT = (p_new < 0.0) ? 0.5 * p_INT : p_new;
+        newton_step(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, p_INT, v_L, v_R, k1, k3, bL, bR);
         ++n;
-        v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
-        v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
         gap = fabs(v_L - v_R);
     }
     r.p_star = p_INT;
     r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
-    r.iters = n;
+    r.iters = n + done;
     return r;
 }
 
@@ -211,9 +251,11 @@ __device__ __forceinline__ void side_trivial(double D[4], double U[4], double rh
 
 // linear_GRP_solver_Edir_Q1D, single-fluid members, with atc = eps.
 // TL/TR: tangential derivatives (rho, u, v, p) of the two sides.
+// tol: Newton tolerance (the reference passes eps); p_guess: see riemann_exact.
 __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L, const Side &R,
                                         const double TL[4], const double TR[4],
-                                        const Gas &GL, const Gas &GR, double eps)
+                                        const Gas &GL, const Gas &GR, double eps, double tol,
+                                        double p_guess)
 {
     const double gL = GL.g, gR = GR.g;
     const double lu = 0.0, lv = 0.0;
@@ -240,7 +282,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         rs_L = rho_L; cs_L = c_L; speed_L = u_star - cs_L;
         rs_R = rho_R; cs_R = c_R; speed_R = u_star + cs_R;
     } else {
-        const StarResult st = riemann_exact(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, eps, 500);
+        const StarResult st = riemann_exact(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, 500, p_guess);
         u_star = st.u_star; p_star = st.p_star; crw0 = st.crw0; crw1 = st.crw1;
         if (crw0) {
             const PowBase b = pow_base(p_star / p_L);
@@ -483,9 +525,11 @@ __device__ __forceinline__ int star_dire_bad(const double mid[4], const double d
 // the split path's zero tangential derivatives (+0.0 in x, -0.0 in y).
 // On success F = (F_rho, F_u, F_v, F_e), I = (rho, u, v, p) at t + tau; on a
 // star/derivative failure F and I are zero and the check code is returned.
+// tol, p_guess: Newton tolerance and learned starting pressure (riemann_exact);
+// the reference's path is tol = eps, p_guess <= 0.
 template <int DIR>
 __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double tau, double eps,
-                                           double F[4], double I[4])
+                                           double F[4], double I[4], double tol, double p_guess = -1.0)
 {
     constexpr double n_x = DIR == 0 ? 1.0 : 0.0;
     constexpr double n_y = DIR == 0 ? 0.0 : 1.0;
@@ -504,7 +548,7 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double 
     const double T[4] = {tz, t_u, t_v, tz};
 
     double dire[4], mid[4];
-    grp_q1d(dire, mid, L, R, T, T, G, G, eps);
+    grp_q1d(dire, mid, L, R, T, T, G, G, eps, tol, p_guess);
     const int bad = star_dire_bad(mid, dire, eps);
     if (bad) {
         F[0] = F[1] = F[2] = F[3] = 0.0;

@@ -20,8 +20,8 @@ __global__ void flux_batch_kernel(const double *L, const double *R, int count, d
     Side SR = {r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
     double F[4], I[4];
     const Gas G = make_gas(l[0]);
-    const int code = dir == 0 ? grp_2d_flux<0>(SL, SR, G, tau, eps, F, I)
-                              : grp_2d_flux<1>(SL, SR, G, tau, eps, F, I);
+    const int code = dir == 0 ? grp_2d_flux<0>(SL, SR, G, tau, eps, F, I, eps)
+                              : grp_2d_flux<1>(SL, SR, G, tau, eps, F, I, eps);
     double *o = out + (size_t)k * 9;
     o[0] = code;
     for (int q = 0; q < 4; ++q) { o[1 + q] = F[q]; o[5 + q] = I[q]; }

@@ -153,11 +153,26 @@ __device__ __forceinline__ Side recon(const double w[4], const double s[4], doub
 
 template <int DIR>
 __device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &SL, const Side &SR,
-                                                unsigned long long loop_index, double F[4], double I[4])
+                                                unsigned long long loop_index, double F[4], double I[4],
+                                                double p_guess = -1.0)
 {
     int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
-    if (grp_2d_flux<DIR>(SL, SR, a.gas, a.tau, a.eps, F, I)) code = 2;
+    if (grp_2d_flux<DIR>(SL, SR, a.gas, a.tau, a.eps, F, I, a.tol, p_guess)) code = 2;
     if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
+}
+
+// Learned starting pressure for the interface (SL | SR) normal to DIR, for
+// every lane of the wave (uniform call); -1 where Newton will not run (the
+// Q1D acoustic shortcut dist < eps) or the lane has no interface.
+template <int DIR>
+__device__ __forceinline__ double interface_guess(const SweepArgs &a, bool active, const Side &SL, const Side &SR,
+                                                  uint32_t *lds)
+{
+    const double uL = DIR == 0 ? SL.u : SL.v, uR = DIR == 0 ? SR.u : SR.v;
+    const double dist = sqrt((SL.rho - SR.rho) * (SL.rho - SR.rho) + (uL - uR) * (uL - uR)
+                             + (SL.p - SR.p) * (SL.p - SR.p));
+    const bool want = active && !(dist < a.eps);
+    return learned_guess_wave(a.mlp, want, SL.rho, uL, SL.p, SR.rho, uR, SR.p, a.gas.g, a.gas.g, lds);
 }
 
 // X sweep: thread = x-line i, segment of TJ cells in j.
@@ -208,7 +223,7 @@ __global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
 // are exchanged through LDS.  Wave w solves interface c+1 (between rows c and
 // c+1); waves 1..W-1 update rows j0 .. j0+W-2.  Every interface row is one
 // wave, so the GRP work is spread over all rows of the grid at once.
-template <int W, int MINW>
+template <int W, int MINW, bool LEARNED>
 __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 {
     __shared__ double xa[W][4][64];   // values, then fluxes
@@ -261,10 +276,18 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
     const int f = c + 1;
+    double pg = -1.0;
+    if constexpr (LEARNED) {
+        // xa[w] is not read again before the flux exchange: it holds this wave's network inputs
+        const Side SL = recon(v, s, hh, true);
+        const Side SR = recon(vr, sr, hh, false);
+        pg = interface_guess<0>(a, col && f <= m, SL, SR, (uint32_t *)&xa[w][0][0]);
+    }
     if (col && f <= m) {
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f, F, I);
+        solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f, F, I,
+                           pg);
     }
     __syncthreads();                               // everyone is done reading values/slopes
 #pragma unroll
@@ -281,9 +304,10 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 // Y sweep: a wave owns cells c0-1 .. c0+62 of y-line j (lane t: cell c0-1+t),
 // computes interfaces c0 .. c0+63 (lane t: interface c0+t) and updates cells
 // c0 .. c0+62 (lane t >= 1).
-template <int MINW>
+template <int MINW, bool LEARNED>
 __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chunks)
 {
+    __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
     if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -323,10 +347,17 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
     const int f = c + 1;
+    double pg = -1.0;
+    if constexpr (LEARNED) {
+        const Side SL = recon(w, s, hh, true);
+        const Side SR = recon(wr, sr, hh, false);
+        pg = interface_guess<1>(a, f <= n, SL, SR, mlp_lds[(threadIdx.x >> 6) & 3]);
+    }
     if (f <= n) {
         const Side SL = recon(w, s, hh, true);
         const Side SR = recon(wr, sr, hh, false);
-        solve_interface<1>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I);
+        solve_interface<1>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I,
+                           pg);
     }
     double Fl[4], Il[4];
 #pragma unroll
@@ -394,6 +425,14 @@ static int sweep_y_variant()
 
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
 {
+    if (a.mlp) {
+        dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
+        if (sweep_x_variant(a) == 4)
+            hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4, true>), grid, dim3(64 * 8), 0, st, a);
+        else
+            hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1, true>), grid, dim3(64 * 8), 0, st, a);
+        return hipGetLastError();
+    }
     switch (sweep_x_variant(a)) {
     case 3: {
         const int bx = a.n >= 256 ? 256 : (a.n >= 128 ? 128 : 64);
@@ -403,17 +442,17 @@ hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
     }
     case 2: {
         dim3 grid((a.n + 63) / 64, (a.m + 14) / 15);
-        hipLaunchKernelGGL((sweep_x_rows_kernel<16, 4>), grid, dim3(64 * 16), 0, st, a);
+        hipLaunchKernelGGL((sweep_x_rows_kernel<16, 4, false>), grid, dim3(64 * 16), 0, st, a);
         break;
     }
     case 4: {
         dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4>), grid, dim3(64 * 8), 0, st, a);
+        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4, false>), grid, dim3(64 * 8), 0, st, a);
         break;
     }
     default: {
         dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1>), grid, dim3(64 * 8), 0, st, a);
+        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1, false>), grid, dim3(64 * 8), 0, st, a);
         break;
     }
     }
@@ -425,30 +464,39 @@ hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st)
     const int chunks = (a.n + 62) / 63;
     const long waves = (long)a.m * chunks;
     const long blocks = (waves + 3) / 4;
+    if (a.mlp) {
+        if (sweep_y_variant() == 1)
+            hipLaunchKernelGGL((sweep_y_kernel<1, true>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+        else
+            hipLaunchKernelGGL((sweep_y_kernel<4, true>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+        return hipGetLastError();
+    }
     switch (sweep_y_variant()) {
-    case 1: hipLaunchKernelGGL(sweep_y_kernel<1>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
-    case 2: hipLaunchKernelGGL(sweep_y_kernel<3>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
-    default: hipLaunchKernelGGL(sweep_y_kernel<4>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    case 1: hipLaunchKernelGGL((sweep_y_kernel<1, false>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    case 2: hipLaunchKernelGGL((sweep_y_kernel<3, false>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    default: hipLaunchKernelGGL((sweep_y_kernel<4, false>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
     }
     return hipGetLastError();
 }
 
-const char *sweep_kernel_name(int dir, int m, int n)
+const char *sweep_kernel_name(int dir, int m, int n, bool learned)
 {
     SweepArgs a{};
     a.m = m; a.n = n;
     if (dir == 1) {
+        if (learned) return sweep_y_variant() == 1 ? "sweep_y_kernel<1, true>" : "sweep_y_kernel<4, true>";
         switch (sweep_y_variant()) {
-        case 1: return "sweep_y_kernel<1>";
-        case 2: return "sweep_y_kernel<3>";
-        default: return "sweep_y_kernel<4>";
+        case 1: return "sweep_y_kernel<1, false>";
+        case 2: return "sweep_y_kernel<3, false>";
+        default: return "sweep_y_kernel<4, false>";
         }
     }
+    if (learned) return sweep_x_variant(a) == 4 ? "sweep_x_rows_kernel<8, 4, true>" : "sweep_x_rows_kernel<8, 1, true>";
     switch (sweep_x_variant(a)) {
     case 3: return "sweep_x_kernel<32>";
-    case 2: return "sweep_x_rows_kernel<16, 4>";
-    case 4: return "sweep_x_rows_kernel<8, 4>";
-    default: return "sweep_x_rows_kernel<8, 1>";
+    case 2: return "sweep_x_rows_kernel<16, 4, false>";
+    case 4: return "sweep_x_rows_kernel<8, 4, false>";
+    default: return "sweep_x_rows_kernel<8, 1, false>";
     }
 }
 

@@ -16,6 +16,7 @@
 #include <stdint.h>
 
 #include "grp_device.hpp"
+#include "mlp_guess.hpp"
 
 namespace hgrp {
 
@@ -38,6 +39,8 @@ struct SweepArgs {
     int bc;                                   // boundary condition along the sweep
     int limiter3;                             // 0: minmod2 (first sweep of this direction), 1: minmod3
     double h, hh, tau, nu, eps, alpha;        // h, h/2, flux time, update factor, ...
+    double tol;                               // Newton tolerance (the reference: eps)
+    const MlpPacked *mlp;                     // learned p* guess, or nullptr (the reference's guess)
     Gas gas;                                  // gamma and its derived constants
     SweepStatus *status;                      // this sweep
     const SweepStatus *prev;                  // the sweep before (nullptr for the first)

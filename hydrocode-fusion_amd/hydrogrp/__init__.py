@@ -47,6 +47,15 @@ def lib():
     L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
     L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
     L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
+    _FP = C.POINTER(C.c_float)
+    L.hgrp2d_set_newton.argtypes = [C.c_void_p, _D, _FP]
+    L.hgrp2d_sweep_kernel.argtypes = [C.c_void_p, C.c_int]
+    L.hgrp2d_sweep_kernel.restype = C.c_char_p
+    L.hgrp_mlp_data.argtypes = [C.c_ulonglong, C.c_int, _D, _FP, _FP, _DP, _IP]
+    L.hgrp_mlp_loss_grad.argtypes = [_FP, _FP, _FP, C.c_int, _DP, _FP]
+    L.hgrp_mlp_infer.argtypes = [_FP, _FP, C.c_int, _FP]
+    L.hgrp_mlp_train.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int, _D, _D, _FP, _FP]
+    L.hgrp_riemann_guess_batch.argtypes = [_DP, C.c_int, C.c_int, _FP, _DP]
     _lib = L
     return L
 
@@ -124,9 +133,16 @@ class Solver2D:
 
     def sweep_kernel(self, direction):
         """Name of the kernel launched for an x (0) or y (1) sweep of this grid."""
-        f = lib().hgrp2d_sweep_kernel
-        f.restype = C.c_char_p
-        return f(self._h, direction).decode()
+        return lib().hgrp2d_sweep_kernel(self._h, direction).decode()
+
+    def set_newton(self, tol=0.0, params=None):
+        """Newton tolerance for p* (<= 0: eps, the reference's) and the learned
+        starting guess (params: master vector from hydrogrp.mlp, None: off)."""
+        fp = None
+        if params is not None:
+            params = np.ascontiguousarray(params, dtype=np.float32)
+            fp = params.ctypes.data_as(C.POINTER(C.c_float))
+        _check(lib().hgrp2d_set_newton(self._h, tol, fp), "hgrp2d_set_newton")
 
     def bench(self, steps):
         t, x, y = _D(), _D(), _D()
@@ -134,8 +150,9 @@ class Solver2D:
         return t.value, x.value, y.value
 
 
-def run_split2d(m, n, init, config, time_plot=None, N_T=None):
-    """Whole run like the oracle's split2d(): returns (frames, N_plot, time_plot, config, status)."""
+def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None):
+    """Whole run like the oracle's split2d(): returns (frames, N_plot, time_plot, config, status).
+    newton_tol / guess_params: see Solver2D.set_newton (defaults: the reference's solve)."""
     config = np.array(config, dtype=np.float64)
     if time_plot is None:
         time_plot = np.array([0.0, config[1]])
@@ -144,6 +161,8 @@ def run_split2d(m, n, init, config, time_plot=None, N_T=None):
         N_T = min(5, len(time_plot))
     s = Solver2D(m, n, config)
     try:
+        if newton_tol or guess_params is not None:
+            s.set_newton(newton_tol, guess_params)
         s.upload(init["rho"], init["u"], init["v"], init["p"], init.get("e"))
         frames = np.zeros((5, N_T, m, n))
         frames[:, 0] = s.download()

@@ -79,6 +79,13 @@ int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double 
 /* h/S_max of the uploaded initial data (the frozen CFL reduction). */
 int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
 
+/* Newton solve for p* (no counterpart in the reference, which always uses
+ * tol = eps and its own starting guess): tol > 0 sets the Newton tolerance on
+ * the velocity gap (<= 0: eps, the reference's); params (HGRP_MLP_NP floats,
+ * the master vector of hgrp_mlp_train) switches on the learned starting guess,
+ * NULL switches it off.  The other uses of eps are unchanged. */
+int hgrp2d_set_newton(hgrp2d *ctx, double tol, const float *params);
+
 /* Name of the kernel the context launches for an x (dir 0) or y (dir 1) sweep
  * (it depends on the grid size); "" for a bad argument.  For profiling tools. */
 const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
@@ -107,6 +114,46 @@ int hgrp_pow_batch(const double *x, const double *y, int count, double *out);
  * kernel, copy_probe_kernel); the calibration point for HBM byte counters and
  * the measured copy bandwidth.  ms_per_copy: average device time of one copy. */
 int hgrp_copy_probe(size_t bytes, int reps, double *ms_per_copy);
+
+/* ------------------------------------------------------------------------- */
+/* Learned starting guess for p* (no counterpart in the reference).  A 6-32-32-1 */
+/* GELU MLP predicts log(p* / p_TR), p_TR the two-rarefaction estimate; inputs   */
+/* log(rho_L/rho_R)/ln1e3, log(p_L/p_R)/ln1e4, (u_R-u_L)/(5c_L), log(c_R/c_L)/5, */
+/* gamma_L-1.4, gamma_R-1.4.  Master parameters: W1[32][6] b1[32] W2[32][32]   */
+/* b2[32] w3[32] b3 (FP32); the device runs BF16 weights with FP32 accumulation */
+/* on MFMA.  Newton takes at most 3 steps from the guess, then restarts from   */
+/* the reference's guess.                                                       */
+/* ------------------------------------------------------------------------- */
+#define HGRP_MLP_NP 1313
+
+/* count samples of the training distribution (left state (1,0,1), rho_L/rho_R
+ * log-uniform 1e-3..1e3, p_L/p_R log-uniform 1e-4..1e4, (u_R-u_L)/c_L uniform
+ * -5..5, gamma 1.4; counter-based from seed), their p* from the exact solver at
+ * tolerance tol: feat[count*6], target[count] = log(p* / p_TR), ok[count];
+ * states[count*7] (may be NULL) = rho_L u_L p_L rho_R u_R p_R p*. */
+int hgrp_mlp_data(unsigned long long seed, int count, double tol, float *feat, float *target,
+                  double *states, int *ok);
+
+/* Mean squared error over count samples and its gradient (HGRP_MLP_NP floats)
+ * with respect to the master parameters, from the GPU forward/backward kernels. */
+int hgrp_mlp_loss_grad(const float *params, const float *feat, const float *target, int count,
+                       double *loss, float *grad);
+
+/* Network outputs for count samples (MFMA inference path). */
+int hgrp_mlp_infer(const float *params, const float *feat, int count, float *out);
+
+/* Trains on n_samples fresh samples (hgrp_mlp_data(seed, ...)) for `epochs`
+ * passes of Adam in minibatches of `batch` (a multiple of 64), learning rate lr
+ * with cosine decay to lr/100.  params: initial values in, trained values out.
+ * epoch_loss[epochs] (may be NULL): mean squared error of each pass. */
+int hgrp_mlp_train(unsigned long long seed, int n_samples, int epochs, int batch, double lr, double tol,
+                   float *params, float *epoch_loss);
+
+/* Exact Riemann solve per interface with an optional learned guess (params
+ * NULL: the reference's guess).  in[count*9]: rho_L u_L p_L rho_R u_R p_R gamma
+ * eps tol; out[count*6]: u* p* CRW[0] CRW[1] Newton-iterations starting-guess
+ * (-1 where the reference's guess was used). */
+int hgrp_riemann_guess_batch(const double *in, int count, int N, const float *params, double *out);
 
 /* Name of the device the library runs on (empty if none). */
 int hgrp_device_name(char *buf, int len);

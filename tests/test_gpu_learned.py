@@ -1,0 +1,133 @@
+"""GPU tests of the learned p* guess: training data against the oracle's exact
+solver, the GPU gradient and the BF16 MFMA inference against the NumPy
+reference (tests/mlp_ref.py), training, and what the trained network gains:
+fewer Newton iterations with the converged p* and the 2D result unchanged."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+import mlp_ref
+from conftest import rel_linf
+
+pytestmark = pytest.mark.gpu
+TOL12 = 1e-12
+
+
+@pytest.fixture(scope="module")
+def mlp():
+    from hydrogrp import mlp as m
+    import hydrogrp
+    hydrogrp.lib()
+    return m
+
+
+@pytest.fixture(scope="module")
+def trained(mlp):
+    if os.path.exists(mlp.WEIGHTS):
+        return mlp.load()
+    params, _ = mlp.train(mlp.init_params(3), seed=3, n_samples=1 << 21, epochs=6)
+    return params
+
+
+def test_training_targets_match_oracle(mlp, oracle):
+    feat, target, states, ok = mlp.data(11, 4096, TOL12)
+    assert ok.mean() > 0.99
+    g = 1.4
+    worst = 0.0
+    for k in np.flatnonzero(ok)[:300]:
+        rl, ul, pl, rr, ur, pr, ps = states[k]
+        cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
+        want = oracle.riemann(g, g, ul, ur, pl, pr, cl, cr, 1e-9, TOL12, 500)[1]
+        worst = max(worst, abs(ps - want) / want)
+    print("p* vs oracle: max rel", worst, flush=True)
+    assert worst <= 1e-10
+    # the two-rarefaction estimate is exact where both waves are rarefactions
+    two_rare = ok & (states[:, 6] < np.minimum(states[:, 2], states[:, 5]))
+    assert two_rare.sum() > 100
+    assert np.max(np.abs(target[two_rare])) < 2e-4
+    # the device features agree with the FP64 restatement
+    x64, _ = mlp_ref.features(states[ok, 0], states[ok, 1], states[ok, 2], states[ok, 3], states[ok, 4], states[ok, 5])
+    assert np.max(np.abs(feat[ok] - x64)) < 1e-5
+
+
+def test_gradient_matches_reference(mlp):
+    feat, target, states, ok = mlp.data(12, 3000, TOL12)
+    feat, target = feat[ok], target[ok]
+    P = mlp.init_params(4) + np.random.default_rng(4).normal(0, 0.05, mlp.NP).astype(np.float32)
+    loss, grad = mlp.loss_grad(P, feat, target)
+    loss_ref, grad_ref = mlp_ref.loss_grad(P, feat, target, quantize=True)
+    rel = np.linalg.norm(grad - grad_ref) / np.linalg.norm(grad_ref)
+    print(f"loss {loss:.6e} ref {loss_ref:.6e}; gradient rel L2 {rel:.2e}", flush=True)
+    assert abs(loss - loss_ref) <= 1e-4 * loss_ref
+    assert rel <= 2e-3
+    for name, sl in (("W1", slice(0, 192)), ("b1", slice(192, 224)), ("W2", slice(224, 1248)),
+                     ("b2", slice(1248, 1280)), ("w3", slice(1280, 1312)), ("b3", slice(1312, 1313))):
+        r = np.linalg.norm(grad[sl] - grad_ref[sl]) / max(np.linalg.norm(grad_ref[sl]), 1e-30)
+        assert r <= 5e-3, (name, r)
+
+
+def test_mfma_inference_matches_reference(mlp, trained):
+    feat, _, _, ok = mlp.data(13, 20000, TOL12)
+    feat = feat[ok]
+    for P in (trained, mlp.init_params(5)):
+        y = mlp.infer(P, feat)
+        y_ref, _ = mlp_ref.forward(P, feat, quantize=True)
+        d = np.abs(y - y_ref)
+        print(f"inference |y - y_ref|: median {np.median(d):.2e} max {d.max():.2e}", flush=True)
+        assert np.median(d) <= 1e-5
+        assert d.max() <= 2e-2
+    # rows of a partial wave
+    y = mlp.infer(trained, feat[:37])
+    assert np.allclose(y, mlp.infer(trained, feat[:64])[:37], rtol=0, atol=1e-6)
+
+
+def test_training_lowers_loss(mlp):
+    p0 = mlp.init_params(6)
+    params, losses = mlp.train(p0, seed=6, n_samples=1 << 18, epochs=3, batch=4096)
+    print("training mse per pass:", losses, flush=True)
+    feat, target, _, ok = mlp.data(99, 8192, TOL12)
+    l0, _ = mlp.loss_grad(p0, feat[ok], target[ok])
+    l1, _ = mlp.loss_grad(params, feat[ok], target[ok])
+    print(f"held-out mse: {l0:.3e} -> {l1:.3e}", flush=True)
+    assert l1 < 0.2 * l0 and losses[-1] < losses[0]
+
+
+def test_learned_guess_cuts_newton_iterations(mlp, trained):
+    _, _, states, ok = mlp.data(777, 1 << 16, TOL12)
+    st = states[ok]
+    plain = mlp.riemann_guess_batch(st, tol=TOL12)
+    learned = mlp.riemann_guess_batch(st, tol=TOL12, params=trained)
+    ip, il = plain[:, 4], learned[:, 4]
+    fallbacks = np.mean(il > 3)
+    dp = np.abs(learned[:, 1] - plain[:, 1]) / plain[:, 1]
+    print(f"Newton iterations: plain {ip.mean():.3f} (max {ip.max():.0f}), learned {il.mean():.3f} "
+          f"(max {il.max():.0f}); fallbacks {fallbacks * 100:.3f}%; p* rel diff median {np.median(dp):.1e} "
+          f"max {dp.max():.1e}", flush=True)
+    assert np.array_equal(plain[:, 2:4], learned[:, 2:4])        # wave types do not depend on the guess
+    assert il.mean() <= 0.6 * ip.mean()
+    assert fallbacks <= 0.02
+    assert np.median(dp) <= 1e-12 and dp.max() <= 1e-9
+    # the plain form of this entry point is the reference's solver
+    from hydrogrp import riemann_batch
+    g = 1.4
+    cl, cr = np.sqrt(g * st[:, 2] / st[:, 0]), np.sqrt(g * st[:, 5] / st[:, 3])
+    inp = np.stack([np.full(len(st), g), np.full(len(st), g), st[:, 1], st[:, 4], st[:, 2], st[:, 5], cl, cr,
+                    np.full(len(st), 1e-9), np.full(len(st), TOL12)], axis=1)
+    ref = riemann_batch(inp, 500)
+    assert np.array_equal(ref[:, 1], plain[:, 1]) and np.array_equal(ref[:, 4], plain[:, 4])
+
+
+def test_2d_learned_equals_plain(mlp, trained):
+    import hydrogrp
+    m = n = 128
+    init = cases.four_quadrant(m, n)
+    cfg = cases.four_quadrant_config(m, n)
+    fp, npp, tpp, cp, rp = hydrogrp.run_split2d(m, n, init, cfg, newton_tol=TOL12)
+    fl, npl, tpl, cl, rl = hydrogrp.run_split2d(m, n, init, cfg, newton_tol=TOL12, guess_params=trained)
+    assert (cp[5], npp, tpp[npp - 1]) == (cl[5], npl, tpl[npl - 1]) and rp.status == rl.status == 0
+    d = rel_linf(fl[:, npl - 1], fp[:, npp - 1])
+    l1 = np.abs(fl[:, npl - 1] - fp[:, npp - 1]).sum(axis=(1, 2)) / np.abs(fp[:, npp - 1]).sum(axis=(1, 2))
+    print("learned vs plain (tol 1e-12): rel Linf", d, "rel L1", l1, flush=True)
+    assert d.max() <= 1e-10 and l1.max() <= 1e-10
