@@ -12,6 +12,9 @@ x-sweep, two launches per step) against HBM peak with the algorithmic bytes of
 a sweep (128 B per cell: read and write rho u v p and their four slopes);
 `cpu_baseline` times the oracle (the C99 restatement of the reference, OpenMP)
 on a bounded number of steps of the same workload on the host's cores.
+`learned_guess` compares, at 4096^2 with Newton tolerance 1e-12, the plain
+solve with the one started from the learned p* guess (rate, and the relative L1
+difference of the two final states).
 """
 import argparse
 import json
@@ -64,6 +67,43 @@ def gpu_leg(steps, warmup):
     return ms_total, ms_x, ms_y, kernels, hydrogrp.device_name(), copy_gbs
 
 
+def learned_leg(size=4096, steps=200):
+    """The learned-guess workload: the same four-quadrant problem at size^2 with
+    Newton tolerance 1e-12, plain (the reference's starting guess) against the
+    learned guess, from t = 0 for `steps` steps each; plus Newton iteration
+    counts of both on held-out states of the training distribution."""
+    import numpy as np
+    import hydrogrp
+    from hydrogrp import mlp
+    import cases
+    params = mlp.load()
+    cfg = cases.four_quadrant_config(size, size)
+    init = cases.four_quadrant(size, size)
+    out, states = {}, {}
+    for name, p in (("plain", None), ("learned", params)):
+        s = hydrogrp.Solver2D(size, size, cfg)
+        s.set_newton(1e-12, p)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.bench(3)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        ms, ms_x, ms_y = s.bench(steps)
+        states[name] = s.download()
+        s.close()
+        out[name] = {"value": size * size * steps / (ms / 1000.0), "ms_per_step": ms / steps,
+                     "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
+    a, b = states["learned"], states["plain"]
+    rel_l1 = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
+    _, _, st, ok = mlp.data(777, 1 << 16, 1e-12)
+    it_p = mlp.riemann_guess_batch(st[ok], tol=1e-12)[:, 4]
+    it_l = mlp.riemann_guess_batch(st[ok], tol=1e-12, params=params)[:, 4]
+    return {"workload": f"2D four-quadrant {size}x{size}, Newton tol 1e-12, {steps} steps from t=0",
+            "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
+            "speedup": out["learned"]["value"] / out["plain"]["value"], "rel_l1_difference": rel_l1,
+            "newton_iterations_held_out": {"states": int(ok.sum()), "plain": float(it_p.mean()),
+                                           "learned": float(it_l.mean()),
+                                           "fallback_share": float(np.mean(it_l > 3))}}
+
+
 def cpu_leg(target_seconds=12.0):
     """The oracle on the host cores, fixed-tau mode (exactly the frozen tau), bounded sample."""
     import numpy as np
@@ -99,6 +139,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--size", type=int, default=1024, help="grid cells per side (default: the 1024^2 workload)")
+    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess comparison at 4096^2")
     args = ap.parse_args()
     global M, N
     M = N = args.size
@@ -134,6 +175,8 @@ def main():
                      "ms_per_launch": {kernels[0]: ms_x, kernels[1]: ms_y}},
         "wall_s": wall,
     }
+    if not args.no_learned:
+        line["learned_guess"] = learned_leg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg()
     print(json.dumps(line), flush=True)

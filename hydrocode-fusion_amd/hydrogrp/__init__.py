@@ -56,6 +56,13 @@ def lib():
     L.hgrp_mlp_infer.argtypes = [_FP, _FP, C.c_int, _FP]
     L.hgrp_mlp_train.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int, _D, _D, _FP, _FP]
     L.hgrp_riemann_guess_batch.argtypes = [_DP, C.c_int, C.c_int, _FP, _DP]
+    L.hgrp1d_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _IP, _DP, _DP]
+    L.hgrp1d_destroy.argtypes = [C.c_void_p]
+    L.hgrp1d_upload.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP]
+    L.hgrp1d_download.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP]
+    L.hgrp1d_run.argtypes = [C.c_void_p, C.c_int, _DP, C.POINTER(C.c_longlong)]
+    L.hgrp1d_results.argtypes = [C.c_void_p, _IP, _IP, _DP]
+    L.hgrp1d_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP]
     _lib = L
     return L
 
@@ -178,6 +185,75 @@ def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, gu
     config[5] = res.k
     time_plot[res.nt] = res.time_c
     return frames, res.nt + 1, time_plot, config, res
+
+
+class Ensemble1D:
+    """Independent 1D GRP problems advanced together (hgrp1d_* entry points)."""
+
+    def __init__(self, cells, h, config):
+        self.cells = np.ascontiguousarray(cells, dtype=np.int32)
+        self.h = np.ascontiguousarray(h, dtype=np.float64)
+        self.config = np.ascontiguousarray(config, dtype=np.float64)
+        assert self.config.size == N_CONF and len(self.h) == len(self.cells)
+        self.offsets = np.concatenate([[0], np.cumsum(self.cells)])
+        self._h = C.c_void_p()
+        _check(lib().hgrp1d_create(C.byref(self._h), len(self.cells), self.cells.ctypes.data_as(_IP),
+                                   _p(self.h), _p(self.config)), "hgrp1d_create")
+
+    def close(self):
+        if self._h:
+            lib().hgrp1d_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, rho, u, p, e=None):
+        arr = [np.ascontiguousarray(a, dtype=np.float64) for a in (rho, u, p)]
+        e = None if e is None else np.ascontiguousarray(e, dtype=np.float64)
+        _check(lib().hgrp1d_upload(self._h, *(_p(a) for a in arr), _p(e)), "hgrp1d_upload")
+
+    def download(self):
+        n = int(self.offsets[-1])
+        out = np.zeros((4, n))
+        _check(lib().hgrp1d_download(self._h, *(_p(out[q]) for q in range(4))), "hgrp1d_download")
+        return out
+
+    def run(self, graph=True):
+        ms, cu = _D(), C.c_longlong()
+        _check(lib().hgrp1d_run(self._h, int(graph), C.byref(ms), C.byref(cu)), "hgrp1d_run")
+        return ms.value, cu.value
+
+    def results(self):
+        n = len(self.cells)
+        steps, status, t = np.zeros(n, np.int32), np.zeros(n, np.int32), np.zeros(n)
+        _check(lib().hgrp1d_results(self._h, steps.ctypes.data_as(_IP), status.ctypes.data_as(_IP), _p(t)),
+               "hgrp1d_results")
+        return steps, status, t
+
+    def tune(self, probe_steps=4):
+        a, b = C.c_int(), C.c_int()
+        _check(lib().hgrp1d_tune(self._h, probe_steps, C.byref(a), C.byref(b)), "hgrp1d_tune")
+        return a.value, b.value
+
+
+def run_grp1d(m, init, config):
+    """One 1D problem through the batched engine, shaped like the oracle's
+    grp1d(): (final rho u p e [4, m], N_plot, time_plot, config, status)."""
+    config = np.array(config, dtype=np.float64)
+    ens = Ensemble1D([m], [config[10]], config)
+    try:
+        ens.upload(init["rho"], init["u"], init["p"], init.get("e"))
+        ens.run()
+        fin = ens.download()
+        steps, status, t = ens.results()
+    finally:
+        ens.close()
+    config[5] = steps[0]
+    return fin, 2, np.array([0.0, t[0]]), config, int(status[0])
 
 
 def flux_batch(L, R, tau, eps, direction):

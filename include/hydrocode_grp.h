@@ -116,6 +116,37 @@ int hgrp_pow_batch(const double *x, const double *y, int count, double *out);
 int hgrp_copy_probe(size_t bytes, int reps, double *ms_per_copy);
 
 /* ------------------------------------------------------------------------- */
+/* Batched 1D Eulerian GRP: independent problems of uneven sizes advanced    */
+/* together on one GPU, each with its own CFL time step.  Each instance runs  */
+/* GRP_solver_EUL_source (src/finite_volume/grp_solver_EUL_source.c:27-334,   */
+/* src/include/finite_volume.h:37) on its own cells: same limiter, ghosts,    */
+/* linear_GRP_solver_Edir, per-step tau from the reconstructed states, stop   */
+/* and abort rules; only the final state is kept (no plot frames).            */
+/* ------------------------------------------------------------------------- */
+typedef struct hgrp1d hgrp1d;
+
+/* n_inst instances with cells[i] cells of width h[i]; config[] as the
+ * reference's (1 t_all, 4 eps, 5 max steps, 6 gamma, 7 CFL, 16 fixed tau,
+ * 17 boundary condition, 41 minmod alpha), shared by all instances. */
+int hgrp1d_create(hgrp1d **ctx, int n_inst, const int *cells, const double *h, const double *config);
+int hgrp1d_destroy(hgrp1d *ctx);
+/* Initial data, instances concatenated (sum of cells[] doubles each); e may
+ * be NULL (then E = u^2/2 + p/(gamma-1)/rho, src/hydrocode_1D/hydrocode.c:235). */
+int hgrp1d_upload(hgrp1d *ctx, const double *rho, const double *u, const double *p, const double *e);
+int hgrp1d_download(hgrp1d *ctx, double *rho, double *u, double *p, double *e);
+/* Runs every instance to its end.  use_graph: launch steps as captured
+ * hipGraphs (1) or one by one (0).  ms: device time; cell_updates: cells times
+ * steps that updated them, summed over instances (either may be NULL). */
+int hgrp1d_run(hgrp1d *ctx, int use_graph, double *ms, long long *cell_updates);
+/* Per instance: the reference's step counter k (config[5] on return), status
+ * (0 normal end, 1 aborted, 2 stopped by a check), and the time reached. */
+int hgrp1d_results(hgrp1d *ctx, int *steps, int *status, double *time_end);
+/* Autotuner: times probe_steps steps from the uploaded data for each pair of
+ * block sizes (64..512) of the interface and update kernels, keeps the
+ * fastest, restores the uploaded data.  The chosen sizes are returned. */
+int hgrp1d_tune(hgrp1d *ctx, int probe_steps, int *bs_iface, int *bs_update);
+
+/* ------------------------------------------------------------------------- */
 /* Learned starting guess for p* (no counterpart in the reference).  A 6-32-32-1 */
 /* GELU MLP predicts log(p* / p_TR), p_TR the two-rarefaction estimate; inputs   */
 /* log(rho_L/rho_R)/ln1e3, log(p_L/p_R)/ln1e4, (u_R-u_L)/(5c_L), log(c_R/c_L)/5, */

@@ -14,6 +14,7 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
+ORACLE_DEVPOW_SO = os.path.join(ROOT, "oracle", "liboracle_devpow.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref.so")
 PKG_DIR = os.path.join(ROOT, "hydrocode-fusion_amd")
 if PKG_DIR not in sys.path:

@@ -107,3 +107,30 @@ def interface_pairs(count, seed, gamma=1.4):
         tmask = rng.random((count, 1)) < 0.33
         A[:, 11:15] = rng.normal(0, 1, (count, 4)) * tmask
     return L, R
+
+
+def ensemble_1d(n_inst, seed, n_lo=256, n_hi=16384, gamma=1.4):
+    """Independent 1D Riemann problems on [0, 1] (jump at x = 0.5) with cell
+    counts uniform in [n_lo, n_hi] and states from the learned-guess training
+    distribution: rho_L/rho_R log-uniform 1e-3..1e3, p_L/p_R log-uniform
+    1e-4..1e4, (u_R - u_L)/c_L uniform -5..5; each instance is scaled so that
+    its larger sound speed is 1 (rho_L = 1, u_L = 0).
+    Returns (cells, h, rho, u, p) with the instances concatenated."""
+    rng = np.random.default_rng(seed)
+    cells = rng.integers(n_lo, n_hi + 1, n_inst)
+    rho, u, p = [], [], []
+    for m in cells:
+        r_ratio = 10 ** rng.uniform(-3, 3)
+        p_ratio = 10 ** rng.uniform(-4, 4)
+        du = rng.uniform(-5, 5)
+        rho_L, p_L = 1.0, 1.0
+        rho_R, p_R = rho_L / r_ratio, p_L / p_ratio
+        c_max = max(np.sqrt(gamma * p_L / rho_L), np.sqrt(gamma * p_R / rho_R))
+        p_L, p_R = p_L / c_max ** 2, p_R / c_max ** 2
+        c_L = np.sqrt(gamma * p_L / rho_L)
+        x = (np.arange(m) + 0.5) / m
+        left = x < 0.5
+        rho.append(np.where(left, rho_L, rho_R))
+        u.append(np.where(left, 0.0, du * c_L))
+        p.append(np.where(left, p_L, p_R))
+    return cells, 1.0 / cells, np.concatenate(rho), np.concatenate(u), np.concatenate(p)

@@ -22,6 +22,13 @@ def oracle():
 
 
 @pytest.fixture(scope="session")
+def oracle_devpow():
+    """The oracle with pow() taken from the device library (oracle/liboracle_devpow.so)."""
+    from _libs import Oracle, ORACLE_DEVPOW_SO
+    return Oracle(ORACLE_DEVPOW_SO)
+
+
+@pytest.fixture(scope="session")
 def reflib():
     from _libs import RefLib, REF_SO
     if not os.path.exists(REF_SO):

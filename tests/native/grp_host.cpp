@@ -1,0 +1,33 @@
+// Host build of the device GRP solvers (hydrocode-fusion_amd/csrc/grp_device.hpp,
+// grp1d_device.hpp) so that their arithmetic can be compared with the oracle
+// on the CPU: same source, same expression order, no FMA contraction.
+#include <hip/hip_runtime.h>
+#include "grp_device.hpp"
+#include "grp1d_device.hpp"
+
+using namespace hgrp;
+
+// L, R: gamma lambda_u lambda_v rho u v p d_rho d_u d_v d_p ... (orc_ifv order, 15 doubles)
+extern "C" void hg_edir_many(const double *L, const double *R, int n, double eps, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 15 * k, *r = R + 15 * k;
+        Side1 a{l[3], l[4], l[6], l[7], l[8], l[10]}, b{r[3], r[4], r[6], r[7], r[8], r[10]};
+        double D[3], U[3];
+        grp_edir(D, U, a, b, make_gas(l[0]), eps);
+        for (int q = 0; q < 3; ++q) { out[6 * k + q] = D[q]; out[6 * k + 3 + q] = U[q]; }
+    }
+}
+
+extern "C" void hg_flux_many(const double *L, const double *R, int n, double tau, double eps, int dir, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 15 * k, *r = R + 15 * k;
+        Side a{l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]}, b{r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+        double F[4], I[4];
+        const int rc = dir == 0 ? grp_2d_flux<0>(a, b, make_gas(l[0]), tau, eps, F, I, eps)
+                                : grp_2d_flux<1>(a, b, make_gas(l[0]), tau, eps, F, I, eps);
+        out[9 * k] = rc;
+        for (int q = 0; q < 4; ++q) { out[9 * k + 1 + q] = F[q]; out[9 * k + 5 + q] = I[q]; }
+    }
+}

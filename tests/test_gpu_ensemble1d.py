@@ -1,0 +1,162 @@
+"""GPU parity of the batched 1D GRP engine (hgrp1d_*) with the reference's 1D
+path: the golden Sod-1000 and boundary fixtures, ensembles of uneven problems
+against the oracle instance by instance, stop/abort paths, and the launch
+forms (captured graphs, block sizes, autotuner) giving identical results."""
+import os
+
+import numpy as np
+import pytest
+
+import cases
+from _libs import make_config
+from conftest import rel_linf
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def hg():
+    import hydrogrp
+    hydrogrp.lib()
+    return hydrogrp
+
+
+def _cfg(t_end, bc=-4, **kw):
+    return make_config(t_all=t_end, bc_x=bc, CFL=0.45, **kw)
+
+
+def test_sod_1000(hg, golden_1d):
+    g = golden_1d
+    cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
+    fin, npl, tp, c, st = hg.run_grp1d(1000, cases.sod_1d(1000), cfg)
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["sod1000_meta"]) and st == 0
+    d = rel_linf(fin, g["sod1000_final"])
+    print("Sod-1000 rel Linf", d, flush=True)
+    assert d.max() <= TOL
+
+
+def test_boundary_conditions(hg, golden_1d):
+    g = golden_1d
+    init = dict(zip(("rho", "u", "p"), g["bc_init"]))
+    for k, (bc, steps, npl0, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.1, h_x=1 / 200, bc_x=bc, CFL=0.45)
+        fin, npl, tp, c, st = hg.run_grp1d(200, init, cfg)
+        assert (c[5], npl, tp[npl - 1]) == (steps, npl0, t_end), bc
+        assert rel_linf(fin, g["bc_final"][k]).max() <= TOL, bc
+
+
+def _oracle_each(oracle, cells, h, rho, u, p, cfg):
+    off = np.concatenate([[0], np.cumsum(cells)])
+    out = []
+    for i, m in enumerate(cells):
+        c = cfg.copy()
+        c[10] = h[i]
+        sl = slice(off[i], off[i + 1])
+        fr, npl, tp, cc, st = oracle.grp1d(int(m), {"rho": rho[sl], "u": u[sl], "p": p[sl]}, c)
+        out.append((fr[:, npl - 1], int(cc[5]), st, tp[npl - 1]))
+    return off, out
+
+
+def _final_diff(got, f):
+    ok = np.isfinite(f)
+    assert np.array_equal(ok, np.isfinite(got))
+    if not ok.any():
+        return 0.0
+    a, b = np.where(ok, got, 0.0), np.where(ok, f, 0.0)
+    return rel_linf(a, b).max()
+
+
+def test_ensemble_matches_oracle_per_instance(hg, oracle, oracle_devpow):
+    """Every instance against the oracle (glibc pow, as the reference) and
+    against the oracle whose pow is the device's.  The GPU must match the
+    second to 1e-12: that pins every operation but pow.  Against the first
+    the bar is 1e-10, except where the two oracles themselves differ by more
+    (a strong-shock run can amplify last-bit pow differences through limiter
+    switches); there the GPU must sit exactly where the device-pow oracle is."""
+    cells, h, rho, u, p = cases.ensemble_1d(48, seed=5, n_lo=40, n_hi=400)
+    cfg = _cfg(0.15)
+    ens = hg.Ensemble1D(cells, h, cfg)
+    ens.upload(rho, u, p)
+    ms, cu = ens.run()
+    fin = ens.download()
+    steps, status, t = ens.results()
+    ens.close()
+    off, ref = _oracle_each(oracle, cells, h, rho, u, p, cfg)
+    _, ref_dp = _oracle_each(oracle_devpow, cells, h, rho, u, p, cfg)
+    worst, worst_dp, n_stop, amplified = 0.0, 0.0, 0, []
+    for i, ((f, k, st, te), (fd, kd, sd, ted)) in enumerate(zip(ref, ref_dp)):
+        got = fin[:, off[i]:off[i + 1]]
+        assert (steps[i], status[i], t[i]) == (kd, sd, ted), (i, steps[i], status[i], kd, sd)
+        d_dp = _final_diff(got, fd)
+        worst_dp = max(worst_dp, d_dp)
+        assert d_dp <= 1e-12, (i, d_dp)
+        n_stop += st != 0
+        oracles = _final_diff(fd, f) if (k, st) == (kd, sd) else np.inf
+        if oracles > TOL:
+            amplified.append((i, oracles))
+            continue
+        assert (steps[i], status[i], t[i]) == (k, st, te), i
+        d = _final_diff(got, f)
+        worst = max(worst, d)
+        assert d <= TOL, (i, d)
+    print(f"48 instances ({cells.sum()} cells, {cu} cell-updates, {ms:.1f} ms): rel Linf vs oracle "
+          f"{worst:.2e}, vs device-pow oracle {worst_dp:.2e}; {n_stop} ended by a check; "
+          f"amplified pow differences (oracle vs oracle): {amplified}", flush=True)
+    assert len(amplified) <= 2
+
+
+def test_launch_forms_agree_bitwise(hg):
+    cells, h, rho, u, p = cases.ensemble_1d(24, seed=8, n_lo=100, n_hi=3000)
+    cfg = _cfg(0.05)
+    outs = []
+    for graph, blocks in ((True, None), (False, None), (True, "64,512"), (True, "512,64")):
+        if blocks:
+            os.environ["HGRP_1D_BLOCKS"] = blocks
+        try:
+            ens = hg.Ensemble1D(cells, h, cfg)
+        finally:
+            os.environ.pop("HGRP_1D_BLOCKS", None)
+        ens.upload(rho, u, p)
+        ens.run(graph=graph)
+        outs.append((ens.download(), ens.results()))
+        ens.close()
+    for fin, res in outs[1:]:
+        assert np.array_equal(fin, outs[0][0], equal_nan=True)
+        for a, b in zip(res, outs[0][1]):
+            assert np.array_equal(a, b)
+
+
+def test_autotuner(hg):
+    cells, h, rho, u, p = cases.ensemble_1d(32, seed=9, n_lo=1000, n_hi=4000)
+    cfg = _cfg(0.03)
+    ens = hg.Ensemble1D(cells, h, cfg)
+    ens.upload(rho, u, p)
+    ens.run()
+    want = ens.download()
+    ens.upload(rho, u, p)
+    bi, bu = ens.tune(4)
+    print("autotuned block sizes (interface, update):", bi, bu, flush=True)
+    assert bi in (64, 128, 256, 512) and bu in (64, 128, 256, 512)
+    ens.run()
+    assert np.array_equal(ens.download(), want, equal_nan=True)
+    ens.close()
+
+
+def test_error_paths(hg, oracle):
+    m = 60
+    base = cases.sod_1d(m)
+    for mutate, kw in ((("p", 20, -1.0), {}), (None, {"CFL": 1e-12}), (("rho", 30, 1e-12), {})):
+        init = {k: v.copy() for k, v in base.items()}
+        if mutate:
+            init[mutate[0]][mutate[1]] = mutate[2]
+        cfg = make_config(t_all=0.1, h_x=1 / m, bc_x=-4, **kw)
+        fo, npo, tpo, co, st = oracle.grp1d(m, init, cfg)
+        fin, npl, tp, c, sg = hg.run_grp1d(m, init, cfg)
+        assert (co[5], st) == (c[5], sg), (mutate, kw)
+        assert tpo[npo - 1] == tp[npl - 1]
+        ok = np.isfinite(fo[:, npo - 1])
+        assert np.array_equal(ok, np.isfinite(fin))
+        assert np.max(np.abs(fo[:, npo - 1][ok] - fin[ok])) <= 1e-9
+    with pytest.raises(RuntimeError):
+        hg.Ensemble1D([10], [0.1], make_config(t_all=0.1, bc_x=-3))
