@@ -14,7 +14,8 @@ a sweep (128 B per cell: read and write rho u v p and their four slopes);
 on a bounded number of steps of the same workload on the host's cores.
 `learned_guess` compares, at 4096^2 with Newton tolerance 1e-12, the plain
 solve with the one started from the learned p* guess (rate, and the relative L1
-difference of the two final states).
+difference of the two final states).  `ensemble_1d` runs 512 independent 1D
+GRP problems of uneven sizes through the batched engine.
 """
 import argparse
 import json
@@ -104,6 +105,30 @@ def learned_leg(size=4096, steps=200):
                                            "fallback_share": float(np.mean(it_l > 3))}}
 
 
+def ensemble_leg(n_inst=512, seed=2024):
+    """The batched 1D workload: n_inst independent Riemann problems, cell counts
+    uniform in [256, 16384], states from the learned-guess training
+    distribution, each to t = 0.15 with its own CFL time step.  Device time of
+    the whole run after the autotuner has picked the block sizes."""
+    import numpy as np
+    import hydrogrp
+    import cases
+    from _libs import make_config
+    cells, h, rho, u, p = cases.ensemble_1d(n_inst, seed=seed)
+    cfg = make_config(t_all=0.15, bc_x=-4, CFL=0.45)
+    ens = hydrogrp.Ensemble1D(cells, h, cfg)
+    ens.upload(rho, u, p)
+    bs = ens.tune(4)
+    ms, cu = ens.run()
+    steps, status, _ = ens.results()
+    ens.close()
+    return {"workload": f"{n_inst} independent 1D GRP Riemann problems, {int(cells.sum())} cells "
+                        f"(N uniform in [256, 16384], seed {seed}), FP64, t_end 0.15, per-instance CFL",
+            "value": cu / (ms / 1000.0), "unit": "cell-updates/s", "cell_updates": int(cu), "ms": ms,
+            "max_steps": int(steps.max()), "block_sizes": {"iface": bs[0], "update": bs[1]},
+            "ended_by_check": int(np.sum(status != 0))}
+
+
 def cpu_leg(target_seconds=12.0):
     """The oracle on the host cores, fixed-tau mode (exactly the frozen tau), bounded sample."""
     import numpy as np
@@ -140,6 +165,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--size", type=int, default=1024, help="grid cells per side (default: the 1024^2 workload)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess comparison at 4096^2")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 1D ensemble")
     args = ap.parse_args()
     global M, N
     M = N = args.size
@@ -177,6 +203,8 @@ def main():
     }
     if not args.no_learned:
         line["learned_guess"] = learned_leg()
+    if not args.no_ensemble:
+        line["ensemble_1d"] = ensemble_leg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg()
     print(json.dumps(line), flush=True)

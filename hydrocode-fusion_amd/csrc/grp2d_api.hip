@@ -251,6 +251,7 @@ static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double 
     a.limiter3 = dir == 0 ? c->found_x : c->found_y;
     a.h = c->cfg[dir == 0 ? 10 : 11];
     a.hh = 0.5 * a.h;
+    a.div_h = recip_div(a.h);
     a.tau = tau_flux;
     a.nu = nu;
     a.gas = make_gas(c->cfg[6]);

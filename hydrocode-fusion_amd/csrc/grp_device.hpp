@@ -68,6 +68,37 @@ __host__ __device__ inline Gas make_gas(double g)
     return s;
 }
 
+// a / b for a divisor b known when the kernel starts, from y = RN(1/b)
+// (computed once): q = a y, r = a - q b (exact in an FMA), q + r y rounded.
+// For |b| in [2^-100, 2^100] and |q| in [2^-800, 2^800] this is the correctly
+// rounded quotient (Markstein's theorem: y correctly rounded, q within one ulp,
+// no underflow in r), i.e. bit for bit what IEEE a / b gives; a = +-0 gives
+// q = a y = +-0 exactly.  Anything else (zeros of b, infinities, NaN, the
+// extremes of the range) takes the division itself.
+struct RecipDiv {
+    double b, y;
+    bool ok;
+};
+__host__ __device__ inline RecipDiv recip_div(double b)
+{
+    RecipDiv d;
+    d.b = b;
+    d.y = 1.0 / b;
+    const double ab = fabs(b);
+    d.ok = ab >= 0x1p-100 && ab <= 0x1p100;
+    return d;
+}
+__device__ __forceinline__ double div_by(double a, const RecipDiv &d)
+{
+    const double q = a * d.y;
+    const double r = fma(-q, d.b, a);
+    const double q2 = fma(r, d.y, q);
+    const double aq = fabs(q);
+    if (__builtin_expect(d.ok && aq >= 0x1p-800 && aq <= 0x1p800, 1)) return q2;
+    if (a == 0.0 && d.ok) return q;
+    return a / d.b;
+}
+
 __device__ __forceinline__ double minmod2(double s_L, double s_R)
 {
     if (s_L * s_R <= 0.0) return 0.0;

@@ -80,17 +80,29 @@ struct Line {
             return;
         }
     }
-    // limited slope of cell k from its neighbours and its previous slope (slope_limiter_2D_x.c:177-211)
-    __device__ void limited(int k, const double vm[4], const double v0[4], const double vp[4], double s[4]) const
+    // previous slopes of cell k (read by minmod3 only)
+    __device__ void prev_slopes(int k, double sp[4]) const
     {
         const size_t o = at(k);
-        const double h = a.h;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) sp[q] = a.limiter3 ? a.s[q][o] : 0.0;
+    }
+    // limited slope from the neighbours and the previous slope sp (slope_limiter_2D_x.c:177-211)
+    __device__ void limited_sp(const double vm[4], const double v0[4], const double vp[4], const double sp[4],
+                               double s[4]) const
+    {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const double s_L = (v0[q] - vm[q]) / h;
-            const double s_R = (vp[q] - v0[q]) / h;
-            s[q] = a.limiter3 ? minmod3(a.alpha * s_L, a.alpha * s_R, a.s[q][o]) : minmod2(s_L, s_R);
+            const double s_L = div_by(v0[q] - vm[q], a.div_h);
+            const double s_R = div_by(vp[q] - v0[q], a.div_h);
+            s[q] = a.limiter3 ? minmod3(a.alpha * s_L, a.alpha * s_R, sp[q]) : minmod2(s_L, s_R);
         }
+    }
+    __device__ void limited(int k, const double vm[4], const double v0[4], const double vp[4], double s[4]) const
+    {
+        double sp[4];
+        prev_slopes(k, sp);
+        limited_sp(vm, v0, vp, sp, s);
     }
     __device__ void limited_at(int k, double s[4]) const
     {
@@ -122,9 +134,14 @@ struct Line {
     __device__ void update(int k, const double w[4], const double Fl[4], const double Fr[4],
                            const double Il[4], const double Ir[4]) const
     {
+        update_e(k, w, a.e[at(k)], Fl, Fr, Il, Ir);
+    }
+    // the same with E of the cell already loaded
+    __device__ void update_e(int k, const double w[4], const double e, const double Fl[4], const double Fr[4],
+                             const double Il[4], const double Ir[4]) const
+    {
         const size_t o = at(k);
         const double nu = a.nu;
-        const double e = a.e[o];
         const double mom_x = w[0] * w[1] - nu * (Fr[1] - Fl[1]);
         const double mom_y = w[0] * w[2] - nu * (Fr[2] - Fl[2]);
         const double ene = w[0] * e - nu * (Fr[3] - Fl[3]);
@@ -135,7 +152,7 @@ struct Line {
         a.rho_o[o] = rho; a.u_o[o] = u; a.v_o[o] = v; a.e_o[o] = ene / rho; a.p_o[o] = p;
         if (p < a.eps || rho < a.eps) atomicOr(&a.status->flags, 1u);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a.s_o[q][o] = (Ir[q] - Il[q]) / a.h;
+        for (int q = 0; q < 4; ++q) a.s_o[q][o] = div_by(Ir[q] - Il[q], a.div_h);
     }
 };
 
@@ -220,14 +237,16 @@ __global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
 // X sweep, row-parallel form: a block of W waves covers 64 consecutive i and
 // the cell rows j0-1 .. j0+W-2 (wave w: row c = j0-1+w, lane: column i).
 // Neighbours along x live in the next wave, so values, slopes and then fluxes
-// are exchanged through LDS.  Wave w solves interface c+1 (between rows c and
+// are exchanged through LDS (one array each: three barriers per block).  Wave w solves interface c+1 (between rows c and
 // c+1); waves 1..W-1 update rows j0 .. j0+W-2.  Every interface row is one
 // wave, so the GRP work is spread over all rows of the grid at once.
 template <int W, int MINW, bool LEARNED>
 __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 {
-    __shared__ double xa[W][4][64];   // values, then fluxes
-    __shared__ double xb[W][4][64];   // slopes, then interface values
+    __shared__ double xa[W][4][64];   // values
+    __shared__ double xb[W][4][64];   // slopes
+    __shared__ double xf[W][4][64];   // fluxes
+    __shared__ double xi[W][4][64];   // interface values at t + tau
     if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -238,12 +257,25 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     const Line<0> L(a, col ? i : 0);
     const double hh = a.hh;
 
-    double v[4] = {1.0, 0.0, 0.0, 1.0}, vl[4], vr[4];
+    // every global load of the owned row up front: values, previous slopes, E
+    const bool own = col && c >= 0 && c < m;
+    double v[4] = {1.0, 0.0, 0.0, 1.0}, vl[4], vr[4], sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
     if (col && c <= m) L.value(c, v);
+    if (own) {
+        L.prev_slopes(c, sp);
+        e_c = a.e[L.at(c)];
+    }
 #pragma unroll
     for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
     if (col && w == 0 && c >= 0) L.value(c - 1, vl);
+    // the last wave also limits row c + 1 (the right side of its interface): its loads go out now too
+    double vrr[4], sp1[4];
+    const bool next_own = col && w == W - 1 && c + 1 < m;
     if (col && w == W - 1 && c + 1 <= m) L.value(c + 1, vr);
+    if (next_own) {
+        L.value(c + 2, vrr);
+        L.prev_slopes(c + 1, sp1);
+    }
     __syncthreads();
     if (w > 0)
 #pragma unroll
@@ -254,14 +286,12 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 
     double s[4] = {0.0, 0.0, 0.0, 0.0}, sr[4] = {0.0, 0.0, 0.0, 0.0};
     if (col) {
-        if (c >= 0 && c < m) L.limited(c, vl, v, vr, s);
+        if (own) L.limited_sp(vl, v, vr, sp, s);
         else if (c == -1) L.ghost_slope(true, s);
         else if (c == m) L.ghost_slope(false, s);
         if (w == W - 1) {
-            if (c + 1 < m) {
-                double vrr[4];
-                L.value(c + 2, vrr);
-                L.limited(c + 1, v, vr, vrr, sr);
+            if (next_own) {
+                L.limited_sp(v, vr, vrr, sp1, sr);
             } else if (c + 1 == m) {
                 L.ghost_slope(false, sr);
             }
@@ -278,7 +308,7 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     const int f = c + 1;
     double pg = -1.0;
     if constexpr (LEARNED) {
-        // xa[w] is not read again before the flux exchange: it holds this wave's network inputs
+        // xa[w] is not read again: it holds this wave's network inputs
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
         pg = interface_guess<0>(a, col && f <= m, SL, SR, (uint32_t *)&xa[w][0][0]);
@@ -289,15 +319,14 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
         solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f, F, I,
                            pg);
     }
-    __syncthreads();                               // everyone is done reading values/slopes
 #pragma unroll
-    for (int q = 0; q < 4; ++q) { xa[w][q][lane] = F[q]; xb[w][q][lane] = I[q]; }
+    for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
     __syncthreads();
-    if (col && w >= 1 && c >= 0 && c < m) {
+    if (own && w >= 1) {
         double Fl[4], Il[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { Fl[q] = xa[w - 1][q][lane]; Il[q] = xb[w - 1][q][lane]; }
-        L.update(c, v, Fl, F, Il, I);
+        for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; }
+        L.update_e(c, v, e_c, Fl, F, Il, I);
     }
 }
 
@@ -319,8 +348,13 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     const Line<1> L(a, j);
     const double hh = a.hh;
 
-    double w[4] = {1.0, 0.0, 0.0, 1.0};
+    const bool own = c >= 0 && c < n;
+    double w[4] = {1.0, 0.0, 0.0, 1.0}, sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
     if (c <= n) L.value(c, w);
+    if (own) {                                         // every global load of the owned cell up front
+        L.prev_slopes(c, sp);
+        e_c = a.e[L.at(c)];
+    }
     double wl[4], wr[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
@@ -328,7 +362,7 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     if (lane == 63 && c + 1 <= n) L.value(c + 1, wr);
 
     double s[4] = {0.0, 0.0, 0.0, 0.0};
-    if (c >= 0 && c < n) L.limited(c, wl, w, wr, s);
+    if (own) L.limited_sp(wl, w, wr, sp, s);
     else if (c == -1) L.ghost_slope(true, s);
     else if (c == n) L.ghost_slope(false, s);
 
@@ -362,7 +396,7 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     double Fl[4], Il[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
-    if (lane >= 1 && c >= 0 && c < n) L.update(c, w, Fl, F, Il, I);
+    if (lane >= 1 && own) L.update_e(c, w, e_c, Fl, F, Il, I);
 }
 
 // min over cells of min(h_x, h_y) / (|c| + |u| + |v|)  (grp_solver_2D_split_EUL_source.c:185-191).
@@ -392,13 +426,11 @@ __global__ void __launch_bounds__(256) cfl_kernel(const double *rho, const doubl
 // ---- launchers -----------------------------------------------------------------
 static constexpr int TJ_X = 32;
 
-// Launch variants.  The defaults were picked by timing full four-quadrant runs
-// at 1024^2 and 4096^2 on MI355X:
-//   X: 8 rows per block; with the register budget for 4 waves/SIMD (128 VGPRs,
-//      a few spills) up to 2^22 cells, where the grid is only ~9 blocks per CU
-//      and the extra resident block hides the tail; with the compiler's free
-//      budget (2 waves/SIMD, no spills) above that.
-//   Y: register budget for 4 waves/SIMD at every size.
+// Launch variants.  The defaults were picked by timing four-quadrant runs at
+// 1024^2 (whole run) and 4096^2 (200 steps) on MI355X: the X sweep with 8 rows
+// per block and the register budget for 4 waves/SIMD (128 VGPRs, a few
+// spills), the Y sweep with the same budget; both beat the spill-free
+// 2-waves/SIMD forms at every size measured.
 // HGRP_SWEEP_X / HGRP_SWEEP_Y override them (read once, for tuning runs):
 //   HGRP_SWEEP_X  1 = 8 rows, free budget;  2 = 16 rows, 4 waves/SIMD;
 //                 3 = sliding register window;  4 = 8 rows, 4 waves/SIMD
@@ -413,8 +445,8 @@ static int sweep_x_variant(const SweepArgs &a)
 {
     static int v = -1;
     if (v < 0) v = env_variant("HGRP_SWEEP_X", 4);
-    if (v != 0) return v;
-    return (long)a.m * a.n <= (1l << 22) ? 4 : 1;
+    (void)a;
+    return v != 0 ? v : 4;
 }
 static int sweep_y_variant()
 {

@@ -40,6 +40,7 @@ struct SweepArgs {
     int limiter3;                             // 0: minmod2 (first sweep of this direction), 1: minmod3
     double h, hh, tau, nu, eps, alpha;        // h, h/2, flux time, update factor, ...
     double tol;                               // Newton tolerance (the reference: eps)
+    RecipDiv div_h;                           // division by h
     const MlpPacked *mlp;                     // learned p* guess, or nullptr (the reference's guess)
     Gas gas;                                  // gamma and its derived constants
     SweepStatus *status;                      // this sweep

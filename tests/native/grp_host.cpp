@@ -31,3 +31,9 @@ extern "C" void hg_flux_many(const double *L, const double *R, int n, double tau
         for (int q = 0; q < 4; ++q) { out[9 * k + 1 + q] = F[q]; out[9 * k + 5 + q] = I[q]; }
     }
 }
+
+extern "C" void hg_div_by_many(const double *a, int n, double b, double *out)
+{
+    const RecipDiv d = recip_div(b);
+    for (int k = 0; k < n; ++k) out[k] = div_by(a[k], d);
+}

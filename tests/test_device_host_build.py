@@ -58,3 +58,23 @@ def test_flux_against_reference_and_devpow_oracle(dev, golden_iface, oracle_devp
         ok = want[:, 0] == 0
         assert np.array_equal(out[:, 0] != 0, want[:, 0] != 0)
         assert np.array_equal(out[ok], want[ok]), direction
+
+
+def test_reciprocal_division_is_ieee_division(dev):
+    """div_by (a / b from a precomputed reciprocal, Markstein's correction)
+    against IEEE division, bit for bit, signs of zero included."""
+    rng = np.random.default_rng(12)
+    n = 2_000_000
+    mant = rng.uniform(1.0, 2.0, n) * rng.choice([-1.0, 1.0], n)
+    a = np.ldexp(mant, rng.integers(-1070, 1020, n))
+    a[:1000] = np.ldexp(rng.uniform(1, 2, 1000), rng.integers(-60, 60, 1000))
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, -5e-324, 2.2250738585072014e-308,
+                        1.7976931348623157e308, -1.7976931348623157e308, 1.0, -1.0])
+    a = np.ascontiguousarray(np.concatenate([a, special]))
+    for b in (1 / 1024, 1 / 4096, 1e-3, 1 / 12, 1 / 10, 0.7, 3.0, 1e-5, 1 / 3, 2.0 ** -40, -0.37, 1e-200, 0.0):
+        out = np.zeros_like(a)
+        dev.hg_div_by_many(_dp(a), len(a), C.c_double(b), _dp(out))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            want = a / b
+        same = (out.view(np.int64) == want.view(np.int64)) | (np.isnan(out) & np.isnan(want))
+        assert same.all(), (b, a[~same][:5], out[~same][:5], want[~same][:5])

@@ -96,7 +96,8 @@ def test_ensemble_matches_oracle_per_instance(hg, oracle, oracle_devpow):
         if oracles > TOL:
             amplified.append((i, oracles))
             continue
-        assert (steps[i], status[i], t[i]) == (k, st, te), i
+        # tau comes from the state every step, so pow's last bit reaches the time too
+        assert (steps[i], status[i]) == (k, st) and abs(t[i] - te) <= 1e-12 * te, i
         d = _final_diff(got, f)
         worst = max(worst, d)
         assert d <= TOL, (i, d)
