@@ -247,6 +247,7 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     __shared__ double xb[W][4][64];   // slopes
     __shared__ double xf[W][4][64];   // fluxes
     __shared__ double xi[W][4][64];   // interface values at t + tau
+    __shared__ double xe[W][64];      // E of the owned row (read back for the update)
     if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
@@ -267,6 +268,7 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
+    xe[w][lane] = e_c;
     if (col && w == 0 && c >= 0) L.value(c - 1, vl);
     // the last wave also limits row c + 1 (the right side of its interface): its loads go out now too
     double vrr[4], sp1[4];
@@ -308,10 +310,10 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     const int f = c + 1;
     double pg = -1.0;
     if constexpr (LEARNED) {
-        // xa[w] is not read again: it holds this wave's network inputs
+        // xf[w] is not written before the solve: it holds this wave's network inputs
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        pg = interface_guess<0>(a, col && f <= m, SL, SR, (uint32_t *)&xa[w][0][0]);
+        pg = interface_guess<0>(a, col && f <= m, SL, SR, (uint32_t *)&xf[w][0][0]);
     }
     if (col && f <= m) {
         const Side SL = recon(v, s, hh, true);
@@ -323,10 +325,11 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
     __syncthreads();
     if (own && w >= 1) {
-        double Fl[4], Il[4];
+        // the cell's values and E come back from LDS, so no register holds them across the solve
+        double Fl[4], Il[4], vc[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; }
-        L.update_e(c, v, e_c, Fl, F, Il, I);
+        for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; vc[q] = xa[w][q][lane]; }
+        L.update_e(c, vc, xe[w][lane], Fl, F, Il, I);
     }
 }
 
@@ -337,6 +340,7 @@ template <int MINW, bool LEARNED>
 __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chunks)
 {
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
+    __shared__ double stash[4][5][64];   // this lane's cell values and E, read back for the update
     if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -355,6 +359,11 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
         L.prev_slopes(c, sp);
         e_c = a.e[L.at(c)];
     }
+    // each lane reads back only its own slots: no barrier needed
+    double(*st)[64] = stash[(threadIdx.x >> 6) & 3];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) st[q][lane] = w[q];
+    st[4][lane] = e_c;
     double wl[4], wr[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
@@ -396,7 +405,12 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     double Fl[4], Il[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
-    if (lane >= 1 && own) L.update_e(c, w, e_c, Fl, F, Il, I);
+    if (lane >= 1 && own) {
+        double wc[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wc[q] = st[q][lane];
+        L.update_e(c, wc, st[4][lane], Fl, F, Il, I);
+    }
 }
 
 // min over cells of min(h_x, h_y) / (|c| + |u| + |v|)  (grp_solver_2D_split_EUL_source.c:185-191).
