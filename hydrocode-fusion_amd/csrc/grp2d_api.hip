@@ -41,6 +41,13 @@ using namespace hgrp;
     } while (0)
 
 static constexpr int STATUS_SLOTS = 96;    // sweeps between host checks
+
+// Where the reference's time loop stands (hgrp2d_solve's locals), for
+// checkpoints taken in a plot-frame callback and for resuming from them.
+struct LoopPos {
+    int valid, k, DS, nt, nt_plot;
+    double time_c, tau, half_tau, half_nu, mu;
+};
 static constexpr int ARG_ERROR = -1000;
 
 struct hgrp2d {
@@ -57,6 +64,8 @@ struct hgrp2d {
     int found_x, found_y;       // the reference's find_bound_x / find_bound_y
     double h_S_max;
     bool uploaded;
+    LoopPos at;                 // inside a frame callback: where the loop goes on after it
+    LoopPos resume;             // set by hgrp2d_restore: the next solve starts here
     double newton_tol;          // <= 0: the reference's (eps)
     MlpPacked *mlp;             // device copy of the learned-guess weights
     bool use_mlp;
@@ -136,6 +145,7 @@ extern "C" int hgrp2d_upload(hgrp2d *c, const double *rho, const double *u, cons
                              const double *p, const double *e)
 {
     if (!c || !rho || !u || !v || !p) return ARG_ERROR - 1;
+    c->resume.valid = 0;                  // fresh initial data: the loop starts at t = 0
     const int m = c->m, n = c->n;
     const double gamma = c->cfg[6];
     const size_t mn = (size_t)m * n;
@@ -318,6 +328,83 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x
     return 0;
 }
 
+// ---- checkpoints --------------------------------------------------------------
+// Layout: CkptHead, then state rho u v p E, x-slopes, y-slopes (each m*n
+// doubles, j outer), then the BC -1 ghost tables (2n*4 + 2m*4 doubles).
+struct CkptHead {
+    char magic[8];
+    int version, m, n, found_x, found_y, pad;
+    LoopPos pos;
+    double h_S_max;
+    double cfg[HGRP_N_CONF];
+};
+static const char CKPT_MAGIC[8] = {'H', 'G', 'R', 'P', 'C', 'K', '2', 'D'};
+
+static size_t ckpt_bytes(const hgrp2d *c)
+{
+    const size_t cells = (size_t)c->m * c->n;
+    return sizeof(CkptHead) + (13 * cells + 8 * ((size_t)c->m + c->n)) * sizeof(double);
+}
+
+extern "C" int hgrp2d_checkpoint_size(hgrp2d *c, size_t *bytes)
+{
+    if (!c || !bytes) return ARG_ERROR - 1;
+    *bytes = ckpt_bytes(c);
+    return 0;
+}
+
+extern "C" int hgrp2d_checkpoint(hgrp2d *c, void *buf, size_t bytes)
+{
+    if (!c || !buf || !c->uploaded || bytes < ckpt_bytes(c)) return ARG_ERROR - 1;
+    CkptHead *h = (CkptHead *)buf;
+    memset(h, 0, sizeof *h);
+    memcpy(h->magic, CKPT_MAGIC, 8);
+    h->version = 1; h->m = c->m; h->n = c->n; h->found_x = c->found_x; h->found_y = c->found_y;
+    h->pos = c->at;                       // valid only inside a frame callback
+    h->h_S_max = c->h_S_max;
+    memcpy(h->cfg, c->cfg, sizeof h->cfg);
+    double *d = (double *)(h + 1);
+    const size_t cells = (size_t)c->m * c->n, row = (size_t)c->n * 8, pitch = (size_t)c->ld * 8;
+    HCHECK(hipStreamSynchronize(c->stream));
+    double *planes[13];
+    for (int q = 0; q < 5; ++q) planes[q] = c->st[c->cs][q];
+    for (int q = 0; q < 4; ++q) { planes[5 + q] = c->sx[c->cx][q]; planes[9 + q] = c->sy[c->cy][q]; }
+    for (int q = 0; q < 13; ++q)
+        HCHECK(hipMemcpy2DAsync(d + q * cells, row, planes[q], pitch, row, (size_t)c->m, hipMemcpyDeviceToHost,
+                                c->stream));
+    double *b = d + 13 * cells;
+    HCHECK(hipMemcpyAsync(b, c->bnd[0], (size_t)(8 * c->n + 8 * c->m) * 8, hipMemcpyDeviceToHost, c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+extern "C" int hgrp2d_restore(hgrp2d *c, const void *buf, size_t bytes)
+{
+    if (!c || !buf || bytes < sizeof(CkptHead)) return ARG_ERROR - 1;
+    const CkptHead *h = (const CkptHead *)buf;
+    if (memcmp(h->magic, CKPT_MAGIC, 8) != 0 || h->version != 1 || h->m != c->m || h->n != c->n ||
+        bytes < ckpt_bytes(c))
+        return ARG_ERROR - 2;
+    const double *d = (const double *)(h + 1);
+    const size_t cells = (size_t)c->m * c->n, row = (size_t)c->n * 8, pitch = (size_t)c->ld * 8;
+    HCHECK(hipStreamSynchronize(c->stream));
+    double *planes[13];
+    for (int q = 0; q < 5; ++q) planes[q] = c->st[c->cs][q];
+    for (int q = 0; q < 4; ++q) { planes[5 + q] = c->sx[c->cx][q]; planes[9 + q] = c->sy[c->cy][q]; }
+    for (int q = 0; q < 13; ++q)
+        HCHECK(hipMemcpy2DAsync(planes[q], pitch, d + q * cells, row, row, (size_t)c->m, hipMemcpyHostToDevice,
+                                c->stream));
+    HCHECK(hipMemcpyAsync(c->bnd[0], d + 13 * cells, (size_t)(8 * c->n + 8 * c->m) * 8, hipMemcpyHostToDevice,
+                          c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    c->found_x = h->found_x;
+    c->found_y = h->found_y;
+    c->h_S_max = h->h_S_max;              // the frozen CFL bound of the original initial data
+    c->resume = h->pos;
+    c->uploaded = true;
+    return 0;
+}
+
 // ---- the reference's time loop -------------------------------------------------
 namespace {
 struct Loop {
@@ -359,9 +446,17 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
 
     Loop L;
     L.c = c;
-    int k = 0, nt = 0, nt_plot = 0, DS = 1, status = 0;
+    int k = 1, nt = 0, nt_plot = 0, DS = 1, status = 0;
     double time_c = 0.0, half_tau = 0.0, half_nu = 0.0, mu = 0.0;
-    bool stop_t = false;
+    bool stop_t = false, skip_frame = false;
+    if (c->resume.valid) {                  // continue where a checkpoint was taken
+        const LoopPos &r = c->resume;
+        k = r.k; DS = r.DS; nt = r.nt; nt_plot = r.nt_plot;
+        time_c = r.time_c; tau = r.tau; half_tau = r.half_tau; half_nu = r.half_nu; mu = r.mu;
+        skip_frame = true;                  // that iteration's frame was handled before the checkpoint
+        c->resume.valid = 0;
+        if (nt > N_T - 1) nt = N_T - 1;
+    }
     HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
 
     // finish at a sweep that failed on the device: restore the loop state it
@@ -408,15 +503,19 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
     };
 
     bool ended = false;
-    for (k = 1; k <= N; DS ? k : ++k) {
-        if (time_c >= time_plot[nt_plot] && nt_plot < (N_plot - 1)) {          // plot frame
+    for (; k <= N; DS ? k : ++k) {
+        if (!skip_frame && time_c >= time_plot[nt_plot] && nt_plot < (N_plot - 1)) {   // plot frame
             const int f = flush();
             if (f < 0) return -2;
             if (f) { ended = true; break; }
-            if (fn && fn(user, nt, nt_plot, time_c)) { status = 1; ended = true; break; }
+            c->at = {1, k, DS, nt < (N_T - 1) ? nt + 1 : nt, nt_plot + 1, time_c, tau, half_tau, half_nu, mu};
+            const int stop = fn ? fn(user, nt, nt_plot, time_c) : 0;
+            c->at.valid = 0;
+            if (stop) { status = 1; ended = true; break; }
             nt_plot++;
             if (nt < (N_T - 1)) nt++;
         }
+        skip_frame = false;
         if (DS) {                                                             // time step
             const double h_S_max = c->h_S_max;
             if (isfinite(t_all) || !isfinite(c->cfg[16]) || c->cfg[16] <= 0.0) {

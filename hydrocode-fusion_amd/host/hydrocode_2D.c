@@ -91,6 +91,30 @@ int main(int argc, char *argv[])
             CV[0].E[j][i] += 0.5 * CV[0].V[j][i] * CV[0].V[j][i];
         }
 
+    /* restart at full precision: a FLU_VAR.f64 in the input folder replaces the
+     * initial data by its last frame (time starts again at 0) */
+    {
+        char add_in[FILENAME_MAX + 40], file[FILENAME_MAX + 60];
+        example_io(argv[1], add_in, 1);
+        snprintf(file, sizeof file, "%sFLU_VAR.f64", add_in);
+        int rx, ry, rn;
+        double *rtp = NULL, *rfr = NULL;
+        const int rr = file_2D_read_f64(file, &rx, &ry, &rn, &rtp, &rfr);
+        if (rr == 0 && (rx != n_x || ry != n_y)) {
+            printf("FLU_VAR.f64 is %d x %d, the case is %d x %d: not used.\n", rx, ry, n_x, n_y);
+        } else if (rr == 0) {
+            const size_t cells = (size_t)n_x * n_y;
+            const double *last = rfr + (size_t)(rn - 1) * 5 * cells;
+            double **dst[5] = {CV[0].RHO, CV[0].U, CV[0].V, CV[0].P, CV[0].E};
+            for (int q = 0; q < 5; ++q)
+                for (int j = 0; j < n_x; ++j)
+                    memcpy(dst[q][j], last + q * cells + (size_t)j * n_y, (size_t)n_y * sizeof(double));
+            printf("Restart from %s (frame time %.10g).\n", file, rtp[rn - 1]);
+        }
+        free(rtp);
+        free(rfr);
+    }
+
     if (strcmp(argv[4], "EUL") != 0) {
         printf("NOT appropriate coordinate framework! The framework is %s.\n", argv[4]);
         retval = 4;
@@ -114,6 +138,13 @@ int main(int argc, char *argv[])
 #ifndef NODATPLOT
     file_2D_write(n_x, n_y, N_plot, CV, X, Y, cpu_time, argv[2], time_plot);
 #endif
+    {
+        char add_out[FILENAME_MAX + 40], file[FILENAME_MAX + 60];
+        example_io(argv[2], add_out, 0);
+        snprintf(file, sizeof file, "%sFLU_VAR.f64", add_out);
+        if (file_2D_write_f64(file, n_x, n_y, N_plot, CV, time_plot))
+            retval = 1;
+    }
 #ifndef NOTECPLOT
     file_2D_write_POINT_TEC(n_x, n_y, 1, CV + N_plot - 1, X, Y, cpu_time, argv[2], time_plot + N_plot - 1);
 #endif

@@ -379,3 +379,60 @@ void file_2D_write_POINT_TEC(const int n_x, const int n_y, const int N, const st
     }
     fclose(fp);
 }
+
+/* ---- full-precision frames (no counterpart in the reference) ----------------
+ * FLU_VAR.f64, little-endian IEEE doubles: "HGRPF64\0", int32 version (1),
+ * n_x, n_y, N; time_plot[N]; then for each frame k the fields RHO U V P E,
+ * each n_x*n_y doubles in CV order (j = x index outer, i = y index inner). */
+static const char F64_MAGIC[8] = {'H', 'G', 'R', 'P', 'F', '6', '4', '\0'};
+
+int file_2D_write_f64(const char *file, const int n_x, const int n_y, const int N, const struct cell_var_stru CV[],
+                      const double time_plot[])
+{
+    FILE *fp = fopen(file, "wb");
+    if (fp == NULL) {
+        printf("Cannot open full-precision output file: %s\n", file);
+        return 1;
+    }
+    const int head[4] = {1, n_x, n_y, N};
+    int ok = fwrite(F64_MAGIC, 1, 8, fp) == 8 && fwrite(head, sizeof(int), 4, fp) == 4 &&
+             fwrite(time_plot, sizeof(double), (size_t)N, fp) == (size_t)N;
+    for (int k = 0; ok && k < N; ++k) {
+        double **f[5] = {CV[k].RHO, CV[k].U, CV[k].V, CV[k].P, CV[k].E};
+        for (int q = 0; ok && q < 5; ++q)
+            for (int j = 0; ok && j < n_x; ++j)
+                ok = fwrite(f[q][j], sizeof(double), (size_t)n_y, fp) == (size_t)n_y;
+    }
+    if (fclose(fp) != 0) ok = 0;
+    if (!ok) printf("Write error in full-precision output file: %s\n", file);
+    return ok ? 0 : 1;
+}
+
+int file_2D_read_f64(const char *file, int *n_x, int *n_y, int *N, double **time_plot, double **frames)
+{
+    FILE *fp = fopen(file, "rb");
+    if (fp == NULL) return 1;
+    char magic[8];
+    int head[4];
+    int ok = fread(magic, 1, 8, fp) == 8 && memcmp(magic, F64_MAGIC, 8) == 0 && fread(head, sizeof(int), 4, fp) == 4 &&
+             head[0] == 1 && head[1] > 0 && head[2] > 0 && head[3] > 0;
+    double *tp = NULL, *fr = NULL;
+    if (ok) {
+        const size_t cells = (size_t)head[1] * (size_t)head[2];
+        tp = (double *)malloc((size_t)head[3] * sizeof(double));
+        fr = (double *)malloc((size_t)head[3] * 5 * cells * sizeof(double));
+        ok = tp && fr && fread(tp, sizeof(double), (size_t)head[3], fp) == (size_t)head[3] &&
+             fread(fr, sizeof(double), (size_t)head[3] * 5 * cells, fp) == (size_t)head[3] * 5 * cells;
+    }
+    fclose(fp);
+    if (!ok) {
+        free(tp);
+        free(fr);
+        printf("Not a readable full-precision frame file: %s\n", file);
+        return 2;
+    }
+    *n_x = head[1]; *n_y = head[2]; *N = head[3];
+    *time_plot = tp;
+    *frames = fr;
+    return 0;
+}

@@ -49,6 +49,9 @@ def lib():
     L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
     _FP = C.POINTER(C.c_float)
     L.hgrp2d_set_newton.argtypes = [C.c_void_p, _D, _FP]
+    L.hgrp2d_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
+    L.hgrp2d_checkpoint.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+    L.hgrp2d_restore.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     L.hgrp2d_sweep_kernel.argtypes = [C.c_void_p, C.c_int]
     L.hgrp2d_sweep_kernel.restype = C.c_char_p
     L.hgrp_mlp_data.argtypes = [C.c_ulonglong, C.c_int, _D, _FP, _FP, _DP, _IP]
@@ -150,6 +153,18 @@ class Solver2D:
             params = np.ascontiguousarray(params, dtype=np.float32)
             fp = params.ctypes.data_as(C.POINTER(C.c_float))
         _check(lib().hgrp2d_set_newton(self._h, tol, fp), "hgrp2d_set_newton")
+
+    def checkpoint(self):
+        """Full-precision device state (and loop position inside a frame callback) as bytes."""
+        n = C.c_size_t()
+        _check(lib().hgrp2d_checkpoint_size(self._h, C.byref(n)), "hgrp2d_checkpoint_size")
+        buf = C.create_string_buffer(n.value)
+        _check(lib().hgrp2d_checkpoint(self._h, buf, n.value), "hgrp2d_checkpoint")
+        return buf.raw
+
+    def restore(self, blob):
+        buf = C.create_string_buffer(blob, len(blob))
+        _check(lib().hgrp2d_restore(self._h, buf, len(blob)), "hgrp2d_restore")
 
     def bench(self, steps):
         t, x, y = _D(), _D(), _D()

@@ -42,6 +42,17 @@ void file_2D_write_POINT_TEC(const int n_x, const int n_y, const int N, const st
                              double **X, double **Y, const double *cpu_time, const char *problem,
                              const double time_plot[]);                          /* file_2D_out.c:103-151 */
 
+/* Full-precision frames, written beside the reference's outputs (no counterpart
+ * in the reference, whose .dat/.tec files carry %.10g and whose 2D HDF5 output
+ * is float32).  FLU_VAR.f64: magic "HGRPF64\0", int32 version 1, n_x, n_y, N,
+ * double time_plot[N], then per frame RHO U V P E as n_x*n_y doubles each
+ * (j outer, i inner).  Returns 0 on success. */
+int file_2D_write_f64(const char *file, const int n_x, const int n_y, const int N,
+                      const struct cell_var_stru CV[], const double time_plot[]);
+/* Reads such a file; *time_plot (N) and *frames (N*5*n_x*n_y) are malloc'd.
+ * Returns 0, 1 (cannot open) or 2 (not a frame file / short). */
+int file_2D_read_f64(const char *file, int *n_x, int *n_y, int *N, double **time_plot, double **frames);
+
 /* replaces src/tools/sys_pro.c:37-100 */
 void DispPro(const double pro, const int step);
 int CreateDir(const char *pPath);

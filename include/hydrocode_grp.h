@@ -70,6 +70,18 @@ int hgrp2d_download(hgrp2d *ctx, double *rho, double *u, double *v, double *p, d
 int hgrp2d_solve(hgrp2d *ctx, const double *time_plot, int N_plot, int N_T,
                  hgrp2d_frame_fn fn, void *user, hgrp2d_result *res);
 
+/* Checkpoints (no counterpart in the reference): the device state at full
+ * precision (state, both slope sets, limiter switches, the frozen h/S_max,
+ * BC -1 ghosts) and, when taken inside the frame callback of hgrp2d_solve, the
+ * position of the time loop.  hgrp2d_restore into a context of the same size
+ * makes the next hgrp2d_solve (same time_plot) continue from that position,
+ * bit for bit as if the run had not been interrupted; a checkpoint taken
+ * outside a solve restarts the loop at t = 0 from the saved state.  The
+ * context's own config[] is kept (t_all may be raised to run longer). */
+int hgrp2d_checkpoint_size(hgrp2d *ctx, size_t *bytes);
+int hgrp2d_checkpoint(hgrp2d *ctx, void *buf, size_t bytes);
+int hgrp2d_restore(hgrp2d *ctx, const void *buf, size_t bytes);
+
 /* Benchmark form: `steps` full steps X(tau/2) Y(tau) X(tau/2) with the frozen
  * tau = CFL * h/S_max, no time clipping, plot frames or host syncs inside.
  * ms_total: device time of all steps (HIP events on the solver's stream);

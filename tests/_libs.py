@@ -123,7 +123,8 @@ class Oracle:
         return np.array(list(Dd) + list(U))
 
     def split2d(self, m, n, init, config, time_plot=None, N_T=None):
-        """init: dict rho,u,v,p (m x n arrays, [j, i]); returns (frames, N_plot, time_plot, config, status)."""
+        """init: dict rho,u,v,p (m x n arrays, [j, i]), optionally e (else formed as the
+        reference's main does); returns (frames, N_plot, time_plot, config, status)."""
         if time_plot is None:
             time_plot = np.array([0.0, config[1]])
         time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
@@ -134,7 +135,7 @@ class Oracle:
         frames = np.zeros((5, N_T, m, n))
         r, u, v, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "v", "p"))
         frames[0, 0], frames[1, 0], frames[2, 0], frames[3, 0] = r, u, v, p
-        frames[4, 0] = initial_energy(r, u, v, p, gamma)
+        frames[4, 0] = init["e"] if init.get("e") is not None else initial_energy(r, u, v, p, gamma)
         ptrs = [(DP * N_T)(*[frames[f, k].ctypes.data_as(DP) for k in range(N_T)]) for f in range(5)]
         cfg = np.array(config, dtype=np.float64)
         npl = C.c_int(N_plot)

@@ -121,7 +121,8 @@ def test_program_matches_reference_program(tmp_path):
     init = cases.four_quadrant(m, n)
     ours = _run_program(PROGRAM, tmp_path, "gpu", m, n, init, [0.07])
     ref = _run_program(REF_PROGRAM, tmp_path, "ref", m, n, init, [0.07])
-    names = sorted(p.name for p in ours.iterdir())
+    # FLU_VAR.f64 (full-precision frames) is this build's addition to the outputs
+    names = sorted(p.name for p in ours.iterdir() if p.suffix != ".f64")
     assert names == sorted(p.name for p in ref.iterdir())
     for name in names:
         a, b = open(ours / name).read().split("\n"), open(ref / name).read().split("\n")
@@ -134,3 +135,72 @@ def test_program_matches_reference_program(tmp_path):
                     continue
                 fx, fy = float(x), float(y)          # numbers printed with %.10g
                 assert abs(fx - fy) <= 1e-9 * max(abs(fy), 1e-300) or abs(fx - fy) <= 1e-12, (name, x, y)
+
+
+def _read_f64(path):
+    """FLU_VAR.f64: (time_plot[N], frames[N, 5, n_x, n_y]) (layout in include/hydrocode_dropin.h)."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"HGRPF64\0"
+    ver, nx, ny, N = np.frombuffer(raw, "<i4", 4, 8)
+    assert ver == 1
+    tp = np.frombuffer(raw, "<f8", N, 24)
+    fr = np.frombuffer(raw, "<f8", N * 5 * nx * ny, 24 + 8 * N).reshape(N, 5, nx, ny)
+    return tp, fr
+
+
+def test_program_full_precision_output_and_restart(oracle, tmp_path):
+    m, n = 40, 32
+    init = cases.four_quadrant(m, n)
+    out = _run_program(PROGRAM, tmp_path, "gpu", m, n, init, [0.05])
+    tp, fr = _read_f64(out / "FLU_VAR.f64")
+    cfg = cases.four_quadrant_config(m, n, t_all=0.15)
+    fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg, time_plot=np.array([0.0, 0.05, 0.15]))
+    assert len(tp) == npo and np.array_equal(tp, tpo[:npo])
+    for k in range(npo):
+        assert rel_linf(fr[k], fo[:, k]).max() <= 1e-10, k
+    # the %.10g tables hold the same numbers, rounded
+    rho_txt = _read_frames(out / "RHO.dat", m, n)
+    assert np.allclose(rho_txt, np.transpose(fr[:, 0], (0, 2, 1)), rtol=1e-9, atol=0)
+    # restart: the frame file in the input folder replaces the initial data by its last frame
+    src = tmp_path / "gpu" / "data_in" / "two-dim" / "fqcase" / "FLU_VAR.f64"
+    src.write_bytes((out / "FLU_VAR.f64").read_bytes())
+    run = tmp_path / "gpu" / "run" / "here"
+    p = subprocess.run([PROGRAM, "fqcase", "fqout2", "2_GRP", "EUL", "33=1", "1=0.05"], cwd=run,
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
+    assert "Restart from" in p.stdout
+    tp2, fr2 = _read_f64(tmp_path / "gpu" / "data_out" / "two-dim" / "EUL_2_order" / "fqout2" / "FLU_VAR.f64")
+    assert np.array_equal(fr2[0], fr[-1])
+    # and continuing from it is the oracle's run from that state
+    start = {k: fr[-1][q] for q, k in enumerate(("rho", "u", "v", "p", "e"))}
+    cfg2 = cases.four_quadrant_config(m, n, t_all=0.05)
+    fo2, npo2, tpo2, co2, st2 = oracle.split2d(m, n, start, cfg2)
+    assert rel_linf(fr2[-1], fo2[:, npo2 - 1]).max() <= 1e-10
+
+
+def test_checkpoint_resume_is_bitwise():
+    import hydrogrp
+    m, n = 48, 40
+    init = cases.four_quadrant(m, n)
+    cfg = cases.four_quadrant_config(m, n, t_all=0.2)
+    tp = np.array([0.0, 0.07, 0.2])
+    a = hydrogrp.Solver2D(m, n, cfg)
+    a.upload(init["rho"], init["u"], init["v"], init["p"])
+    fresh = a.checkpoint()                        # outside a solve: restart at t = 0 from this state
+    saved = {}
+
+    def on_frame(nt, ntp, tc):
+        if ntp == 1:
+            saved["blob"] = a.checkpoint()
+        return 0
+    ra = a.solve(tp, 3, on_frame)
+    fa = a.download()
+    a.close()
+    for blob in (saved["blob"], fresh):
+        b = hydrogrp.Solver2D(m, n, cfg)
+        b.restore(blob)
+        rb = b.solve(tp, 3)
+        fb = b.download()
+        b.close()
+        assert (ra.k, ra.status, ra.time_c) == (rb.k, rb.status, rb.time_c)
+        assert np.array_equal(fa.view(np.int64), fb.view(np.int64))

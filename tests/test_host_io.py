@@ -141,3 +141,34 @@ def test_writers_match_reference_byte_for_byte(host, refio, tmp_path):
     assert set(outs[0]) == set(outs[1]) and len(outs[0]) == 10, sorted(outs[0])
     for name in outs[0]:
         assert outs[0][name] == outs[1][name], name
+
+
+def test_full_precision_frames_roundtrip(host, tmp_path):
+    """FLU_VAR.f64 (the full-precision frames written beside the reference's
+    outputs): every double comes back bit for bit, headers are checked."""
+    rng = np.random.default_rng(4)
+    m, n, N = 7, 5, 3
+    cv = CVFrames(m, n, N)
+    cv.data[:] = rng.normal(size=cv.data.shape) * 10.0 ** rng.integers(-300, 300, cv.data.shape)
+    cv.data[0, 0, 0, 0] = -0.0
+    tp = np.ascontiguousarray([0.0, 0.1, 0.2 + 1e-17])
+    DP = C.POINTER(C.c_double)
+    host.file_2D_write_f64.argtypes = [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_void_p, DP]
+    host.file_2D_read_f64.argtypes = [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                      C.POINTER(DP), C.POINTER(DP)]
+    f = str(tmp_path / "FLU_VAR.f64").encode()
+    assert host.file_2D_write_f64(f, m, n, N, cv.CV, tp.ctypes.data_as(DP)) == 0
+    assert os.path.getsize(f) == 8 + 16 + 8 * N + 8 * N * 5 * m * n
+    rx, ry, rn = C.c_int(), C.c_int(), C.c_int()
+    tpp, frp = DP(), DP()
+    assert host.file_2D_read_f64(f, C.byref(rx), C.byref(ry), C.byref(rn), C.byref(tpp), C.byref(frp)) == 0
+    assert (rx.value, ry.value, rn.value) == (m, n, N)
+    back = np.ctypeslib.as_array(frp, (N * 5 * m * n,)).reshape(N, 5, m, n)
+    assert np.array_equal(np.ctypeslib.as_array(tpp, (N,)).view(np.int64), tp.view(np.int64))
+    assert np.array_equal(back.transpose(1, 0, 2, 3).view(np.int64), cv.data.view(np.int64))
+    bad = tmp_path / "bad.f64"
+    bad.write_bytes(b"NOTAFILE" + bytes(64))
+    assert host.file_2D_read_f64(str(bad).encode(), C.byref(rx), C.byref(ry), C.byref(rn), C.byref(tpp),
+                                 C.byref(frp)) == 2
+    assert host.file_2D_read_f64(str(tmp_path / "missing.f64").encode(), C.byref(rx), C.byref(ry), C.byref(rn),
+                                 C.byref(tpp), C.byref(frp)) == 1
