@@ -24,6 +24,12 @@ namespace hgrp {
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st);
 hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st);
 const char *sweep_kernel_name(int dir, int m, int n, bool learned);
+hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st);
+hipError_t launch_sweep_y_tf(const SweepArgs &a, hipStream_t st);
+hipError_t launch_cfl_tf(const double *rho, const double *u, const double *v, const double *p, const double *fa,
+                         int m, int n, int ld, const TwoGas &two, double hmin, unsigned long long *out, hipStream_t st);
+hipError_t launch_energy_tf(const double *rho, const double *u, const double *v, const double *p, const double *fa,
+                            double *e, int m, int n, int ld, const TwoGas &two, hipStream_t st);
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 }
@@ -69,6 +75,10 @@ struct hgrp2d {
     double newton_tol;          // <= 0: the reference's (eps)
     MlpPacked *mlp;             // device copy of the learned-guess weights
     bool use_mlp;
+    bool two_fluid;             // volume fraction of fluid a carried (hgrp2d_set_fluids)
+    double *tf_block;           // fa[2], a-slopes in x [2] and in y [2]
+    double *fa[2], *sax[2], *say[2];
+    TwoGas two;
     hipEvent_t ev[2];
 };
 
@@ -127,6 +137,7 @@ extern "C" int hgrp2d_destroy(hgrp2d *c)
     if (c->block) (void)hipFree(c->block);
     if (c->status) (void)hipFree(c->status);
     if (c->mlp) (void)hipFree(c->mlp);
+    if (c->tf_block) (void)hipFree(c->tf_block);
     if (c->ev[0]) (void)hipEventDestroy(c->ev[0]);
     if (c->ev[1]) (void)hipEventDestroy(c->ev[1]);
     if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -146,6 +157,7 @@ extern "C" int hgrp2d_upload(hgrp2d *c, const double *rho, const double *u, cons
 {
     if (!c || !rho || !u || !v || !p) return ARG_ERROR - 1;
     c->resume.valid = 0;                  // fresh initial data: the loop starts at t = 0
+    c->two_fluid = false;                 // a new state needs its volume fraction again
     const int m = c->m, n = c->n;
     const double gamma = c->cfg[6];
     const size_t mn = (size_t)m * n;
@@ -220,6 +232,55 @@ extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
     return sweep_kernel_name(dir, c->m, c->n, c->use_mlp);
 }
 
+extern "C" int hgrp2d_set_fluids(hgrp2d *c, double gamma_b, const double *frac_a)
+{
+    if (!c || !c->uploaded || !frac_a || !(gamma_b > 1.0)) return ARG_ERROR - 1;
+    for (int d = 17; d <= 18; ++d) {
+        const int b = (int)c->cfg[d];
+        if (b != -2 && b != -4 && b != -7 && b != -24) {
+            printf("The two-fluid sweeps take boundary conditions -2, -4, -7 and -24 only (config[%d] = %d).\n", d, b);
+            return ARG_ERROR - 4;
+        }
+    }
+    const size_t cells = (size_t)c->m * c->ld;
+    if (!c->tf_block) {
+        HCHECK(hipMalloc((void **)&c->tf_block, 6 * cells * sizeof(double)));
+        for (int b = 0; b < 2; ++b) {
+            c->fa[b] = c->tf_block + b * cells;
+            c->sax[b] = c->tf_block + (2 + b) * cells;
+            c->say[b] = c->tf_block + (4 + b) * cells;
+        }
+    }
+    HCHECK(hipMemsetAsync(c->tf_block, 0, 6 * cells * sizeof(double), c->stream));
+    HCHECK(hipMemcpy2DAsync(c->fa[c->cs], (size_t)c->ld * 8, frac_a, (size_t)c->n * 8, (size_t)c->n * 8,
+                            (size_t)c->m, hipMemcpyHostToDevice, c->stream));
+    c->two.a = make_gas(c->cfg[6]);
+    c->two.b = make_gas(gamma_b);
+    HCHECK(launch_energy_tf(c->st[c->cs][0], c->st[c->cs][1], c->st[c->cs][2], c->st[c->cs][3], c->fa[c->cs],
+                            c->st[c->cs][4], c->m, c->n, c->ld, c->two, c->stream));
+    // the frozen CFL bound with each cell's own gamma
+    const double hmin = fmin(c->cfg[10], c->cfg[11]);
+    const unsigned long long inf_bits = 0x7ff0000000000000ull;
+    HCHECK(hipMemcpyAsync(c->cfl_bits, &inf_bits, 8, hipMemcpyHostToDevice, c->stream));
+    HCHECK(launch_cfl_tf(c->st[c->cs][0], c->st[c->cs][1], c->st[c->cs][2], c->st[c->cs][3], c->fa[c->cs], c->m, c->n,
+                         c->ld, c->two, hmin, c->cfl_bits, c->stream));
+    unsigned long long bits = 0;
+    HCHECK(hipMemcpyAsync(&bits, c->cfl_bits, 8, hipMemcpyDeviceToHost, c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    memcpy(&c->h_S_max, &bits, 8);
+    c->two_fluid = true;
+    return 0;
+}
+
+extern "C" int hgrp2d_download_fraction(hgrp2d *c, double *frac_a)
+{
+    if (!c || !frac_a || !c->two_fluid) return ARG_ERROR - 1;
+    HCHECK(hipMemcpy2DAsync(frac_a, (size_t)c->n * 8, c->fa[c->cs], (size_t)c->ld * 8, (size_t)c->n * 8, (size_t)c->m,
+                            hipMemcpyDeviceToHost, c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
 extern "C" int hgrp2d_set_newton(hgrp2d *c, double tol, const float *params)
 {
     if (!c || !(tol == tol)) return ARG_ERROR - 1;
@@ -268,6 +329,12 @@ static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double 
     a.eps = c->cfg[4];
     a.alpha = c->cfg[41];
     a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
+    if (c->two_fluid) {
+        a.fa = c->fa[in]; a.fa_o = c->fa[out];
+        a.sa = dir == 0 ? c->sax[sin] : c->say[sin];
+        a.sa_o = dir == 0 ? c->sax[1 - sin] : c->say[1 - sin];
+        a.two = c->two;
+    }
     a.mlp = c->use_mlp ? c->mlp : nullptr;
     a.status = c->status + slot;
     a.prev = prev_slot >= 0 ? c->status + prev_slot : nullptr;
@@ -277,7 +344,10 @@ static int enqueue_sweep(hgrp2d *c, int dir, double tau_flux, double nu, int slo
 {
     SweepArgs a;
     fill_args(c, a, dir, tau_flux, nu, slot, prev_slot);
-    HCHECK(dir == 0 ? launch_sweep_x(a, c->stream) : launch_sweep_y(a, c->stream));
+    if (c->two_fluid)
+        HCHECK(dir == 0 ? launch_sweep_x_tf(a, c->stream) : launch_sweep_y_tf(a, c->stream));
+    else
+        HCHECK(dir == 0 ? launch_sweep_x(a, c->stream) : launch_sweep_y(a, c->stream));
     c->cs = 1 - c->cs;
     if (dir == 0) { c->cx = 1 - c->cx; c->found_x = 1; }
     else          { c->cy = 1 - c->cy; c->found_y = 1; }
@@ -355,7 +425,7 @@ extern "C" int hgrp2d_checkpoint_size(hgrp2d *c, size_t *bytes)
 
 extern "C" int hgrp2d_checkpoint(hgrp2d *c, void *buf, size_t bytes)
 {
-    if (!c || !buf || !c->uploaded || bytes < ckpt_bytes(c)) return ARG_ERROR - 1;
+    if (!c || !buf || !c->uploaded || bytes < ckpt_bytes(c) || c->two_fluid) return ARG_ERROR - 1;
     CkptHead *h = (CkptHead *)buf;
     memset(h, 0, sizeof *h);
     memcpy(h->magic, CKPT_MAGIC, 8);

@@ -283,10 +283,16 @@ __device__ __forceinline__ void side_trivial(double D[4], double U[4], double rh
 // linear_GRP_solver_Edir_Q1D, single-fluid members, with atc = eps.
 // TL/TR: tangential derivatives (rho, u, v, p) of the two sides.
 // tol: Newton tolerance (the reference passes eps); p_guess: see riemann_exact.
+// TF (two fluids, the reference's MULTIFLUID_BASICS members of Q1D, :55-61 and
+// the U[4]/D[4] lines): FA = {a_L, d_a_L, a_R, d_a_R}, the volume fraction of
+// fluid a and its normal derivative on each side; FO receives its value on the
+// t-axis and its time derivative (carried with the contact like v; the split
+// path's tangential derivatives of a are zero).
+template <bool TF = false>
 __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L, const Side &R,
                                         const double TL[4], const double TR[4],
                                         const Gas &GL, const Gas &GR, double eps, double tol,
-                                        double p_guess)
+                                        double p_guess, const double *FA = nullptr, double *FO = nullptr)
 {
     const double gL = GL.g, gR = GR.g;
     const double lu = 0.0, lv = 0.0;
@@ -339,10 +345,12 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
 
     if (speed_L > lu) {
         side_trivial(D, U, rho_L, u_L, v_L, p_L, d_rho_L, d_u_L, d_v_L, d_p_L, t_rho_L, t_u_L, t_v_L, t_p_L, c_L, lu, lv);
+        if constexpr (TF) { FO[0] = FA[0]; FO[1] = -(u_L - lu) * FA[1]; }
         return;
     }
     if (speed_R < lu) {
         side_trivial(D, U, rho_R, u_R, v_R, p_R, d_rho_R, d_u_R, d_v_R, d_p_R, t_rho_R, t_u_R, t_v_R, t_p_R, c_R, lu, lv);
+        if constexpr (TF) { FO[0] = FA[2]; FO[1] = -(u_R - lu) * FA[3]; }
         return;
     }
 
@@ -373,6 +381,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         D[1] = -(U[1] - lu) * D_u   - (U[2] - lv) * T_u   - D_p / U[0];
         D[2] = -(U[1] - lu) * D_v   - (U[2] - lv) * T_v   - T_p / U[0];
         D[3] = -(U[1] - lu) * D_p   - (U[2] - lv) * T_p   - U[0] * C * C * (D_u + T_v);
+        if constexpr (TF) {
+            const bool left = u_star > lu;
+            FO[0] = left ? FA[0] : FA[2];
+            FO[1] = -(U[1] - lu) * (left ? FA[1] : FA[3]);
+        }
         return;
     }
 
@@ -393,6 +406,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GL.gm1;
         D[0] = (D[0] + D[3]) / C / C;
         D[2] = -(U[1] - lu) * d_v_L * U[0] / rho_L;
+        if constexpr (TF) { FO[0] = FA[0]; FO[1] = -(U[1] - lu) * FA[1] * U[0] / rho_L; }
     } else if (crw1 && ((u_star + cs_R) < lu)) {          // t-axis inside the 3-CRW
         U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1);
         C = lu - U[1];
@@ -410,6 +424,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GR.gm1;
         D[0] = (D[0] + D[3]) / C / C;
         D[2] = -(U[1] - lu) * d_v_R * U[0] / rho_R;
+        if constexpr (TF) { FO[0] = FA[2]; FO[1] = -(U[1] - lu) * FA[3] * U[0] / rho_R; }
     } else {                                              // non-sonic
         double a_L, b_L, d_L, a_R, b_R, d_R;
         if (u_star < lu) {
@@ -481,6 +496,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[0] = (D[0] + D[3]) / cs_R / cs_R;
                 D[2] = -U[1] * d_v_R * U[0] / rho_R;
                 D[2] = D[2] + lu * d_v_R;
+                if constexpr (TF) { FO[0] = FA[2]; FO[1] = -U[1] * FA[3] * U[0] / rho_R + lu * FA[3]; }
             } else {
                 const double SmUs = sqrt(0.5 * (GR.gp1 * p_R + GR.gm1 * p_star) / rs_R);
                 const double SmUR = sqrt(0.5 * (GR.gp1 * p_star + GR.gm1 * p_R) / rho_R);
@@ -498,6 +514,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[0] = D[0] + lu * rho_x;
                 D[2] = -U[1] * SmUR * d_v_R / SmUs;
                 D[2] = D[2] + lu * d_v_R;
+                if constexpr (TF) { FO[0] = FA[2]; FO[1] = -U[1] * SmUR * FA[3] / SmUs + lu * FA[3]; }
             }
         } else {
             if (crw0) {
@@ -505,6 +522,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[0] = (D[0] + D[3]) / cs_L / cs_L;
                 D[2] = -U[1] * d_v_L * U[0] / rho_L;
                 D[2] = D[2] + lu * d_v_L;
+                if constexpr (TF) { FO[0] = FA[0]; FO[1] = -U[1] * FA[1] * U[0] / rho_L + lu * FA[1]; }
             } else {
                 const double SmUs = -sqrt(0.5 * (GL.gp1 * p_L + GL.gm1 * p_star) / rs_L);
                 const double SmUL = -sqrt(0.5 * (GL.gp1 * p_star + GL.gm1 * p_L) / rho_L);
@@ -522,6 +540,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[0] = D[0] + lu * rho_x;
                 D[2] = -U[1] * SmUL * d_v_L / SmUs;
                 D[2] = D[2] + lu * d_v_L;
+                if constexpr (TF) { FO[0] = FA[0]; FO[1] = -U[1] * SmUL * FA[1] / SmUs + lu * FA[1]; }
             }
         }
     }
@@ -599,6 +618,88 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double 
     I[2] = (mid[1] + tau * dire[1]) * n_y + (mid[2] + tau * dire[2]) * n_x;
     I[0] = mid[0] + tau * dire[0];
     I[3] = mid[3] + tau * dire[3];
+    return 0;
+}
+
+// Two gamma-law fluids a and b mixed by the volume fraction a of fluid a:
+// gamma = 1/(a/(gamma_a-1) + (1-a)/(gamma_b-1)) + 1
+// (src/inter_process_unstruct/cons_qty_calc.c:70).  Pure cells use their
+// fluid's constants as they are, so a = 1 everywhere is the single-fluid path.
+struct TwoGas {
+    Gas a, b;
+};
+__device__ __forceinline__ double mix_gamma(double fa, const TwoGas &T)
+{
+    return 1.0 / (fa / (T.a.g - 1.0) + (1.0 - fa) / (T.b.g - 1.0)) + 1.0;
+}
+__device__ __forceinline__ Gas gas_of(double fa, const TwoGas &T)
+{
+    if (fa == 1.0) return T.a;
+    if (fa == 0.0) return T.b;
+    return make_gas(mix_gamma(fa, T));
+}
+
+// GRP_2D_flux with the MULTIFLUID_BASICS members (src/flux_calc/flux_solver.c:158-237,
+// :203-235): gamma per side from the reconstructed volume fractions, gamma at
+// t + tau/2 from a_mid for the energy flux.  FA = {a_L, d_a_L, a_R, d_a_R};
+// on success FZ = {u_n a at t + tau/2, u_n at t + tau/2, a at t + tau}.
+template <int DIR>
+__device__ __forceinline__ int grp_2d_flux_tf(Side L, Side R, const TwoGas &T, const double FA[4], double tau,
+                                              double eps, double F[4], double I[4], double FZ[3], double tol,
+                                              double p_guess = -1.0)
+{
+    constexpr double n_x = DIR == 0 ? 1.0 : 0.0;
+    constexpr double n_y = DIR == 0 ? 0.0 : 1.0;
+    constexpr double tz = DIR == 0 ? 0.0 : -0.0;
+    const Gas GL = gas_of(FA[0], T), GR = gas_of(FA[2], T);
+    const double u = L.u * n_x + L.v * n_y;
+    const double u_R = R.u * n_x + R.v * n_y;
+    const double d_u = L.d_u * n_x + L.d_v * n_y;
+    const double d_u_R = R.d_u * n_x + R.d_v * n_y;
+    constexpr double t_u = tz * n_x + tz * n_y;
+    constexpr double t_v = -tz * n_y + tz * n_x;
+    L.v = -L.u * n_y + L.v * n_x;
+    R.v = -R.u * n_y + R.v * n_x;
+    L.d_v = -L.d_u * n_y + L.d_v * n_x;
+    R.d_v = -R.d_u * n_y + R.d_v * n_x;
+    L.u = u; R.u = u_R; L.d_u = d_u; R.d_u = d_u_R;
+    const double Tz[4] = {tz, t_u, t_v, tz};
+
+    double dire[4], mid[4], fo[2];
+    grp_q1d<true>(dire, mid, L, R, Tz, Tz, GL, GR, eps, tol, p_guess, FA, fo);
+    const int bad = star_dire_bad(mid, dire, eps);
+    if (bad) {
+        F[0] = F[1] = F[2] = F[3] = 0.0;
+        I[0] = I[1] = I[2] = I[3] = 0.0;
+        FZ[0] = FZ[1] = FZ[2] = 0.0;
+        return bad;
+    }
+    const double rho_mid = mid[0] + 0.5 * tau * dire[0];
+    const double u_mid = (mid[1] + 0.5 * tau * dire[1]) * n_x - (mid[2] + 0.5 * tau * dire[2]) * n_y;
+    const double v_mid = (mid[1] + 0.5 * tau * dire[1]) * n_y + (mid[2] + 0.5 * tau * dire[2]) * n_x;
+    const double p_mid = mid[3] + 0.5 * tau * dire[3];
+    const double a_mid = fo[0] + 0.5 * tau * fo[1];
+    const double g_gm1 = a_mid == 1.0 ? T.a.g_gm1 : (a_mid == 0.0 ? T.b.g_gm1 : 0.0);
+    double ge;
+    if (a_mid == 1.0 || a_mid == 0.0) {
+        ge = g_gm1;
+    } else {
+        const double gamma_mid = mix_gamma(a_mid, T);
+        ge = gamma_mid / (gamma_mid - 1.0);
+    }
+    F[0] = rho_mid * (u_mid * n_x + v_mid * n_y);
+    F[1] = F[0] * u_mid + p_mid * n_x;
+    F[2] = F[0] * v_mid + p_mid * n_y;
+    F[3] = ge * p_mid / rho_mid + 0.5 * (u_mid * u_mid + v_mid * v_mid);
+    F[3] = F[0] * F[3];
+    I[1] = (mid[1] + tau * dire[1]) * n_x - (mid[2] + tau * dire[2]) * n_y;
+    I[2] = (mid[1] + tau * dire[1]) * n_y + (mid[2] + tau * dire[2]) * n_x;
+    I[0] = mid[0] + tau * dire[0];
+    I[3] = mid[3] + tau * dire[3];
+    const double un_mid = u_mid * n_x + v_mid * n_y;
+    FZ[0] = un_mid * a_mid;
+    FZ[1] = un_mid;
+    FZ[2] = fo[0] + tau * fo[1];
     return 0;
 }
 

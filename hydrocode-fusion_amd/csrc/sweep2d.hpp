@@ -43,6 +43,12 @@ struct SweepArgs {
     RecipDiv div_h;                           // division by h
     const MlpPacked *mlp;                     // learned p* guess, or nullptr (the reference's guess)
     Gas gas;                                  // gamma and its derived constants
+    // two fluids (sweep2d_tf.hip only): volume fraction of fluid a and its slope
+    const double *fa;                         // in
+    double *fa_o;                             // out
+    const double *sa;                         // slope of this direction, in
+    double *sa_o;                             // refreshed slope, out
+    TwoGas two;                               // fluids a (config[6]) and b (config[106])
     SweepStatus *status;                      // this sweep
     const SweepStatus *prev;                  // the sweep before (nullptr for the first)
 };

@@ -49,6 +49,8 @@ def lib():
     L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
     _FP = C.POINTER(C.c_float)
     L.hgrp2d_set_newton.argtypes = [C.c_void_p, _D, _FP]
+    L.hgrp2d_set_fluids.argtypes = [C.c_void_p, _D, _DP]
+    L.hgrp2d_download_fraction.argtypes = [C.c_void_p, _DP]
     L.hgrp2d_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
     L.hgrp2d_checkpoint.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
     L.hgrp2d_restore.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
@@ -153,6 +155,17 @@ class Solver2D:
             params = np.ascontiguousarray(params, dtype=np.float32)
             fp = params.ctypes.data_as(C.POINTER(C.c_float))
         _check(lib().hgrp2d_set_newton(self._h, tol, fp), "hgrp2d_set_newton")
+
+    def set_fluids(self, gamma_b, frac_a):
+        """Two fluids: volume fraction of fluid a (gamma = config[6]); fluid b has gamma_b."""
+        fa = np.ascontiguousarray(frac_a, dtype=np.float64)
+        assert fa.shape == (self.m, self.n)
+        _check(lib().hgrp2d_set_fluids(self._h, gamma_b, _p(fa)), "hgrp2d_set_fluids")
+
+    def download_fraction(self):
+        out = np.empty((self.m, self.n))
+        _check(lib().hgrp2d_download_fraction(self._h, _p(out)), "hgrp2d_download_fraction")
+        return out
 
     def checkpoint(self):
         """Full-precision device state (and loop position inside a frame callback) as bytes."""

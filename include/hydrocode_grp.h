@@ -91,6 +91,18 @@ int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double 
 /* h/S_max of the uploaded initial data (the frozen CFL reduction). */
 int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
 
+/* Two gamma-law fluids (no structured counterpart in the reference; its
+ * MULTIFLUID_BASICS members of GRP_2D_flux / Q1D, src/flux_calc/flux_solver.c:203-235):
+ * after hgrp2d_upload, frac_a (m*n, j*n+i) gives the volume fraction of fluid a
+ * (gamma = config[6]); fluid b has gamma_b (the reference's config[106]).  The
+ * mixture's gamma is 1/(a/(gamma_a-1) + (1-a)/(gamma_b-1)) + 1, the volume
+ * fraction is advected in quasi-conservative form, and cells with a = 1 are
+ * computed exactly as in the single-fluid path.  Recomputes E from the uploaded
+ * primitives and the frozen CFL bound with each cell's gamma.  Boundary conditions -2, -4, -7, -24; no
+ * checkpoints in this mode. */
+int hgrp2d_set_fluids(hgrp2d *ctx, double gamma_b, const double *frac_a);
+int hgrp2d_download_fraction(hgrp2d *ctx, double *frac_a);
+
 /* Newton solve for p* (no counterpart in the reference, which always uses
  * tol = eps and its own starting guess): tol > 0 sets the Newton tolerance on
  * the velocity gap (<= 0: eps, the reference's); params (HGRP_MLP_NP floats,

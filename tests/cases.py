@@ -6,6 +6,8 @@ reference's case definitions:
   (same states and split at (0.5, 0.5); value_start.m:1-4,27-48), any n_x x n_y.
 * sod_1d: Toro's Sod tube, (1, 0, 1) | (0.125, 0, 0.1), split at x = 0.5.
 """
+import math
+
 import numpy as np
 
 # (rho, u, v, p) of the four quadrants
@@ -134,3 +136,44 @@ def ensemble_1d(n_inst, seed, n_lo=256, n_hi=16384, gamma=1.4):
         u.append(np.where(left, 0.0, du * c_L))
         p.append(np.where(left, p_L, p_R))
     return cells, 1.0 / cells, np.concatenate(rho), np.concatenate(u), np.concatenate(p)
+
+
+def _disc_fraction(m, n, lx, ly, cx, cy, r, sub=8):
+    """Fraction of each cell (domain [0,lx]x[0,ly], m x n cells) inside the
+    circle of radius r at (cx, cy), by sub x sub point sampling."""
+    hx, hy = lx / m, ly / n
+    off = (np.arange(sub) + 0.5) / sub
+    xs = (np.arange(m)[:, None] + off[None, :]) * hx          # m x sub
+    ys = (np.arange(n)[:, None] + off[None, :]) * hy          # n x sub
+    inside = ((xs[:, None, :, None] - cx) ** 2 + (ys[None, :, None, :] - cy) ** 2) <= r * r
+    return inside.mean(axis=(2, 3))
+
+
+def shock_bubble(m, n, mach=1.22, gamma_a=1.4, gamma_b=1.67, lx=4.0, ly=1.0):
+    """Two-fluid shock-bubble interaction: ambient fluid a (rho 1, p 1) on
+    [0,lx]x[0,ly], a bubble of fluid b (rho 0.138, p 1) of radius 0.25 at
+    (1, 0.5), a planar shock of Mach `mach` at x = 0.5 moving into the ambient
+    gas (post-shock state from the Rankine-Hugoniot relations).  Returns
+    (init dict rho u v p, volume fraction of fluid a)."""
+    g = gamma_a
+    M2 = mach * mach
+    p2 = 1.0 + 2.0 * g / (g + 1.0) * (M2 - 1.0)
+    r2 = (g + 1.0) * M2 / ((g - 1.0) * M2 + 2.0)
+    u2 = math.sqrt(g) * 2.0 / (g + 1.0) * (mach - 1.0 / mach)
+    x = (np.arange(m) + 0.5) * lx / m
+    post = np.repeat((x < 0.5)[:, None], n, axis=1)
+    inside = _disc_fraction(m, n, lx, ly, 1.0, 0.5, 0.25)
+    frac_a = 1.0 - inside
+    rho = np.where(post, r2, frac_a * 1.0 + inside * 0.138)
+    u = np.where(post, u2, 0.0)
+    p = np.where(post, p2, 1.0)
+    return {"rho": rho, "u": u, "v": np.zeros((m, n)), "p": p}, frac_a
+
+
+def material_interface(m, n, u0=0.5, v0=0.25):
+    """Uniform pressure and velocity with a circular blob of a second fluid
+    (Abgrall's test): only the contact moves, p and u must stay uniform."""
+    inside = _disc_fraction(m, n, 1.0, 1.0, 0.5, 0.5, 0.25)
+    frac_a = 1.0 - inside
+    rho = frac_a * 1.0 + inside * 0.138
+    return ({"rho": rho, "u": np.full((m, n), u0), "v": np.full((m, n), v0), "p": np.ones((m, n))}, frac_a)

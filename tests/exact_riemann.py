@@ -24,18 +24,20 @@ def _f(p, rho, pk, c, g):
     return 2.0 * c / (g - 1.0) * (r - 1.0), 1.0 / (rho * c) * (p / pk) ** (-(g + 1.0) / (2.0 * g))
 
 
-def star_state(rhoL, uL, pL, rhoR, uR, pR, g=1.4):
-    cL, cR = math.sqrt(g * pL / rhoL), math.sqrt(g * pR / rhoR)
+def star_state(rhoL, uL, pL, rhoR, uR, pR, g=1.4, gR=None):
+    """Star pressure and velocity; gR (default g) is the right state's gamma."""
+    gL, gR = g, (g if gR is None else gR)
+    cL, cR = math.sqrt(gL * pL / rhoL), math.sqrt(gR * pR / rhoR)
     du = uR - uL
-    if 2.0 * (cL + cR) / (g - 1.0) <= du:
+    if 2.0 * cL / (gL - 1.0) + 2.0 * cR / (gR - 1.0) <= du:
         raise ValueError("vacuum")
     lo, hi = 0.0, max(pL, pR)
-    while _f(hi, rhoL, pL, cL, g)[0] + _f(hi, rhoR, pR, cR, g)[0] + du < 0:
+    while _f(hi, rhoL, pL, cL, gL)[0] + _f(hi, rhoR, pR, cR, gR)[0] + du < 0:
         hi *= 2.0
     p = 0.5 * (pL + pR)
     for _ in range(200):
-        fl, dl = _f(p, rhoL, pL, cL, g)
-        fr, dr = _f(p, rhoR, pR, cR, g)
+        fl, dl = _f(p, rhoL, pL, cL, gL)
+        fr, dr = _f(p, rhoR, pR, cR, gR)
         F = fl + fr + du
         if F > 0:
             hi = min(hi, p)
@@ -48,23 +50,25 @@ def star_state(rhoL, uL, pL, rhoR, uR, pR, g=1.4):
             p = pn
             break
         p = pn
-    fl = _f(p, rhoL, pL, cL, g)[0]
-    fr = _f(p, rhoR, pR, cR, g)[0]
+    fl = _f(p, rhoL, pL, cL, gL)[0]
+    fr = _f(p, rhoR, pR, cR, gR)[0]
     u = 0.5 * (uL + uR) + 0.5 * (fr - fl)
     return p, u
 
 
-def sample(rhoL, uL, pL, rhoR, uR, pR, xi, g=1.4):
-    """(rho, u, p) at similarity coordinates xi = x / t (array)."""
-    ps, us = star_state(rhoL, uL, pL, rhoR, uR, pR, g)
-    cL, cR = math.sqrt(g * pL / rhoL), math.sqrt(g * pR / rhoR)
+def sample(rhoL, uL, pL, rhoR, uR, pR, xi, g=1.4, gR=None):
+    """(rho, u, p) at similarity coordinates xi = x / t (array); gR: the right gamma."""
+    gL, gR = g, (g if gR is None else gR)
+    ps, us = star_state(rhoL, uL, pL, rhoR, uR, pR, gL, gR)
+    cL, cR = math.sqrt(gL * pL / rhoL), math.sqrt(gR * pR / rhoR)
     xi = np.asarray(xi, dtype=np.float64)
     rho = np.empty_like(xi)
     u = np.empty_like(xi)
     p = np.empty_like(xi)
-    gm, gp = g - 1.0, g + 1.0
     for k, s in enumerate(xi.flat):
         if s <= us:                                            # left of contact
+            g = gL
+            gm, gp = g - 1.0, g + 1.0
             if ps > pL:
                 SL = uL - cL * math.sqrt(gp / (2 * g) * ps / pL + gm / (2 * g))
                 if s <= SL:
@@ -82,6 +86,8 @@ def sample(rhoL, uL, pL, rhoR, uR, pR, xi, g=1.4):
                     uu = 2.0 / gp * (cL + gm / 2.0 * uL + s)
                     st = (rhoL * (c / cL) ** (2.0 / gm), uu, pL * (c / cL) ** (2.0 * g / gm))
         else:
+            g = gR
+            gm, gp = g - 1.0, g + 1.0
             if ps > pR:
                 SR = uR + cR * math.sqrt(gp / (2 * g) * ps / pR + gm / (2 * g))
                 if s >= SR:

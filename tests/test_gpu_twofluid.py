@@ -1,0 +1,143 @@
+"""GPU tests of the two-fluid split path (hgrp2d_set_fluids): it has no
+structured counterpart in the reference, so it is pinned by its single-fluid
+limit (bit for bit against the reference-parity path), Abgrall's uniform
+pressure/velocity invariant, conservation, and the exact two-gamma Riemann
+solution (tests/exact_riemann.py)."""
+import numpy as np
+import pytest
+
+import cases
+import exact_riemann
+from _libs import make_config
+from conftest import rel_linf
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def hg():
+    import hydrogrp
+    hydrogrp.lib()
+    return hydrogrp
+
+
+def _run(hg, m, n, init, frac, cfg, gamma_b=1.67, time_plot=None):
+    s = hg.Solver2D(m, n, cfg)
+    try:
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        if frac is not None:
+            s.set_fluids(gamma_b, frac)
+        tp = np.array([0.0, cfg[1]]) if time_plot is None else time_plot
+        res = s.solve(tp, 2)
+        out = s.download()
+        fa = s.download_fraction() if frac is not None else None
+    finally:
+        s.close()
+    return out, fa, res
+
+
+def test_single_fluid_limit_is_bitwise(hg):
+    m, n = 96, 80
+    init = cases.four_quadrant(m, n)
+    cfg = cases.four_quadrant_config(m, n, t_all=0.2)
+    one, _, r1 = _run(hg, m, n, init, None, cfg)
+    two, fa, r2 = _run(hg, m, n, init, np.ones((m, n)), cfg)
+    assert (r1.k, r1.status, r1.time_c) == (r2.k, r2.status, r2.time_c)
+    assert np.array_equal(one.view(np.int64), two.view(np.int64))
+    assert np.all(fa == 1.0)
+
+
+def test_abgrall_pressure_and_velocity_stay_uniform(hg):
+    m = n = 64
+    init, frac = cases.material_interface(m, n)
+    cfg = make_config(t_all=0.2, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-7, split=1)
+    out, fa, res = _run(hg, m, n, init, frac, cfg)
+    assert res.status == 0
+    dp = np.max(np.abs(out[3] - 1.0))
+    du = max(np.max(np.abs(out[1] - 0.5)), np.max(np.abs(out[2] - 0.25)))
+    moved = np.abs(fa - frac).max()
+    print(f"material interface after {res.k} steps: max |p-1| {dp:.2e}, max |u-u0| {du:.2e}, "
+          f"volume fraction moved by up to {moved:.2f}", flush=True)
+    assert moved > 0.5                       # the blob has travelled
+    assert dp <= 1e-11 and du <= 1e-11
+
+
+def test_conservation_with_periodic_boundaries(hg):
+    m, n = 80, 64
+    init, frac = cases.material_interface(m, n, u0=0.3, v0=-0.2)
+    rng = np.random.default_rng(3)
+    init["p"] = init["p"] + 0.3 * np.sin(2 * np.pi * (np.arange(m)[:, None] + 0.5) / m) * np.ones((1, n))
+    init["u"] = init["u"] + 0.1 * rng.uniform(-1, 1, (m, n))
+    cfg = make_config(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-7, split=1)
+    s = hg.Solver2D(m, n, cfg)
+    s.upload(init["rho"], init["u"], init["v"], init["p"])
+    s.set_fluids(1.67, frac)
+    before = s.download()
+    res = s.solve(np.array([0.0, 0.1]), 2)
+    after = s.download()
+    s.close()
+    assert res.status == 0
+
+    def totals(st):
+        rho, u, v, p, e = st
+        return np.array([rho.sum(), (rho * u).sum(), (rho * v).sum(), (rho * e).sum()])
+    t0, t1 = totals(before), totals(after)
+    rel = np.abs(t1 - t0) / np.maximum(np.abs(t0), 1.0)
+    print("mass, momentum x/y, energy drift:", rel, flush=True)
+    assert rel.max() <= 1e-12
+
+
+def test_two_gamma_shock_tube_converges_to_exact(hg):
+    errs = []
+    for m in (200, 400, 800):
+        n = 4
+        x = (np.arange(m) + 0.5) / m
+        left = np.repeat((x < 0.5)[:, None], n, axis=1)
+        init = {"rho": np.where(left, 1.0, 0.125), "u": np.zeros((m, n)), "v": np.zeros((m, n)),
+                "p": np.where(left, 1.0, 0.1)}
+        frac = np.where(left, 1.0, 0.0)
+        cfg = make_config(t_all=0.2, h_x=1 / m, h_y=1 / m, bc_x=-4, bc_y=-4, split=1)
+        out, fa, res = _run(hg, m, n, init, frac, cfg, gamma_b=1.67)
+        assert res.status == 0
+        er, eu, ep = exact_riemann.sample(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, (x - 0.5) / res.time_c, 1.4, 1.67)
+        errs.append(np.mean(np.abs(out[0][:, 0] - er)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    print("two-gamma shock tube: L1(rho)", errs, "orders", rates, flush=True)
+    assert errs[-1] < 5e-3 and np.all(rates > 0.6)
+
+
+def test_shock_bubble_runs_and_stays_bounded(hg):
+    m, n = 256, 64
+    init, frac = cases.shock_bubble(m, n)
+    cfg = make_config(t_all=0.4, h_x=4.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-2, split=1)
+    out, fa, res = _run(hg, m, n, init, frac, cfg)
+    print(f"shock-bubble {m}x{n}: {res.k} steps, status {res.status}, t {res.time_c:.3f}", flush=True)
+    assert res.status == 0 and np.all(np.isfinite(out))
+    assert out[3].min() > 0 and out[0].min() > 0
+    assert fa.min() >= 0.0 and fa.max() <= 1.0
+    x = (np.arange(m) + 0.5) * 4.0 / m
+    w0, w1 = 1.0 - frac, 1.0 - fa
+    c0 = (w0 * x[:, None]).sum() / w0.sum()
+    c1 = (w1 * x[:, None]).sum() / w1.sum()
+    assert c1 > c0 + 0.01                     # the shocked bubble is pushed downstream
+
+
+def test_two_gamma_riemann_vectors(hg, oracle):
+    rng = np.random.default_rng(17)
+    rows, ok = [], 0
+    for _ in range(300):
+        gL, gR = rng.choice([1.4, 1.67, 1.2]), rng.choice([1.4, 1.67, 3.0])
+        rL, rR = 10 ** rng.uniform(-1, 1), 10 ** rng.uniform(-1, 1)
+        pL, pR = 10 ** rng.uniform(-1, 1), 10 ** rng.uniform(-1, 1)
+        cL, cR = np.sqrt(gL * pL / rL), np.sqrt(gR * pR / rR)
+        uL, uR = 0.0, rng.uniform(-2, 1) * min(cL, cR)
+        rows.append([gL, gR, uL, uR, pL, pR, cL, cR, 1e-9, 1e-12])
+    inp = np.array(rows)
+    out = hg.riemann_batch(inp, 500)
+    for k, r in enumerate(rows):
+        us, ps, c0, c1, gap, it = oracle.riemann(*r[:10], 500)
+        assert (out[k, 2], out[k, 3]) == (c0, c1)
+        assert abs(out[k, 1] - ps) <= 1e-12 * ps
+        p_ex, u_ex = exact_riemann.star_state(r[4] / r[6] ** 2 * r[0], 0.0, r[4], r[5] / r[7] ** 2 * r[1], r[3], r[5],
+                                              r[0], r[1])
+        assert abs(out[k, 1] - p_ex) <= 1e-9 * p_ex

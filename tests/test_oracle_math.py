@@ -275,3 +275,19 @@ def test_error_paths_match_reference(oracle, reflib):
         assert co[5] == cr[5] and npo == npr and tpo[npo - 1] == tpr[npr - 1]
         if st != 2:   # a GRP failure stores a schedule-dependent flux in the reference
             assert np.array_equal(fo[:, npo - 1], fr[:, npr - 1], equal_nan=True)
+
+
+def test_two_gamma_star_state_against_independent_solver(oracle):
+    """Riemann_solver_exact with different gammas on the two sides (the
+    two-fluid path) against the independent exact solver."""
+    from exact_riemann import star_state
+    rng = np.random.default_rng(21)
+    for _ in range(200):
+        gL, gR = rng.choice([1.2, 1.4, 1.67]), rng.choice([1.4, 1.67, 3.0])
+        rL, rR, pL, pR = 10 ** rng.uniform(-1, 1, 4)
+        cL, cR = np.sqrt(gL * pL / rL), np.sqrt(gR * pR / rR)
+        uR = rng.uniform(-2, 1) * min(cL, cR)
+        us, ps, *_ = oracle.riemann(gL, gR, 0.0, uR, pL, pR, cL, cR, 1e-9, 1e-13, 500)
+        p_ex, u_ex = star_state(rL, 0.0, pL, rR, uR, pR, gL, gR)
+        assert abs(ps - p_ex) <= 1e-10 * p_ex
+        assert abs(us - u_ex) <= 1e-9 * (cL + cR)
