@@ -15,7 +15,8 @@ on a bounded number of steps of the same workload on the host's cores.
 `learned_guess` compares, at 4096^2 with Newton tolerance 1e-12, the plain
 solve with the one started from the learned p* guess (rate, and the relative L1
 difference of the two final states).  `ensemble_1d` runs 512 independent 1D
-GRP problems of uneven sizes through the batched engine.
+GRP problems of uneven sizes through the batched engine; `two_fluid` runs the
+shock-bubble interaction (4096 x 1024) with and without the learned guess.
 """
 import argparse
 import json
@@ -129,6 +130,38 @@ def ensemble_leg(n_inst=512, seed=2024):
             "ended_by_check": int(np.sum(status != 0))}
 
 
+def two_fluid_leg(m=4096, n=1024, steps=300):
+    """The two-fluid shock-bubble workload (4 x 1 domain, Mach 1.22 shock, gamma
+    1.4 / 1.67 bubble, reflective top and bottom), Newton tolerance 1e-12, the
+    reference's starting guess against the learned one, `steps` steps from t = 0
+    each (the whole run to t = 1 is about 7900 steps)."""
+    import numpy as np
+    import hydrogrp
+    from hydrogrp import mlp
+    import cases
+    from _libs import make_config
+    init, frac = cases.shock_bubble(m, n)
+    cfg = make_config(t_all=1.0, h_x=4.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-2, split=1)
+    out = {}
+    for name, p in (("plain", None), ("learned", mlp.load())):
+        s = hydrogrp.Solver2D(m, n, cfg)
+        s.set_newton(1e-12, p)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.set_fluids(1.67, frac)
+        s.bench(2)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.set_fluids(1.67, frac)
+        ms, ms_x, ms_y = s.bench(steps)
+        state = s.download()
+        s.close()
+        assert np.all(np.isfinite(state)), "non-finite two-fluid state"
+        out[name] = {"value": m * n * steps / (ms / 1000.0), "ms_per_step": ms / steps,
+                     "ms_per_launch": {"sweep_x_tf": ms_x, "sweep_y_tf": ms_y}}
+    return {"workload": f"two-fluid shock-bubble {m}x{n} (4x1 domain), Newton tol 1e-12, {steps} steps from t=0",
+            "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
+            "speedup": out["learned"]["value"] / out["plain"]["value"]}
+
+
 def cpu_leg(target_seconds=12.0):
     """The oracle on the host cores, fixed-tau mode (exactly the frozen tau), bounded sample."""
     import numpy as np
@@ -166,6 +199,7 @@ def main():
     ap.add_argument("--size", type=int, default=1024, help="grid cells per side (default: the 1024^2 workload)")
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess comparison at 4096^2")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 1D ensemble")
+    ap.add_argument("--no-two-fluid", action="store_true", help="skip the two-fluid shock-bubble")
     args = ap.parse_args()
     global M, N
     M = N = args.size
@@ -205,6 +239,8 @@ def main():
         line["learned_guess"] = learned_leg()
     if not args.no_ensemble:
         line["ensemble_1d"] = ensemble_leg()
+    if not args.no_two_fluid:
+        line["two_fluid"] = two_fluid_leg()
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg()
     print(json.dumps(line), flush=True)

@@ -4,7 +4,8 @@
 // src/riemann_solver/riemann_solver_exact_Ben.c:87-93) always starts Newton
 // from half the intersection of the two acoustic tangents.  Here a small MLP
 // predicts y = log(p* / p_TR), the correction to the two-rarefaction estimate
-// p_TR (exact when both waves are rarefactions), from six dimensionless inputs
+// p_TR (exact when both waves are rarefactions of one gas; two Newton steps
+// from the mean-exponent form for two gases), from six dimensionless inputs
 //   log(rho_L/rho_R)/ln 1e3, log(p_L/p_R)/ln 1e4, (u_R-u_L)/(5 c_L),
 //   log(c_R/c_L)/5, gamma_L - 1.4, gamma_R - 1.4,
 // and Newton starts from p_TR * exp(y).  The solver stays the judge: after at
@@ -67,8 +68,11 @@ __device__ __forceinline__ float gelu_f(float x)
 }
 
 // Inputs of the network and the two-rarefaction estimate, from the states
-// normal to the interface.  Returns false where no estimate exists (the
-// two-rarefaction numerator is not positive: the data would open a vacuum).
+// normal to the interface.  p_TR solves
+//   2c_L/(g_L-1) ((p/p_L)^z_L - 1) + 2c_R/(g_R-1) ((p/p_R)^z_R - 1) + (u_R - u_L) = 0,
+// z = (g-1)/(2g): the closed form with the mean exponent (exact for one gas),
+// then two Newton steps in log p for two gases.  Returns false where no
+// estimate exists (the data would open a vacuum).
 __device__ __forceinline__ bool mlp_features(double rho_L, double u_L, double p_L, double rho_R, double u_R,
                                              double p_R, double g_L, double g_R, float x[MLP_IN], float *p_tr)
 {
@@ -83,12 +87,21 @@ __device__ __forceinline__ bool mlp_features(double rho_L, double u_L, double p_
     x[3] = 0.5f * ((lpr - lrr + __logf(gr)) - (lpl - lrl + __logf(gl))) * 0.2f;
     x[4] = gl - 1.4f;
     x[5] = gr - 1.4f;
-    // p_TR = ((c_L + c_R - (g-1)/2 du) / (c_L p_L^-z + c_R p_R^-z))^(1/z), z = (g-1)/(2g), g = gamma_L
-    const float z = (gl - 1.0f) / (2.0f * gl);
-    const float num = cl + cr - 0.5f * (gl - 1.0f) * du;
-    const float den = cl * __expf(-z * lpl) + cr * __expf(-z * lpr);
-    const bool ok = num > 0.0f && den > 0.0f && cl > 0.0f && cr > 0.0f && __builtin_isfinite(x[0] + x[1] + x[2] + x[3]);
-    *p_tr = ok ? __expf(__logf(num / den) / z) : 0.0f;
+    const float zl = (gl - 1.0f) / (2.0f * gl), zr = (gr - 1.0f) / (2.0f * gr), z = 0.5f * (zl + zr);
+    const float al = 2.0f * cl / (gl - 1.0f), ar = 2.0f * cr / (gr - 1.0f);
+    const float num = al + ar - du;
+    const float den = al * __expf(-z * lpl) + ar * __expf(-z * lpr);
+    bool ok = num > 0.0f && den > 0.0f && cl > 0.0f && cr > 0.0f && __builtin_isfinite(x[0] + x[1] + x[2] + x[3]);
+    float y = ok ? __logf(num / den) / z : 0.0f;             // log p
+    if (ok && gl != gr) {
+        for (int it = 0; it < 2; ++it) {
+            const float ql = __expf(zl * (y - lpl)), qr = __expf(zr * (y - lpr));
+            const float F = al * (ql - 1.0f) + ar * (qr - 1.0f) + du;
+            const float dF = al * zl * ql + ar * zr * qr;
+            y -= F / dF;
+        }
+    }
+    *p_tr = ok ? __expf(y) : 0.0f;
     return ok && *p_tr > 0.0f && __builtin_isfinite(*p_tr);
 }
 
@@ -171,6 +184,10 @@ __device__ inline float mlp_eval_wave(const MlpPacked *__restrict__ W, const flo
 
 // Learned starting pressure for this lane's interface, or -1 where the lane
 // has none (want == false, or no two-rarefaction estimate).  Wave-uniform call.
+// Weak interfaces (pressure, velocity and density jumps all within about 10 %)
+// start from p_TR itself, which is already within O(jump^3) of p* and where
+// the network's correction is nil; the network runs only in waves that hold a
+// strong interface.
 __device__ inline double learned_guess_wave(const MlpPacked *__restrict__ W, bool want, double rho_L, double u_L,
                                             double p_L, double rho_R, double u_R, double p_R, double g_L,
                                             double g_R, uint32_t *lds)
@@ -179,14 +196,17 @@ __device__ inline double learned_guess_wave(const MlpPacked *__restrict__ W, boo
     float p_tr = 0.f;
     bool ok = false;
     if (want) ok = mlp_features(rho_L, u_L, p_L, rho_R, u_R, p_R, g_L, g_R, x, &p_tr);
-    if (!ok) {
+    // |log(p_L/p_R)| < 0.1, |u_R-u_L| < 0.1 c_L, |log(rho_L/rho_R)| < 0.1, one gas
+    const bool strong = ok && (fabsf(x[1]) > 0.0109f || fabsf(x[2]) > 0.02f || fabsf(x[0]) > 0.0145f ||
+                               x[4] != x[5]);
+    if (!strong) {
 #pragma unroll
         for (int k = 0; k < MLP_IN; ++k) x[k] = 0.f;
     }
-    if (!__any(ok)) return -1.0;
-    const float y = mlp_eval_wave(W, x, lds);
+    float y = 0.f;
+    if (__any(strong)) y = mlp_eval_wave(W, x, lds);
     if (!ok) return -1.0;
-    const double p0 = (double)p_tr * (double)__expf(y);
+    const double p0 = (double)p_tr * (strong ? (double)__expf(y) : 1.0);
     return (p0 > 0.0 && p0 < 1e300) ? p0 : -1.0;
 }
 

@@ -73,31 +73,37 @@ __device__ __forceinline__ double unif(uint64_t seed, uint64_t idx, int stream)
 }
 
 // One sample: left state (1, 0, 1), rho_L/rho_R log-uniform in [1e-3, 1e3],
-// p_L/p_R log-uniform in [1e-4, 1e4], (u_R - u_L)/c_L uniform in [-5, 5],
-// gamma 1.4; p* from the exact solver (the reference's guess, tolerance tol).
-// states: rho_L u_L p_L rho_R u_R p_R p*.
+// p_L/p_R log-uniform in [1e-4, 1e4], (u_R - u_L)/c_L uniform in [-5, 5];
+// half the samples have gamma 1.4 on both sides, the other half gamma_L and
+// gamma_R independently uniform in [1.2, 1.8] (two-fluid interfaces); p* from
+// the exact solver (the reference's guess, tolerance tol).  Samples whose
+// solve does not converge (above gamma 1.4 a jump of 5 c_L can open a vacuum)
+// are marked not ok and left out of training.
+// states: rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R p*.
 __global__ void mlp_data_kernel(uint64_t seed, int count, double tol, float *feat, float *target, double *states,
                                 int *ok)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= count) return;
-    const double g = 1.4;
+    const bool mixed = unif(seed, (uint64_t)k, 3) < 0.5;
+    const double gL = mixed ? 1.2 + 0.6 * unif(seed ^ 0x5bd1e995ull, (uint64_t)k, 0) : 1.4;
+    const double gR = mixed ? 1.2 + 0.6 * unif(seed ^ 0x5bd1e995ull, (uint64_t)k, 1) : 1.4;
     const double rho_L = 1.0, p_L = 1.0, u_L = 0.0;
     const double rho_R = rho_L / pow(10.0, -3.0 + 6.0 * unif(seed, (uint64_t)k, 0));
     const double p_R = p_L / pow(10.0, -4.0 + 8.0 * unif(seed, (uint64_t)k, 1));
-    const double c_L = sqrt(g * p_L / rho_L), c_R = sqrt(g * p_R / rho_R);
+    const double c_L = sqrt(gL * p_L / rho_L), c_R = sqrt(gR * p_R / rho_R);
     const double u_R = u_L + (-5.0 + 10.0 * unif(seed, (uint64_t)k, 2)) * c_L;
-    const Gas G = make_gas(g);
-    const StarResult r = riemann_exact(G, G, u_L, u_R, p_L, p_R, c_L, c_R, 1e-9, tol, 500);
+    const StarResult r = riemann_exact(make_gas(gL), make_gas(gR), u_L, u_R, p_L, p_R, c_L, c_R, 1e-9, tol, 500);
     float x[MLP_IN], p_tr;
-    const bool good = mlp_features(rho_L, u_L, p_L, rho_R, u_R, p_R, g, g, x, &p_tr) && r.p_star > 0.0 &&
+    const bool good = mlp_features(rho_L, u_L, p_L, rho_R, u_R, p_R, gL, gR, x, &p_tr) && r.p_star > 0.0 &&
                       isfinite(r.p_star) && r.iters < 500;
     for (int q = 0; q < MLP_IN; ++q) feat[(size_t)k * MLP_IN + q] = good ? x[q] : 0.f;
     target[k] = good ? (float)log(r.p_star / (double)p_tr) : 0.f;
     ok[k] = good;
     if (states) {
-        double *s = states + (size_t)k * 7;
-        s[0] = rho_L; s[1] = u_L; s[2] = p_L; s[3] = rho_R; s[4] = u_R; s[5] = p_R; s[6] = r.p_star;
+        double *s = states + (size_t)k * 9;
+        s[0] = rho_L; s[1] = u_L; s[2] = p_L; s[3] = rho_R; s[4] = u_R; s[5] = p_R; s[6] = gL; s[7] = gR;
+        s[8] = r.p_star;
     }
 }
 
@@ -237,7 +243,7 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(const MlpPacked *W, cons
     if (k < count) out[k] = y;
 }
 
-// in: rho_L u_L p_L rho_R u_R p_R gamma eps tol per interface;
+// in: rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R eps tol per interface;
 // out: u* p* crw0 crw1 iterations starting-guess (-1: the reference's guess)
 __global__ void __launch_bounds__(256) riemann_guess_kernel(const double *in, int count, int N, const MlpPacked *W,
                                                             double *out)
@@ -245,15 +251,14 @@ __global__ void __launch_bounds__(256) riemann_guess_kernel(const double *in, in
     __shared__ uint32_t lds[4][64 * 8];
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool act = k < count;
-    const double *a = in + (size_t)(act ? k : 0) * 9;
-    const double rho_L = a[0], u_L = a[1], p_L = a[2], rho_R = a[3], u_R = a[4], p_R = a[5], g = a[6];
-    const double eps = a[7], tol = a[8];
+    const double *a = in + (size_t)(act ? k : 0) * 10;
+    const double rho_L = a[0], u_L = a[1], p_L = a[2], rho_R = a[3], u_R = a[4], p_R = a[5], gL = a[6], gR = a[7];
+    const double eps = a[8], tol = a[9];
     double pg = -1.0;
-    if (W) pg = learned_guess_wave(W, act, rho_L, u_L, p_L, rho_R, u_R, p_R, g, g, lds[threadIdx.x >> 6]);
+    if (W) pg = learned_guess_wave(W, act, rho_L, u_L, p_L, rho_R, u_R, p_R, gL, gR, lds[threadIdx.x >> 6]);
     if (!act) return;
-    const Gas G = make_gas(g);
-    const double c_L = sqrt(g * p_L / rho_L), c_R = sqrt(g * p_R / rho_R);
-    const StarResult r = riemann_exact(G, G, u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, N, pg);
+    const double c_L = sqrt(gL * p_L / rho_L), c_R = sqrt(gR * p_R / rho_R);
+    const StarResult r = riemann_exact(make_gas(gL), make_gas(gR), u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, N, pg);
     double *o = out + (size_t)k * 6;
     o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters; o[5] = pg;
 }
@@ -278,7 +283,7 @@ extern "C" int hgrp_mlp_data(unsigned long long seed, int count, double tol, flo
     MCHECK(df.alloc((size_t)count * MLP_IN));
     MCHECK(dt.alloc(count));
     MCHECK(dk.alloc(count));
-    if (states) MCHECK(ds.alloc((size_t)count * 7));
+    if (states) MCHECK(ds.alloc((size_t)count * 9));
     hipLaunchKernelGGL(mlp_data_kernel, dim3((count + 255) / 256), dim3(256), 0, 0, (uint64_t)seed, count, tol, df.p,
                        dt.p, states ? ds.p : nullptr, dk.p);
     MCHECK(hipGetLastError());
@@ -286,7 +291,7 @@ extern "C" int hgrp_mlp_data(unsigned long long seed, int count, double tol, flo
     MCHECK(hipMemcpy(feat, df.p, (size_t)count * MLP_IN * 4, hipMemcpyDeviceToHost));
     MCHECK(hipMemcpy(target, dt.p, (size_t)count * 4, hipMemcpyDeviceToHost));
     MCHECK(hipMemcpy(ok, dk.p, (size_t)count * 4, hipMemcpyDeviceToHost));
-    if (states) MCHECK(hipMemcpy(states, ds.p, (size_t)count * 7 * 8, hipMemcpyDeviceToHost));
+    if (states) MCHECK(hipMemcpy(states, ds.p, (size_t)count * 9 * 8, hipMemcpyDeviceToHost));
     return 0;
 }
 
@@ -350,9 +355,9 @@ extern "C" int hgrp_riemann_guess_batch(const double *in, int count, int N, cons
         MCHECK(hipMemcpy(dw.p, &h, sizeof h, hipMemcpyHostToDevice));
     }
     Dev<double> din, dout;
-    MCHECK(din.alloc((size_t)count * 9));
+    MCHECK(din.alloc((size_t)count * 10));
     MCHECK(dout.alloc((size_t)count * 6));
-    MCHECK(hipMemcpy(din.p, in, (size_t)count * 9 * 8, hipMemcpyHostToDevice));
+    MCHECK(hipMemcpy(din.p, in, (size_t)count * 10 * 8, hipMemcpyHostToDevice));
     hipLaunchKernelGGL(riemann_guess_kernel, dim3((count + 255) / 256), dim3(256), 0, 0, din.p, count, N,
                        params ? dw.p : (const MlpPacked *)nullptr, dout.p);
     MCHECK(hipGetLastError());

@@ -180,7 +180,8 @@ __device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &
 
 // Learned starting pressure for the interface (SL | SR) normal to DIR, for
 // every lane of the wave (uniform call); -1 where Newton will not run (the
-// Q1D acoustic shortcut dist < eps) or the lane has no interface.
+// Q1D acoustic shortcut dist < eps, the solver's equal-state exit) or the lane
+// has no interface.
 template <int DIR>
 __device__ __forceinline__ double interface_guess(const SweepArgs &a, bool active, const Side &SL, const Side &SR,
                                                   uint32_t *lds)
@@ -188,7 +189,8 @@ __device__ __forceinline__ double interface_guess(const SweepArgs &a, bool activ
     const double uL = DIR == 0 ? SL.u : SL.v, uR = DIR == 0 ? SR.u : SR.v;
     const double dist = sqrt((SL.rho - SR.rho) * (SL.rho - SR.rho) + (uL - uR) * (uL - uR)
                              + (SL.p - SR.p) * (SL.p - SR.p));
-    const bool want = active && !(dist < a.eps);
+    // Newton runs only past the acoustic shortcut and the equal-state exit of the exact solver
+    const bool want = active && !(dist < a.eps) && !(fabs(uL - uR) < a.tol && fabs(SL.p - SR.p) < a.tol);
     return learned_guess_wave(a.mlp, want, SL.rho, uL, SL.p, SR.rho, uR, SR.p, a.gas.g, a.gas.g, lds);
 }
 

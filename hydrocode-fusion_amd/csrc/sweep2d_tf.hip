@@ -147,15 +147,36 @@ template <int DIR>
 __device__ __forceinline__ void solve_interface_tf(const SweepArgs &a, const double wl[NQ], const double sl[NQ],
                                                    const double wr[NQ], const double sr[NQ],
                                                    unsigned long long loop_index, double F[4], double I[4],
-                                                   double Z[3])
+                                                   double Z[3], double p_guess = -1.0)
 {
     Side SL, SR;
     double FA[4];
     recon_tf(wl, sl, a.hh, true, SL, FA[0], FA[1]);
     recon_tf(wr, sr, a.hh, false, SR, FA[2], FA[3]);
     int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
-    if (grp_2d_flux_tf<DIR>(SL, SR, a.two, FA, a.tau, a.eps, F, I, Z, a.tol)) code = 2;
+    if (grp_2d_flux_tf<DIR>(SL, SR, a.two, FA, a.tau, a.eps, F, I, Z, a.tol, p_guess)) code = 2;
     if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
+}
+
+// learned starting pressure for the interface (wl, sl | wr, sr), every lane of
+// the wave (uniform call); gamma of each side from its reconstructed fraction
+template <int DIR>
+__device__ __forceinline__ double interface_guess_tf(const SweepArgs &a, bool active, const double wl[NQ],
+                                                     const double sl[NQ], const double wr[NQ], const double sr[NQ],
+                                                     uint32_t *lds)
+{
+    Side SL, SR;
+    double aL, daL, aR, daR;
+    recon_tf(wl, sl, a.hh, true, SL, aL, daL);
+    recon_tf(wr, sr, a.hh, false, SR, aR, daR);
+    const double uL = DIR == 0 ? SL.u : SL.v, uR = DIR == 0 ? SR.u : SR.v;
+    const double dist = sqrt((SL.rho - SR.rho) * (SL.rho - SR.rho) + (uL - uR) * (uL - uR)
+                             + (SL.p - SR.p) * (SL.p - SR.p));
+    // Newton runs only past the acoustic shortcut and the equal-state exit of the exact solver
+    const bool want = active && !(dist < a.eps) && !(fabs(uL - uR) < a.tol && fabs(SL.p - SR.p) < a.tol);
+    const double gL = aL == 1.0 ? a.two.a.g : (aL == 0.0 ? a.two.b.g : mix_gamma(aL, a.two));
+    const double gR = aR == 1.0 ? a.two.a.g : (aR == 0.0 ? a.two.b.g : mix_gamma(aR, a.two));
+    return learned_guess_wave(a.mlp, want, SL.rho, uL, SL.p, SR.rho, uR, SR.p, gL, gR, lds);
 }
 
 __device__ __forceinline__ bool tf_halted(const SweepArgs &a)
@@ -170,7 +191,7 @@ __device__ __forceinline__ bool tf_halted(const SweepArgs &a)
 }
 
 // X sweep, row-parallel (the layout of sweep_x_rows_kernel with five quantities)
-template <int W>
+template <int W, bool LEARNED>
 __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 {
     __shared__ double xa[W][NQ][64];   // values
@@ -233,9 +254,12 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0}, Z[3] = {0.0, 0.0, 0.0};
     const int f = c + 1;
+    double pg = -1.0;
+    if constexpr (LEARNED)   // xf[w] is not written before the solve: it holds the network inputs
+        pg = interface_guess_tf<0>(a, col && f <= m, v, s, vr, sr, (uint32_t *)&xf[w][0][0]);
     if (col && f <= m)
         solve_interface_tf<0>(a, v, s, vr, sr, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f,
-                              F, I, Z);
+                              F, I, Z, pg);
 #pragma unroll
     for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
 #pragma unroll
@@ -254,9 +278,11 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 }
 
 // Y sweep: a wave owns 64 consecutive cells of a y-line (as sweep_y_kernel)
+template <bool LEARNED>
 __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chunks)
 {
     __shared__ double stash[4][NQ + 1][64];
+    __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
     if (tf_halted(a)) return;
     const int lane = threadIdx.x & 63;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
@@ -304,9 +330,11 @@ __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chu
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0}, Z[3] = {0.0, 0.0, 0.0};
     const int f = c + 1;
+    double pg = -1.0;
+    if constexpr (LEARNED) pg = interface_guess_tf<1>(a, f <= n, w, s, wr, sr, mlp_lds[(threadIdx.x >> 6) & 3]);
     if (f <= n)
         solve_interface_tf<1>(a, w, s, wr, sr, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f,
-                              F, I, Z);
+                              F, I, Z, pg);
     double Fl[4], Il[4], Zl[3];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
@@ -380,7 +408,10 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st)
 {
     dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-    hipLaunchKernelGGL(sweep_x_tf_kernel<8>, grid, dim3(64 * 8), 0, st, a);
+    if (a.mlp)
+        hipLaunchKernelGGL((sweep_x_tf_kernel<8, true>), grid, dim3(64 * 8), 0, st, a);
+    else
+        hipLaunchKernelGGL((sweep_x_tf_kernel<8, false>), grid, dim3(64 * 8), 0, st, a);
     return hipGetLastError();
 }
 
@@ -389,7 +420,10 @@ hipError_t launch_sweep_y_tf(const SweepArgs &a, hipStream_t st)
     const int chunks = (a.n + 62) / 63;
     const long waves = (long)a.m * chunks;
     const long blocks = (waves + 3) / 4;
-    hipLaunchKernelGGL(sweep_y_tf_kernel, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+    if (a.mlp)
+        hipLaunchKernelGGL(sweep_y_tf_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+    else
+        hipLaunchKernelGGL(sweep_y_tf_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
     return hipGetLastError();
 }
 

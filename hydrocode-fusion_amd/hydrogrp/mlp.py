@@ -38,10 +38,11 @@ def init_params(seed=0):
 
 
 def data(seed, count, tol=1e-12):
-    """Training-distribution samples: (features[count,6], target[count], states[count,7], ok[count])."""
+    """Training-distribution samples: (features[count,6], target[count],
+    states[count,9] = rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R p*, ok[count])."""
     feat = np.zeros((count, N_IN), np.float32)
     target = np.zeros(count, np.float32)
-    states = np.zeros((count, 7))
+    states = np.zeros((count, 9))
     ok = np.zeros(count, np.int32)
     _check(lib().hgrp_mlp_data(seed, count, tol, _f(feat), _f(target), _p(states),
                                ok.ctypes.data_as(C.POINTER(C.c_int))), "hgrp_mlp_data")
@@ -77,12 +78,17 @@ def train(params, seed=1, n_samples=1 << 20, epochs=4, batch=8192, lr=3e-3, tol=
 
 
 def riemann_guess_batch(states, gamma=1.4, eps=1e-9, tol=1e-12, params=None, N=500):
-    """Exact solves of (rho_L u_L p_L rho_R u_R p_R) rows; params None: the reference's guess.
+    """Exact solves of (rho_L u_L p_L rho_R u_R p_R [gamma_L gamma_R ...]) rows (without
+    the gamma columns every row has `gamma`); params None: the reference's guess.
     Returns rows (u*, p*, CRW0, CRW1, iterations, starting guess or -1)."""
     states = np.asarray(states, np.float64)
-    inp = np.zeros((len(states), 9))
+    inp = np.zeros((len(states), 10))
     inp[:, :6] = states[:, :6]
-    inp[:, 6], inp[:, 7], inp[:, 8] = gamma, eps, tol
+    if states.shape[1] >= 8:
+        inp[:, 6:8] = states[:, 6:8]
+    else:
+        inp[:, 6:8] = gamma
+    inp[:, 8], inp[:, 9] = eps, tol
     out = np.zeros((len(states), 6))
     fp = None if params is None else _f(np.ascontiguousarray(params, np.float32))
     _check(lib().hgrp_riemann_guess_batch(_p(inp), len(inp), N, fp, _p(out)), "hgrp_riemann_guess_batch")

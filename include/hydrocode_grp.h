@@ -183,9 +183,10 @@ int hgrp1d_tune(hgrp1d *ctx, int probe_steps, int *bs_iface, int *bs_update);
 
 /* count samples of the training distribution (left state (1,0,1), rho_L/rho_R
  * log-uniform 1e-3..1e3, p_L/p_R log-uniform 1e-4..1e4, (u_R-u_L)/c_L uniform
- * -5..5, gamma 1.4; counter-based from seed), their p* from the exact solver at
+ * -5..5; half with gamma 1.4 on both sides, half with gamma_L, gamma_R uniform
+ * in 1.2..1.8; counter-based from seed), their p* from the exact solver at
  * tolerance tol: feat[count*6], target[count] = log(p* / p_TR), ok[count];
- * states[count*7] (may be NULL) = rho_L u_L p_L rho_R u_R p_R p*. */
+ * states[count*9] (may be NULL) = rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R p*. */
 int hgrp_mlp_data(unsigned long long seed, int count, double tol, float *feat, float *target,
                   double *states, int *ok);
 
@@ -205,9 +206,9 @@ int hgrp_mlp_train(unsigned long long seed, int n_samples, int epochs, int batch
                    float *params, float *epoch_loss);
 
 /* Exact Riemann solve per interface with an optional learned guess (params
- * NULL: the reference's guess).  in[count*9]: rho_L u_L p_L rho_R u_R p_R gamma
- * eps tol; out[count*6]: u* p* CRW[0] CRW[1] Newton-iterations starting-guess
- * (-1 where the reference's guess was used). */
+ * NULL: the reference's guess).  in[count*10]: rho_L u_L p_L rho_R u_R p_R
+ * gamma_L gamma_R eps tol; out[count*6]: u* p* CRW[0] CRW[1] Newton-iterations
+ * starting-guess (-1 where the reference's guess was used). */
 int hgrp_riemann_guess_batch(const double *in, int count, int N, const float *params, double *out);
 
 /* Name of the device the library runs on (empty if none). */

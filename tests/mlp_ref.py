@@ -70,12 +70,21 @@ def loss_grad(P, X, T, quantize=True):
     return float(np.mean(r * r)), grad
 
 
-def features(rho_L, u_L, p_L, rho_R, u_R, p_R, g=1.4):
-    """Network inputs and the two-rarefaction estimate p_TR, in FP64."""
-    c_L, c_R = np.sqrt(g * p_L / rho_L), np.sqrt(g * p_R / rho_R)
-    x = np.stack([np.log(rho_L / rho_R) / np.log(1e3), np.log(p_L / p_R) / np.log(1e4), (u_R - u_L) / c_L / 5,
-                  np.log(c_R / c_L) / 5, np.zeros_like(c_L), np.zeros_like(c_L)], axis=-1)
-    z = (g - 1) / (2 * g)
-    num = c_L + c_R - 0.5 * (g - 1) * (u_R - u_L)
-    den = c_L * p_L ** -z + c_R * p_R ** -z
-    return x, (num / den) ** (1 / z)
+def features(rho_L, u_L, p_L, rho_R, u_R, p_R, g=1.4, gR=None):
+    """Network inputs and the two-rarefaction estimate p_TR, in FP64: the closed
+    form with the mean exponent, two Newton steps in log p where the gammas differ."""
+    gR = np.asarray(g if gR is None else gR, dtype=np.float64)
+    g = np.asarray(g, dtype=np.float64)
+    c_L, c_R = np.sqrt(g * p_L / rho_L), np.sqrt(gR * p_R / rho_R)
+    x = np.stack(np.broadcast_arrays(np.log(rho_L / rho_R) / np.log(1e3), np.log(p_L / p_R) / np.log(1e4),
+                                     (u_R - u_L) / c_L / 5, np.log(c_R / c_L) / 5, g - 1.4, gR - 1.4), axis=-1)
+    zl, zr = (g - 1) / (2 * g), (gR - 1) / (2 * gR)
+    z = 0.5 * (zl + zr)
+    al, ar = 2 * c_L / (g - 1), 2 * c_R / (gR - 1)
+    du = u_R - u_L
+    y = np.log((al + ar - du) / (al * p_L ** -z + ar * p_R ** -z)) / z
+    for _ in range(2):
+        ql, qr = np.exp(zl * (y - np.log(p_L))), np.exp(zr * (y - np.log(p_R)))
+        F = al * (ql - 1) + ar * (qr - 1) + du
+        y = np.where(g != gR, y - F / (al * zl * ql + ar * zr * qr), y)
+    return x, np.exp(y)

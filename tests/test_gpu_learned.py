@@ -33,22 +33,26 @@ def trained(mlp):
 
 def test_training_targets_match_oracle(mlp, oracle):
     feat, target, states, ok = mlp.data(11, 4096, TOL12)
-    assert ok.mean() > 0.99
-    g = 1.4
+    # with gamma up to 1.8 a jump of 5 c_L can open a vacuum (2 c_L/(gamma_L-1) < 5 c_L):
+    # those samples have no star pressure and are left out
+    assert ok.mean() > 0.97
+    assert ok[states[:, 6] == states[:, 7]].mean() > 0.99
     worst = 0.0
     for k in np.flatnonzero(ok)[:300]:
-        rl, ul, pl, rr, ur, pr, ps = states[k]
-        cl, cr = np.sqrt(g * pl / rl), np.sqrt(g * pr / rr)
-        want = oracle.riemann(g, g, ul, ur, pl, pr, cl, cr, 1e-9, TOL12, 500)[1]
+        rl, ul, pl, rr, ur, pr, gl, gr, ps = states[k]
+        cl, cr = np.sqrt(gl * pl / rl), np.sqrt(gr * pr / rr)
+        want = oracle.riemann(gl, gr, ul, ur, pl, pr, cl, cr, 1e-9, TOL12, 500)[1]
         worst = max(worst, abs(ps - want) / want)
     print("p* vs oracle: max rel", worst, flush=True)
     assert worst <= 1e-10
-    # the two-rarefaction estimate is exact where both waves are rarefactions
-    two_rare = ok & (states[:, 6] < np.minimum(states[:, 2], states[:, 5]))
-    assert two_rare.sum() > 100
+    # the two-rarefaction estimate is exact where both waves are rarefactions of one gas
+    one_gas = states[:, 6] == states[:, 7]
+    two_rare = ok & one_gas & (states[:, 8] < np.minimum(states[:, 2], states[:, 5]))
+    assert two_rare.sum() > 50 and np.mean(one_gas) > 0.4
     assert np.max(np.abs(target[two_rare])) < 2e-4
     # the device features agree with the FP64 restatement
-    x64, _ = mlp_ref.features(states[ok, 0], states[ok, 1], states[ok, 2], states[ok, 3], states[ok, 4], states[ok, 5])
+    x64, _ = mlp_ref.features(states[ok, 0], states[ok, 1], states[ok, 2], states[ok, 3], states[ok, 4], states[ok, 5],
+                              states[ok, 6], states[ok, 7])
     assert np.max(np.abs(feat[ok] - x64)) < 1e-5
 
 
@@ -111,9 +115,9 @@ def test_learned_guess_cuts_newton_iterations(mlp, trained):
     assert np.median(dp) <= 1e-12 and dp.max() <= 1e-9
     # the plain form of this entry point is the reference's solver
     from hydrogrp import riemann_batch
-    g = 1.4
-    cl, cr = np.sqrt(g * st[:, 2] / st[:, 0]), np.sqrt(g * st[:, 5] / st[:, 3])
-    inp = np.stack([np.full(len(st), g), np.full(len(st), g), st[:, 1], st[:, 4], st[:, 2], st[:, 5], cl, cr,
+    gl, gr = st[:, 6], st[:, 7]
+    cl, cr = np.sqrt(gl * st[:, 2] / st[:, 0]), np.sqrt(gr * st[:, 5] / st[:, 3])
+    inp = np.stack([gl, gr, st[:, 1], st[:, 4], st[:, 2], st[:, 5], cl, cr,
                     np.full(len(st), 1e-9), np.full(len(st), TOL12)], axis=1)
     ref = riemann_batch(inp, 500)
     assert np.array_equal(ref[:, 1], plain[:, 1]) and np.array_equal(ref[:, 4], plain[:, 4])

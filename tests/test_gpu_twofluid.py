@@ -141,3 +141,30 @@ def test_two_gamma_riemann_vectors(hg, oracle):
         p_ex, u_ex = exact_riemann.star_state(r[4] / r[6] ** 2 * r[0], 0.0, r[4], r[5] / r[7] ** 2 * r[1], r[3], r[5],
                                               r[0], r[1])
         assert abs(out[k, 1] - p_ex) <= 1e-9 * p_ex
+
+
+def test_learned_guess_in_two_fluid_sweeps(hg):
+    """Shock-bubble with Newton tolerance 1e-12, the learned guess against the
+    reference's guess: the same solution to the tolerance's reach."""
+    from hydrogrp import mlp
+    params = mlp.load()
+    m, n = 256, 64
+    init, frac = cases.shock_bubble(m, n)
+    cfg = make_config(t_all=0.3, h_x=4.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-2, split=1)
+    outs = []
+    for p in (None, params):
+        s = hg.Solver2D(m, n, cfg)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.set_fluids(1.67, frac)
+        s.set_newton(1e-12, p)
+        res = s.solve(np.array([0.0, 0.3]), 2)
+        assert res.status == 0
+        outs.append((s.download(), s.download_fraction(), res.k))
+        s.close()
+    (a, fa, ka), (b, fb, kb) = outs
+    assert ka == kb
+    d = rel_linf(b, a)
+    l1 = np.abs(b - a).sum(axis=(1, 2)) / np.abs(a).sum(axis=(1, 2))
+    print("two-fluid learned vs plain: rel Linf", d, "rel L1", l1, flush=True)
+    assert l1.max() <= 1e-10 and d.max() <= 1e-8
+    assert np.abs(fb - fa).max() <= 1e-8

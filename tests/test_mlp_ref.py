@@ -49,3 +49,23 @@ def test_two_rarefaction_estimate_is_exact_for_two_rarefactions(oracle):
         worst = max(worst, abs(p_tr - ps) / ps)
     assert n > 100
     assert worst < 1e-9, worst
+
+
+def test_two_gamma_two_rarefaction_estimate(oracle):
+    """With two gases the estimate takes two Newton steps from the mean-exponent
+    closed form: close to the exact star pressure of two rarefactions."""
+    rng = np.random.default_rng(8)
+    worst, n = 0.0, 0
+    for _ in range(400):
+        gL, gR = rng.uniform(1.2, 1.8, 2)
+        rho_L, p_L, rho_R, p_R = 10 ** rng.uniform(-1, 1, 4)
+        c_L, c_R = np.sqrt(gL * p_L / rho_L), np.sqrt(gR * p_R / rho_R)
+        u_R = rng.uniform(0.0, 3.0) * min(c_L, c_R)
+        us, ps, *_ = oracle.riemann(gL, gR, 0.0, u_R, p_L, p_R, c_L, c_R, 1e-9, 1e-13, 500)
+        if ps >= min(p_L, p_R):
+            continue
+        n += 1
+        _, p_tr = mlp_ref.features(rho_L, 0.0, p_L, rho_R, u_R, p_R, gL, gR)
+        worst = max(worst, abs(p_tr - ps) / ps)
+    assert n > 100
+    assert worst < 5e-4, worst            # two Newton steps from the mean-exponent form
