@@ -3,11 +3,12 @@
 // and the batched interface entry points.  C ABI in include/hydrocode_grp.h.
 //
 // The time loop restates GRP_solver_2D_split_EUL_source
-// (src/finite_volume/grp_solver_2D_split_EUL_source.c:147-341) on the host side
-// of a GPU pipeline: sweeps are enqueued without waiting; each sweep records
-// its failures in a status slot and a sweep whose predecessor failed does
-// nothing, so the host can check every few dozen sweeps and still end exactly
-// where the reference ends.
+// (src/finite_volume/grp_solver_2D_split_EUL_source.c:147-341), or
+// GRP_solver_2D_EUL_source (src/finite_volume/grp_solver_2D_EUL_source.c:147-300)
+// when config[33] is 0, on the host side of a GPU pipeline: sweeps (steps) are
+// enqueued without waiting; each records its failures in a status slot and one
+// whose predecessor failed does nothing, so the host can check every few dozen
+// launches and still end exactly where the reference ends.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include <stdio.h>
@@ -19,6 +20,7 @@
 #include "../../include/hydrocode_grp.h"
 #include "grp_device.hpp"
 #include "sweep2d.hpp"
+#include "unsplit2d.hpp"
 
 namespace hgrp {
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st);
@@ -32,6 +34,8 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
                             double *e, int m, int n, int ld, const TwoGas &two, hipStream_t st);
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
+hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st);
+const char *unsplit_kernel_name();
 }
 
 using namespace hgrp;
@@ -228,8 +232,30 @@ extern "C" int hgrp2d_h_over_smax(hgrp2d *c, double *h)
 
 extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
 {
-    if (!c || (dir != 0 && dir != 1)) return "";
+    if (!c || dir < 0 || dir > 2) return "";
+    if (dir == 2) return unsplit_kernel_name();
     return sweep_kernel_name(dir, c->m, c->n, c->use_mlp);
+}
+
+// h/S_max of the current state: the reference's CFL loop reads CV[0], which is
+// the live state until the first plot frame has been copied off
+// (grp_solver_2D_split_EUL_source.c:183-191 with nt == 0)
+static int refresh_cfl(hgrp2d *c)
+{
+    const double hmin = fmin(c->cfg[10], c->cfg[11]);
+    const unsigned long long inf_bits = 0x7ff0000000000000ull;
+    double *const *st = c->st[c->cs];
+    HCHECK(hipMemcpyAsync(c->cfl_bits, &inf_bits, 8, hipMemcpyHostToDevice, c->stream));
+    if (c->two_fluid)
+        HCHECK(launch_cfl_tf(st[0], st[1], st[2], st[3], c->fa[c->cs], c->m, c->n, c->ld, c->two, hmin, c->cfl_bits,
+                             c->stream));
+    else
+        HCHECK(launch_cfl(st[0], st[1], st[2], st[3], c->m, c->n, c->ld, c->cfg[6], hmin, c->cfl_bits, c->stream));
+    unsigned long long bits = 0;
+    HCHECK(hipMemcpyAsync(&bits, c->cfl_bits, 8, hipMemcpyDeviceToHost, c->stream));
+    HCHECK(hipStreamSynchronize(c->stream));
+    memcpy(&c->h_S_max, &bits, 8);
+    return 0;
 }
 
 extern "C" int hgrp2d_set_fluids(hgrp2d *c, double gamma_b, const double *frac_a)
@@ -302,6 +328,7 @@ struct SweepRec {
     int dir;              // 0 x, 1 y
     int k, DS;            // loop counter and pass kind when it was enqueued
     double time_c;        // time_c when it was enqueued
+    double tau;           // the step's tau
     int in_s, in_d;       // state buffer and slope buffer read
 };
 
@@ -503,10 +530,15 @@ struct Loop {
 };
 }  // namespace
 
+static int solve_unsplit(hgrp2d *c, const double *time_plot, int N_plot, int N_T, hgrp2d_frame_fn fn, void *user,
+                         hgrp2d_result *res);
+
 extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int N_T,
                             hgrp2d_frame_fn fn, void *user, hgrp2d_result *res)
 {
     if (!c || !c->uploaded || !time_plot || N_plot < 2 || N_T < 1 || !res) return ARG_ERROR - 1;
+    if (c->cfg[33] == 0.0)                 // hydrocode.c:264-267: (_Bool)config[33] picks the split solver
+        return solve_unsplit(c, time_plot, N_plot, N_T, fn, user, res);
     const double t_all = c->cfg[1], eps = c->cfg[4], CFL = c->cfg[7];
     const int N = (int)c->cfg[5];
     const double h_x = c->cfg[10], h_y = c->cfg[11];
@@ -534,7 +566,7 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
     auto settle = [&](int r, unsigned long long key, unsigned flags) {
         const SweepRec &R = L.recs[(size_t)r];
         const int code = key ? (int)((key - 1ull) & 3ull) : 0;
-        k = R.k; DS = R.DS; time_c = R.time_c;
+        k = R.k; DS = R.DS; time_c = R.time_c; tau = R.tau;
         L.sweeps += r + 1 - (int)L.recs.size();
         if (code == 1) {                       // ifvar_check failed: goto return_NULL, no update
             printf("<0.0 or non-finite reconstruction in the %c sweep of step %d\n", R.dir ? 'y' : 'x', k);
@@ -544,7 +576,7 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
             if (code == 2) printf("GRP star/derivative error in the %c sweep of step %d\n", R.dir ? 'y' : 'x', k);
             if (flags & 1u) printf("<0.0 error in the update of the %c sweep of step %d\n", R.dir ? 'y' : 'x', k);
             c->cs = 1 - R.in_s;
-            if (R.dir == 1) time_c += tau;
+            if (R.dir == 1) time_c += R.tau;
             status = 2;
         }
         (void)flags;
@@ -566,7 +598,7 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
         }
         const int slot = (int)L.recs.size();
         const int prev = L.recs.empty() ? -1 : slot - 1;
-        L.recs.push_back({dir, k, DS, time_c, c->cs, dir == 0 ? c->cx : c->cy});
+        L.recs.push_back({dir, k, DS, time_c, tau, c->cs, dir == 0 ? c->cx : c->cy});
         ++L.sweeps;
         int rc = enqueue_sweep(c, dir, tf, nu, slot, prev);
         return rc ? rc : 0;
@@ -587,6 +619,13 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
         }
         skip_frame = false;
         if (DS) {                                                             // time step
+            if (nt == 0) {                    // no frame copied off yet: CV[0] is the live state
+                const int f = flush();
+                if (f < 0) return -2;
+                if (f) { ended = true; break; }
+                const int rc = refresh_cfl(c);
+                if (rc) return rc;
+            }
             const double h_S_max = c->h_S_max;
             if (isfinite(t_all) || !isfinite(c->cfg[16]) || c->cfg[16] <= 0.0) {
                 tau = CFL * h_S_max;
@@ -599,7 +638,7 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
                     printf("NAN or INFinite error on [%d, %g, %g] (t_n, time_c, tau) - CFL\n", k, time_c, tau);
                     const int f = flush();
                     if (f < 0) return -2;
-                    status = 1;
+                    if (!f) status = 1;
                     ended = true;
                     break;
                 }
@@ -655,5 +694,233 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
     res->time_c = time_c;
     res->tau = tau;
     res->seconds = secs;
+    return 0;
+}
+
+// ---- the unsplit scheme ----------------------------------------------------------
+// One launch per step; status slots come in pairs ([0] x-interfaces and the
+// update, [1] y-interfaces).
+static constexpr int UNSPLIT_SLOTS = STATUS_SLOTS / 2;
+
+static void fill_unsplit_args(hgrp2d *c, UnsplitArgs &a, double tau, double nu, double mu, int slot, int prev_slot)
+{
+    const int in = c->cs, out = 1 - c->cs;
+    a.rho = c->st[in][0]; a.u = c->st[in][1]; a.v = c->st[in][2]; a.p = c->st[in][3]; a.e = c->st[in][4];
+    a.rho_o = c->st[out][0]; a.u_o = c->st[out][1]; a.v_o = c->st[out][2]; a.p_o = c->st[out][3];
+    a.e_o = c->st[out][4];
+    for (int q = 0; q < 4; ++q) {
+        a.s[q] = c->sx[c->cx][q]; a.s_o[q] = c->sx[1 - c->cx][q];
+        a.t[q] = c->sy[c->cy][q]; a.t_o[q] = c->sy[1 - c->cy][q];
+    }
+    a.bx_lo = c->bnd[0]; a.bx_hi = c->bnd[1]; a.by_lo = c->bnd[2]; a.by_hi = c->bnd[3];
+    a.m = c->m; a.n = c->n; a.ld = c->ld;
+    a.bc_x = (int)c->cfg[17]; a.bc_y = (int)c->cfg[18];
+    a.lim3_x = c->found_x; a.lim3_y = c->found_y;
+    a.hh_x = 0.5 * c->cfg[10]; a.hh_y = 0.5 * c->cfg[11];
+    a.div_hx = recip_div(c->cfg[10]); a.div_hy = recip_div(c->cfg[11]);
+    a.tau = tau; a.nu = nu; a.mu = mu;
+    a.eps = c->cfg[4];
+    a.alpha = c->cfg[41];
+    a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
+    a.gas = make_gas(c->cfg[6]);
+    a.status = c->status + 2 * slot;
+    a.prev = prev_slot >= 0 ? c->status + 2 * prev_slot : nullptr;
+}
+
+static int enqueue_unsplit(hgrp2d *c, double tau, double nu, double mu, int slot, int prev_slot)
+{
+    UnsplitArgs a;
+    fill_unsplit_args(c, a, tau, nu, mu, slot, prev_slot);
+    HCHECK(launch_unsplit_step(a, c->stream));
+    c->cs = 1 - c->cs;
+    c->cx = 1 - c->cx;
+    c->cy = 1 - c->cy;
+    c->found_x = c->found_y = 1;
+    return 0;
+}
+
+static bool unsplit_supported(hgrp2d *c)
+{
+    if (c->two_fluid) {
+        printf("The unsplit scheme (config[33] = 0) runs one gamma-law gas only.\n");
+        return false;
+    }
+    return true;
+}
+
+// GRP_solver_2D_EUL_source (src/finite_volume/grp_solver_2D_EUL_source.c:147-300)
+static int solve_unsplit(hgrp2d *c, const double *time_plot, int N_plot, int N_T, hgrp2d_frame_fn fn, void *user,
+                         hgrp2d_result *res)
+{
+    if (!unsplit_supported(c)) return ARG_ERROR - 4;
+    const double t_all = c->cfg[1], eps = c->cfg[4], CFL = c->cfg[7];
+    const int N = (int)c->cfg[5];
+    const double h_x = c->cfg[10], h_y = c->cfg[11];
+    double tau = c->cfg[16];
+    const int bc_x = (int)c->cfg[17], bc_y = (int)c->cfg[18];
+    const auto t_start = std::chrono::steady_clock::now();
+
+    int k = 1, nt = 0, nt_plot = 0, status = 0, launched = 0;
+    double time_c = 0.0, nu = 0.0, mu = 0.0;
+    bool stop_t = false, skip_frame = false, ended = false;
+    if (c->resume.valid) {
+        const LoopPos &r = c->resume;
+        k = r.k; nt = r.nt; nt_plot = r.nt_plot;
+        time_c = r.time_c; tau = r.tau; nu = r.half_nu; mu = r.mu;
+        skip_frame = true;
+        c->resume.valid = 0;
+        if (nt > N_T - 1) nt = N_T - 1;
+    }
+    HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
+
+    struct Rec { int k; double time_c, tau; int in_s; };
+    std::vector<Rec> recs;
+    // reads the pending steps' status; at the first failed one, restores the
+    // loop state it ran in and applies the reference's rule.  Returns 1 if a
+    // failure ended the run, 0 if none, -2 on a HIP error.
+    auto flush = [&]() -> int {
+        if (recs.empty()) return 0;
+        std::vector<SweepStatus> st(2 * recs.size());
+        if (hipMemcpyAsync(st.data(), c->status, st.size() * sizeof(SweepStatus), hipMemcpyDeviceToHost,
+                           c->stream) != hipSuccess ||
+            hipStreamSynchronize(c->stream) != hipSuccess)
+            return -2;
+        for (size_t r = 0; r < recs.size(); ++r) {
+            const SweepStatus &x = st[2 * r], &y = st[2 * r + 1];
+            if (!x.fail_key && !y.fail_key && !(x.flags & 1u)) continue;
+            const int cx = x.fail_key ? (int)((x.fail_key - 1ull) & 3ull) : 0;
+            const int cy = y.fail_key ? (int)((y.fail_key - 1ull) & 3ull) : 0;
+            const Rec &R = recs[r];
+            k = R.k; time_c = R.time_c; tau = R.tau;
+            launched += (int)r + 1 - (int)recs.size();
+            if (cx == 1 || cy == 1) {                   // flux_generator_{x,y} returned 1: goto return_NULL
+                printf("<0.0 or non-finite reconstruction at the %c-interfaces of step %d\n", cx == 1 ? 'x' : 'y', k);
+                c->cs = R.in_s;
+                status = 1;
+            } else {                                    // GRP or positivity failure: update kept, stop
+                if (cx == 2) printf("GRP star/derivative error at the x-interfaces of step %d\n", k);
+                if (cy == 2) printf("GRP star/derivative error at the y-interfaces of step %d\n", k);
+                if (x.flags & 1u) printf("<0.0 error on step %d - Update\n", k);
+                c->cs = 1 - R.in_s;
+                time_c += R.tau;
+                status = 2;
+            }
+            return 1;
+        }
+        recs.clear();
+        return 0;
+    };
+
+    for (; k <= N; ++k) {
+        if (!skip_frame && time_c >= time_plot[nt_plot] && nt_plot < (N_plot - 1)) {   // plot frame (:157-176)
+            const int f = flush();
+            if (f < 0) return -2;
+            if (f) { ended = true; break; }
+            c->at = {1, k, 1, nt < (N_T - 1) ? nt + 1 : nt, nt_plot + 1, time_c, tau, 0.0, nu, mu};
+            const int stop = fn ? fn(user, nt, nt_plot, time_c) : 0;
+            c->at.valid = 0;
+            if (stop) { status = 1; ended = true; break; }
+            nt_plot++;
+            if (nt < (N_T - 1)) nt++;
+        }
+        skip_frame = false;
+        if (nt == 0) {                     // CV[0] is the live state: its CFL bound moves (:183-191)
+            const int f = flush();
+            if (f < 0) return -2;
+            if (f) { ended = true; break; }
+            const int rc = refresh_cfl(c);
+            if (rc) return rc;
+        }
+        if (isfinite(t_all) || !isfinite(c->cfg[16]) || c->cfg[16] <= 0.0) {          // (:193-207)
+            tau = CFL * c->h_S_max;
+            if (tau < eps) {
+                printf("\nThe length of the time step is so small on [%d, %g, %g] (t_n, time_c, tau)\n", k, time_c, tau);
+                stop_t = true;
+            } else if ((time_c + tau) > (t_all - eps)) {
+                tau = t_all - time_c;
+            } else if (!isfinite(tau)) {
+                printf("NAN or INFinite error on [%d, %g, %g] (t_n, time_c, tau) - CFL\n", k, time_c, tau);
+                const int f = flush();
+                if (f < 0) return -2;
+                if (!f) status = 1;
+                ended = true;
+                break;
+            }
+        }
+        nu = tau / h_x;
+        mu = tau / h_y;
+        if (!valid_bc(bc_x) || !valid_bc(bc_y)) {
+            printf("No suitable boundary coditions in %c direction!\n", valid_bc(bc_x) ? 'y' : 'x');
+            const int f = flush();
+            if (f < 0) return -2;
+            if (!f) status = 1;
+            ended = true;
+            break;
+        }
+        if ((int)recs.size() + 1 > UNSPLIT_SLOTS) {
+            const int f = flush();
+            if (f < 0) return -2;
+            if (f) { ended = true; break; }
+        }
+        const int slot = (int)recs.size();
+        recs.push_back({k, time_c, tau, c->cs});
+        ++launched;
+        const int rc = enqueue_unsplit(c, tau, nu, mu, slot, slot - 1);
+        if (rc) return rc;
+        time_c += tau;
+        if (stop_t || time_c > (t_all - eps) || !isfinite(time_c)) break;
+    }
+    if (!ended) {
+        const int f = flush();
+        if (f < 0) return -2;
+        if (f == 0 && stop_t) status = 2;
+    }
+    HCHECK(hipStreamSynchronize(c->stream));
+    const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    if (status != 1) {
+        printf("\nTime is up at time step %d.\n", k);
+        printf("The cost of CPU time for genuinely 2D-GRP Eulerian scheme without dimension splitting for this problem is %g seconds.\n", secs);
+    }
+    res->k = k;
+    res->nt = nt;
+    res->status = status;
+    res->sweeps = launched;
+    res->time_c = time_c;
+    res->tau = tau;
+    res->seconds = secs;
+    return 0;
+}
+
+extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, double *ms_total, double *ms_step)
+{
+    if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
+    if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
+    if (!unsplit_supported(c)) return ARG_ERROR - 4;
+    const double tau = c->cfg[7] * c->h_S_max;
+    const double nu = tau / c->cfg[10], mu = tau / c->cfg[11];
+    HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
+    std::vector<hipEvent_t> evs((size_t)2 * steps);
+    for (auto &e : evs) HCHECK(hipEventCreate(&e));
+    HCHECK(hipEventRecord(c->ev[0], c->stream));
+    for (int s = 0; s < steps; ++s) {
+        HCHECK(hipEventRecord(evs[(size_t)2 * s], c->stream));
+        // status slots are not chained here: the benchmark must run every step
+        const int rc = enqueue_unsplit(c, tau, nu, mu, s % UNSPLIT_SLOTS, -1);
+        if (rc) return rc;
+        HCHECK(hipEventRecord(evs[(size_t)2 * s + 1], c->stream));
+    }
+    HCHECK(hipEventRecord(c->ev[1], c->stream));
+    HCHECK(hipEventSynchronize(c->ev[1]));
+    float ms = 0.f;
+    HCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
+    if (ms_total) *ms_total = ms;
+    double sum = 0.0;
+    for (int s = 0; s < steps; ++s) {
+        float t = 0.f;
+        HCHECK(hipEventElapsedTime(&t, evs[(size_t)2 * s], evs[(size_t)2 * s + 1]));
+        sum += t;
+    }
+    if (ms_step) *ms_step = sum / steps;
+    for (auto &e : evs) (void)hipEventDestroy(e);
     return 0;
 }

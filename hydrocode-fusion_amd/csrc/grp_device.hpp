@@ -571,6 +571,41 @@ __device__ __forceinline__ int star_dire_bad(const double mid[4], const double d
     return 0;
 }
 
+// Fluxes over tau and interface values at t + tau from the Q1D mid state and
+// its time derivative, rotated back to grid axes (flux_solver.c:190-236);
+// star_dire_check failures return their code with zero F and I.  The normal
+// (n_x, n_y) is (1, 0) or (0, 1); a compile-time constant in the split sweeps.
+__device__ __forceinline__ int flux_from_mid_n(const double mid[4], const double dire[4], const Gas &G, double n_x,
+                                               double n_y, double tau, double eps, double F[4], double I[4])
+{
+    const int bad = star_dire_bad(mid, dire, eps);
+    if (bad) {
+        F[0] = F[1] = F[2] = F[3] = 0.0;
+        I[0] = I[1] = I[2] = I[3] = 0.0;
+        return bad;
+    }
+    const double rho_mid = mid[0] + 0.5 * tau * dire[0];
+    const double u_mid = (mid[1] + 0.5 * tau * dire[1]) * n_x - (mid[2] + 0.5 * tau * dire[2]) * n_y;
+    const double v_mid = (mid[1] + 0.5 * tau * dire[1]) * n_y + (mid[2] + 0.5 * tau * dire[2]) * n_x;
+    const double p_mid = mid[3] + 0.5 * tau * dire[3];
+    F[0] = rho_mid * (u_mid * n_x + v_mid * n_y);
+    F[1] = F[0] * u_mid + p_mid * n_x;
+    F[2] = F[0] * v_mid + p_mid * n_y;
+    F[3] = G.g_gm1 * p_mid / rho_mid + 0.5 * (u_mid * u_mid + v_mid * v_mid);
+    F[3] = F[0] * F[3];
+    I[1] = (mid[1] + tau * dire[1]) * n_x - (mid[2] + tau * dire[2]) * n_y;
+    I[2] = (mid[1] + tau * dire[1]) * n_y + (mid[2] + tau * dire[2]) * n_x;
+    I[0] = mid[0] + tau * dire[0];
+    I[3] = mid[3] + tau * dire[3];
+    return 0;
+}
+template <int DIR>
+__device__ __forceinline__ int flux_from_mid(const double mid[4], const double dire[4], const Gas &G, double tau,
+                                             double eps, double F[4], double I[4])
+{
+    return flux_from_mid_n(mid, dire, G, DIR == 0 ? 1.0 : 0.0, DIR == 0 ? 0.0 : 1.0, tau, eps, F, I);
+}
+
 // GRP_2D_flux for the axis normal DIR (0: x, n = (1,0); 1: y, n = (0,1)) with
 // the split path's zero tangential derivatives (+0.0 in x, -0.0 in y).
 // On success F = (F_rho, F_u, F_v, F_e), I = (rho, u, v, p) at t + tau; on a
@@ -599,26 +634,31 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double 
 
     double dire[4], mid[4];
     grp_q1d(dire, mid, L, R, T, T, G, G, eps, tol, p_guess);
-    const int bad = star_dire_bad(mid, dire, eps);
-    if (bad) {
-        F[0] = F[1] = F[2] = F[3] = 0.0;
-        I[0] = I[1] = I[2] = I[3] = 0.0;
-        return bad;
-    }
-    const double rho_mid = mid[0] + 0.5 * tau * dire[0];
-    const double u_mid = (mid[1] + 0.5 * tau * dire[1]) * n_x - (mid[2] + 0.5 * tau * dire[2]) * n_y;
-    const double v_mid = (mid[1] + 0.5 * tau * dire[1]) * n_y + (mid[2] + 0.5 * tau * dire[2]) * n_x;
-    const double p_mid = mid[3] + 0.5 * tau * dire[3];
-    F[0] = rho_mid * (u_mid * n_x + v_mid * n_y);
-    F[1] = F[0] * u_mid + p_mid * n_x;
-    F[2] = F[0] * v_mid + p_mid * n_y;
-    F[3] = G.g_gm1 * p_mid / rho_mid + 0.5 * (u_mid * u_mid + v_mid * v_mid);
-    F[3] = F[0] * F[3];
-    I[1] = (mid[1] + tau * dire[1]) * n_x - (mid[2] + tau * dire[2]) * n_y;
-    I[2] = (mid[1] + tau * dire[1]) * n_y + (mid[2] + tau * dire[2]) * n_x;
-    I[0] = mid[0] + tau * dire[0];
-    I[3] = mid[3] + tau * dire[3];
-    return 0;
+    return flux_from_mid<DIR>(mid, dire, G, tau, eps, F, I);
+}
+
+// GRP_2D_flux with the transversal derivatives of both sides (the unsplit
+// step; flux_generator_{x,y}.c:96-121 with Transversal = true) and a run-time
+// axis normal (n_x, n_y) = (1, 0) or (0, 1).  TL, TR hold (t_rho, t_u, t_v,
+// t_p) in grid axes; they are rotated with the velocities (flux_solver.c:167-185).
+__device__ __forceinline__ int grp_2d_flux_n(Side L, Side R, const double TL[4], const double TR[4], const Gas &G,
+                                             double n_x, double n_y, double tau, double eps, double F[4], double I[4],
+                                             double tol)
+{
+    const double u = L.u * n_x + L.v * n_y;
+    const double u_R = R.u * n_x + R.v * n_y;
+    const double d_u = L.d_u * n_x + L.d_v * n_y;
+    const double d_u_R = R.d_u * n_x + R.d_v * n_y;
+    const double TLr[4] = {TL[0], TL[1] * n_x + TL[2] * n_y, -TL[1] * n_y + TL[2] * n_x, TL[3]};
+    const double TRr[4] = {TR[0], TR[1] * n_x + TR[2] * n_y, -TR[1] * n_y + TR[2] * n_x, TR[3]};
+    L.v = -L.u * n_y + L.v * n_x;
+    R.v = -R.u * n_y + R.v * n_x;
+    L.d_v = -L.d_u * n_y + L.d_v * n_x;
+    R.d_v = -R.d_u * n_y + R.d_v * n_x;
+    L.u = u; R.u = u_R; L.d_u = d_u; R.d_u = d_u_R;
+    double dire[4], mid[4];
+    grp_q1d(dire, mid, L, R, TLr, TRr, G, G, eps, tol, -1.0);
+    return flux_from_mid_n(mid, dire, G, n_x, n_y, tau, eps, F, I);
 }
 
 // Two gamma-law fluids a and b mixed by the volume fraction a of fluid a:
@@ -709,6 +749,14 @@ __device__ __forceinline__ bool ifvar_bad(const Side &L, const Side &R, double e
     if (L.p < eps || R.p < eps || L.rho < eps || R.rho < eps) return true;
     return !isfinite(L.d_p) || !isfinite(R.d_p) || !isfinite(L.d_u) || !isfinite(R.d_u)
         || !isfinite(L.d_v) || !isfinite(R.d_v) || !isfinite(L.d_rho) || !isfinite(R.d_rho);
+}
+// the same with the transversal derivatives checked too (fluid_var_check.c:42-47)
+__device__ __forceinline__ bool ifvar_bad_t(const Side &L, const Side &R, const double TL[4], const double TR[4],
+                                            double eps)
+{
+    if (ifvar_bad(L, R, eps)) return true;
+    return !isfinite(TL[3]) || !isfinite(TR[3]) || !isfinite(TL[1]) || !isfinite(TR[1])
+        || !isfinite(TL[2]) || !isfinite(TR[2]) || !isfinite(TL[0]) || !isfinite(TR[0]);
 }
 
 }  // namespace hgrp

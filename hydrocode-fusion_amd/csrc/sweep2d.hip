@@ -156,18 +156,6 @@ struct Line {
     }
 };
 
-__device__ __forceinline__ Side recon(const double w[4], const double s[4], double hh, bool left)
-{
-    Side f;
-    if (left) {
-        f.rho = w[0] + hh * s[0]; f.u = w[1] + hh * s[1]; f.v = w[2] + hh * s[2]; f.p = w[3] + hh * s[3];
-    } else {
-        f.rho = w[0] - hh * s[0]; f.u = w[1] - hh * s[1]; f.v = w[2] - hh * s[2]; f.p = w[3] - hh * s[3];
-    }
-    f.d_rho = s[0]; f.d_u = s[1]; f.d_v = s[2]; f.d_p = s[3];
-    return f;
-}
-
 template <int DIR>
 __device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &SL, const Side &SR,
                                                 unsigned long long loop_index, double F[4], double I[4],

@@ -53,4 +53,19 @@ struct SweepArgs {
     const SweepStatus *prev;                  // the sweep before (nullptr for the first)
 };
 
+// Interface state from a cell value w and its slope s along the interface
+// normal: w + h/2 s on the left (lower) side, w - h/2 s on the right (upper)
+// side, with the slope as the normal derivative (flux_generator_x.c:45-86).
+__device__ __forceinline__ Side recon(const double w[4], const double s[4], double hh, bool left)
+{
+    Side f;
+    if (left) {
+        f.rho = w[0] + hh * s[0]; f.u = w[1] + hh * s[1]; f.v = w[2] + hh * s[2]; f.p = w[3] + hh * s[3];
+    } else {
+        f.rho = w[0] - hh * s[0]; f.u = w[1] - hh * s[1]; f.v = w[2] - hh * s[2]; f.p = w[3] - hh * s[3];
+    }
+    f.d_rho = s[0]; f.d_u = s[1]; f.d_v = s[2]; f.d_p = s[3];
+    return f;
+}
+
 }  // namespace hgrp

@@ -1,0 +1,332 @@
+// unsplit2d.hip -- the genuinely 2D GRP step (no dimension splitting), one
+// launch per time step.
+//
+//   unsplit_step_kernel  a block of W waves covers 62 consecutive y-cells
+//                        (lanes: cells i0-1 .. i0+62) of the cell rows
+//                        j0-1 .. j0+W-2 (wave w: row j0-1+w).  Every lane forms
+//                        the "side data" of its cell -- value, limited x-slope,
+//                        limited y-slope, or the ghost's value and slopes at the
+//                        edges -- and parks it in LDS.  Waves 0..W-2 solve the
+//                        x-interface between their row and the next; waves
+//                        1..W-2 also solve the y-interfaces of their row (lane t:
+//                        between lanes t and t+1) and update their 62 cells with
+//                        both flux differences.  Rows j0-1 and j0+W-2 are halo
+//                        rows; interface rows and columns on tile edges are
+//                        solved by both tiles, everything else once.
+//
+// Semantics follow src/finite_volume/grp_solver_2D_EUL_source.c:181-262,
+// src/inter_process/bound_cond_slope_limiter_{x,y}.c:27-129 (including the
+// ghost slopes across directions, :111-126) and
+// src/flux_calc/flux_generator_{x,y}.c:31-165 with Transversal = true.
+#include <hip/hip_runtime.h>
+#include <stdlib.h>
+#include "grp_device.hpp"
+#include "unsplit2d.hpp"
+
+namespace hgrp {
+
+namespace {
+
+// Values, limited slopes and interface side data around one step.  Rows c
+// (x index) run over [-1, m], columns ci (y index) over [-1, n]; -1, m and n
+// are ghost cells.
+struct Grid {
+    const UnsplitArgs &a;
+    __device__ explicit Grid(const UnsplitArgs &args) : a(args) {}
+
+    __device__ __forceinline__ size_t at(int c, int ci) const { return (size_t)c * (size_t)a.ld + (size_t)ci; }
+    __device__ __forceinline__ void cell(int c, int ci, double w[4]) const
+    {
+        const size_t o = at(c, ci);
+        w[0] = a.rho[o]; w[1] = a.u[o]; w[2] = a.v[o]; w[3] = a.p[o];
+    }
+    // value of row c in [-1, m] of column ci in [0, n): x-ghosts per config[17]
+    // (bound_cond_slope_limiter_x.c:34-82)
+    __device__ __forceinline__ void xval(int c, int ci, double w[4]) const
+    {
+        if (c >= 0 && c < a.m) { cell(c, ci, w); return; }
+        const bool lo = c < 0;
+        switch (a.bc_x) {
+        case BC_INITIAL: {
+            const double *b = (lo ? a.bx_lo : a.bx_hi) + (size_t)ci * 4;
+            w[0] = b[0]; w[1] = b[1]; w[2] = b[2]; w[3] = b[3];
+            return;
+        }
+        case BC_PERIODIC: cell(lo ? a.m - 1 : 0, ci, w); return;
+        case BC_REFLECT: cell(lo ? 0 : a.m - 1, ci, w); w[1] = -w[1]; return;
+        case BC_REFLECT_FREE: cell(lo ? 0 : a.m - 1, ci, w); if (lo) w[1] = -w[1]; return;
+        default: cell(lo ? 0 : a.m - 1, ci, w); return;
+        }
+    }
+    // value of column ci in [-1, n] of row c in [0, m): y-ghosts per config[18]
+    // (bound_cond_slope_limiter_y.c:34-82)
+    __device__ __forceinline__ void yval(int c, int ci, double w[4]) const
+    {
+        if (ci >= 0 && ci < a.n) { cell(c, ci, w); return; }
+        const bool lo = ci < 0;
+        switch (a.bc_y) {
+        case BC_INITIAL: {
+            const double *b = (lo ? a.by_lo : a.by_hi) + (size_t)c * 4;
+            w[0] = b[0]; w[1] = b[1]; w[2] = b[2]; w[3] = b[3];
+            return;
+        }
+        case BC_PERIODIC: cell(c, lo ? a.n - 1 : 0, w); return;
+        case BC_REFLECT: cell(c, lo ? 0 : a.n - 1, w); w[2] = -w[2]; return;
+        case BC_REFLECT_FREE: cell(c, lo ? 0 : a.n - 1, w); if (lo) w[2] = -w[2]; return;
+        default: cell(c, lo ? 0 : a.n - 1, w); return;
+        }
+    }
+    // minmod slope from the neighbours (slope_limiter_2D_x.c:32-84, slope_limiter.c:31-83)
+    __device__ __forceinline__ void limit(const double vm[4], const double v0[4], const double vp[4], const double *const prev[4],
+                          size_t o, int lim3, const RecipDiv &d, double s[4]) const
+    {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double s_L = div_by(v0[q] - vm[q], d);
+            const double s_R = div_by(vp[q] - v0[q], d);
+            s[q] = lim3 ? minmod3(a.alpha * s_L, a.alpha * s_R, prev[q][o]) : minmod2(s_L, s_R);
+        }
+    }
+    __device__ __forceinline__ void lim_x(int c, int ci, const double v0[4], double s[4]) const
+    {
+        double vm[4], vp[4];
+        xval(c - 1, ci, vm);
+        xval(c + 1, ci, vp);
+        limit(vm, v0, vp, a.s, at(c, ci), a.lim3_x, a.div_hx, s);
+    }
+    __device__ __forceinline__ void lim_y(int c, int ci, const double v0[4], double t[4]) const
+    {
+        double vm[4], vp[4];
+        yval(c, ci - 1, vm);
+        yval(c, ci + 1, vp);
+        limit(vm, v0, vp, a.t, at(c, ci), a.lim3_y, a.div_hy, t);
+    }
+    __device__ __forceinline__ void lim_x_at(int c, int ci, double s[4]) const
+    {
+        double v0[4];
+        cell(c, ci, v0);
+        lim_x(c, ci, v0, s);
+    }
+    __device__ __forceinline__ void lim_y_at(int c, int ci, double t[4]) const
+    {
+        double v0[4];
+        cell(c, ci, v0);
+        lim_y(c, ci, v0, t);
+    }
+
+    // Side data of (c, ci): value v, x-slope s, y-slope t.  A cell: its limited
+    // slopes.  An x-ghost (c = -1 or m): bfv_L/R -- ghost value, the x-slope
+    // rule of bound_cond_slope_limiter_x.c:94-109 and, across, the y-slopes of
+    // the boundary cell (bound_cond_slope_limiter_y.c:111-126).  A y-ghost
+    // (ci = -1 or n): bfv_D/U with the mirrored rules.  Slopes the reference
+    // never sets stay zero (its ghost arrays are calloc'ed).  Not for corners.
+    __device__ __forceinline__ void side(int c, int ci, double v[4], double s[4], double t[4]) const
+    {
+        const bool xin = c >= 0 && c < a.m, yin = ci >= 0 && ci < a.n;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { s[q] = 0.0; t[q] = 0.0; }
+        if (xin && yin) {
+            cell(c, ci, v);
+            lim_x(c, ci, v, s);
+            lim_y(c, ci, v, t);
+        } else if (yin) {
+            const bool lo = c < 0;
+            const int near = lo ? 0 : a.m - 1, far = lo ? a.m - 1 : 0;
+            xval(c, ci, v);
+            double q4[4];
+            switch (a.bc_x) {
+            case BC_REFLECT: lim_x_at(near, ci, q4); s[1] = q4[1]; break;
+            case BC_PERIODIC: lim_x_at(far, ci, s); break;
+            case BC_REFLECT_FREE: if (lo) { lim_x_at(0, ci, q4); s[1] = q4[1]; } break;
+            default: break;
+            }
+            switch (a.bc_x) {
+            case BC_REFLECT: case BC_FREE: case BC_REFLECT_FREE: lim_y_at(near, ci, t); break;
+            case BC_PERIODIC: lim_y_at(far, ci, t); break;
+            default: break;
+            }
+        } else {
+            const bool lo = ci < 0;
+            const int near = lo ? 0 : a.n - 1, far = lo ? a.n - 1 : 0;
+            yval(c, ci, v);
+            double q4[4];
+            switch (a.bc_y) {
+            case BC_REFLECT: lim_y_at(c, near, q4); t[2] = q4[2]; break;
+            case BC_PERIODIC: lim_y_at(c, far, t); break;
+            case BC_REFLECT_FREE: if (lo) { lim_y_at(c, 0, q4); t[2] = q4[2]; } break;
+            default: break;
+            }
+            switch (a.bc_y) {
+            case BC_REFLECT: case BC_FREE: case BC_REFLECT_FREE: lim_x_at(c, near, s); break;
+            case BC_PERIODIC: lim_x_at(c, far, s); break;
+            default: break;
+            }
+        }
+    }
+};
+
+// GRP solve of one interface with transversal derivatives, normal x (dir 0)
+// or y (dir 1); failures are recorded in status[dir] as
+// ((loop index << 2) | code) + 1, the reference's loop order
+__device__ __forceinline__ void solve_t(const UnsplitArgs &a, int dir, const Side &SL, const Side &SR,
+                                        const double TL[4], const double TR[4], unsigned long long loop_index,
+                                        double F[4], double I[4])
+{
+    int code = ifvar_bad_t(SL, SR, TL, TR, a.eps) ? 1 : 0;
+    if (grp_2d_flux_n(SL, SR, TL, TR, a.gas, dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, a.tau, a.eps, F, I, a.tol)) code = 2;
+    if (code) atomicMax(&a.status[dir].fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
+}
+
+}  // namespace
+
+template <int W, int MINW>
+__global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs a)
+{
+    __shared__ double sd[W][12][64];   // side data of row c: value, x-slope, y-slope
+    __shared__ double fx[W - 1][8][64];  // x-interface c+1: flux, values at t + tau
+    if (a.prev != nullptr) {           // a failed step ends the run: later steps do nothing
+        const SweepStatus p0 = a.prev[0], p1 = a.prev[1];
+        if (p0.fail_key != 0ull || p0.flags != 0u || p1.fail_key != 0ull) {
+            if (threadIdx.x == 0) atomicOr(&a.status[0].flags, 2u);
+            return;
+        }
+    }
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int m = a.m, n = a.n;
+    const int i0 = blockIdx.x * 62, ci = i0 - 1 + lane;
+    const int j0 = blockIdx.y * (W - 2), c = j0 - 1 + w;
+    const Grid G(a);
+    const bool xin = c >= 0 && c < m, yin = ci >= 0 && ci < n;
+    {
+        double v[4] = {1.0, 0.0, 0.0, 1.0}, s[4] = {0.0, 0.0, 0.0, 0.0}, t[4] = {0.0, 0.0, 0.0, 0.0};
+        if (c <= m && ci <= n && (xin || yin)) G.side(c, ci, v, s, t);   // a cell or a ghost, not a corner
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { sd[w][q][lane] = v[q]; sd[w][4 + q][lane] = s[q]; sd[w][8 + q][lane] = t[q]; }
+    }
+    __syncthreads();
+
+    // pass 0: x-interface f = c + 1 between rows c and c + 1 (flux_generator_x.c:42-121),
+    //         the right side from the next wave;
+    // pass 1: y-interface ci + 1 between lanes t and t + 1 of row c (flux_generator_y.c:42-121),
+    //         normal derivative = y-slope, transversal = -(x-slope).
+    // One solve site for both, so the GRP code and its registers are laid out once.
+    const int f = c + 1;
+    double Gy[4], Iy[4];
+#pragma unroll 1
+    for (int pass = 0; pass < 2; ++pass) {
+        Side SL, SR;
+        double TL[4], TR[4];
+        unsigned long long idx = 0;
+        bool active;
+        if (pass == 0) {
+            active = w <= W - 2 && lane >= 1 && lane <= 62 && yin && f <= m;
+            if (active) {
+                double vl[4], sl[4], vr[4], sr[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    vl[q] = sd[w][q][lane];     sl[q] = sd[w][4 + q][lane];     TL[q] = sd[w][8 + q][lane];
+                    vr[q] = sd[w + 1][q][lane]; sr[q] = sd[w + 1][4 + q][lane]; TR[q] = sd[w + 1][8 + q][lane];
+                }
+                SL = recon(vl, sl, a.hh_x, true);
+                SR = recon(vr, sr, a.hh_x, false);
+                idx = (unsigned long long)ci * (unsigned long long)(m + 1) + (unsigned long long)f;
+            }
+        } else {
+            active = w >= 1 && w <= W - 2 && xin && lane <= 62 && ci <= n - 1;
+            if (active) {
+                double vd[4], td[4], vu[4], tu[4];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    vd[q] = sd[w][q][lane];     td[q] = sd[w][8 + q][lane];     TL[q] = -sd[w][4 + q][lane];
+                    vu[q] = sd[w][q][lane + 1]; tu[q] = sd[w][8 + q][lane + 1]; TR[q] = -sd[w][4 + q][lane + 1];
+                }
+                SL = recon(vd, td, a.hh_y, true);
+                SR = recon(vu, tu, a.hh_y, false);
+                idx = (unsigned long long)c * (unsigned long long)(n + 1) + (unsigned long long)(ci + 1);
+            }
+        }
+        double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
+        if (active) solve_t(a, pass, SL, SR, TL, TR, idx, F, I);
+        if (pass == 0 && w <= W - 2) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { Gy[q] = F[q]; Iy[q] = I[q]; }
+    }
+    double Gl[4], Iyl[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) { Gl[q] = __shfl_up(Gy[q], 1); Iyl[q] = __shfl_up(Iy[q], 1); }
+    __syncthreads();
+
+    // combined update and slope refresh (grp_solver_2D_EUL_source.c:231-262)
+    if (w >= 1 && w <= W - 2 && xin && yin && lane >= 1 && lane <= 62) {
+        const size_t o = G.at(c, ci);
+        const double e_c = a.e[o];
+        double vc[4], Fl[4], Ixl[4], Fr[4], Ixr[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            vc[q] = sd[w][q][lane];
+            Fl[q] = fx[w - 1][q][lane]; Ixl[q] = fx[w - 1][4 + q][lane];
+            Fr[q] = fx[w][q][lane];     Ixr[q] = fx[w][4 + q][lane];
+        }
+        const double nu = a.nu, mu = a.mu;
+        const double mom_x = vc[0] * vc[1] - nu * (Fr[1] - Fl[1]) - mu * (Gy[1] - Gl[1]);
+        const double mom_y = vc[0] * vc[2] - nu * (Fr[2] - Fl[2]) - mu * (Gy[2] - Gl[2]);
+        const double ene = vc[0] * e_c - nu * (Fr[3] - Fl[3]) - mu * (Gy[3] - Gl[3]);
+        const double rho = vc[0] + (-nu * (Fr[0] - Fl[0]) - mu * (Gy[0] - Gl[0]));
+        const double uu = mom_x / rho;
+        const double vv = mom_y / rho;
+        const double p = (ene - 0.5 * mom_x * uu - 0.5 * mom_y * vv) * a.gas.gm1;
+        a.rho_o[o] = rho; a.u_o[o] = uu; a.v_o[o] = vv; a.e_o[o] = ene / rho; a.p_o[o] = p;
+        if (p < a.eps || rho < a.eps) atomicOr(&a.status[0].flags, 1u);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            a.s_o[q][o] = div_by(Ixr[q] - Ixl[q], a.div_hx);
+            a.t_o[q][o] = div_by(Iy[q] - Iyl[q], a.div_hy);
+        }
+    }
+}
+
+// ---- launchers -----------------------------------------------------------------
+// HGRP_UNSPLIT picks the block shape (read once, for tuning runs):
+//   1 = 8 waves (6 rows) per block, 4 waves/SIMD (default);
+//   2 = 4 waves (2 rows), 4 waves/SIMD;  3 = 8 waves, free register budget
+static int unsplit_variant()
+{
+    static int v = -1;
+    if (v < 0) {
+        const char *e = getenv("HGRP_UNSPLIT");
+        v = e ? atoi(e) : 0;
+        if (v < 1 || v > 3) v = 1;
+    }
+    return v;
+}
+
+// rows of cells one block updates
+static int unsplit_rows() { return unsplit_variant() == 2 ? 2 : 6; }
+
+int unsplit_max_rows() { return 65535 * unsplit_rows(); }
+
+hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
+{
+    const int rows = unsplit_rows();
+    const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + rows - 1) / rows));
+    switch (unsplit_variant()) {
+    case 2: hipLaunchKernelGGL((unsplit_step_kernel<4, 4>), grid, dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((unsplit_step_kernel<8, 1>), grid, dim3(512), 0, st, a); break;
+    default: hipLaunchKernelGGL((unsplit_step_kernel<8, 4>), grid, dim3(512), 0, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+const char *unsplit_kernel_name()
+{
+    switch (unsplit_variant()) {
+    case 2: return "unsplit_step_kernel<4, 4>";
+    case 3: return "unsplit_step_kernel<8, 1>";
+    default: return "unsplit_step_kernel<8, 4>";
+    }
+}
+
+}  // namespace hgrp

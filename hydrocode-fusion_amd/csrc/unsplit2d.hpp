@@ -1,0 +1,37 @@
+// unsplit2d.hpp -- one fused launch per time step of the genuinely 2D GRP
+// scheme (no dimension splitting).
+//
+// One launch performs everything the reference's unsplit step spreads over
+// bound_cond_slope_limiter_x + bound_cond_slope_limiter_y + flux_generator_x +
+// flux_generator_y (Transversal = true) + the update loop of
+// GRP_solver_2D_EUL_source (src/finite_volume/grp_solver_2D_EUL_source.c:209-262):
+// ghost values, both minmod limiters with the ghost-slope rules across
+// directions, x- and y-interface GRP solves over tau with the transversal
+// derivatives of the cells beside each interface, the combined conservative
+// update and the refresh of both slope sets from the interface values at
+// t + tau.  State and slopes are read from one buffer set and written to the
+// other (ping-pong), as in the split sweeps.
+#pragma once
+#include "sweep2d.hpp"
+
+namespace hgrp {
+
+struct UnsplitArgs {
+    const double *rho, *u, *v, *p, *e;        // state in
+    double *rho_o, *u_o, *v_o, *p_o, *e_o;    // state out
+    const double *s[4], *t[4];                // x- and y-slopes (rho u v p) of the previous step
+    double *s_o[4], *t_o[4];                  // refreshed slopes, out
+    const double *bx_lo, *bx_hi;              // BC -1 ghosts of the x-ends from the initial data: [i*4 + q]
+    const double *by_lo, *by_hi;              // ... of the y-ends: [j*4 + q]
+    int m, n, ld;                             // cells in x, in y, row pitch
+    int bc_x, bc_y;                           // config[17], config[18]
+    int lim3_x, lim3_y;                       // 0: minmod2 (first step), 1: minmod3 with the previous slope
+    double hh_x, hh_y;                        // h/2
+    double tau, nu, mu, eps, alpha, tol;      // tau, tau/h_x, tau/h_y, ...
+    RecipDiv div_hx, div_hy;                  // division by h_x, h_y
+    Gas gas;
+    SweepStatus *status;                      // [0]: x-interfaces and the update, [1]: y-interfaces
+    const SweepStatus *prev;                  // the previous step's pair (nullptr for the first)
+};
+
+}  // namespace hgrp

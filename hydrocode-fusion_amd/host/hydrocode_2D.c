@@ -6,9 +6,9 @@
  *
  *     hydrocode_2D.out <in-case> <out-case> <order>[_<scheme>] EUL [n=C ...]
  *
- * run from a directory two levels below data_in/ and data_out/.  The split
- * scheme (config[33] = 1) runs on the GPU; the unsplit scheme is not part of
- * this build and is reported as such (exit code 4).
+ * run from a directory two levels below data_in/ and data_out/.  config[33]
+ * picks the scheme as in the reference: 1 the dimension-split solver, 0 (the
+ * default) the unsplit one; both run on the GPU.
  * Exit codes: 1 directory, 2 data, 4 arguments, 5 memory.
  */
 #include <math.h>
@@ -128,12 +128,10 @@ int main(int argc, char *argv[])
         retval = 4;
         goto cleanup;
     }
-    if (!dim_split) {
-        printf("The unsplit 2D GRP scheme is not part of this GPU build; run with 33=1 (dimension splitting).\n");
-        retval = 4;
-        goto cleanup;
-    }
-    GRP_solver_2D_split_EUL_source(n_x, n_y, CV, X, Y, cpu_time, argv[2], N, &N_plot, time_plot);
+    if (dim_split)       /* hydrocode.c:264-267 */
+        GRP_solver_2D_split_EUL_source(n_x, n_y, CV, X, Y, cpu_time, argv[2], N, &N_plot, time_plot);
+    else
+        GRP_solver_2D_EUL_source(n_x, n_y, CV, X, Y, cpu_time, argv[2], N, &N_plot, time_plot);
 
 #ifndef NODATPLOT
     file_2D_write(n_x, n_y, N_plot, CV, X, Y, cpu_time, argv[2], time_plot);

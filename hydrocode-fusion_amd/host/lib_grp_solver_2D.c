@@ -1,14 +1,15 @@
 /*
- * lib_grp_solver_2D_split.c -- GRP_solver_2D_split_EUL_source on the GPU.
+ * lib_grp_solver_2D.c -- GRP_solver_2D_split_EUL_source and
+ * GRP_solver_2D_EUL_source on the GPU.
  *
- * Drop-in for src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379: same
- * signature, same use of config[] (reads 1, 4, 5, 6, 7, 10, 11, 16, 17, 18, 41;
- * writes config[5] = k on exit), same CV frames, N_plot and time_plot results
- * and the same Tecplot frames at plot times.  The time loop runs on one GPU
- * through the C ABI of hydrocode_grp.h; this file only moves the caller's row
- * pointers in and out.  The scratch members of CV[0] (slopes, fluxes,
- * interface values) are never allocated and stay NULL, as the reference leaves
- * them on return.
+ * Drop-ins for src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379 and
+ * src/finite_volume/grp_solver_2D_EUL_source.c:75-330: same signatures, same
+ * use of config[] (reads 1, 4, 5, 6, 7, 10, 11, 16, 17, 18, 41; writes
+ * config[5] = k on exit), same CV frames, N_plot and time_plot results and the
+ * same Tecplot frames at plot times.  The time loop runs on one GPU through
+ * the C ABI of hydrocode_grp.h; this file only moves the caller's row pointers
+ * in and out.  The scratch members of CV[0] (slopes, fluxes, interface values)
+ * are never allocated and stay NULL, as the reference leaves them on return.
  */
 #include <math.h>
 #include <stdio.h>
@@ -76,9 +77,9 @@ static const char *bc_name(int bc)
     }
 }
 
-void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
-                                    double *cpu_time, const char *problem, int N_T, int *N_plot,
-                                    double time_plot[])
+/* split = 1: the dimension-split scheme, 0: the unsplit one (config[33] of the context) */
+static void run_2d(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y, double *cpu_time,
+                   const char *problem, int N_T, int *N_plot, double time_plot[], int split)
 {
     char dev[256];
     hgrp_device_name(dev, (int)sizeof dev);
@@ -97,7 +98,12 @@ void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_st
         printf("NOT enough memory! host staging buffer\n");
         goto done;
     }
-    rc = hgrp2d_create(&f.ctx, m, n, config);
+    {
+        double cfg[HGRP_N_CONF];
+        memcpy(cfg, config, sizeof cfg);
+        cfg[33] = split ? 1.0 : 0.0;
+        rc = hgrp2d_create(&f.ctx, m, n, cfg);
+    }
     if (rc) {
         printf("GPU solver set-up failed (%d)\n", rc);
         goto done;
@@ -133,4 +139,17 @@ done:
     if (f.ctx)
         hgrp2d_destroy(f.ctx);
     free(f.buf);
+}
+
+void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
+                                    double *cpu_time, const char *problem, int N_T, int *N_plot,
+                                    double time_plot[])
+{
+    run_2d(m, n, CV, X, Y, cpu_time, problem, N_T, N_plot, time_plot, 1);
+}
+
+void GRP_solver_2D_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
+                              double *cpu_time, const char *problem, int N_T, int *N_plot, double time_plot[])
+{
+    run_2d(m, n, CV, X, Y, cpu_time, problem, N_T, N_plot, time_plot, 0);
 }

@@ -41,6 +41,7 @@ def lib():
     L.hgrp2d_download.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
     L.hgrp2d_solve.argtypes = [C.c_void_p, _DP, C.c_int, C.c_int, FRAME_FN, C.c_void_p, C.POINTER(Result)]
     L.hgrp2d_bench.argtypes = [C.c_void_p, C.c_int, _DP, _DP, _DP]
+    L.hgrp2d_bench_unsplit.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
     L.hgrp2d_h_over_smax.argtypes = [C.c_void_p, _DP]
     L.hgrp_grp2d_flux_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _DP]
     L.hgrp_riemann_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
@@ -184,11 +185,30 @@ class Solver2D:
         _check(lib().hgrp2d_bench(self._h, steps, C.byref(t), C.byref(x), C.byref(y)), "hgrp2d_bench")
         return t.value, x.value, y.value
 
+    def bench_unsplit(self, steps):
+        """(ms of all steps, average ms of one unsplit step launch)."""
+        t, s = _D(), _D()
+        _check(lib().hgrp2d_bench_unsplit(self._h, steps, C.byref(t), C.byref(s)), "hgrp2d_bench_unsplit")
+        return t.value, s.value
+
 
 def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None):
     """Whole run like the oracle's split2d(): returns (frames, N_plot, time_plot, config, status).
     newton_tol / guess_params: see Solver2D.set_newton (defaults: the reference's solve)."""
     config = np.array(config, dtype=np.float64)
+    config[33] = 1.0
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params)
+
+
+def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0):
+    """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0), like the
+    oracle's unsplit2d(); results as run_split2d."""
+    config = np.array(config, dtype=np.float64)
+    config[33] = 0.0
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, None)
+
+
+def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params):
     if time_plot is None:
         time_plot = np.array([0.0, config[1]])
     time_plot = np.array(time_plot, dtype=np.float64)

@@ -25,6 +25,11 @@ void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_st
                                     double *cpu_time, const char *problem, int N_T, int *N_plot,
                                     double time_plot[]);
 
+/* replaces GRP_solver_2D_EUL_source, src/include/finite_volume.h:43-44
+ * (src/finite_volume/grp_solver_2D_EUL_source.c:75-330): the unsplit scheme */
+void GRP_solver_2D_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
+                              double *cpu_time, const char *problem, int N_T, int *N_plot, double time_plot[]);
+
 /* replaces the 2D program's file I/O, src/include/file_io.h:13-54 */
 void example_io(const char *example, char *add_mkdir, const int i_or_o);        /* io_control.c:40-115   */
 int flu_var_count(FILE *fp, const char *add);                                    /* io_control.c:124-153  */

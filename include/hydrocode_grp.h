@@ -27,9 +27,12 @@ extern "C" {
 #define HGRP_N_CONF 400
 
 /* ------------------------------------------------------------------------- */
-/* 2D dimension-split Eulerian GRP solver resident on one GPU.                */
-/* Implements GRP_solver_2D_split_EUL_source                                  */
-/* (src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379) on device      */
+/* 2D Eulerian GRP solver resident on one GPU.  Implements                    */
+/* GRP_solver_2D_split_EUL_source                                             */
+/* (src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379) and, when      */
+/* config[33] is 0, the unsplit GRP_solver_2D_EUL_source                      */
+/* (src/finite_volume/grp_solver_2D_EUL_source.c:75-300) -- the choice the    */
+/* reference's main makes (src/hydrocode_2D/hydrocode.c:264-267) -- on device */
 /* buffers.  Host arrays handed in/out are contiguous m*n doubles with cell   */
 /* (j, i) at j*n + i, the reference's CV[k].X[j][i] with its rows packed.     */
 /* ------------------------------------------------------------------------- */
@@ -53,7 +56,8 @@ typedef int (*hgrp2d_frame_fn)(void *user, int nt, int nt_plot, double time_c);
 
 /* config: the reference's config[] (HGRP_N_CONF doubles, src/include/var_struc.h:44);
  * indices read: 1 t_all, 4 eps, 5 max steps, 6 gamma, 7 CFL, 10/11 h_x/h_y,
- * 16 fixed tau, 17/18 boundary conditions, 41 minmod alpha. */
+ * 16 fixed tau, 17/18 boundary conditions, 33 split (0: unsplit scheme),
+ * 41 minmod alpha. */
 int hgrp2d_create(hgrp2d **ctx, int m, int n, const double *config);
 int hgrp2d_destroy(hgrp2d *ctx);
 
@@ -64,9 +68,12 @@ int hgrp2d_upload(hgrp2d *ctx, const double *rho, const double *u, const double 
 /* Current state (rho, u, v, p, E); any pointer may be NULL. */
 int hgrp2d_download(hgrp2d *ctx, double *rho, double *u, double *v, double *p, double *e);
 
-/* The reference's whole time loop: frozen-CFL tau, X(tau/2) Y(tau) X(tau/2)
- * schedule, plot frames at time_plot[0..N_plot-1] (N_T frames kept), stop and
- * abort rules.  fn may be NULL. */
+/* The reference's whole time loop: CFL tau from frame 0, X(tau/2) Y(tau)
+ * X(tau/2) schedule (split) or one combined step per tau (config[33] = 0,
+ * unsplit: both limiters, x- and y-interfaces with transversal derivatives, one
+ * update), plot frames at time_plot[0..N_plot-1] (N_T frames kept), stop and
+ * abort rules.  fn may be NULL.  For the unsplit scheme res->sweeps counts
+ * steps. */
 int hgrp2d_solve(hgrp2d *ctx, const double *time_plot, int N_plot, int N_T,
                  hgrp2d_frame_fn fn, void *user, hgrp2d_result *res);
 
@@ -87,6 +94,11 @@ int hgrp2d_restore(hgrp2d *ctx, const void *buf, size_t bytes);
  * ms_total: device time of all steps (HIP events on the solver's stream);
  * ms_x, ms_y (may be NULL): average device time of one X / one Y sweep launch. */
 int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double *ms_y);
+
+/* Benchmark form of the unsplit scheme: `steps` steps with tau = CFL * h/S_max.
+ * ms_total: device time of all steps; ms_step (may be NULL): average device
+ * time of one step launch (unsplit_step_kernel). */
+int hgrp2d_bench_unsplit(hgrp2d *ctx, int steps, double *ms_total, double *ms_step);
 
 /* h/S_max of the uploaded initial data (the frozen CFL reduction). */
 int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
@@ -111,7 +123,8 @@ int hgrp2d_download_fraction(hgrp2d *ctx, double *frac_a);
 int hgrp2d_set_newton(hgrp2d *ctx, double tol, const float *params);
 
 /* Name of the kernel the context launches for an x (dir 0) or y (dir 1) sweep
- * (it depends on the grid size); "" for a bad argument.  For profiling tools. */
+ * (it depends on the grid size), or for an unsplit step (dir 2); "" for a bad
+ * argument.  For profiling tools. */
 const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
 
 /* ------------------------------------------------------------------------- */

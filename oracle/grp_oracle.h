@@ -93,6 +93,14 @@ int orc_split2d(int m, int n, double *const *rho, double *const *u, double *cons
                 double *const *p, double *const *e, int N_T, int *N_plot,
                 double *time_plot, double *cpu_time, double *config);
 
+/* GRP_solver_2D_EUL_source (src/finite_volume/grp_solver_2D_EUL_source.c:75-300):
+ * the unsplit step (both limiters, x and y fluxes over tau with transversal
+ * derivatives, one combined update).  Arguments, config[] use and return
+ * values as orc_split2d. */
+int orc_unsplit2d(int m, int n, double *const *rho, double *const *u, double *const *v,
+                  double *const *p, double *const *e, int N_T, int *N_plot,
+                  double *time_plot, double *cpu_time, double *config);
+
 /* linear_GRP_solver_Edir (src/riemann_solver/linear_grp_solver_Edir.c:33-337):
  * D[3], U[3] = (rho, u, p). */
 void orc_grp_edir(double D[3], double U[3], const orc_ifv *L, const orc_ifv *R,

@@ -94,10 +94,10 @@ void ref_grp_edir(const double *Ls, const double *Rs, double eps, double atc, do
     for (int q = 0; q < 3; ++q) { out[q] = D[q]; out[3 + q] = U[q]; }
 }
 
-/* Runs GRP_solver_2D_split_EUL_source.  fields: 5*N_T blocks of m*n doubles,
- * order rho u v p e, frame-major ([f][k][j*n+i]); frame 0 = initial data.
- * config[] must be prepared by the caller (ref_config_*). */
-void ref_split2d(int m, int n, double *fields, int N_T, int *N_plot, double *time_plot)
+/* Runs one of the 2D solvers.  fields: 5*N_T blocks of m*n doubles, order
+ * rho u v p e, frame-major ([f][k][j*n+i]); frame 0 = initial data.  config[]
+ * must be prepared by the caller (ref_config_*). */
+static void ref_2d(int m, int n, double *fields, int N_T, int *N_plot, double *time_plot, int split)
 {
     struct cell_var_stru *CV = calloc((size_t)N_T, sizeof *CV);
     double **rows = malloc((size_t)5 * N_T * m * sizeof(double *));
@@ -121,9 +121,24 @@ void ref_split2d(int m, int n, double *fields, int N_T, int *N_plot, double *tim
         for (int i = 0; i <= n; ++i) { X[j][i] = j * config[10]; Y[j][i] = i * config[11]; }
     }
     double *cpu_time = calloc((size_t)N_T, sizeof(double));
-    GRP_solver_2D_split_EUL_source(m, n, CV, X, Y, cpu_time, "glue", N_T, N_plot, time_plot);
+    if (split)
+        GRP_solver_2D_split_EUL_source(m, n, CV, X, Y, cpu_time, "glue", N_T, N_plot, time_plot);
+    else
+        GRP_solver_2D_EUL_source(m, n, CV, X, Y, cpu_time, "glue", N_T, N_plot, time_plot);
     for (int j = 0; j <= m; ++j) { free(X[j]); free(Y[j]); }
     free(X); free(Y); free(cpu_time); free(rows); free(CV);
+}
+
+/* Runs GRP_solver_2D_split_EUL_source. */
+void ref_split2d(int m, int n, double *fields, int N_T, int *N_plot, double *time_plot)
+{
+    ref_2d(m, n, fields, N_T, N_plot, time_plot, 1);
+}
+
+/* Runs GRP_solver_2D_EUL_source (the unsplit step); arguments as ref_split2d. */
+void ref_unsplit2d(int m, int n, double *fields, int N_T, int *N_plot, double *time_plot)
+{
+    ref_2d(m, n, fields, N_T, N_plot, time_plot, 0);
 }
 
 /* Runs GRP_solver_EUL_source (1D).  fields: 4*N_T blocks of m doubles (rho u p e). */

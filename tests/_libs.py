@@ -77,8 +77,9 @@ class Oracle:
         L.orc_grp_edir.restype = None
         L.orc_grp_edir.argtypes = [DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), D, D]
         PP = C.POINTER(DP)
-        L.orc_split2d.restype = C.c_int
-        L.orc_split2d.argtypes = [C.c_int, C.c_int, PP, PP, PP, PP, PP, C.c_int, IP, DP, DP, DP]
+        for fn in (L.orc_split2d, L.orc_unsplit2d):
+            fn.restype = C.c_int
+            fn.argtypes = [C.c_int, C.c_int, PP, PP, PP, PP, PP, C.c_int, IP, DP, DP, DP]
         L.orc_grp1d.restype = C.c_int
         L.orc_grp1d.argtypes = [C.c_int, PP, PP, PP, PP, IP, DP, DP, DP]
         L.orc_pow_many.argtypes = [DP, DP, C.c_int, DP]
@@ -125,6 +126,13 @@ class Oracle:
     def split2d(self, m, n, init, config, time_plot=None, N_T=None):
         """init: dict rho,u,v,p (m x n arrays, [j, i]), optionally e (else formed as the
         reference's main does); returns (frames, N_plot, time_plot, config, status)."""
+        return self._run2d(self.lib.orc_split2d, m, n, init, config, time_plot, N_T)
+
+    def unsplit2d(self, m, n, init, config, time_plot=None, N_T=None):
+        """GRP_solver_2D_EUL_source (the unsplit step); arguments and results as split2d."""
+        return self._run2d(self.lib.orc_unsplit2d, m, n, init, config, time_plot, N_T)
+
+    def _run2d(self, fn, m, n, init, config, time_plot, N_T):
         if time_plot is None:
             time_plot = np.array([0.0, config[1]])
         time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
@@ -140,7 +148,7 @@ class Oracle:
         cfg = np.array(config, dtype=np.float64)
         npl = C.c_int(N_plot)
         cpu = np.zeros(N_T)
-        st = self.lib.orc_split2d(m, n, *ptrs, N_T, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
+        st = fn(m, n, *ptrs, N_T, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
         return frames, npl.value, time_plot, cfg, st
 
     def grp1d(self, m, init, config, N_T=2, time_plot=None):
@@ -185,6 +193,7 @@ class RefLib:
         L.ref_grp_2d_flux.restype = C.c_int
         L.ref_grp_edir.argtypes = [DP, DP, D, D, DP]
         L.ref_split2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
+        L.ref_unsplit2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
         L.ref_grp1d.argtypes = [C.c_int, DP, C.c_int, IP, DP]
 
     def set_config(self, cfg):
@@ -227,6 +236,12 @@ class RefLib:
         return out
 
     def split2d(self, m, n, init, config, time_plot=None, N_T=None):
+        return self._run2d(self.lib.ref_split2d, m, n, init, config, time_plot, N_T)
+
+    def unsplit2d(self, m, n, init, config, time_plot=None, N_T=None):
+        return self._run2d(self.lib.ref_unsplit2d, m, n, init, config, time_plot, N_T)
+
+    def _run2d(self, fn, m, n, init, config, time_plot, N_T):
         if time_plot is None:
             time_plot = np.array([0.0, config[1]])
         time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
@@ -240,7 +255,7 @@ class RefLib:
         frames[4, 0] = initial_energy(r, u, v, p, gamma)
         self.set_config(config)
         npl = C.c_int(N_plot)
-        self.lib.ref_split2d(m, n, dptr(frames), N_T, C.byref(npl), dptr(time_plot))
+        fn(m, n, dptr(frames), N_T, C.byref(npl), dptr(time_plot))
         cfg = np.array([self.lib.ref_config_get(k) for k in range(N_CONF)])
         return frames, npl.value, time_plot, cfg
 

@@ -32,9 +32,19 @@ def four_quadrant(m, n):
     return out
 
 
-def four_quadrant_config(m, n, t_all=0.3, **kw):
+def four_quadrant_config(m, n, t_all=0.3, split=1, **kw):
     from _libs import make_config
-    return make_config(t_all=t_all, h_x=1.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-4, split=1, **kw)
+    return make_config(t_all=t_all, h_x=1.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-4, split=split, **kw)
+
+
+def blast_box(m, n):
+    """High-pressure disc (p = 10 inside r^2 < 0.02 around (0.4, 0.55), 0.1 outside)
+    in a unit box of gas at rest with rho = 1."""
+    x = (np.arange(m)[:, None] + 0.5) / m
+    y = (np.arange(n)[None, :] + 0.5) / n
+    r2 = (x - 0.4) ** 2 + (y - 0.55) ** 2
+    return {"rho": np.ones((m, n)), "u": np.zeros((m, n)), "v": np.zeros((m, n)),
+            "p": np.where(r2 < 0.02, 10.0, 0.1)}
 
 
 def smooth_random(m, n, seed, amp=0.2):

@@ -51,6 +51,11 @@ def golden_2d():
 
 
 @pytest.fixture(scope="session")
+def golden_unsplit():
+    return load_golden("unsplit2d_cases.npz")
+
+
+@pytest.fixture(scope="session")
 def golden_1d():
     return load_golden("grp1d_cases.npz")
 

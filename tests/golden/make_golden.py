@@ -113,6 +113,74 @@ def split2d_cases(ref):
     np.savez_compressed(os.path.join(HERE, "split2d_cases.npz"), **out)
 
 
+def unsplit2d_cases(ref):
+    """GRP_solver_2D_EUL_source (the unsplit step, config[33] = 0)."""
+    out = {}
+    m = n = 64
+    cfg = cases.four_quadrant_config(m, n, split=0)
+    fr, npl, tp, c = ref.unsplit2d(m, n, cases.four_quadrant(m, n), cfg)
+    out["fq64_final"] = fr[:, npl - 1]
+    out["fq64_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    # 32 x 48 with 5 frames
+    m, n = 32, 48
+    cfg = cases.four_quadrant_config(m, n, t_all=0.2, split=0)
+    tpin = np.array([0.0, 0.05, 0.1, 0.15, 0.2])
+    fr, npl, tp, c = ref.unsplit2d(m, n, cases.four_quadrant(m, n), cfg, time_plot=tpin)
+    out["frames_all"] = fr
+    out["frames_meta"] = np.concatenate([[c[5], npl], tp])
+    # first plot time after t = 0: frame 0 is the live state for a while, and
+    # the CFL bound follows it there
+    m, n = 24, 20
+    init = cases.smooth_random(m, n, 11)
+    cfg = make_config(t_all=0.3, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-7, split=0)
+    tpin = np.array([0.1, 0.2, 0.3])
+    out["late_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    fr, npl, tp, c = ref.unsplit2d(m, n, init, cfg, time_plot=tpin, N_T=3)
+    out["late_frames"] = fr
+    out["late_meta"] = np.concatenate([[c[5], npl], tp])
+    # every boundary-condition pair on a 12 x 10 smooth field, t = 0.2
+    bcs = (-1, -2, -4, -7, -24)
+    init = cases.smooth_random(12, 10, 5)
+    out["bc_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    finals, metas = [], []
+    for bx in bcs:
+        for by in bcs:
+            cfg = make_config(t_all=0.2, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=0)
+            fr, npl, tp, c = ref.unsplit2d(12, 10, init, cfg)
+            finals.append(fr[:, npl - 1])
+            metas.append([bx, by, c[5], npl, tp[npl - 1]])
+    out["bc_final"] = np.stack(finals)
+    out["bc_meta"] = np.array(metas, dtype=np.float64)
+    # reflective box with a blast: walls on all four sides
+    m, n = 40, 36
+    init = cases.blast_box(m, n)
+    cfg = make_config(t_all=0.15, h_x=1 / m, h_y=1 / n, bc_x=-2, bc_y=-2, split=0)
+    fr, npl, tp, c = ref.unsplit2d(m, n, init, cfg)
+    out["blast_final"] = fr[:, npl - 1]
+    out["blast_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    # fixed-tau mode (t_all infinite, config[16] > 0, 7 steps)
+    cfg = make_config(N=7, tau=1e-3, h_x=1 / 16, h_y=1 / 16, bc_x=-24, bc_y=-1, split=0)
+    cfg[1] = np.inf
+    init = cases.smooth_random(16, 16, 9)
+    out["fixtau_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    fr, npl, tp, c = ref.unsplit2d(16, 16, init, cfg, time_plot=np.array([0.0, np.inf]))
+    out["fixtau_final"] = fr[:, npl - 1]
+    out["fixtau_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    # four-quadrant 256^2 to t = 0.3: sums, L1 norms, probes and every 37th x-line
+    m = n = 256
+    cfg = cases.four_quadrant_config(m, n, split=0)
+    fr, npl, tp, c = ref.unsplit2d(m, n, cases.four_quadrant(m, n), cfg)
+    fin = fr[:, npl - 1]
+    probes = np.array([[3, 5], [100, 200], [128, 128], [200, 31], [255, 255], [77, 160], [160, 77]])
+    out["fq256_sum"] = fin.reshape(5, -1).sum(axis=1)
+    out["fq256_l1"] = np.abs(fin).reshape(5, -1).sum(axis=1)
+    out["fq256_probes"] = probes.astype(np.float64)
+    out["fq256_probe_vals"] = fin[:, probes[:, 0], probes[:, 1]]
+    out["fq256_rows"] = fin[:, ::37, :]
+    out["fq256_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    np.savez_compressed(os.path.join(HERE, "unsplit2d_cases.npz"), **out)
+
+
 def grp1d_cases(ref):
     out = {}
     cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
@@ -134,9 +202,11 @@ def grp1d_cases(ref):
 
 def main():
     ref = RefLib()
-    interface_vectors(ref)
-    split2d_cases(ref)
-    grp1d_cases(ref)
+    only = sys.argv[1:]
+    for name, fn in (("interface", interface_vectors), ("split2d", split2d_cases),
+                     ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
+        if not only or name in only:
+            fn(ref)
     for f in sorted(os.listdir(HERE)):
         if f.endswith(".npz"):
             print(f, os.path.getsize(os.path.join(HERE, f)))

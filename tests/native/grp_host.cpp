@@ -32,6 +32,20 @@ extern "C" void hg_flux_many(const double *L, const double *R, int n, double tau
     }
 }
 
+// the unsplit path's flux: tangential derivatives of both sides (members 11..14), run-time normal
+extern "C" void hg_flux_t_many(const double *L, const double *R, int n, double tau, double eps, int dir, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 15 * k, *r = R + 15 * k;
+        Side a{l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]}, b{r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+        double F[4], I[4];
+        const int rc = grp_2d_flux_n(a, b, l + 11, r + 11, make_gas(l[0]), dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, tau,
+                                     eps, F, I, eps);
+        out[9 * k] = rc;
+        for (int q = 0; q < 4; ++q) { out[9 * k + 1 + q] = F[q]; out[9 * k + 5 + q] = I[q]; }
+    }
+}
+
 extern "C" void hg_div_by_many(const double *a, int n, double b, double *out)
 {
     const RecipDiv d = recip_div(b);

@@ -60,6 +60,27 @@ def test_flux_against_reference_and_devpow_oracle(dev, golden_iface, oracle_devp
         assert np.array_equal(out[ok], want[ok]), direction
 
 
+def test_transversal_flux_against_reference_and_devpow_oracle(dev, golden_iface, oracle_devpow):
+    """The unsplit path's GRP_2D_flux (tangential derivatives on a third of the
+    golden cases, both normals) against the reference's own outputs and, bit for
+    bit, against the oracle with the device's pow."""
+    L, R, tau = (np.ascontiguousarray(golden_iface["L"]), np.ascontiguousarray(golden_iface["R"]),
+                 float(golden_iface["tau"]))
+    assert np.mean(np.any(L[:, 11:15] != 0, axis=1)) > 0.25
+    for direction, nxy, key in ((0, (1.0, 0.0), "flux_x"), (1, (0.0, 1.0), "flux_y")):
+        out = np.zeros((len(L), 9))
+        dev.hg_flux_t_many(_dp(L), _dp(R), len(L), C.c_double(tau), C.c_double(EPS), direction, _dp(out))
+        ref = golden_iface[key]
+        ok = ref[:, 0] == 0
+        assert np.array_equal(out[:, 0] != 0, ref[:, 0] != 0)
+        d = np.abs(out[ok, 1:] - ref[ok, 1:]) / np.maximum(np.abs(ref[ok, 1:]), 1.0)
+        print(f"{key}: {np.mean(np.all(out[ok] == ref[ok], axis=1)) * 100:.2f}% rows bit-identical, max rel {d.max():.2e}")
+        assert d.max() <= 1e-8
+        want = np.array([[rc, *o] for rc, o in (oracle_devpow.flux2d(L[k], R[k], tau, EPS, *nxy)
+                                                 for k in range(len(L)))])
+        assert np.array_equal(out[ok], want[ok]), direction
+
+
 def test_reciprocal_division_is_ieee_division(dev):
     """div_by (a / b from a precomputed reciprocal, Markstein's correction)
     against IEEE division, bit for bit, signs of zero included."""

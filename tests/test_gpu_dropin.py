@@ -1,7 +1,8 @@
-"""GPU: the drop-in entry point GRP_solver_2D_split_EUL_source (called with the
-reference's CV row-pointer frames through ctypes) and the whole 2D program
-(bin/hydrocode_2D.out) against the oracle, and against the reference's own
-program where it has been built (oracle/_ref)."""
+"""GPU: the drop-in entry points GRP_solver_2D_split_EUL_source and
+GRP_solver_2D_EUL_source (called with the reference's CV row-pointer frames
+through ctypes) and the whole 2D program (bin/hydrocode_2D.out) against the
+oracle, and against the reference's own program where it has been built
+(oracle/_ref)."""
 import ctypes as C
 import os
 import subprocess
@@ -26,14 +27,17 @@ def host():
     L.GRP_solver_2D_split_EUL_source.argtypes = [C.c_int, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
                                                  C.POINTER(C.c_double), C.c_char_p, C.c_int,
                                                  C.POINTER(C.c_int), C.POINTER(C.c_double)]
+    L.GRP_solver_2D_EUL_source.restype = None
+    L.GRP_solver_2D_EUL_source.argtypes = L.GRP_solver_2D_split_EUL_source.argtypes
     L.cfg = (C.c_double * 400).in_dll(L, "config")
     return L
 
 
-def test_dropin_entry_point(host, oracle, tmp_path):
+@pytest.mark.parametrize("split", [1, 0])
+def test_dropin_entry_point(host, oracle, tmp_path, split):
     m, n = 40, 36
     init = cases.four_quadrant(m, n)
-    cfg = cases.four_quadrant_config(m, n, t_all=0.2)
+    cfg = cases.four_quadrant_config(m, n, t_all=0.2, split=split)
     cfg[8], cfg[9] = 0.0, 2.0
     time_plot = [0.0, 0.05, 0.12, 0.2]
     N_T = 4
@@ -49,11 +53,13 @@ def test_dropin_entry_point(host, oracle, tmp_path):
     cwd = os.getcwd()
     os.chdir(run)
     try:
-        host.GRP_solver_2D_split_EUL_source(m, n, C.cast(fr.CV, C.c_void_p), C.cast(fr.X, C.c_void_p),
-                                            C.cast(fr.Y, C.c_void_p), cpu, b"dropin", N_T, C.byref(N_plot), tp)
+        fn = host.GRP_solver_2D_split_EUL_source if split else host.GRP_solver_2D_EUL_source
+        fn(m, n, C.cast(fr.CV, C.c_void_p), C.cast(fr.X, C.c_void_p), C.cast(fr.Y, C.c_void_p), cpu, b"dropin",
+           N_T, C.byref(N_plot), tp)
     finally:
         os.chdir(cwd)
-    fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg, time_plot=np.array(time_plot), N_T=N_T)
+    run_oracle = oracle.split2d if split else oracle.unsplit2d
+    fo, npo, tpo, co, st = run_oracle(m, n, init, cfg, time_plot=np.array(time_plot), N_T=N_T)
     assert (host.cfg[5], N_plot.value) == (co[5], npo)
     assert np.array_equal(np.array(tp[:]), tpo)
     for k in range(npo):
@@ -83,7 +89,7 @@ def _read_frames(path, n_x, n_y):
     return np.array(frames)
 
 
-def _run_program(exe, tmp_path, tag, m, n, init, time_plot):
+def _run_program(exe, tmp_path, tag, m, n, init, time_plot, split=1):
     import hydrogrp.datain as datain
     base = tmp_path / tag
     datain.write_case_2d(str(base / "data_in" / "two-dim" / "fqcase"), init,
@@ -92,7 +98,7 @@ def _run_program(exe, tmp_path, tag, m, n, init, time_plot):
     run = base / "run" / "here"
     run.mkdir(parents=True)
     print(f"running {os.path.basename(exe)} ({tag})", flush=True)
-    p = subprocess.run([exe, "fqcase", "fqout", "2_GRP", "EUL", "33=1"], cwd=run, capture_output=True,
+    p = subprocess.run([exe, "fqcase", "fqout", "2_GRP", "EUL", f"33={split}"], cwd=run, capture_output=True,
                        text=True, timeout=300)
     assert p.returncode == 0, p.stdout[-2000:] + p.stderr[-2000:]
     return base / "data_out" / "two-dim" / "EUL_2_order" / "fqout"
@@ -114,13 +120,14 @@ def test_program_end_to_end(oracle, tmp_path):
     assert f"A total of {int(co[5])} time steps are computed." in open(out / "log.dat").read()
 
 
-def test_program_matches_reference_program(tmp_path):
+@pytest.mark.parametrize("split", [1, 0])
+def test_program_matches_reference_program(tmp_path, split):
     if not os.path.exists(REF_PROGRAM):
         pytest.skip("reference program not built here (oracle/_ref)")
     m, n = 32, 24
     init = cases.four_quadrant(m, n)
-    ours = _run_program(PROGRAM, tmp_path, "gpu", m, n, init, [0.07])
-    ref = _run_program(REF_PROGRAM, tmp_path, "ref", m, n, init, [0.07])
+    ours = _run_program(PROGRAM, tmp_path, "gpu", m, n, init, [0.07], split)
+    ref = _run_program(REF_PROGRAM, tmp_path, "ref", m, n, init, [0.07], split)
     # FLU_VAR.f64 (full-precision frames) is this build's addition to the outputs
     names = sorted(p.name for p in ours.iterdir() if p.suffix != ".f64")
     assert names == sorted(p.name for p in ref.iterdir())

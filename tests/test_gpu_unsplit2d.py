@@ -1,0 +1,159 @@
+"""GPU parity of the unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0):
+the HIP library against the golden fixtures made by the reference itself, and
+against the oracle on the same seeded inputs.
+
+Bar: relative L-infinity difference per field <= 1e-10 at final time, as for
+the split path (the device's FP64 pow may differ from glibc's in the last bit;
+everything else follows the reference's expression order).
+"""
+import numpy as np
+import pytest
+
+import cases
+from _libs import make_config
+from conftest import rel_linf
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-10
+
+
+@pytest.fixture(scope="module")
+def hg():
+    import hydrogrp
+    hydrogrp.lib()
+    print("device:", hydrogrp.device_name(), flush=True)
+    return hydrogrp
+
+
+def _run(hg, m, n, init, cfg, time_plot=None, N_T=None):
+    print(f"unsplit run {m}x{n} bc=({cfg[17]:g},{cfg[18]:g})", flush=True)
+    return hg.run_unsplit2d(m, n, init, cfg, time_plot=time_plot, N_T=N_T)
+
+
+def test_four_quadrant_64_and_frames(hg, golden_unsplit):
+    g = golden_unsplit
+    fr, npl, tp, cfg, res = _run(hg, 64, 64, cases.four_quadrant(64, 64), cases.four_quadrant_config(64, 64, split=0))
+    assert (cfg[5], npl, tp[npl - 1]) == tuple(g["fq64_meta"]) and res.status == 0
+    d = rel_linf(fr[:, npl - 1], g["fq64_final"])
+    print("fq64 rel Linf per field:", d, flush=True)
+    assert d.max() <= TOL
+    tpin = np.array([0.0, 0.05, 0.1, 0.15, 0.2])
+    fr, npl, tp, cfg, res = _run(hg, 32, 48, cases.four_quadrant(32, 48),
+                                 cases.four_quadrant_config(32, 48, t_all=0.2, split=0), time_plot=tpin)
+    meta = g["frames_meta"]
+    assert (cfg[5], npl) == (meta[0], meta[1]) and np.array_equal(tp, meta[2:])
+    for k in range(npl):
+        assert rel_linf(fr[:, k], g["frames_all"][:, k]).max() <= TOL, k
+
+
+def test_live_frame_zero(hg, golden_unsplit):
+    """First plot time after t = 0: the CFL bound follows frame 0 while it is the live state."""
+    g = golden_unsplit
+    init = dict(zip(("rho", "u", "v", "p"), g["late_init"]))
+    cfg = make_config(t_all=0.3, h_x=1 / 24, h_y=1 / 20, bc_x=-4, bc_y=-7, split=0)
+    fr, npl, tp, c, res = _run(hg, 24, 20, init, cfg, time_plot=np.array([0.1, 0.2, 0.3]), N_T=3)
+    meta = g["late_meta"]
+    assert (c[5], npl) == (meta[0], meta[1])
+    assert rel_linf(tp, meta[2:]).max() <= 1e-13
+    for k in range(npl):
+        assert rel_linf(fr[:, k], g["late_frames"][:, k]).max() <= TOL, k
+
+
+def test_all_boundary_conditions(hg, golden_unsplit):
+    g = golden_unsplit
+    init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))
+    worst = 0.0
+    for k, (bx, by, steps, npl0, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.2, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=0)
+        fr, npl, tp, c, res = _run(hg, 12, 10, init, cfg)
+        assert (c[5], npl, tp[npl - 1]) == (steps, npl0, t_end), (bx, by)
+        d = rel_linf(fr[:, npl - 1], g["bc_final"][k]).max()
+        worst = max(worst, d)
+        assert d <= TOL, (bx, by, d)
+    print("boundary conditions: worst rel Linf", worst, flush=True)
+
+
+def test_blast_and_fixed_tau(hg, golden_unsplit):
+    g = golden_unsplit
+    cfg = make_config(t_all=0.15, h_x=1 / 40, h_y=1 / 36, bc_x=-2, bc_y=-2, split=0)
+    fr, npl, tp, c, res = _run(hg, 40, 36, cases.blast_box(40, 36), cfg)
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["blast_meta"])
+    assert rel_linf(fr[:, npl - 1], g["blast_final"]).max() <= TOL
+    cfg = make_config(N=7, tau=1e-3, h_x=1 / 16, h_y=1 / 16, bc_x=-24, bc_y=-1, split=0)
+    cfg[1] = np.inf
+    init = dict(zip(("rho", "u", "v", "p"), g["fixtau_init"]))
+    fr, npl, tp, c, res = _run(hg, 16, 16, init, cfg, time_plot=np.array([0.0, np.inf]))
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["fixtau_meta"])
+    assert rel_linf(fr[:, npl - 1], g["fixtau_final"]).max() <= TOL
+
+
+def test_four_quadrant_256(hg, golden_unsplit):
+    g = golden_unsplit
+    fr, npl, tp, c, res = _run(hg, 256, 256, cases.four_quadrant(256, 256),
+                               cases.four_quadrant_config(256, 256, split=0))
+    fin = fr[:, npl - 1]
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["fq256_meta"])
+    probes = g["fq256_probes"].astype(int)
+    d_rows = rel_linf(fin[:, ::37, :], g["fq256_rows"])
+    d_probe = rel_linf(fin[:, probes[:, 0], probes[:, 1]], g["fq256_probe_vals"])
+    print("fq256 rel Linf rows", d_rows.max(), "probes", d_probe.max(), flush=True)
+    assert d_rows.max() <= TOL and d_probe.max() <= TOL
+    assert np.allclose(fin.reshape(5, -1).sum(axis=1), g["fq256_sum"], rtol=1e-12, atol=1e-9)
+
+
+# tile edges: 62 cells per wave in y, 6 rows per block in x
+@pytest.mark.parametrize("m,n,bx,by", [(1, 50, -4, -7), (50, 1, -7, -2), (6, 62, -2, -24), (7, 63, -24, -1),
+                                       (13, 125, -7, -7), (127, 129, -1, -4), (65, 200, -2, -2)])
+def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by):
+    init = cases.smooth_random(m, n, m * 1000 + n)
+    cfg = make_config(t_all=0.04, h_x=1 / m, h_y=1 / n, bc_x=bx, bc_y=by, split=0)
+    fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg)
+    fg, npg, tpg, cg, res = _run(hg, m, n, init, cfg)
+    assert (co[5], npo, tpo[npo - 1]) == (cg[5], npg, tpg[npg - 1])
+    assert rel_linf(fg[:, npg - 1], fo[:, npo - 1]).max() <= TOL
+
+
+def test_error_paths_vs_oracle(hg, oracle):
+    m, n = 8, 6
+    for mutate, kw in ((("p", (3, 2), 1e-12), {}), (("p", (3, 2), -1.0), {}), (None, {"CFL": 1e-12}),
+                       (("rho", (5, 1), 1e-12), {}), (None, {"bc_y": -3}), (("u", (0, 0), np.nan), {})):
+        init = cases.smooth_random(m, n, 2)
+        if mutate:
+            init[mutate[0]][mutate[1]] = mutate[2]
+        base = dict(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-4, split=0)
+        base.update(kw)
+        cfg = make_config(**base)
+        fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg)
+        fg, npg, tpg, cg, res = _run(hg, m, n, init, cfg)
+        assert (co[5], npo, st) == (cg[5], npg, res.status), (mutate, kw)
+        assert tpo[npo - 1] == tpg[npg - 1] or (np.isnan(tpo[npo - 1]) and np.isnan(tpg[npg - 1])), (mutate, kw)
+        assert np.array_equal(np.isfinite(fo[:, npo - 1]), np.isfinite(fg[:, npg - 1]))
+        ok = np.isfinite(fo[:, npo - 1])
+        assert np.max(np.abs(fo[:, npo - 1][ok] - fg[:, npg - 1][ok])) <= 1e-9, (mutate, kw)
+
+
+def test_conservation_full_size(hg):
+    """Periodic in both directions: mass, momentum and energy sums are conserved
+    (to rounding) over 40 steps of a 2048 x 2048 four-quadrant state."""
+    m = n = 2048
+    cfg = cases.four_quadrant_config(m, n, split=0, N=40)
+    cfg[17] = cfg[18] = -7
+    cfg[1] = np.inf
+    cfg[16] = 0.45 * (1.0 / m) / 3.0
+    init = cases.four_quadrant(m, n)
+    s = hg.Solver2D(m, n, cfg)
+    try:
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        r0, u0, v0, p0, e0 = s.download()
+        res = s.solve(np.array([0.0, np.inf]), 2)
+        r1, u1, v1, p1, e1 = s.download()
+    finally:
+        s.close()
+    assert res.status == 0 and res.k == 41
+    for name, a, b in (("mass", r0, r1), ("x-momentum", r0 * u0, r1 * u1), ("y-momentum", r0 * v0, r1 * v1),
+                       ("energy", r0 * e0, r1 * e1)):
+        s0, s1 = a.sum(), b.sum()
+        scale = np.abs(a).sum()
+        print(f"{name}: relative drift {(s1 - s0) / scale:.2e}", flush=True)
+        assert abs(s1 - s0) <= 1e-11 * scale, name
+    assert np.all(np.isfinite(r1)) and r1.min() > 0 and p1.min() > 0

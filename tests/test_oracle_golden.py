@@ -118,6 +118,78 @@ def test_split2d_four_quadrant_256(oracle, golden_2d):
     assert _eq(fin.reshape(5, -1).sum(axis=1), g["fq256_sum"])
 
 
+def test_unsplit2d_four_quadrant_and_frames(oracle, golden_unsplit):
+    g = golden_unsplit
+    fr, npl, tp, cfg, st = oracle.unsplit2d(64, 64, cases.four_quadrant(64, 64),
+                                            cases.four_quadrant_config(64, 64, split=0))
+    assert st == 0 and (cfg[5], npl, tp[npl - 1]) == tuple(g["fq64_meta"])
+    assert _eq(fr[:, npl - 1], g["fq64_final"])
+    tpin = np.array([0.0, 0.05, 0.1, 0.15, 0.2])
+    fr, npl, tp, cfg, st = oracle.unsplit2d(32, 48, cases.four_quadrant(32, 48),
+                                            cases.four_quadrant_config(32, 48, t_all=0.2, split=0), time_plot=tpin)
+    meta = g["frames_meta"]
+    assert (cfg[5], npl) == (meta[0], meta[1]) and _eq(tp, meta[2:])
+    assert _eq(fr, g["frames_all"])
+
+
+def test_unsplit2d_live_frame_zero(oracle, golden_unsplit):
+    """First plot time after t = 0: the CFL bound reads frame 0 while it is the live state."""
+    g = golden_unsplit
+    init = dict(zip(("rho", "u", "v", "p"), g["late_init"]))
+    cfg = make_config(t_all=0.3, h_x=1 / 24, h_y=1 / 20, bc_x=-4, bc_y=-7, split=0)
+    fr, npl, tp, c, st = oracle.unsplit2d(24, 20, init, cfg, time_plot=np.array([0.1, 0.2, 0.3]), N_T=3)
+    meta = g["late_meta"]
+    assert (c[5], npl) == (meta[0], meta[1]) and _eq(tp, meta[2:])
+    assert _eq(fr, g["late_frames"])
+
+
+def test_unsplit2d_all_boundary_conditions(oracle, golden_unsplit):
+    g = golden_unsplit
+    init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))
+    for k, (bx, by, steps, npl, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.2, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=0)
+        fr, n_plot, tp, c, st = oracle.unsplit2d(12, 10, init, cfg)
+        assert (c[5], n_plot, tp[n_plot - 1]) == (steps, npl, t_end), (bx, by)
+        assert _eq(fr[:, n_plot - 1], g["bc_final"][k]), (bx, by)
+
+
+def test_unsplit2d_blast_and_fixed_tau(oracle, golden_unsplit):
+    g = golden_unsplit
+    cfg = make_config(t_all=0.15, h_x=1 / 40, h_y=1 / 36, bc_x=-2, bc_y=-2, split=0)
+    fr, npl, tp, c, st = oracle.unsplit2d(40, 36, cases.blast_box(40, 36), cfg)
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["blast_meta"])
+    assert _eq(fr[:, npl - 1], g["blast_final"])
+    cfg = make_config(N=7, tau=1e-3, h_x=1 / 16, h_y=1 / 16, bc_x=-24, bc_y=-1, split=0)
+    cfg[1] = np.inf
+    init = dict(zip(("rho", "u", "v", "p"), g["fixtau_init"]))
+    fr, npl, tp, c, st = oracle.unsplit2d(16, 16, init, cfg, time_plot=np.array([0.0, np.inf]))
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["fixtau_meta"]) and int(c[5]) == 8
+    assert _eq(fr[:, npl - 1], g["fixtau_final"])
+
+
+def test_unsplit2d_four_quadrant_256(oracle, golden_unsplit):
+    g = golden_unsplit
+    fr, npl, tp, c, st = oracle.unsplit2d(256, 256, cases.four_quadrant(256, 256),
+                                          cases.four_quadrant_config(256, 256, split=0))
+    fin = fr[:, npl - 1]
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["fq256_meta"])
+    probes = g["fq256_probes"].astype(int)
+    assert _eq(fin[:, probes[:, 0], probes[:, 1]], g["fq256_probe_vals"])
+    assert _eq(fin[:, ::37, :], g["fq256_rows"])
+    assert _eq(fin.reshape(5, -1).sum(axis=1), g["fq256_sum"])
+
+
+def test_unsplit_differs_from_split(oracle):
+    """The transversal terms and the single update are really there: the two
+    schemes agree to discretisation error, not to rounding."""
+    m = n = 32
+    init = cases.four_quadrant(m, n)
+    fs = oracle.split2d(m, n, init, cases.four_quadrant_config(m, n, t_all=0.1))[0]
+    fu = oracle.unsplit2d(m, n, init, cases.four_quadrant_config(m, n, t_all=0.1, split=0))[0]
+    d = np.abs(fs[0, -1] - fu[0, -1]).max()
+    assert 1e-4 < d < 0.5
+
+
 def test_grp1d_sod_and_boundaries(oracle, golden_1d):
     g = golden_1d
     cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
@@ -148,3 +220,7 @@ def test_against_live_reference(oracle, reflib):
         fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg)
         fr, npr, tpr, cr = reflib.split2d(m, n, init, cfg)
         assert co[5] == cr[5] and _eq(fo[:, :npo], fr[:, :npr]), (m, n)
+        cfg = make_config(t_all=0.03, h_x=1 / m, h_y=1 / n, bc_x=-2, bc_y=-7, split=0)
+        fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg)
+        fr, npr, tpr, cr = reflib.unsplit2d(m, n, init, cfg)
+        assert co[5] == cr[5] and _eq(fo[:, :npo], fr[:, :npr]), ("unsplit", m, n)
