@@ -16,7 +16,9 @@ on a bounded number of steps of the same workload on the host's cores.
 solve with the one started from the learned p* guess (rate, and the relative L1
 difference of the two final states).  `ensemble_1d` runs 512 independent 1D
 GRP problems of uneven sizes through the batched engine; `two_fluid` runs the
-shock-bubble interaction (4096 x 1024) with and without the learned guess.
+shock-bubble interaction (4096 x 1024) with and without the learned guess;
+`unsplit` runs the same four-quadrant workload through the unsplit scheme
+(config[33] = 0) with its own roofline and CPU baseline.
 """
 import argparse
 import json
@@ -162,6 +164,61 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
             "speedup": out["learned"]["value"] / out["plain"]["value"]}
 
 
+UNSPLIT_BYTES_PER_CELL = 208.0   # read and write rho u v p E and both slope sets (26 doubles)
+
+
+def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
+    """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0) on the same
+    four-quadrant workload: `steps` steps of one fused launch each with
+    tau = CFL * h / S_max from t = 0 (2017 steps = the whole run to t = 0.3),
+    device time from HIP events; roofline of the step kernel against HBM peak
+    with 208 B per cell (rho u v p E and the x- and y-slopes, read and
+    written); and the oracle's unsplit loop on the host cores on a bounded
+    number of steps."""
+    import numpy as np
+    import hydrogrp
+    import cases
+    from _libs import Oracle, make_config
+    cfg = cases.four_quadrant_config(M, N, split=0)
+    init = cases.four_quadrant(M, N)
+    s = hydrogrp.Solver2D(M, N, cfg)
+    s.upload(init["rho"], init["u"], init["v"], init["p"])
+    s.bench_unsplit(5)
+    s.upload(init["rho"], init["u"], init["v"], init["p"])
+    ms, ms_step = s.bench_unsplit(steps)
+    kernel = s.sweep_kernel(2)
+    state = s.download()
+    s.close()
+    assert np.all(np.isfinite(state)), "non-finite state after the unsplit benchmark"
+    achieved = UNSPLIT_BYTES_PER_CELL * M * N / (ms_step / 1000.0) / 1e9
+    out = {"workload": f"2D four-quadrant Riemann problem {M}x{N}, unsplit GRP (one step per tau), FP64, "
+                       f"{steps} steps from t=0",
+           "value": M * N * steps / (ms / 1000.0), "unit": "cell-updates/s", "ms_per_step": ms / steps,
+           "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "ms_per_launch": ms_step}}
+    if with_cpu:
+        orc = Oracle()
+        h = 1.0 / M
+        c = np.sqrt(1.4 * init["p"] / init["rho"])
+        tau = 0.45 * np.min(h / (c + np.abs(init["u"]) + np.abs(init["v"])))
+
+        def run(k):
+            cc = make_config(N=k, tau=tau, h_x=h, h_y=h, bc_x=-4, bc_y=-4, split=0)
+            cc[1] = np.inf
+            t0 = time.perf_counter()
+            orc.unsplit2d(M, N, init, cc, time_plot=np.array([0.0, np.inf]))
+            return time.perf_counter() - t0
+
+        k = max(2, int(cpu_seconds / max(run(1), 1e-3)))
+        t = run(k)
+        threads = int(orc.lib.orc_threads())
+        out["cpu_baseline"] = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+                               "sample": f"{k} unsplit steps of the same {M}x{N} workload ({t:.1f} s), "
+                                         f"oracle/liboracle.so (gcc -O2 -fopenmp, {threads} threads)"}
+    return out
+
+
 def cpu_leg(target_seconds=12.0):
     """The oracle on the host cores, fixed-tau mode (exactly the frozen tau), bounded sample."""
     import numpy as np
@@ -200,6 +257,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess comparison at 4096^2")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 1D ensemble")
     ap.add_argument("--no-two-fluid", action="store_true", help="skip the two-fluid shock-bubble")
+    ap.add_argument("--no-unsplit", action="store_true", help="skip the unsplit scheme")
     args = ap.parse_args()
     global M, N
     M = N = args.size
@@ -235,14 +293,25 @@ def main():
                      "ms_per_launch": {kernels[0]: ms_x, kernels[1]: ms_y}},
         "wall_s": wall,
     }
+    def note(what):
+        print(f"bench.py: {what} done ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    note("main leg")
     if not args.no_learned:
         line["learned_guess"] = learned_leg()
+        note("learned-guess leg")
     if not args.no_ensemble:
         line["ensemble_1d"] = ensemble_leg()
+        note("1D ensemble leg")
     if not args.no_two_fluid:
         line["two_fluid"] = two_fluid_leg()
+        note("two-fluid leg")
+    if not args.no_unsplit:
+        line["unsplit"] = unsplit_leg(with_cpu=not args.no_cpu)
+        note("unsplit leg")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_leg()
+        note("CPU baseline")
     print(json.dumps(line), flush=True)
 
 

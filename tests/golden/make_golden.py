@@ -66,6 +66,14 @@ def split2d_cases(ref):
     fr, npl, tp, c = ref.split2d(m, n, cases.four_quadrant(m, n), cfg, time_plot=tpin)
     out["frames_all"] = fr
     out["frames_meta"] = np.concatenate([[c[5], npl], tp])
+    # first plot time after t = 0: frame 0 is the live state for a while
+    m, n = 24, 20
+    init = cases.smooth_random(m, n, 11)
+    cfg = make_config(t_all=0.3, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-7, split=1)
+    out["late_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    fr, npl, tp, c = ref.split2d(m, n, init, cfg, time_plot=np.array([0.1, 0.2, 0.3]), N_T=3)
+    out["late_frames"] = fr
+    out["late_meta"] = np.concatenate([[c[5], npl], tp])
     # every boundary-condition pair on a 12 x 10 smooth field, t = 0.05
     bcs = (-1, -2, -4, -7, -24)
     init = cases.smooth_random(12, 10, 5)

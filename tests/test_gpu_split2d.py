@@ -96,6 +96,20 @@ def test_four_quadrant_64(hg, golden_2d):
     assert d.max() <= TOL
 
 
+def test_live_frame_zero(hg, golden_2d):
+    """First plot time after t = 0: the CFL bound follows frame 0 while it is
+    the live state and freezes with its last state."""
+    g = golden_2d
+    init = dict(zip(("rho", "u", "v", "p"), g["late_init"]))
+    cfg = make_config(t_all=0.3, h_x=1 / 24, h_y=1 / 20, bc_x=-4, bc_y=-7, split=1)
+    fr, npl, tp, c, res = hg.run_split2d(24, 20, init, cfg, time_plot=np.array([0.1, 0.2, 0.3]), N_T=3)
+    meta = g["late_meta"]
+    assert (c[5], npl) == (meta[0], meta[1])
+    assert rel_linf(tp, meta[2:]).max() <= 1e-13
+    for k in range(npl):
+        assert rel_linf(fr[:, k], g["late_frames"][:, k]).max() <= TOL, k
+
+
 def test_all_boundary_conditions(hg, golden_2d):
     g = golden_2d
     init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))

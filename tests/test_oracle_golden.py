@@ -78,6 +78,16 @@ def test_split2d_frames(oracle, golden_2d):
     assert _eq(fr, g["frames_all"])
 
 
+def test_split2d_live_frame_zero(oracle, golden_2d):
+    g = golden_2d
+    init = dict(zip(("rho", "u", "v", "p"), g["late_init"]))
+    cfg = make_config(t_all=0.3, h_x=1 / 24, h_y=1 / 20, bc_x=-4, bc_y=-7, split=1)
+    fr, npl, tp, c, st = oracle.split2d(24, 20, init, cfg, time_plot=np.array([0.1, 0.2, 0.3]), N_T=3)
+    meta = g["late_meta"]
+    assert (c[5], npl) == (meta[0], meta[1]) and _eq(tp, meta[2:])
+    assert _eq(fr, g["late_frames"])
+
+
 def test_split2d_all_boundary_conditions(oracle, golden_2d):
     g = golden_2d
     init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))
