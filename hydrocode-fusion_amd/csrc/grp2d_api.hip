@@ -610,6 +610,10 @@ extern "C" int hgrp2d_solve(hgrp2d *c, const double *time_plot, int N_plot, int 
             const int f = flush();
             if (f < 0) return -2;
             if (f) { ended = true; break; }
+            if (nt == 0 && nt < N_T - 1) {   // the copy freezes CV[0]: the CFL bound of its last state
+                const int rc = refresh_cfl(c);
+                if (rc) return rc;
+            }
             c->at = {1, k, DS, nt < (N_T - 1) ? nt + 1 : nt, nt_plot + 1, time_c, tau, half_tau, half_nu, mu};
             const int stop = fn ? fn(user, nt, nt_plot, time_c) : 0;
             c->at.valid = 0;
@@ -816,6 +820,10 @@ static int solve_unsplit(hgrp2d *c, const double *time_plot, int N_plot, int N_T
             const int f = flush();
             if (f < 0) return -2;
             if (f) { ended = true; break; }
+            if (nt == 0 && nt < N_T - 1) {   // the copy freezes CV[0]: the CFL bound of its last state
+                const int rc = refresh_cfl(c);
+                if (rc) return rc;
+            }
             c->at = {1, k, 1, nt < (N_T - 1) ? nt + 1 : nt, nt_plot + 1, time_c, tau, 0.0, nu, mu};
             const int stop = fn ? fn(user, nt, nt_plot, time_c) : 0;
             c->at.valid = 0;
