@@ -36,6 +36,7 @@ hipError_t launch_cfl(const double *rho, const double *u, const double *v, const
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st);
 const char *unsplit_kernel_name();
+int unsplit_max_rows();
 }
 
 using namespace hgrp;
@@ -747,6 +748,10 @@ static bool unsplit_supported(hgrp2d *c)
 {
     if (c->two_fluid) {
         printf("The unsplit scheme (config[33] = 0) runs one gamma-law gas only.\n");
+        return false;
+    }
+    if (c->m > unsplit_max_rows()) {
+        printf("The unsplit scheme takes at most %d cells in x (n_x = %d).\n", unsplit_max_rows(), c->m);
         return false;
     }
     return true;
