@@ -39,7 +39,8 @@ WORKLOAD = "2D four-quadrant Riemann problem {m}x{n}, dimension-split GRP (X/2 Y
 def measured_traffic(kernel, m, n):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (profiles/*_hbm_traffic.md, a line
-    `traffic_bytes: {...}`), or None where no such profile exists."""
+    `traffic_bytes: {...}`), or None where no such profile exists for this
+    kernel and size."""
     path = os.path.join(ROOT, "profiles", f"sweep_kernels_fourquadrant{m}_hbm_traffic.md")
     if m != n or not os.path.exists(path):
         return None
@@ -195,7 +196,7 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
                        f"{steps} steps from t=0",
            "value": M * N * steps / (ms / 1000.0), "unit": "cell-updates/s", "ms_per_step": ms / steps,
            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, M, N),
                         "ms_per_launch": ms_step}}
     if with_cpu:
         orc = Oracle()
