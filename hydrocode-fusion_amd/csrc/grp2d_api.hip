@@ -384,6 +384,9 @@ static int enqueue_sweep(hgrp2d *c, int dir, double tau_flux, double nu, int slo
 
 static bool valid_bc(int b) { return b == -1 || b == -2 || b == -4 || b == -7 || b == -24; }
 
+// Steps between per-launch event samples in the benchmark entry points.
+static int bench_sample_every(int steps) { return steps >= 64 ? 8 : 1; }
+
 extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x, double *ms_y)
 {
     if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
@@ -391,20 +394,26 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x
     const double tau = c->cfg[7] * c->h_S_max;
     const double half_tau = tau * 0.5, half_nu = half_tau / c->cfg[10], mu = tau / c->cfg[11];
     HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
-    // per-sweep events for the average launch duration of each sweep kind
-    std::vector<hipEvent_t> evs((size_t)6 * steps);
+    // per-sweep events for the average launch duration of each sweep kind, on
+    // every EVERY-th step only (a uniform sample of the run), so that the timed
+    // stream carries few event packets besides the sweeps themselves
+    const int every = bench_sample_every(steps);
+    const int ns = (steps + every - 1) / every;
+    std::vector<hipEvent_t> evs((size_t)6 * ns);
     for (auto &e : evs) HCHECK(hipEventCreate(&e));
     HCHECK(hipEventRecord(c->ev[0], c->stream));
     int slot = 0;
     for (int s = 0; s < steps; ++s) {
         const int dirs[3] = {0, 1, 0};
+        const bool sampled = s % every == 0;
+        const size_t e0 = (size_t)6 * (s / every);
         for (int w = 0; w < 3; ++w) {
             const int d = dirs[w];
-            HCHECK(hipEventRecord(evs[(size_t)6 * s + 2 * w], c->stream));
+            if (sampled) HCHECK(hipEventRecord(evs[e0 + 2 * w], c->stream));
             // status slots are not chained here: the benchmark must run every sweep
             int rc = enqueue_sweep(c, d, d == 0 ? half_tau : tau, d == 0 ? half_nu : mu, slot % STATUS_SLOTS, -1);
             if (rc) return rc;
-            HCHECK(hipEventRecord(evs[(size_t)6 * s + 2 * w + 1], c->stream));
+            if (sampled) HCHECK(hipEventRecord(evs[e0 + 2 * w + 1], c->stream));
             ++slot;
         }
     }
@@ -414,14 +423,14 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x
     HCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     if (ms_total) *ms_total = ms;
     double sx = 0, sy = 0;
-    for (int s = 0; s < steps; ++s)
+    for (int k = 0; k < ns; ++k)
         for (int w = 0; w < 3; ++w) {
             float t = 0.f;
-            HCHECK(hipEventElapsedTime(&t, evs[(size_t)6 * s + 2 * w], evs[(size_t)6 * s + 2 * w + 1]));
+            HCHECK(hipEventElapsedTime(&t, evs[(size_t)6 * k + 2 * w], evs[(size_t)6 * k + 2 * w + 1]));
             if (w == 1) sy += t; else sx += t;
         }
-    if (ms_x) *ms_x = sx / (2.0 * steps);
-    if (ms_y) *ms_y = sy / steps;
+    if (ms_x) *ms_x = sx / (2.0 * ns);
+    if (ms_y) *ms_y = sy / ns;
     for (auto &e : evs) (void)hipEventDestroy(e);
     return 0;
 }
@@ -912,15 +921,18 @@ extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, double *ms_total, doub
     const double tau = c->cfg[7] * c->h_S_max;
     const double nu = tau / c->cfg[10], mu = tau / c->cfg[11];
     HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
-    std::vector<hipEvent_t> evs((size_t)2 * steps);
+    const int every = bench_sample_every(steps);
+    const int ns = (steps + every - 1) / every;
+    std::vector<hipEvent_t> evs((size_t)2 * ns);
     for (auto &e : evs) HCHECK(hipEventCreate(&e));
     HCHECK(hipEventRecord(c->ev[0], c->stream));
     for (int s = 0; s < steps; ++s) {
-        HCHECK(hipEventRecord(evs[(size_t)2 * s], c->stream));
+        const bool sampled = s % every == 0;
+        if (sampled) HCHECK(hipEventRecord(evs[(size_t)2 * (s / every)], c->stream));
         // status slots are not chained here: the benchmark must run every step
         const int rc = enqueue_unsplit(c, tau, nu, mu, s % UNSPLIT_SLOTS, -1);
         if (rc) return rc;
-        HCHECK(hipEventRecord(evs[(size_t)2 * s + 1], c->stream));
+        if (sampled) HCHECK(hipEventRecord(evs[(size_t)2 * (s / every) + 1], c->stream));
     }
     HCHECK(hipEventRecord(c->ev[1], c->stream));
     HCHECK(hipEventSynchronize(c->ev[1]));
@@ -928,12 +940,12 @@ extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, double *ms_total, doub
     HCHECK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]));
     if (ms_total) *ms_total = ms;
     double sum = 0.0;
-    for (int s = 0; s < steps; ++s) {
+    for (int k = 0; k < ns; ++k) {
         float t = 0.f;
-        HCHECK(hipEventElapsedTime(&t, evs[(size_t)2 * s], evs[(size_t)2 * s + 1]));
+        HCHECK(hipEventElapsedTime(&t, evs[(size_t)2 * k], evs[(size_t)2 * k + 1]));
         sum += t;
     }
-    if (ms_step) *ms_step = sum / steps;
+    if (ms_step) *ms_step = sum / ns;
     for (auto &e : evs) (void)hipEventDestroy(e);
     return 0;
 }

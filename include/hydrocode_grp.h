@@ -92,12 +92,13 @@ int hgrp2d_restore(hgrp2d *ctx, const void *buf, size_t bytes);
 /* Benchmark form: `steps` full steps X(tau/2) Y(tau) X(tau/2) with the frozen
  * tau = CFL * h/S_max, no time clipping, plot frames or host syncs inside.
  * ms_total: device time of all steps (HIP events on the solver's stream);
- * ms_x, ms_y (may be NULL): average device time of one X / one Y sweep launch. */
+ * ms_x, ms_y (may be NULL): average device time of one X / one Y sweep launch,
+ * from events around the launches of every 8th step (every step below 64). */
 int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double *ms_y);
 
 /* Benchmark form of the unsplit scheme: `steps` steps with tau = CFL * h/S_max.
  * ms_total: device time of all steps; ms_step (may be NULL): average device
- * time of one step launch (unsplit_step_kernel). */
+ * time of one step launch (unsplit_step_kernel), sampled as in hgrp2d_bench. */
 int hgrp2d_bench_unsplit(hgrp2d *ctx, int steps, double *ms_total, double *ms_step);
 
 /* h/S_max of the uploaded initial data (the frozen CFL reduction). */
