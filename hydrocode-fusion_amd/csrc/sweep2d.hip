@@ -224,31 +224,33 @@ __global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
     }
 }
 
-// X sweep, row-parallel form: a block of W waves covers 64 consecutive i and
-// the cell rows j0-1 .. j0+W-2 (wave w: row c = j0-1+w, lane: column i).
-// Neighbours along x live in the next wave, so values, slopes and then fluxes
-// are exchanged through LDS (one array each: three barriers per block).  Wave w solves interface c+1 (between rows c and
-// c+1); waves 1..W-1 update rows j0 .. j0+W-2.  Every interface row is one
-// wave, so the GRP work is spread over all rows of the grid at once.
-template <int W, int MINW, bool LEARNED>
-__global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
+// Line-parallel sweep along DIR: a block of W waves covers 64 consecutive
+// grid lines of the sweep direction (lane: line; DIR 0: x-line i, DIR 1:
+// y-line j) and the cells c0-1 .. c0+W-2 along them (wave w: cell c = c0-1+w).
+// Neighbours along the sweep live in the next wave, so values, slopes and then
+// fluxes are exchanged through LDS (one array each: three barriers per block).
+// Wave w solves interface c+1 (between cells c and c+1); waves 1..W-1 update
+// cells c0 .. c0+W-2.  The 64 lanes of a wave lie across the sweep, so an
+// interface front (a shock or contact crossing the lines) is met by the whole
+// wave together and the GRP branches stay coherent.  Used for DIR 0, where the
+// lanes read consecutive doubles.  (For DIR 1 each lane would read its own row,
+// stride ld: on MI355X that form ran as fast as sweep_y_kernel at 1024^2 and
+// 3x slower at 4096^2, where the rows no longer stay in cache.)
+template <int DIR, int W, bool LEARNED>
+__device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][64], double (*xb)[4][64],
+                                           double (*xf)[4][64], double (*xi)[4][64], double (*xe)[64])
 {
-    __shared__ double xa[W][4][64];   // values
-    __shared__ double xb[W][4][64];   // slopes
-    __shared__ double xf[W][4][64];   // fluxes
-    __shared__ double xi[W][4][64];   // interface values at t + tau
-    __shared__ double xe[W][64];      // E of the owned row (read back for the update)
     if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int i = blockIdx.x * 64 + lane;
-    const bool col = i < a.n;
-    const int m = a.m;
+    const int line = blockIdx.x * 64 + lane;
+    const bool col = line < (DIR == 0 ? a.n : a.m);
+    const Line<DIR> L(a, col ? line : 0);
+    const int m = L.cnt;                          // cells along the sweep
     const int j0 = blockIdx.y * (W - 1);
-    const int c = j0 - 1 + w;                     // owned row; -1 and m are ghost rows
-    const Line<0> L(a, col ? i : 0);
+    const int c = j0 - 1 + w;                     // owned cell; -1 and m are ghosts
     const double hh = a.hh;
 
-    // every global load of the owned row up front: values, previous slopes, E
+    // every global load of the owned cell up front: values, previous slopes, E
     const bool own = col && c >= 0 && c < m;
     double v[4] = {1.0, 0.0, 0.0, 1.0}, vl[4], vr[4], sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
     if (col && c <= m) L.value(c, v);
@@ -260,7 +262,7 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
     xe[w][lane] = e_c;
     if (col && w == 0 && c >= 0) L.value(c - 1, vl);
-    // the last wave also limits row c + 1 (the right side of its interface): its loads go out now too
+    // the last wave also limits cell c + 1 (the right side of its interface): its loads go out now too
     double vrr[4], sp1[4];
     const bool next_own = col && w == W - 1 && c + 1 < m;
     if (col && w == W - 1 && c + 1 <= m) L.value(c + 1, vr);
@@ -303,13 +305,13 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
         // xf[w] is not written before the solve: it holds this wave's network inputs
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        pg = interface_guess<0>(a, col && f <= m, SL, SR, (uint32_t *)&xf[w][0][0]);
+        pg = interface_guess<DIR>(a, col && f <= m, SL, SR, (uint32_t *)&xf[w][0][0]);
     }
     if (col && f <= m) {
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f, F, I,
-                           pg);
+        solve_interface<DIR>(a, SL, SR, (unsigned long long)line * (unsigned long long)(m + 1) + (unsigned long long)f,
+                             F, I, pg);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
@@ -321,6 +323,19 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
         for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; vc[q] = xa[w][q][lane]; }
         L.update_e(c, vc, xe[w][lane], Fl, F, Il, I);
     }
+}
+
+// X sweep, row-parallel form (line_sweep<0>): lanes are 64 consecutive i,
+// waves the cell rows j0-1 .. j0+W-2.
+template <int W, int MINW, bool LEARNED>
+__global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
+{
+    __shared__ double xa[W][4][64];   // values
+    __shared__ double xb[W][4][64];   // slopes
+    __shared__ double xf[W][4][64];   // fluxes
+    __shared__ double xi[W][4][64];   // interface values at t + tau
+    __shared__ double xe[W][64];      // E of the owned cell (read back for the update)
+    line_sweep<0, W, LEARNED>(a, xa, xb, xf, xi, xe);
 }
 
 // Y sweep: a wave owns cells c0-1 .. c0+62 of y-line j (lane t: cell c0-1+t),
@@ -343,11 +358,23 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     const double hh = a.hh;
 
     const bool own = c >= 0 && c < n;
+    // every global load of the wave up front, so that it pays one memory
+    // latency: the owned cell (values, previous slopes, E); lane 0 also the
+    // cell below it; lane 63 the two cells above it and the previous slopes of
+    // the first (it limits that cell, the right side of its interface)
     double w[4] = {1.0, 0.0, 0.0, 1.0}, sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
     if (c <= n) L.value(c, w);
-    if (own) {                                         // every global load of the owned cell up front
+    if (own) {
         L.prev_slopes(c, sp);
         e_c = a.e[L.at(c)];
+    }
+    const bool first = lane == 0 && c >= 0, last = lane == 63 && c + 1 <= n, last_own = lane == 63 && c + 1 < n;
+    double wl_e[4], wr_e[4], wrr[4], sp1[4];
+    if (first) L.value(c - 1, wl_e);
+    if (last) L.value(c + 1, wr_e);
+    if (last_own) {
+        L.value(c + 2, wrr);
+        L.prev_slopes(c + 1, sp1);
     }
     // each lane reads back only its own slots: no barrier needed
     double(*st)[64] = stash[(threadIdx.x >> 6) & 3];
@@ -357,8 +384,12 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     double wl[4], wr[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
-    if (lane == 0 && c >= 0) L.value(c - 1, wl);
-    if (lane == 63 && c + 1 <= n) L.value(c + 1, wr);
+    if (first)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wl[q] = wl_e[q];
+    if (last)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) wr[q] = wr_e[q];
 
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     if (own) L.limited_sp(wl, w, wr, sp, s);
@@ -369,13 +400,8 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
 #pragma unroll
     for (int q = 0; q < 4; ++q) sr[q] = __shfl_down(s[q], 1);
     if (lane == 63) {
-        if (c + 1 < n) {
-            double wrr[4];
-            L.value(c + 2, wrr);
-            L.limited(c + 1, w, wr, wrr, sr);
-        } else if (c + 1 == n) {
-            L.ghost_slope(false, sr);
-        }
+        if (last_own) L.limited_sp(w, wr, wrr, sp1, sr);
+        else if (c + 1 == n) L.ghost_slope(false, sr);
     }
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
