@@ -35,7 +35,7 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st);
-const char *unsplit_kernel_name();
+const char *unsplit_kernel_name(int m, int n);
 int unsplit_max_rows();
 }
 
@@ -234,7 +234,7 @@ extern "C" int hgrp2d_h_over_smax(hgrp2d *c, double *h)
 extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
 {
     if (!c || dir < 0 || dir > 2) return "";
-    if (dir == 2) return unsplit_kernel_name();
+    if (dir == 2) return unsplit_kernel_name(c->m, c->n);
     return sweep_kernel_name(dir, c->m, c->n, c->use_mlp);
 }
 

@@ -288,33 +288,180 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
     }
 }
 
+// Strip form: one wave per block walks down R cell rows of a 62-cell strip
+// (lanes: cells i0-1 .. i0+62 of the row).  Iteration r (from r0-1) forms the
+// side data of row r+1, solves the y-interfaces of row r and the x-interface
+// r+1 (one solve site, two passes), and updates row r from the x-interface r
+// kept from the iteration before.  No wave waits for another: the block is
+// one wave, its LDS holds two rows of side data and two rows of interface
+// results (16 KiB).  One extra x-interface per R rows.
+__global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int R)
+{
+    __shared__ double sd[2][12][64];   // side data of rows r and r + 1 (alternating)
+    __shared__ double fl[8][64];       // x-interface r: flux, values at t + tau
+    __shared__ double gb[8][64];       // y-interfaces of row r: flux, values at t + tau
+    if (a.prev != nullptr) {
+        const SweepStatus p0 = a.prev[0], p1 = a.prev[1];
+        if (p0.fail_key != 0ull || p0.flags != 0u || p1.fail_key != 0ull) {
+            if (threadIdx.x == 0) atomicOr(&a.status[0].flags, 2u);
+            return;
+        }
+    }
+    const int lane = threadIdx.x;
+    const int m = a.m, n = a.n;
+    const int ci = blockIdx.x * 62 - 1 + lane;
+    const int r0 = blockIdx.y * R, r1 = min(r0 + R, m);   // rows r0 .. r1-1 are updated here
+    const Grid G(a);
+    const bool yin = ci >= 0 && ci < n;
+    const bool colv = ci <= n;                             // a cell or a y-ghost column
+
+    auto put_side = [&](int buf, int c) {
+        double v[4] = {1.0, 0.0, 0.0, 1.0}, s[4] = {0.0, 0.0, 0.0, 0.0}, t[4] = {0.0, 0.0, 0.0, 0.0};
+        if (colv && c <= m && ((c >= 0 && c < m) || yin)) G.side(c, ci, v, s, t);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { sd[buf][q][lane] = v[q]; sd[buf][4 + q][lane] = s[q]; sd[buf][8 + q][lane] = t[q]; }
+    };
+    int cb = 0;
+    put_side(cb, r0 - 1);
+#pragma unroll 1
+    for (int r = r0 - 1; r < r1; ++r) {
+        const int nb = 1 - cb;
+        put_side(nb, r + 1);
+        __syncthreads();
+        const bool xin = r >= r0;                          // row r is updated (r0 <= r < r1 <= m)
+        double F[4], I[4];
+#pragma unroll 1
+        for (int pass = 0; pass < 2; ++pass) {
+            Side SL, SR;
+            double TL[4], TR[4];
+            unsigned long long idx = 0;
+            bool active;
+            if (pass == 0) {   // y-interface ci + 1 of row r: normal = y-slope, transversal = -(x-slope)
+                active = xin && lane <= 62 && ci <= n - 1;
+                if (active) {
+                    double vd[4], td[4], vu[4], tu[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        vd[q] = sd[cb][q][lane];     td[q] = sd[cb][8 + q][lane];     TL[q] = -sd[cb][4 + q][lane];
+                        vu[q] = sd[cb][q][lane + 1]; tu[q] = sd[cb][8 + q][lane + 1]; TR[q] = -sd[cb][4 + q][lane + 1];
+                    }
+                    SL = recon(vd, td, a.hh_y, true);
+                    SR = recon(vu, tu, a.hh_y, false);
+                    idx = (unsigned long long)r * (unsigned long long)(n + 1) + (unsigned long long)(ci + 1);
+                }
+            } else {           // x-interface r + 1 between rows r and r + 1
+                active = lane >= 1 && lane <= 62 && yin && r + 1 <= m;
+                if (active) {
+                    double vl[4], sl[4], vr[4], sr[4];
+#pragma unroll
+                    for (int q = 0; q < 4; ++q) {
+                        vl[q] = sd[cb][q][lane]; sl[q] = sd[cb][4 + q][lane]; TL[q] = sd[cb][8 + q][lane];
+                        vr[q] = sd[nb][q][lane]; sr[q] = sd[nb][4 + q][lane]; TR[q] = sd[nb][8 + q][lane];
+                    }
+                    SL = recon(vl, sl, a.hh_x, true);
+                    SR = recon(vr, sr, a.hh_x, false);
+                    idx = (unsigned long long)ci * (unsigned long long)(m + 1) + (unsigned long long)(r + 1);
+                }
+            }
+            double Fo[4] = {0.0, 0.0, 0.0, 0.0}, Io[4] = {0.0, 0.0, 0.0, 0.0};
+            if (active) solve_t(a, 1 - pass, SL, SR, TL, TR, idx, Fo, Io);
+            if (pass == 0) {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { gb[q][lane] = Fo[q]; gb[4 + q][lane] = Io[q]; }
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { F[q] = Fo[q]; I[q] = Io[q]; }
+        }
+        __syncthreads();
+        if (xin && yin && lane >= 1 && lane <= 62) {
+            const size_t o = G.at(r, ci);
+            const double e_c = a.e[o];
+            double vc[4], Fl[4], Ixl[4], Gy[4], Gl[4], Iy[4], Iyl[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                vc[q] = sd[cb][q][lane];
+                Fl[q] = fl[q][lane]; Ixl[q] = fl[4 + q][lane];
+                Gy[q] = gb[q][lane]; Iy[q] = gb[4 + q][lane];
+                Gl[q] = gb[q][lane - 1]; Iyl[q] = gb[4 + q][lane - 1];
+            }
+            const double nu = a.nu, mu = a.mu;
+            const double mom_x = vc[0] * vc[1] - nu * (F[1] - Fl[1]) - mu * (Gy[1] - Gl[1]);
+            const double mom_y = vc[0] * vc[2] - nu * (F[2] - Fl[2]) - mu * (Gy[2] - Gl[2]);
+            const double ene = vc[0] * e_c - nu * (F[3] - Fl[3]) - mu * (Gy[3] - Gl[3]);
+            const double rho = vc[0] + (-nu * (F[0] - Fl[0]) - mu * (Gy[0] - Gl[0]));
+            const double uu = mom_x / rho;
+            const double vv = mom_y / rho;
+            const double p = (ene - 0.5 * mom_x * uu - 0.5 * mom_y * vv) * a.gas.gm1;
+            a.rho_o[o] = rho; a.u_o[o] = uu; a.v_o[o] = vv; a.e_o[o] = ene / rho; a.p_o[o] = p;
+            if (p < a.eps || rho < a.eps) atomicOr(&a.status[0].flags, 1u);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                a.s_o[q][o] = div_by(I[q] - Ixl[q], a.div_hx);
+                a.t_o[q][o] = div_by(Iy[q] - Iyl[q], a.div_hy);
+            }
+        }
+        // x-interface r + 1 is the next row's lower interface (each lane keeps its own slot)
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { fl[q][lane] = F[q]; fl[4 + q][lane] = I[q]; }
+        cb = nb;
+    }
+}
+
 // ---- launchers -----------------------------------------------------------------
 // HGRP_UNSPLIT picks the block shape (read once, for tuning runs):
-//   1 = 8 waves (6 rows) per block, free register budget (default: no spills,
-//       2 waves/SIMD; 7% faster than the 4-waves/SIMD form with its spills at
+//   1 = 8 waves (6 rows) per block, free register budget (no spills, 2
+//       waves/SIMD; 7% faster than the 4-waves/SIMD form with its spills at
 //       1024^2 and 4096^2 on MI355X);
-//   2 = 4 waves (2 rows), 4 waves/SIMD;  3 = 8 waves, 4 waves/SIMD
-static int unsplit_variant()
+//   2 = 4 waves (2 rows), 4 waves/SIMD;  3 = 8 waves, 4 waves/SIMD;
+//   4 = the strip form (one wave per block walking down its rows).
+// Without it, unsplit_variant picks 1 or 4 by grid size.
+static int unsplit_env()
 {
     static int v = -1;
     if (v < 0) {
         const char *e = getenv("HGRP_UNSPLIT");
         v = e ? atoi(e) : 0;
-        if (v < 1 || v > 3) v = 1;
+        if (v < 1 || v > 4) v = 0;
     }
     return v;
 }
 
-// rows of cells one block updates
-static int unsplit_rows() { return unsplit_variant() == 2 ? 2 : 6; }
+// rows per strip of the strip form: enough strips to give every CU several waves
+static int strip_rows(int m, int n)
+{
+    const long strips = (long)((n + 61) / 62);
+    int R = 32;
+    while (R > 4 && strips * ((m + R - 1) / R) < 4L * 2048) R /= 2;
+    return R;
+}
 
-int unsplit_max_rows() { return 65535 * unsplit_rows(); }
+// The strip form where it gets long strips (R >= 16: its extra x-interface
+// costs 1/R), else the tile form: on MI355X the strip form is 5 % faster at
+// 4096^2 (R = 32) and 1.5 % slower at 1024^2 (R = 4).
+static int unsplit_variant(int m, int n)
+{
+    const int v = unsplit_env();
+    if (v) return v;
+    return strip_rows(m, n) >= 16 ? 4 : 1;
+}
+
+// rows of cells one block of the tile forms updates
+static int unsplit_rows(int v) { return v == 2 ? 2 : 6; }
+
+int unsplit_max_rows() { return 65535 * 4; }
 
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
 {
-    const int rows = unsplit_rows();
+    const int v = unsplit_variant(a.m, a.n);
+    if (v == 4) {
+        const int R = strip_rows(a.m, a.n);
+        hipLaunchKernelGGL(unsplit_strip_kernel, dim3((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R)),
+                           dim3(64), 0, st, a, R);
+        return hipGetLastError();
+    }
+    const int rows = unsplit_rows(v);
     const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + rows - 1) / rows));
-    switch (unsplit_variant()) {
+    switch (v) {
     case 2: hipLaunchKernelGGL((unsplit_step_kernel<4, 4>), grid, dim3(256), 0, st, a); break;
     case 3: hipLaunchKernelGGL((unsplit_step_kernel<8, 4>), grid, dim3(512), 0, st, a); break;
     default: hipLaunchKernelGGL((unsplit_step_kernel<8, 1>), grid, dim3(512), 0, st, a); break;
@@ -322,11 +469,12 @@ hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
     return hipGetLastError();
 }
 
-const char *unsplit_kernel_name()
+const char *unsplit_kernel_name(int m, int n)
 {
-    switch (unsplit_variant()) {
+    switch (unsplit_variant(m, n)) {
     case 2: return "unsplit_step_kernel<4, 4>";
     case 3: return "unsplit_step_kernel<8, 4>";
+    case 4: return "unsplit_strip_kernel";
     default: return "unsplit_step_kernel<8, 1>";
     }
 }
