@@ -338,37 +338,45 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     line_sweep<0, W, LEARNED>(a, xa, xb, xf, xi, xe);
 }
 
-// Y sweep: a wave owns cells c0-1 .. c0+62 of y-line j (lane t: cell c0-1+t),
-// computes interfaces c0 .. c0+63 (lane t: interface c0+t) and updates cells
-// c0 .. c0+62 (lane t >= 1).
-template <int MINW, bool LEARNED>
+// Y sweep: a wave holds 64/G y-lines (rows j) of G lanes each; lane k of a
+// row owns cell c0-1+k, computes interface c0+k and updates cell c0-1+k for
+// k >= 1 (G-1 cells per row).  With G = 64 a wave is one stretch of one line;
+// smaller G stacks rows, so that a front lying along y (constant i) meets
+// fewer waves, each with more lanes on it.
+template <int MINW, bool LEARNED, int G = 64>
 __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chunks)
 {
+    static_assert(G == 64 || G == 32 || G == 16 || G == 8, "lanes per row");
+    constexpr int ROWS = 64 / G;
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
     __shared__ double stash[4][5][64];   // this lane's cell values and E, read back for the update
     if (sweep_halted(a)) return;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, k = lane % G;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int j = (int)(wave / chunks);
-    if (j >= a.m) return;                              // uniform per wave
-    const int c0 = (int)(wave % chunks) * 63;
+    const int jg = (int)(wave / chunks);
+    if (jg * ROWS >= a.m) return;                      // uniform per wave
+    const int j = jg * ROWS + lane / G;
+    const bool rowv = j < a.m;
+    const int c0 = (int)(wave % chunks) * (G - 1);
     const int n = a.n;
-    const int c = c0 - 1 + lane;
-    const Line<1> L(a, j);
+    const int c = c0 - 1 + k;
+    const Line<1> L(a, rowv ? j : 0);
     const double hh = a.hh;
 
-    const bool own = c >= 0 && c < n;
+    const bool own = rowv && c >= 0 && c < n;
     // every global load of the wave up front, so that it pays one memory
-    // latency: the owned cell (values, previous slopes, E); lane 0 also the
-    // cell below it; lane 63 the two cells above it and the previous slopes of
-    // the first (it limits that cell, the right side of its interface)
+    // latency: the owned cell (values, previous slopes, E); the first lane of a
+    // row also the cell below it; the last lane the two cells above it and the
+    // previous slopes of the first (it limits that cell, the right side of its
+    // interface)
     double w[4] = {1.0, 0.0, 0.0, 1.0}, sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
-    if (c <= n) L.value(c, w);
+    if (rowv && c <= n) L.value(c, w);
     if (own) {
         L.prev_slopes(c, sp);
         e_c = a.e[L.at(c)];
     }
-    const bool first = lane == 0 && c >= 0, last = lane == 63 && c + 1 <= n, last_own = lane == 63 && c + 1 < n;
+    const bool first = rowv && k == 0 && c >= 0, last = rowv && k == G - 1 && c + 1 <= n;
+    const bool last_own = rowv && k == G - 1 && c + 1 < n;
     double wl_e[4], wr_e[4], wrr[4], sp1[4];
     if (first) L.value(c - 1, wl_e);
     if (last) L.value(c + 1, wr_e);
@@ -381,6 +389,7 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
 #pragma unroll
     for (int q = 0; q < 4; ++q) st[q][lane] = w[q];
     st[4][lane] = e_c;
+    // neighbours along the row from the next lanes; a row's end lanes take the loaded ones
     double wl[4], wr[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
@@ -393,26 +402,27 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
 
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     if (own) L.limited_sp(wl, w, wr, sp, s);
-    else if (c == -1) L.ghost_slope(true, s);
-    else if (c == n) L.ghost_slope(false, s);
+    else if (rowv && c == -1) L.ghost_slope(true, s);
+    else if (rowv && c == n) L.ghost_slope(false, s);
 
     double sr[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) sr[q] = __shfl_down(s[q], 1);
-    if (lane == 63) {
+    if (rowv && k == G - 1) {
         if (last_own) L.limited_sp(w, wr, wrr, sp1, sr);
         else if (c + 1 == n) L.ghost_slope(false, sr);
     }
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
     const int f = c + 1;
+    const bool face = rowv && f <= n;
     double pg = -1.0;
     if constexpr (LEARNED) {
         const Side SL = recon(w, s, hh, true);
         const Side SR = recon(wr, sr, hh, false);
-        pg = interface_guess<1>(a, f <= n, SL, SR, mlp_lds[(threadIdx.x >> 6) & 3]);
+        pg = interface_guess<1>(a, face, SL, SR, mlp_lds[(threadIdx.x >> 6) & 3]);
     }
-    if (f <= n) {
+    if (face) {
         const Side SL = recon(w, s, hh, true);
         const Side SR = recon(wr, sr, hh, false);
         solve_interface<1>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I,
@@ -421,7 +431,7 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     double Fl[4], Il[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
-    if (lane >= 1 && own) {
+    if (k >= 1 && own) {
         double wc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) wc[q] = st[q][lane];
@@ -459,12 +469,14 @@ static constexpr int TJ_X = 32;
 // Launch variants.  The defaults were picked by timing four-quadrant runs at
 // 1024^2 (whole run) and 4096^2 (200 steps) on MI355X: the X sweep with 8 rows
 // per block and the register budget for 4 waves/SIMD (128 VGPRs, a few
-// spills), the Y sweep with the same budget; both beat the spill-free
-// 2-waves/SIMD forms at every size measured.
+// spills); the Y sweep with the same budget and 4 rows of 16 lanes per wave
+// (12 % faster than one 64-lane stretch per wave at 1024^2, 6 % at 4096^2;
+// 2 rows of 32 and 8 rows of 8 lie between).
 // HGRP_SWEEP_X / HGRP_SWEEP_Y override them (read once, for tuning runs):
 //   HGRP_SWEEP_X  1 = 8 rows, free budget;  2 = 16 rows, 4 waves/SIMD;
 //                 3 = sliding register window;  4 = 8 rows, 4 waves/SIMD
-//   HGRP_SWEEP_Y  1 = free budget;  2 = 3 waves/SIMD;  3 = 4 waves/SIMD
+//   HGRP_SWEEP_Y  1 = one 64-lane row per wave, free budget;  2 = the same, 3 waves/SIMD;
+//                 3 = the same, 4 waves/SIMD;  4, 5, 6 = 2, 4, 8 rows of 32, 16, 8 lanes, 4 waves/SIMD
 static int env_variant(const char *name, int hi)
 {
     const char *e = getenv(name);
@@ -481,8 +493,8 @@ static int sweep_x_variant(const SweepArgs &a)
 static int sweep_y_variant()
 {
     static int v = -1;
-    if (v < 0) v = env_variant("HGRP_SWEEP_Y", 3);
-    return v != 0 ? v : 3;
+    if (v < 0) v = env_variant("HGRP_SWEEP_Y", 6);
+    return v != 0 ? v : 5;
 }
 
 hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
@@ -523,20 +535,26 @@ hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
 
 hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st)
 {
-    const int chunks = (a.n + 62) / 63;
-    const long waves = (long)a.m * chunks;
-    const long blocks = (waves + 3) / 4;
+    const int yv = sweep_y_variant();
+    const int G = yv == 4 ? 32 : yv == 5 ? 16 : yv == 6 ? 8 : 64, rows = 64 / G;
+    const int chunks = (a.n + G - 2) / (G - 1);
+    const long waves = (long)((a.m + rows - 1) / rows) * chunks;
+    const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
     if (a.mlp) {
-        if (sweep_y_variant() == 1)
-            hipLaunchKernelGGL((sweep_y_kernel<1, true>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
-        else
-            hipLaunchKernelGGL((sweep_y_kernel<4, true>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+        switch (yv) {
+        case 1: hipLaunchKernelGGL((sweep_y_kernel<1, true>), grid, block, 0, st, a, chunks); break;
+        case 2: case 3: hipLaunchKernelGGL((sweep_y_kernel<4, true>), grid, block, 0, st, a, chunks); break;
+        default: hipLaunchKernelGGL((sweep_y_kernel<4, true, 16>), grid, block, 0, st, a, chunks); break;
+        }
         return hipGetLastError();
     }
-    switch (sweep_y_variant()) {
-    case 1: hipLaunchKernelGGL((sweep_y_kernel<1, false>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
-    case 2: hipLaunchKernelGGL((sweep_y_kernel<3, false>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
-    default: hipLaunchKernelGGL((sweep_y_kernel<4, false>), dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    switch (yv) {
+    case 1: hipLaunchKernelGGL((sweep_y_kernel<1, false>), grid, block, 0, st, a, chunks); break;
+    case 2: hipLaunchKernelGGL((sweep_y_kernel<3, false>), grid, block, 0, st, a, chunks); break;
+    case 3: hipLaunchKernelGGL((sweep_y_kernel<4, false>), grid, block, 0, st, a, chunks); break;
+    case 4: hipLaunchKernelGGL((sweep_y_kernel<4, false, 32>), grid, block, 0, st, a, chunks); break;
+    case 6: hipLaunchKernelGGL((sweep_y_kernel<4, false, 8>), grid, block, 0, st, a, chunks); break;
+    default: hipLaunchKernelGGL((sweep_y_kernel<4, false, 16>), grid, block, 0, st, a, chunks); break;
     }
     return hipGetLastError();
 }
@@ -546,11 +564,16 @@ const char *sweep_kernel_name(int dir, int m, int n, bool learned)
     SweepArgs a{};
     a.m = m; a.n = n;
     if (dir == 1) {
-        if (learned) return sweep_y_variant() == 1 ? "sweep_y_kernel<1, true>" : "sweep_y_kernel<4, true>";
-        switch (sweep_y_variant()) {
+        const int yv = sweep_y_variant();
+        if (learned) return yv == 1 ? "sweep_y_kernel<1, true>" : yv <= 3 ? "sweep_y_kernel<4, true>"
+                                                                         : "sweep_y_kernel<4, true, 16>";
+        switch (yv) {
         case 1: return "sweep_y_kernel<1, false>";
         case 2: return "sweep_y_kernel<3, false>";
-        default: return "sweep_y_kernel<4, false>";
+        case 3: return "sweep_y_kernel<4, false>";
+        case 4: return "sweep_y_kernel<4, false, 32>";
+        case 6: return "sweep_y_kernel<4, false, 8>";
+        default: return "sweep_y_kernel<4, false, 16>";
         }
     }
     if (learned) return sweep_x_variant(a) == 4 ? "sweep_x_rows_kernel<8, 4, true>" : "sweep_x_rows_kernel<8, 1, true>";

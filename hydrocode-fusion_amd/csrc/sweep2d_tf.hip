@@ -278,27 +278,41 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 }
 
 // Y sweep: a wave owns 64 consecutive cells of a y-line (as sweep_y_kernel)
-template <bool LEARNED>
+template <bool LEARNED, int G = 16>
 __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chunks)
 {
+    // 64/G y-lines of G lanes per wave, as in sweep_y_kernel: lane k of a row
+    // owns cell c0-1+k and updates it for k >= 1
+    constexpr int ROWS = 64 / G;
     __shared__ double stash[4][NQ + 1][64];
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
     if (tf_halted(a)) return;
-    const int lane = threadIdx.x & 63;
+    const int lane = threadIdx.x & 63, k = lane % G;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int j = (int)(wave / chunks);
-    if (j >= a.m) return;
-    const int c0 = (int)(wave % chunks) * 63;
+    const int jg = (int)(wave / chunks);
+    if (jg * ROWS >= a.m) return;
+    const int j = jg * ROWS + lane / G;
+    const bool rowv = j < a.m;
+    const int c0 = (int)(wave % chunks) * (G - 1);
     const int n = a.n;
-    const int c = c0 - 1 + lane;
-    const LineTF<1> L(a, j);
+    const int c = c0 - 1 + k;
+    const LineTF<1> L(a, rowv ? j : 0);
 
-    const bool own = c >= 0 && c < n;
+    const bool own = rowv && c >= 0 && c < n;
     double w[NQ] = {1.0, 0.0, 0.0, 1.0, 1.0}, sp[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
-    if (c <= n) L.value(c, w);
+    if (rowv && c <= n) L.value(c, w);
     if (own) {
         L.prev_slopes(c, sp);
         e_c = a.e[L.at(c)];
+    }
+    const bool first = rowv && k == 0 && c >= 0, last = rowv && k == G - 1 && c + 1 <= n;
+    const bool last_own = rowv && k == G - 1 && c + 1 < n;
+    double wl_e[NQ], wr_e[NQ], wrr[NQ], sp1[NQ];
+    if (first) L.value(c - 1, wl_e);
+    if (last) L.value(c + 1, wr_e);
+    if (last_own) {
+        L.value(c + 2, wrr);
+        L.prev_slopes(c + 1, sp1);
     }
     double(*st)[64] = stash[(threadIdx.x >> 6) & 3];
 #pragma unroll
@@ -307,32 +321,31 @@ __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chu
     double wl[NQ], wr[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
-    if (lane == 0 && c >= 0) L.value(c - 1, wl);
-    if (lane == 63 && c + 1 <= n) L.value(c + 1, wr);
+    if (first)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) wl[q] = wl_e[q];
+    if (last)
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) wr[q] = wr_e[q];
 
     double s[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0};
     if (own) L.limited_sp(wl, w, wr, sp, s);
-    else if (c == -1) L.ghost_slope(true, s);
-    else if (c == n) L.ghost_slope(false, s);
+    else if (rowv && c == -1) L.ghost_slope(true, s);
+    else if (rowv && c == n) L.ghost_slope(false, s);
     double sr[NQ];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) sr[q] = __shfl_down(s[q], 1);
-    if (lane == 63) {
-        if (c + 1 < n) {
-            double wrr[NQ], sp1[NQ];
-            L.value(c + 2, wrr);
-            L.prev_slopes(c + 1, sp1);
-            L.limited_sp(w, wr, wrr, sp1, sr);
-        } else if (c + 1 == n) {
-            L.ghost_slope(false, sr);
-        }
+    if (rowv && k == G - 1) {
+        if (last_own) L.limited_sp(w, wr, wrr, sp1, sr);
+        else if (c + 1 == n) L.ghost_slope(false, sr);
     }
 
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0}, Z[3] = {0.0, 0.0, 0.0};
     const int f = c + 1;
+    const bool face = rowv && f <= n;
     double pg = -1.0;
-    if constexpr (LEARNED) pg = interface_guess_tf<1>(a, f <= n, w, s, wr, sr, mlp_lds[(threadIdx.x >> 6) & 3]);
-    if (f <= n)
+    if constexpr (LEARNED) pg = interface_guess_tf<1>(a, face, w, s, wr, sr, mlp_lds[(threadIdx.x >> 6) & 3]);
+    if (face)
         solve_interface_tf<1>(a, w, s, wr, sr, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f,
                               F, I, Z, pg);
     double Fl[4], Il[4], Zl[3];
@@ -340,7 +353,7 @@ __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chu
     for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
 #pragma unroll
     for (int q = 0; q < 3; ++q) Zl[q] = __shfl_up(Z[q], 1);
-    if (lane >= 1 && own) {
+    if (k >= 1 && own) {
         double wc[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) wc[q] = st[q][lane];
@@ -417,8 +430,9 @@ hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st)
 
 hipError_t launch_sweep_y_tf(const SweepArgs &a, hipStream_t st)
 {
-    const int chunks = (a.n + 62) / 63;
-    const long waves = (long)a.m * chunks;
+    // 4 rows of 16 lanes per wave (the default of the single-fluid Y sweep)
+    const int chunks = (a.n + 14) / 15;
+    const long waves = (long)((a.m + 3) / 4) * chunks;
     const long blocks = (waves + 3) / 4;
     if (a.mlp)
         hipLaunchKernelGGL(sweep_y_tf_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
