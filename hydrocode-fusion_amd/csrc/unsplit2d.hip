@@ -8,11 +8,12 @@
 //                        limited y-slope, or the ghost's value and slopes at the
 //                        edges -- and parks it in LDS.  Waves 0..W-2 solve the
 //                        x-interface between their row and the next; waves
-//                        1..W-2 also solve the y-interfaces of their row (lane t:
-//                        between lanes t and t+1) and update their 62 cells with
-//                        both flux differences.  Rows j0-1 and j0+W-2 are halo
-//                        rows; interface rows and columns on tile edges are
-//                        solved by both tiles, everything else once.
+//                        1..W-2 also solve the y-interfaces of the updated rows
+//                        (each half a pair of rows, 2 x 32 interfaces) and
+//                        update their 62 cells with both flux differences.
+//                        Rows j0-1 and j0+W-2 are halo rows; interface rows and
+//                        columns on tile edges are solved by both tiles,
+//                        everything else once.
 //
 // Semantics follow src/finite_volume/grp_solver_2D_EUL_source.c:181-262,
 // src/inter_process/bound_cond_slope_limiter_{x,y}.c:27-129 (including the
@@ -182,8 +183,9 @@ __device__ __forceinline__ void solve_t(const UnsplitArgs &a, int dir, const Sid
 template <int W, int MINW>
 __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs a)
 {
-    __shared__ double sd[W][12][64];   // side data of row c: value, x-slope, y-slope
+    __shared__ double sd[W][12][64];     // side data of row c: value, x-slope, y-slope
     __shared__ double fx[W - 1][8][64];  // x-interface c+1: flux, values at t + tau
+    __shared__ double gy[W - 2][8][64];  // y-interfaces of the updated rows: flux, values at t + tau
     if (a.prev != nullptr) {           // a failed step ends the run: later steps do nothing
         const SweepStatus p0 = a.prev[0], p1 = a.prev[1];
         if (p0.fail_key != 0ull || p0.flags != 0u || p1.fail_key != 0ull) {
@@ -207,11 +209,16 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
 
     // pass 0: x-interface f = c + 1 between rows c and c + 1 (flux_generator_x.c:42-121),
     //         the right side from the next wave;
-    // pass 1: y-interface ci + 1 between lanes t and t + 1 of row c (flux_generator_y.c:42-121),
-    //         normal derivative = y-slope, transversal = -(x-slope).
+    // pass 1: y-interfaces of the updated rows (flux_generator_y.c:42-121), normal derivative =
+    //         y-slope, transversal = -(x-slope); wave w takes half a row pair (2 rows x 32
+    //         interfaces), so a front lying along y meets half as many waves as with one
+    //         64-lane row per wave.
     // One solve site for both, so the GRP code and its registers are laid out once.
     const int f = c + 1;
-    double Gy[4], Iy[4];
+    const int task = w - 1;                                  // waves 1 .. W-2
+    const int yr = 1 + 2 * (task >> 1) + (lane >> 5);        // tile row of this lane's y-interface
+    const int yt = ((task & 1) << 5) + (lane & 31);          // between tile lanes yt and yt + 1
+    const int yc = j0 - 1 + yr, ycell = i0 - 1 + yt;
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
         Side SL, SR;
@@ -232,17 +239,17 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
                 idx = (unsigned long long)ci * (unsigned long long)(m + 1) + (unsigned long long)f;
             }
         } else {
-            active = w >= 1 && w <= W - 2 && xin && lane <= 62 && ci <= n - 1;
+            active = w >= 1 && w <= W - 2 && yc >= 0 && yc < m && yt <= 62 && ycell <= n - 1;
             if (active) {
                 double vd[4], td[4], vu[4], tu[4];
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    vd[q] = sd[w][q][lane];     td[q] = sd[w][8 + q][lane];     TL[q] = -sd[w][4 + q][lane];
-                    vu[q] = sd[w][q][lane + 1]; tu[q] = sd[w][8 + q][lane + 1]; TR[q] = -sd[w][4 + q][lane + 1];
+                    vd[q] = sd[yr][q][yt];     td[q] = sd[yr][8 + q][yt];     TL[q] = -sd[yr][4 + q][yt];
+                    vu[q] = sd[yr][q][yt + 1]; tu[q] = sd[yr][8 + q][yt + 1]; TR[q] = -sd[yr][4 + q][yt + 1];
                 }
                 SL = recon(vd, td, a.hh_y, true);
                 SR = recon(vu, tu, a.hh_y, false);
-                idx = (unsigned long long)c * (unsigned long long)(n + 1) + (unsigned long long)(ci + 1);
+                idx = (unsigned long long)yc * (unsigned long long)(n + 1) + (unsigned long long)(ycell + 1);
             }
         }
         double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
@@ -251,24 +258,25 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
 #pragma unroll
             for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
         }
+        if (pass == 1 && w >= 1 && w <= W - 2) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { Gy[q] = F[q]; Iy[q] = I[q]; }
+            for (int q = 0; q < 4; ++q) { gy[yr - 1][q][yt] = F[q]; gy[yr - 1][4 + q][yt] = I[q]; }
+        }
     }
-    double Gl[4], Iyl[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { Gl[q] = __shfl_up(Gy[q], 1); Iyl[q] = __shfl_up(Iy[q], 1); }
     __syncthreads();
 
     // combined update and slope refresh (grp_solver_2D_EUL_source.c:231-262)
     if (w >= 1 && w <= W - 2 && xin && yin && lane >= 1 && lane <= 62) {
         const size_t o = G.at(c, ci);
         const double e_c = a.e[o];
-        double vc[4], Fl[4], Ixl[4], Fr[4], Ixr[4];
+        double vc[4], Fl[4], Ixl[4], Fr[4], Ixr[4], Gy[4], Iy[4], Gl[4], Iyl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             vc[q] = sd[w][q][lane];
             Fl[q] = fx[w - 1][q][lane]; Ixl[q] = fx[w - 1][4 + q][lane];
             Fr[q] = fx[w][q][lane];     Ixr[q] = fx[w][4 + q][lane];
+            Gl[q] = gy[w - 1][q][lane - 1]; Iyl[q] = gy[w - 1][4 + q][lane - 1];
+            Gy[q] = gy[w - 1][q][lane];     Iy[q] = gy[w - 1][4 + q][lane];
         }
         const double nu = a.nu, mu = a.mu;
         const double mom_x = vc[0] * vc[1] - nu * (Fr[1] - Fl[1]) - mu * (Gy[1] - Gl[1]);

@@ -6,6 +6,8 @@ Bar: relative L-infinity difference per field <= 1e-10 at final time, as for
 the split path (the device's FP64 pow may differ from glibc's in the last bit;
 everything else follows the reference's expression order).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -144,7 +146,8 @@ def test_conservation_full_size(hg):
     init = cases.four_quadrant(m, n)
     s = hg.Solver2D(m, n, cfg)
     try:
-        assert s.sweep_kernel(2) == "unsplit_strip_kernel"
+        if not os.environ.get("HGRP_UNSPLIT"):          # the launcher's own choice at this size
+            assert s.sweep_kernel(2) == "unsplit_strip_kernel"
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         r0, u0, v0, p0, e0 = s.download()
         res = s.solve(np.array([0.0, np.inf]), 2)
