@@ -194,3 +194,23 @@ def test_checkpoint_resume_is_bitwise(hg):
         b.close()
     assert (ra.k, ra.nt, ra.status, ra.time_c) == (rb.k, rb.nt, rb.status, rb.time_c)
     assert np.array_equal(np.asarray(full), np.asarray(resumed))
+
+
+def test_unsplit_rejects_two_fluids(hg):
+    """The two-fluid mode runs the split scheme only: an unsplit solve of a
+    context with two fluids is refused with an argument error, state untouched."""
+    m, n = 16, 12
+    cfg = make_config(t_all=0.05, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-2, split=0)
+    init = cases.smooth_random(m, n, 4)
+    s = hg.Solver2D(m, n, cfg)
+    try:
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.set_fluids(1.67, np.ones((m, n)))
+        before = np.asarray(s.download())
+        with pytest.raises(RuntimeError, match="hgrp2d_solve"):
+            s.solve(np.array([0.0, 0.05]), 2)
+        with pytest.raises(RuntimeError, match="hgrp2d_bench_unsplit"):
+            s.bench_unsplit(2)
+        assert np.array_equal(before, np.asarray(s.download()))
+    finally:
+        s.close()
