@@ -10,7 +10,8 @@ comes from HIP events on the solver's stream (hgrp2d_bench).
 Prints one JSON line (rank 0).  `roofline` reports the dominant kernel (the
 x-sweep, two launches per step) against HBM peak with the algorithmic bytes of
 a sweep (128 B per cell: read and write rho u v p and their four slopes);
-`cpu_baseline` times the oracle (the C99 restatement of the reference, OpenMP)
+`cpu_baseline` times the reference's own split solver (its sources built in
+oracle/_ref, OpenMP; the C99 restatement in oracle/ where that build is absent)
 on a bounded number of steps of the same workload on the host's cores.
 `learned_guess` compares, at 4096^2 with Newton tolerance 1e-12, the plain
 solve with the one started from the learned p* guess (rate, and the relative L1
@@ -179,7 +180,7 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
     import numpy as np
     import hydrogrp
     import cases
-    from _libs import Oracle, make_config
+    from _libs import make_config
     cfg = cases.four_quadrant_config(M, N, split=0)
     init = cases.four_quadrant(M, N)
     s = hydrogrp.Solver2D(M, N, cfg)
@@ -199,7 +200,7 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, M, N),
                         "ms_per_launch": ms_step}}
     if with_cpu:
-        orc = Oracle()
+        run2d, kind, label, threads = _cpu_solver(split=False)
         h = 1.0 / M
         c = np.sqrt(1.4 * init["p"] / init["rho"])
         tau = 0.45 * np.min(h / (c + np.abs(init["u"]) + np.abs(init["v"])))
@@ -208,24 +209,57 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
             cc = make_config(N=k, tau=tau, h_x=h, h_y=h, bc_x=-4, bc_y=-4, split=0)
             cc[1] = np.inf
             t0 = time.perf_counter()
-            orc.unsplit2d(M, N, init, cc, time_plot=np.array([0.0, np.inf]))
+            run2d(M, N, init, cc, time_plot=np.array([0.0, np.inf]))
             return time.perf_counter() - t0
 
         k = max(2, int(cpu_seconds / max(run(1), 1e-3)))
         t = run(k)
-        threads = int(orc.lib.orc_threads())
-        out["cpu_baseline"] = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-                               "sample": f"{k} unsplit steps of the same {M}x{N} workload ({t:.1f} s), "
-                                         f"oracle/liboracle.so (gcc -O2 -fopenmp, {threads} threads)"}
+        out["cpu_baseline"] = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+                               "sample": f"{k} unsplit steps of the same {M}x{N} workload ({t:.1f} s), {label}, "
+                                         f"{threads} threads"}
     return out
 
 
+def _quiet(fn, *args, **kw):
+    """fn(*args) with the process's C-level stdout sent to /dev/null: the
+    reference's solvers draw a progress bar on every step."""
+    import ctypes
+    sys.stdout.flush()
+    saved, devnull = os.dup(1), os.open(os.devnull, os.O_WRONLY)
+    try:
+        os.dup2(devnull, 1)
+        return fn(*args, **kw)
+    finally:
+        ctypes.CDLL(None).fflush(None)
+        os.dup2(saved, 1)
+        os.close(saved)
+        os.close(devnull)
+
+
+def _cpu_solver(split):
+    """The CPU path timed as the baseline: the reference's own sources built in
+    oracle/_ref where that build is present (kind "reference"), else the oracle
+    (kind "port").  Returns (run2d(m, n, init, cfg, time_plot), kind, label, threads)."""
+    from _libs import Oracle, RefLib
+    orc = Oracle()
+    threads = int(orc.lib.orc_threads())       # the same OpenMP runtime and environment
+    try:
+        ref = RefLib()
+        fn = ref.split2d if split else ref.unsplit2d
+        return (lambda *a, **kw: _quiet(fn, *a, **kw)), "reference", \
+            "oracle/_ref/libhydroref.so (the reference's own sources, gcc -O2 -fopenmp)", threads
+    except OSError:
+        fn = orc.split2d if split else orc.unsplit2d
+        return fn, "port", "oracle/liboracle.so (C99 restatement, gcc -O2 -fopenmp)", threads
+
+
 def cpu_leg(target_seconds=12.0):
-    """The oracle on the host cores, fixed-tau mode (exactly the frozen tau), bounded sample."""
+    """The reference's CPU solver on the host cores, fixed-tau mode (exactly the
+    frozen tau), bounded sample."""
     import numpy as np
     import cases
-    from _libs import Oracle, make_config
-    orc = Oracle()
+    from _libs import make_config
+    run2d, kind, label, threads = _cpu_solver(split=True)
     init = cases.four_quadrant(M, N)
     h = 1.0 / M
     c = np.sqrt(1.4 * init["p"] / init["rho"])
@@ -235,16 +269,14 @@ def cpu_leg(target_seconds=12.0):
         cfg = make_config(N=k, tau=tau, h_x=h, h_y=h, bc_x=-4, bc_y=-4, split=1)
         cfg[1] = np.inf
         t0 = time.perf_counter()
-        orc.split2d(M, N, init, cfg, time_plot=np.array([0.0, np.inf]))
+        run2d(M, N, init, cfg, time_plot=np.array([0.0, np.inf]))
         return time.perf_counter() - t0
 
     t1 = run(1)
     k = max(2, int(target_seconds / max(t1, 1e-3)))
     t = run(k)
-    threads = orc.lib.orc_threads()
-    return {"value": M * N * k / t, "unit": "cell-updates/s", "cores": int(threads), "kind": "port",
-            "sample": f"{k} steps of the same {M}x{N} four-quadrant workload ({t:.1f} s), "
-                      f"oracle/liboracle.so (C99 restatement, gcc -O2 -fopenmp, {threads} threads)"}
+    return {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+            "sample": f"{k} steps of the same {M}x{N} four-quadrant workload ({t:.1f} s), {label}, {threads} threads"}
 
 
 def main():
