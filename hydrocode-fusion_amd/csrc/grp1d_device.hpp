@@ -39,12 +39,12 @@ struct Side1 {
 
 // linear_GRP_solver_Edir with atc = eps: D = time derivatives, U = values of
 // (rho, u, p) on the t-axis.
-__device__ inline void grp_edir(double D[3], double U[3], const Side1 &L, const Side1 &R, const Gas &G, double eps)
+__device__ inline void grp_edir(double D[3], double U[3], const Side1 &L, const Side1 &R, const Gas &G, double eps,
+                                double atc)
 {
     const double rho_L = L.rho, rho_R = R.rho, u_L = L.u, u_R = R.u, p_L = L.p, p_R = R.p;
     const double s_rho_L = L.s_rho, s_rho_R = R.s_rho, s_u_L = L.s_u, s_u_R = R.s_u, s_p_L = L.s_p, s_p_R = R.s_p;
     const double gamma = G.g, zeta = G.z, zts = G.zz;
-    const double atc = eps;
 
     int crw0 = 0, crw1 = 0;
     double u_star, p_star, rs_L, rs_R, cs_L, cs_R;
@@ -217,6 +217,12 @@ __device__ inline void grp_edir(double D[3], double U[3], const Side1 &L, const 
             D[0] = (f * u_star - g_p * p_t_mat - g_u * u_t_mat) / g_rho;
         }
     }
+}
+
+// the 1D solver's call: atc = eps (grp_solver_EUL_source.c)
+__device__ inline void grp_edir(double D[3], double U[3], const Side1 &L, const Side1 &R, const Gas &G, double eps)
+{
+    grp_edir(D, U, L, R, G, eps, eps);
 }
 
 }  // namespace hgrp

@@ -158,6 +158,7 @@ constexpr int GUESS_ITERS = 3;
 
 struct StarResult {
     double u_star, p_star;
+    double gap;    // the solver's return value: |v_L - v_R| at the end (|u_L - u_R| on the early exit)
     int crw0, crw1;
     int iters;     // Newton iterations taken (-1: equal-state early exit)
 };
@@ -228,6 +229,7 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
     if (fabs(u_L - u_R) < tol && fabs(p_L - p_R) < tol) {
         r.p_star = 0.5 * (p_L + p_R);
         r.u_star = 0.5 * (u_L + u_R);
+        r.gap = fabs(u_L - u_R);
         r.iters = -1;
         return r;
     }
@@ -245,6 +247,7 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
         if (gap <= tol) {
             r.p_star = p;
             r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
+            r.gap = gap;
             r.iters = done;
             return r;
         }
@@ -263,6 +266,7 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
     }
     r.p_star = p_INT;
     r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
+    r.gap = gap;
     r.iters = n + done;
     return r;
 }
@@ -288,15 +292,25 @@ __device__ __forceinline__ void side_trivial(double D[4], double U[4], double rh
 // fluid a and its normal derivative on each side; FO receives its value on the
 // t-axis and its time derivative (carried with the contact like v; the split
 // path's tangential derivatives of a are zero).
+// The rest of linear_GRP_solver_Edir_Q1D's interface, for callers that use it
+// as a function on its own (the solver's C-ABI entry points): atc and the
+// grid velocity (lambda_u, lambda_v), which the flux path fixes at eps and 0;
+// wave_speed[2] and U_star[6] out.  The sweeps pass none.
+struct Q1DExtra {
+    double atc, lu, lv;
+    double *ws, *us;
+};
+
 template <bool TF = false>
 __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L, const Side &R,
                                         const double TL[4], const double TR[4],
                                         const Gas &GL, const Gas &GR, double eps, double tol,
-                                        double p_guess, const double *FA = nullptr, double *FO = nullptr)
+                                        double p_guess, const double *FA = nullptr, double *FO = nullptr,
+                                        const Q1DExtra *X = nullptr)
 {
     const double gL = GL.g, gR = GR.g;
-    const double lu = 0.0, lv = 0.0;
-    const double atc = eps;
+    const double lu = X ? X->lu : 0.0, lv = X ? X->lv : 0.0;
+    const double atc = X ? X->atc : eps;
     const double rho_L = L.rho, rho_R = R.rho, u_L = L.u, u_R = R.u;
     const double v_L = L.v, v_R = R.v, p_L = L.p, p_R = R.p;
     const double d_rho_L = L.d_rho, d_rho_R = R.d_rho, d_u_L = L.d_u, d_u_R = R.d_u;
@@ -343,6 +357,10 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         }
     }
 
+    if (X) {
+        X->ws[0] = speed_L; X->ws[1] = speed_R;
+        X->us[0] = rs_L; X->us[1] = u_star; X->us[2] = rs_R; X->us[3] = p_star; X->us[4] = cs_L; X->us[5] = cs_R;
+    }
     if (speed_L > lu) {
         side_trivial(D, U, rho_L, u_L, v_L, p_L, d_rho_L, d_u_L, d_v_L, d_p_L, t_rho_L, t_u_L, t_v_L, t_p_L, c_L, lu, lv);
         if constexpr (TF) { FO[0] = FA[0]; FO[1] = -(u_L - lu) * FA[1]; }
@@ -355,10 +373,24 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
     }
 
     if (dist < atc) {
-        // acoustic case: the averaged star state never lies inside a fan here
-        // (u_star -+ c_star > 0 is exactly the trivial test above), so only the
-        // two contact-side states remain
-        if (u_star > lu) {
+        // acoustic case.  With atc < 2 eps the star state is the averaged one,
+        // which never lies inside a fan (u_star -+ c_star > 0 is exactly the
+        // trivial test above); the flux path (X == nullptr) is that case.  A
+        // larger atc (INFINITY: acoustic only) takes the exact star state,
+        // whose rarefaction fans the t-axis can lie in (:173-192).
+        if (X && crw0 && ((u_star - cs_L) > lu)) {
+            U[1] = zL * (u_L + 2.0 * (c_L + lu) / GL.gm1);
+            C = U[1] - lu;
+            U[3] = pow_with(pow_base(C / c_L), GL.e_sonic) * p_L;
+            U[0] = gL * U[3] / C / C;
+            U[2] = v_L;
+        } else if (X && crw1 && ((u_star + cs_R) < lu)) {
+            U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1);
+            C = lu - U[1];
+            U[3] = pow_with(pow_base(C / c_R), GR.e_sonic) * p_R;
+            U[0] = gR * U[3] / C / C;
+            U[2] = v_R;
+        } else if (u_star > lu) {
             U[0] = rs_L; U[1] = u_star; U[2] = v_L; U[3] = p_star; C = cs_L;
         } else {
             U[0] = rs_R; U[1] = u_star; U[2] = v_R; U[3] = p_star; C = cs_R;
@@ -637,13 +669,13 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double 
     return flux_from_mid<DIR>(mid, dire, G, tau, eps, F, I);
 }
 
-// GRP_2D_flux with the transversal derivatives of both sides (the unsplit
-// step; flux_generator_{x,y}.c:96-121 with Transversal = true) and a run-time
-// axis normal (n_x, n_y) = (1, 0) or (0, 1).  TL, TR hold (t_rho, t_u, t_v,
-// t_p) in grid axes; they are rotated with the velocities (flux_solver.c:167-185).
-__device__ __forceinline__ int grp_2d_flux_n(Side L, Side R, const double TL[4], const double TR[4], const Gas &G,
-                                             double n_x, double n_y, double tau, double eps, double F[4], double I[4],
-                                             double tol)
+// GRP_2D_flux with the transversal derivatives of both sides and a gas per
+// side (GRP_2D_flux itself, flux_solver.c:158-237: any unit normal (n_x, n_y),
+// the energy flux with the left side's gamma).  TL, TR hold (t_rho, t_u, t_v,
+// t_p) in grid axes; they are rotated with the velocities (:167-185).
+__device__ __forceinline__ int grp_2d_flux_g(Side L, Side R, const double TL[4], const double TR[4], const Gas &GL,
+                                             const Gas &GR, double n_x, double n_y, double tau, double eps,
+                                             double F[4], double I[4], double tol)
 {
     const double u = L.u * n_x + L.v * n_y;
     const double u_R = R.u * n_x + R.v * n_y;
@@ -657,8 +689,17 @@ __device__ __forceinline__ int grp_2d_flux_n(Side L, Side R, const double TL[4],
     R.d_v = -R.d_u * n_y + R.d_v * n_x;
     L.u = u; R.u = u_R; L.d_u = d_u; R.d_u = d_u_R;
     double dire[4], mid[4];
-    grp_q1d(dire, mid, L, R, TLr, TRr, G, G, eps, tol, -1.0);
-    return flux_from_mid_n(mid, dire, G, n_x, n_y, tau, eps, F, I);
+    grp_q1d(dire, mid, L, R, TLr, TRr, GL, GR, eps, tol, -1.0);
+    return flux_from_mid_n(mid, dire, GL, n_x, n_y, tau, eps, F, I);
+}
+
+// The unsplit step's form: one gas, normal (1, 0) or (0, 1)
+// (flux_generator_{x,y}.c:96-121 with Transversal = true).
+__device__ __forceinline__ int grp_2d_flux_n(Side L, Side R, const double TL[4], const double TR[4], const Gas &G,
+                                             double n_x, double n_y, double tau, double eps, double F[4], double I[4],
+                                             double tol)
+{
+    return grp_2d_flux_g(L, R, TL, TR, G, G, n_x, n_y, tau, eps, F, I, tol);
 }
 
 // Two gamma-law fluids a and b mixed by the volume fraction a of fluid a:

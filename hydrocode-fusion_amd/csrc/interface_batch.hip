@@ -7,6 +7,7 @@
 
 #include "../../include/hydrocode_grp.h"
 #include "grp_device.hpp"
+#include "grp1d_device.hpp"
 
 using namespace hgrp;
 
@@ -34,8 +35,62 @@ __global__ void riemann_batch_kernel(const double *in, int count, int N, double 
     const double *a = in + (size_t)k * 10;
     const StarResult r = riemann_exact(make_gas(a[0]), make_gas(a[1]), a[2], a[3], a[4], a[5], a[6], a[7], a[8],
                                        a[9], N);
-    double *o = out + (size_t)k * 5;
-    o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters;
+    double *o = out + (size_t)k * 6;
+    o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
+}
+
+// Interface sides of the per-function entry points: 15 doubles in the order
+// gamma, lambda_u, lambda_v, rho, u, v, p, d_rho, d_u, d_v, d_p, t_rho, t_u, t_v, t_p.
+__device__ __forceinline__ Side side_of(const double *a)
+{
+    return Side{a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]};
+}
+
+// linear_GRP_solver_Edir_Q1D with every argument of its own (atc, lambda_u/v of
+// the left side, a gamma per side).  out: wave_speed[2], D[4], U[4], U_star[6].
+__global__ void q1d_batch_kernel(const double *L, const double *R, int count, double eps, double atc, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *l = L + (size_t)k * 15, *r = R + (size_t)k * 15;
+    double *o = out + (size_t)k * 16;
+    double D[4], U[4], ws[2], us[6];
+    const Q1DExtra X = {atc, l[1], l[2], ws, us};
+    grp_q1d(D, U, side_of(l), side_of(r), l + 11, r + 11, make_gas(l[0]), make_gas(r[0]), eps, eps, -1.0, nullptr,
+            nullptr, &X);
+    o[0] = ws[0]; o[1] = ws[1];
+    for (int q = 0; q < 4; ++q) { o[2 + q] = D[q]; o[6 + q] = U[q]; }
+    for (int q = 0; q < 6; ++q) o[10 + q] = us[q];
+}
+
+// GRP_2D_flux for any unit normal, both sides' tangential derivatives and
+// gammas.  out: code, F_rho F_u F_v F_e, RHO_int U_int V_int P_int (zeros
+// when code != 0: the reference returns before it writes them).
+__global__ void flux_general_batch_kernel(const double *L, const double *R, int count, double tau, double eps,
+                                          double n_x, double n_y, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *l = L + (size_t)k * 15, *r = R + (size_t)k * 15;
+    double *o = out + (size_t)k * 9;
+    double F[4], I[4];
+    const int code = grp_2d_flux_g(side_of(l), side_of(r), l + 11, r + 11, make_gas(l[0]), make_gas(r[0]), n_x, n_y,
+                                   tau, eps, F, I, eps);
+    o[0] = code;
+    for (int q = 0; q < 4; ++q) { o[1 + q] = code ? 0.0 : F[q]; o[5 + q] = code ? 0.0 : I[q]; }
+}
+
+// linear_GRP_solver_Edir (1D, one gamma).  out: D[3], U[3] = (rho, u, p).
+__global__ void edir_batch_kernel(const double *L, const double *R, int count, double eps, double atc, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *l = L + (size_t)k * 15, *r = R + (size_t)k * 15;
+    const Side1 a{l[3], l[4], l[6], l[7], l[8], l[10]}, b{r[3], r[4], r[6], r[7], r[8], r[10]};
+    double D[3], U[3];
+    grp_edir(D, U, a, b, make_gas(l[0]), eps, atc);
+    double *o = out + (size_t)k * 6;
+    for (int q = 0; q < 3; ++q) { o[q] = D[q]; o[3 + q] = U[q]; }
 }
 
 // plain streaming copy: the calibration kernel for HBM byte counters
@@ -82,6 +137,18 @@ int run_batch(const void *const *hin, const size_t *in_bytes, int nin, void *hou
 
 struct FluxCtx { int count; double tau, eps; int dir; };
 struct RiemCtx { int count, N; };
+struct SideCtx { int count; double a, b, c, d; };   // per-entry-point scalars
+
+int side_batch(const double *L, const double *R, int count, int out_per, double *out, SideCtx sc,
+               void (*launch)(void *const *din, void *dout, void *ctx))
+{
+    if (count < 0 || (count && (!L || !R || !out))) return -1001;
+    if (count == 0) return 0;
+    const void *in[2] = {L, R};
+    const size_t nb[2] = {(size_t)count * 15 * 8, (size_t)count * 15 * 8};
+    sc.count = count;
+    return run_batch(in, nb, 2, out, (size_t)count * out_per * 8, launch, &sc);
+}
 }  // namespace
 
 extern "C" int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double tau,
@@ -109,13 +176,41 @@ extern "C" int hgrp_riemann_batch(const double *in, int count, int N, double *ou
     const void *hin[1] = {in};
     const size_t nb[1] = {(size_t)count * 10 * 8};
     RiemCtx rc = {count, N};
-    return run_batch(hin, nb, 1, out, (size_t)count * 5 * 8,
+    return run_batch(hin, nb, 1, out, (size_t)count * 6 * 8,
                      [](void *const *d, void *o, void *cx) {
                          const RiemCtx *r = (const RiemCtx *)cx;
                          hipLaunchKernelGGL(riemann_batch_kernel, dim3((r->count + 127) / 128), dim3(128), 0, 0,
                                             (const double *)d[0], r->count, r->N, (double *)o);
                      },
                      &rc);
+}
+
+extern "C" int hgrp_q1d_batch(const double *L, const double *R, int count, double eps, double atc, double *out)
+{
+    return side_batch(L, R, count, 16, out, SideCtx{0, eps, atc, 0.0, 0.0}, [](void *const *d, void *o, void *cx) {
+        const SideCtx *c = (const SideCtx *)cx;
+        hipLaunchKernelGGL(q1d_batch_kernel, dim3((c->count + 127) / 128), dim3(128), 0, 0, (const double *)d[0],
+                           (const double *)d[1], c->count, c->a, c->b, (double *)o);
+    });
+}
+
+extern "C" int hgrp_grp2d_flux_general_batch(const double *L, const double *R, int count, double tau, double eps,
+                                             double n_x, double n_y, double *out)
+{
+    return side_batch(L, R, count, 9, out, SideCtx{0, tau, eps, n_x, n_y}, [](void *const *d, void *o, void *cx) {
+        const SideCtx *c = (const SideCtx *)cx;
+        hipLaunchKernelGGL(flux_general_batch_kernel, dim3((c->count + 127) / 128), dim3(128), 0, 0,
+                           (const double *)d[0], (const double *)d[1], c->count, c->a, c->b, c->c, c->d, (double *)o);
+    });
+}
+
+extern "C" int hgrp_edir_batch(const double *L, const double *R, int count, double eps, double atc, double *out)
+{
+    return side_batch(L, R, count, 6, out, SideCtx{0, eps, atc, 0.0, 0.0}, [](void *const *d, void *o, void *cx) {
+        const SideCtx *c = (const SideCtx *)cx;
+        hipLaunchKernelGGL(edir_batch_kernel, dim3((c->count + 127) / 128), dim3(128), 0, 0, (const double *)d[0],
+                           (const double *)d[1], c->count, c->a, c->b, (double *)o);
+    });
 }
 
 extern "C" int hgrp_pow_batch(const double *x, const double *y, int count, double *out)

@@ -46,6 +46,9 @@ def lib():
     L.hgrp_grp2d_flux_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _DP]
     L.hgrp_riemann_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
     L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
+    L.hgrp_q1d_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
+    L.hgrp_grp2d_flux_general_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _D, _D, _DP]
+    L.hgrp_edir_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
     L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
     L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
     _FP = C.POINTER(C.c_float)
@@ -313,9 +316,45 @@ def flux_batch(L, R, tau, eps, direction):
 
 
 def riemann_batch(inp, N):
+    """Riemann_solver_exact per row of inp (gammaL gammaR u_L u_R p_L p_R c_L c_R
+    eps tol); rows of U_star P_star CRW0 CRW1 gap Newton-iterations."""
     inp = np.ascontiguousarray(inp, dtype=np.float64)
-    out = np.zeros((len(inp), 5))
+    out = np.zeros((len(inp), 6))
     _check(lib().hgrp_riemann_batch(_p(inp), len(inp), N, _p(out)), "riemann batch")
+    return out
+
+
+def _sides(L, R):
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    if L.ndim != 2 or L.shape[1] != 15 or R.shape != L.shape:
+        raise ValueError("interface sides are (count, 15) arrays of equal shape")
+    return L, R
+
+
+def q1d_batch(L, R, eps, atc):
+    """linear_GRP_solver_Edir_Q1D per pair: rows of wave_speed[2], D[4], U[4], U_star[6]."""
+    L, R = _sides(L, R)
+    out = np.zeros((len(L), 16))
+    _check(lib().hgrp_q1d_batch(_p(L), _p(R), len(L), eps, atc, _p(out)), "q1d batch")
+    return out
+
+
+def flux_general_batch(L, R, tau, eps, n_x, n_y):
+    """GRP_2D_flux per pair at the normal (n_x, n_y): rows of code, F_rho F_u F_v F_e,
+    RHO_int U_int V_int P_int."""
+    L, R = _sides(L, R)
+    out = np.zeros((len(L), 9))
+    _check(lib().hgrp_grp2d_flux_general_batch(_p(L), _p(R), len(L), tau, eps, n_x, n_y, _p(out)),
+           "general flux batch")
+    return out
+
+
+def edir_batch(L, R, eps, atc):
+    """linear_GRP_solver_Edir per pair: rows of D[3], U[3] (rho u p)."""
+    L, R = _sides(L, R)
+    out = np.zeros((len(L), 6))
+    _check(lib().hgrp_edir_batch(_p(L), _p(R), len(L), eps, atc, _p(out)), "edir batch")
     return out
 
 

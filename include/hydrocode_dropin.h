@@ -16,7 +16,10 @@
 #include "var_struc.h"
 
 #ifdef __cplusplus
+typedef bool hgrp_bool;     /* C's _Bool and C++'s bool share size and ABI */
 extern "C" {
+#else
+typedef _Bool hgrp_bool;
 #endif
 
 /* replaces GRP_solver_2D_split_EUL_source, src/include/finite_volume.h:48-49
@@ -29,6 +32,29 @@ void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_st
  * (src/finite_volume/grp_solver_2D_EUL_source.c:75-330): the unsplit scheme */
 void GRP_solver_2D_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
                               double *cpu_time, const char *problem, int N_T, int *N_plot, double time_plot[]);
+
+/* The per-interface solvers, one GPU thread per call (host/lib_units.c).
+ * Single-fluid members only (the reference's program build has no
+ * MULTIFLUID_BASICS): D and U of the Q1D solver carry rho u v p.  A failed
+ * device call is reported on stderr and yields NaN outputs (GRP_2D_flux: -1). */
+/* replaces Riemann_solver_exact, src/include/riemann_solver.h:24-27
+ * (src/riemann_solver/riemann_solver_exact_Ben.c:30-211) */
+double Riemann_solver_exact(double *U_star, double *P_star, const double gammaL, const double gammaR,
+                            const double u_L, const double u_R, const double p_L, const double p_R,
+                            const double c_L, const double c_R, hgrp_bool *CRW, const double eps, const double tol,
+                            int N);
+/* replaces linear_GRP_solver_Edir, src/include/riemann_solver.h:60
+ * (src/riemann_solver/linear_grp_solver_Edir.c:33-337) */
+void linear_GRP_solver_Edir(double *D, double *U, const struct i_f_var *ifv_L, const struct i_f_var *ifv_R,
+                            const double eps, const double atc);
+/* replaces linear_GRP_solver_Edir_Q1D, src/include/riemann_solver.h:66
+ * (src/riemann_solver/linear_grp_solver_Edir_Q1D.c:38-554) */
+void linear_GRP_solver_Edir_Q1D(double *wave_speed, double *D, double *U, double *U_star,
+                                const struct i_f_var *ifv_L, const struct i_f_var *ifv_R, const double eps,
+                                const double atc);
+/* replaces GRP_2D_flux, src/include/flux_calc.h:28 (src/flux_calc/flux_solver.c:158-237);
+ * eps is config[4] */
+int GRP_2D_flux(struct i_f_var *ifv, struct i_f_var *ifv_R, const double tau);
 
 /* replaces the 2D program's file I/O, src/include/file_io.h:13-54 */
 void example_io(const char *example, char *add_mkdir, const int i_or_o);        /* io_control.c:40-115   */

@@ -140,10 +140,34 @@ const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
 int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double tau,
                           double eps, int dir, double *out);
 
-/* Riemann_solver_exact (src/riemann_solver/riemann_solver_exact_Ben.c:30-211).
+/* Riemann_solver_exact (src/riemann_solver/riemann_solver_exact_Ben.c:30-211,
+ * declared at src/include/riemann_solver.h:24-27).
  * in: count x 10 = gammaL gammaR u_L u_R p_L p_R c_L c_R eps tol;
- * out: count x 5 = U_star P_star CRW0 CRW1 Newton-iterations. */
+ * out: count x 6 = U_star P_star CRW0 CRW1 gap Newton-iterations, gap being
+ * the reference's return value (the velocity gap left at the end). */
 int hgrp_riemann_batch(const double *in, int count, int N, double *out);
+
+/* linear_GRP_solver_Edir_Q1D (src/riemann_solver/linear_grp_solver_Edir_Q1D.c:38-554,
+ * src/include/riemann_solver.h:66) with all of its arguments: atc (eps: the
+ * quasi-1D GRP with its acoustic case; INFINITY: acoustic only; -0.0:
+ * nonlinear only), the grid velocity of L (lambda_u, lambda_v), a gamma per
+ * side, both sides' tangential derivatives.
+ * out: count x 16 = wave_speed[2], D[4], U[4] (rho u v p), U_star[6]
+ * (rho_star_L u_star rho_star_R p_star c_star_L c_star_R). */
+int hgrp_q1d_batch(const double *L, const double *R, int count, double eps, double atc, double *out);
+
+/* GRP_2D_flux (src/flux_calc/flux_solver.c:158-237, src/include/flux_calc.h:28)
+ * in full: unit normal (n_x, n_y), tangential derivatives and gamma of both
+ * sides (lambda is set to 0 as the reference does; the energy flux uses L's
+ * gamma).  out: count x 9 as hgrp_grp2d_flux_batch (fluxes zero when the
+ * check code is not 0). */
+int hgrp_grp2d_flux_general_batch(const double *L, const double *R, int count, double tau, double eps,
+                                  double n_x, double n_y, double *out);
+
+/* linear_GRP_solver_Edir (src/riemann_solver/linear_grp_solver_Edir.c:33-337,
+ * src/include/riemann_solver.h:60), 1D: reads gamma of L and rho u p, d_rho
+ * d_u d_p of both sides.  out: count x 6 = D[3], U[3] (rho u p). */
+int hgrp_edir_batch(const double *L, const double *R, int count, double eps, double atc, double *out);
 
 /* pow(x[k], y[k]) as evaluated on the device (precision check of the GPU's FP64 pow). */
 int hgrp_pow_batch(const double *x, const double *y, int count, double *out);

@@ -84,6 +84,42 @@ int ref_grp_2d_flux(const double *Ls, const double *Rs, double n_x, double n_y, 
     return rc;
 }
 
+/* The members of struct i_f_var that a single-fluid build has, in their
+ * declaration order (src/include/var_struc.h:116-131): 34 doubles. */
+#define IFV_MEMBERS(X) X(n_x) X(n_y) X(delta_x) X(delta_y) X(length) X(lambda_u) X(lambda_v) X(gamma) \
+    X(RHO) X(P) X(U) X(V) X(RHO_int) X(P_int) X(U_int) X(V_int) X(F_rho) X(F_e) X(F_u) X(F_v)          \
+    X(U_rho) X(U_e) X(U_u) X(U_v) X(d_rho) X(d_e) X(d_u) X(d_v) X(d_p) X(t_rho) X(t_e) X(t_u) X(t_v) X(t_p)
+
+static void ifv_from_members(struct i_f_var *f, const double *a)
+{
+    int k = 0;
+    memset(f, 0, sizeof *f);
+#define SET(name) f->name = a[k++];
+    IFV_MEMBERS(SET)
+#undef SET
+}
+
+static void ifv_to_members(double *a, const struct i_f_var *f)
+{
+    int k = 0;
+#define GET(name) a[k++] = f->name;
+    IFV_MEMBERS(GET)
+#undef GET
+}
+
+/* GRP_2D_flux on whole structs (34 members each, in and out): its side effects
+ * on both sides as well as its results.  config[4] (eps) must be set. */
+int ref_grp_2d_flux_struct(const double *Lm, const double *Rm, double tau, double *Lout, double *Rout)
+{
+    struct i_f_var L, R;
+    ifv_from_members(&L, Lm);
+    ifv_from_members(&R, Rm);
+    int rc = GRP_2D_flux(&L, &R, tau);
+    ifv_to_members(Lout, &L);
+    ifv_to_members(Rout, &R);
+    return rc;
+}
+
 /* out: D[3], U[3] */
 void ref_grp_edir(const double *Ls, const double *Rs, double eps, double atc, double *out)
 {

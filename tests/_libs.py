@@ -175,6 +175,31 @@ def initial_energy(r, u, v, p, gamma):
     return e
 
 
+# struct i_f_var of a single-fluid build, member order (src/include/var_struc.h:116-131)
+IFV_MEMBERS = ("n_x", "n_y", "delta_x", "delta_y", "length", "lambda_u", "lambda_v", "gamma",
+               "RHO", "P", "U", "V", "RHO_int", "P_int", "U_int", "V_int", "F_rho", "F_e", "F_u", "F_v",
+               "U_rho", "U_e", "U_u", "U_v", "d_rho", "d_e", "d_u", "d_v", "d_p", "t_rho", "t_e", "t_u", "t_v", "t_p")
+# the 15-double side vectors (gamma, lambda_u, lambda_v, rho, u, v, p, d_*, t_*) as member names
+SIDE15_MEMBERS = ("gamma", "lambda_u", "lambda_v", "RHO", "U", "V", "P", "d_rho", "d_u", "d_v", "d_p",
+                  "t_rho", "t_u", "t_v", "t_p")
+
+
+class IfvStruct(C.Structure):
+    """struct i_f_var (single-fluid members) as include/var_struc.h lays it out."""
+    _fields_ = [(name, C.c_double) for name in IFV_MEMBERS]
+
+
+def ifv_members(side15, n_x=0.0, n_y=0.0, extra=None):
+    """34-member vector from a 15-double side; other members from `extra` (dict) or 0."""
+    a = np.zeros(34)
+    for k, name in enumerate(SIDE15_MEMBERS):
+        a[IFV_MEMBERS.index(name)] = side15[k]
+    a[0], a[1] = n_x, n_y
+    for name, v in (extra or {}).items():
+        a[IFV_MEMBERS.index(name)] = v
+    return a
+
+
 class RefLib:
     """The reference's own sources built in oracle/_ref (checker only)."""
 
@@ -192,6 +217,8 @@ class RefLib:
         L.ref_grp_2d_flux.argtypes = [DP, DP, D, D, D, DP]
         L.ref_grp_2d_flux.restype = C.c_int
         L.ref_grp_edir.argtypes = [DP, DP, D, D, DP]
+        L.ref_grp_2d_flux_struct.argtypes = [DP, DP, D, DP, DP]
+        L.ref_grp_2d_flux_struct.restype = C.c_int
         L.ref_split2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
         L.ref_unsplit2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
         L.ref_grp1d.argtypes = [C.c_int, DP, C.c_int, IP, DP]
@@ -234,6 +261,16 @@ class RefLib:
         out = np.zeros(6)
         self.lib.ref_grp_edir(dptr(Lv), dptr(Rv), eps, atc, dptr(out))
         return out
+
+    def flux2d_struct(self, Lm, Rm, tau, eps):
+        """GRP_2D_flux on two whole structs (IFV_MEMBERS order, 34 doubles each):
+        returns (code, L after, R after)."""
+        self.lib.ref_config_set(4, eps)
+        Lm = np.ascontiguousarray(Lm, dtype=np.float64)
+        Rm = np.ascontiguousarray(Rm, dtype=np.float64)
+        Lo, Ro = np.zeros(34), np.zeros(34)
+        rc = self.lib.ref_grp_2d_flux_struct(dptr(Lm), dptr(Rm), tau, dptr(Lo), dptr(Ro))
+        return rc, Lo, Ro
 
     def split2d(self, m, n, init, config, time_plot=None, N_T=None):
         return self._run2d(self.lib.ref_split2d, m, n, init, config, time_plot, N_T)

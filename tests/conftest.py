@@ -46,6 +46,11 @@ def golden_iface():
 
 
 @pytest.fixture(scope="session")
+def golden_units():
+    return load_golden("interface_units.npz")
+
+
+@pytest.fixture(scope="session")
 def golden_2d():
     return load_golden("split2d_cases.npz")
 

@@ -17,7 +17,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(HERE))
-from _libs import RefLib, make_config  # noqa: E402
+from _libs import IFV_MEMBERS, RefLib, ifv_members, make_config  # noqa: E402
 import cases  # noqa: E402
 
 EPS = 1e-9
@@ -49,6 +49,68 @@ def interface_vectors(ref):
     np.savez_compressed(os.path.join(HERE, "interface_vectors.npz"), L=L, R=R, tau=tau, eps=EPS,
                         riemann=riem, riemann_tol12=riem12, riemann_ben=ben, q1d=q1d,
                         flux_x=fx, flux_y=fy, edir=edir)
+
+
+def unit_pairs(count, seed):
+    """Interface pairs for the per-interface entry points with every argument
+    in play: a grid velocity on the left side, a second gas (gamma 1.67) on
+    the right side of every other pair."""
+    L, R = cases.interface_pairs(count, seed=seed)
+    rng = np.random.default_rng(seed + 1)
+    c = np.sqrt(L[:, 0] * L[:, 6] / L[:, 3])
+    L[:, 1] = rng.uniform(-1, 1, count) * c
+    L[:, 2] = rng.uniform(-1, 1, count) * c
+    R[1::2, 0] = 1.67
+    return L, R
+
+
+def interface_units(ref):
+    """The per-interface solvers called as functions of their own (atc, grid
+    velocity, two gases, any normal, side effects on the structs)."""
+    L, R = unit_pairs(1000, seed=20240901)
+    n = len(L)
+    tau = 2.5e-3
+    ref.set_config(make_config())
+    cL = np.sqrt(L[:, 0] * L[:, 6] / L[:, 3])
+    cR = np.sqrt(R[:, 0] * R[:, 6] / R[:, 3])
+    riem_in = np.stack([L[:, 0], R[:, 0], L[:, 4], R[:, 4], L[:, 6], R[:, 6], cL, cR,
+                        np.full(n, EPS), np.full(n, EPS)], axis=1)
+    riem = np.array([ref.riemann(riem_in[k], 500) for k in range(n)])
+    atcs = np.array([EPS, np.inf, -0.0])
+    q1d = np.array([[ref.q1d(L[k], R[k], EPS, a) for k in range(n)] for a in atcs])
+    edir_inf = np.array([ref.edir(L[k], R[k], EPS, np.inf) for k in range(n)])
+    normals = np.array([[0.6, 0.8], [-0.28, 0.96], [1.0, 0.0], [0.0, 1.0]])
+    flux = np.zeros((len(normals), n, 9))
+    for a, (nx, ny) in enumerate(normals):
+        for k in range(n):
+            rc, o = ref.flux2d(L[k], R[k], tau, EPS, nx, ny)
+            flux[a, k, 0], flux[a, k, 1:] = rc, o
+    # whole structs: the members the solver does not read hold markers, the
+    # flux members too (GRP_2D_flux leaves them when its check fails)
+    ns = 300
+    rng = np.random.default_rng(7)
+    sin_L, sin_R, sout_L, sout_R, src = (np.zeros((ns, 34)), np.zeros((ns, 34)), np.zeros((ns, 34)),
+                                         np.zeros((ns, 34)), np.zeros(ns))
+    for k in range(ns):
+        nx, ny = normals[k % 2]
+        extra = {name: rng.uniform(1, 2) for name in ("delta_x", "delta_y", "length", "RHO_int", "P_int",
+                                                      "U_int", "V_int", "F_rho", "F_e", "F_u", "F_v",
+                                                      "U_rho", "U_e", "U_u", "U_v", "d_e", "t_e")}
+        sin_L[k] = ifv_members(L[k], nx, ny, extra)
+        sin_R[k] = ifv_members(R[k], nx, ny, extra)
+        # the last four fail the solver's checks: a negative density on a
+        # supersonic interface (code 1), NaN / inf values (2), NaN slopes (3)
+        bad = {ns - 4: {"RHO": -1.0, "P": -1.0, "U": 50.0, "V": 50.0}, ns - 3: {"RHO": np.nan},
+               ns - 2: {"U": np.inf}, ns - 1: {"d_rho": np.nan}}
+        for name, v in bad.get(k, {}).items():
+            sin_L[k, IFV_MEMBERS.index(name)] = v
+            if name == "d_rho":
+                sin_R[k, IFV_MEMBERS.index(name)] = v
+        src[k], sout_L[k], sout_R[k] = ref.flux2d_struct(sin_L[k], sin_R[k], tau, EPS)
+    np.savez_compressed(os.path.join(HERE, "interface_units.npz"), L=L, R=R, tau=tau, eps=EPS,
+                        riemann_in=riem_in, riemann=riem, atc=atcs, q1d=q1d, edir_inf=edir_inf,
+                        normals=normals, flux=flux, struct_in_L=sin_L, struct_in_R=sin_R,
+                        struct_out_L=sout_L, struct_out_R=sout_R, struct_rc=src)
 
 
 def split2d_cases(ref):
@@ -211,7 +273,7 @@ def grp1d_cases(ref):
 def main():
     ref = RefLib()
     only = sys.argv[1:]
-    for name, fn in (("interface", interface_vectors), ("split2d", split2d_cases),
+    for name, fn in (("interface", interface_vectors), ("units", interface_units), ("split2d", split2d_cases),
                      ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
         if not only or name in only:
             fn(ref)

@@ -51,3 +51,56 @@ extern "C" void hg_div_by_many(const double *a, int n, double b, double *out)
     const RecipDiv d = recip_div(b);
     for (int k = 0; k < n; ++k) out[k] = div_by(a[k], d);
 }
+
+// the per-interface entry points' forms: atc, grid velocity of L, a gas per side
+extern "C" void hg_q1d_many(const double *L, const double *R, int n, double eps, double atc, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 15 * k, *r = R + 15 * k;
+        Side a{l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]}, b{r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+        double D[4], U[4], ws[2], us[6];
+        const Q1DExtra X = {atc, l[1], l[2], ws, us};
+        grp_q1d(D, U, a, b, l + 11, r + 11, make_gas(l[0]), make_gas(r[0]), eps, eps, -1.0, nullptr, nullptr, &X);
+        double *o = out + 16 * k;
+        o[0] = ws[0]; o[1] = ws[1];
+        for (int q = 0; q < 4; ++q) { o[2 + q] = D[q]; o[6 + q] = U[q]; }
+        for (int q = 0; q < 6; ++q) o[10 + q] = us[q];
+    }
+}
+
+extern "C" void hg_flux_g_many(const double *L, const double *R, int n, double tau, double eps, double n_x,
+                               double n_y, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 15 * k, *r = R + 15 * k;
+        Side a{l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]}, b{r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+        double F[4], I[4];
+        const int rc = grp_2d_flux_g(a, b, l + 11, r + 11, make_gas(l[0]), make_gas(r[0]), n_x, n_y, tau, eps, F, I,
+                                     eps);
+        out[9 * k] = rc;
+        for (int q = 0; q < 4; ++q) { out[9 * k + 1 + q] = F[q]; out[9 * k + 5 + q] = I[q]; }
+    }
+}
+
+extern "C" void hg_edir_atc_many(const double *L, const double *R, int n, double eps, double atc, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 15 * k, *r = R + 15 * k;
+        Side1 a{l[3], l[4], l[6], l[7], l[8], l[10]}, b{r[3], r[4], r[6], r[7], r[8], r[10]};
+        double D[3], U[3];
+        grp_edir(D, U, a, b, make_gas(l[0]), eps, atc);
+        for (int q = 0; q < 3; ++q) { out[6 * k + q] = D[q]; out[6 * k + 3 + q] = U[q]; }
+    }
+}
+
+// in: gammaL gammaR u_L u_R p_L p_R c_L c_R eps tol; out: U* P* CRW0 CRW1 gap iters
+extern "C" void hg_riemann_many(const double *in, int n, int N, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *a = in + 10 * k;
+        const StarResult r = riemann_exact(make_gas(a[0]), make_gas(a[1]), a[2], a[3], a[4], a[5], a[6], a[7], a[8],
+                                           a[9], N);
+        double *o = out + 6 * k;
+        o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
+    }
+}

@@ -44,6 +44,45 @@ def test_interface_vectors_bitexact(oracle, golden_iface):
     assert not bad, bad[:10]
 
 
+def test_interface_units_bitexact(oracle, golden_units):
+    """The per-interface solvers with every argument in play (atc INFINITY and
+    -0.0, a grid velocity, a second gas, oblique normals): the oracle equals
+    the reference bit for bit."""
+    g = golden_units
+    L, R, tau = g["L"], g["R"], float(g["tau"])
+    bad = []
+    for k in range(len(L)):
+        if not _eq(oracle.riemann(*g["riemann_in"][k], 500)[:5], g["riemann"][k]):
+            bad.append(("riemann", k))
+        for a, atc in enumerate(g["atc"]):
+            if not _eq(oracle.q1d(L[k], R[k], EPS, atc), g["q1d"][a, k]):
+                bad.append(("q1d", a, k))
+        if not _eq(oracle.edir(L[k], R[k], EPS, np.inf), g["edir_inf"][k]):
+            bad.append(("edir_inf", k))
+        for a, (nx, ny) in enumerate(g["normals"]):
+            rc, out = oracle.flux2d(L[k], R[k], tau, EPS, nx, ny)
+            if rc != g["flux"][a, k, 0] or not _eq(out, g["flux"][a, k, 1:]):
+                bad.append(("flux", a, k))
+    assert not bad, bad[:10]
+
+
+def test_interface_units_cover_the_arguments(golden_units):
+    """atc changes the result (acoustic-only and nonlinear-only differ from the
+    default), the grid velocity and the second gas are in play."""
+    g = golden_units
+    q = g["q1d"]
+    assert np.mean(np.any(q[1] != q[0], axis=1)) > 0.5 and np.any(q[2] != q[0])
+    assert np.all(g["L"][:, 1] != 0) and np.sum(g["R"][:, 0] != g["L"][:, 0]) == len(g["L"]) // 2
+    # the flux's side effects: rotated velocities on both structs, lambda zeroed on the left
+    from _libs import IFV_MEMBERS
+    iu, iv, ilu = IFV_MEMBERS.index("U"), IFV_MEMBERS.index("V"), IFV_MEMBERS.index("lambda_u")
+    assert np.all(g["struct_out_L"][:, ilu] == 0) and np.any(g["struct_in_L"][:, ilu] != 0)
+    assert np.any(g["struct_out_R"][:, iu] != g["struct_in_R"][:, iu])
+    sp_in = np.hypot(g["struct_in_R"][:, iu], g["struct_in_R"][:, iv])
+    sp_out = np.hypot(g["struct_out_R"][:, iu], g["struct_out_R"][:, iv])
+    assert np.allclose(sp_in, sp_out, rtol=1e-14)       # a rotation keeps the speed
+
+
 def test_fixture_covers_every_branch(golden_iface):
     """The vectors exercise shocks and rarefactions on both sides, the acoustic
     shortcut, equal states and supersonic (trivial) interfaces."""
