@@ -52,8 +52,9 @@ struct Inst {
     int coff, ioff, m, bc;        // first cell, first interface, cells, boundary condition
     int k, done, status, found;   // step counter (the reference's k), finished, 0/1/2, limiter switch
     unsigned flags;               // FLAG_* raised during the current step
-    int pad;
+    int paused;                   // time_c reached t_pause at the start of a step: waits for the host
     double h, time_c, tau;
+    double t_pause;               // plot time the host waits for (INFINITY: none)
     unsigned long long hsm_bits;  // min over interfaces of h/S, as bits (positive doubles order like them)
     double b0[6];                 // BC -1 ghosts from the initial data: rho u p left, rho u p right
 };
@@ -148,7 +149,7 @@ __global__ void __launch_bounds__(BS) iface_kernel(Tables T, int n_if, Inst *ins
     const bool in = t < n_if;
     const int slot = in ? T.if_slot[t] : -1;
     Inst *Ip = in ? &inst[T.act[slot]] : nullptr;
-    const bool live = in && !Ip->done;
+    const bool live = in && !Ip->done && !Ip->paused;
     double hs = INFINITY;
     if (live) {
         const Inst &I = *Ip;
@@ -208,7 +209,7 @@ __global__ void tau_kernel(const int *act, int n_act, Inst *inst, Params P)
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_act) return;
     Inst &I = inst[act[s]];
-    if (I.done) return;
+    if (I.done || I.paused) return;
     if (I.flags & FLAG_ABORT) {            // reconstruction error: no update, stop here
         I.done = 1;
         I.status = 1;
@@ -257,7 +258,7 @@ __global__ void __launch_bounds__(BS) update_kernel(Tables T, int n_c, Inst *ins
     if (t >= n_c) return;
     const int slot = T.c_slot[t];
     Inst &I = inst[T.act[slot]];
-    if (I.done) return;
+    if (I.done || I.paused) return;
     const int j = t - T.pre_c[slot];
     const double tau = I.tau, h = I.h, gamma = P.gamma;
     const double nu = tau / h;
@@ -281,13 +282,15 @@ __global__ void __launch_bounds__(BS) update_kernel(Tables T, int n_c, Inst *ins
 }
 
 // end of a step (grp_solver_EUL_source.c:279-283): time, stop and end rules,
-// step counter; clears the per-step accumulators
+// step counter; clears the per-step accumulators.  Then the plot-time test the
+// reference makes at the start of the next step (:116-125): an instance whose
+// time has reached its t_pause waits for the host, which copies the frame.
 __global__ void close_kernel(const int *act, int n_act, Inst *inst, Params P)
 {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n_act) return;
     Inst &I = inst[act[s]];
-    if (I.done) return;
+    if (I.done || I.paused) return;
     I.found = 1;
     I.time_c += I.tau;
     const unsigned f = I.flags;
@@ -298,7 +301,11 @@ __global__ void close_kernel(const int *act, int n_act, Inst *inst, Params P)
         I.status = (f & FLAG_STOP) ? 2 : 0;
         return;
     }
-    if (++I.k > P.n_max) I.done = 1;
+    if (++I.k > P.n_max) {
+        I.done = 1;
+        return;
+    }
+    if (I.time_c >= I.t_pause) I.paused = 1;
 }
 
 __global__ void fill_tables_kernel(const int *pre_if, const int *pre_c, int n_act, int *if_slot, int *c_slot)
@@ -384,12 +391,12 @@ static void drop_graph(hgrp1d *c)
     c->graph = nullptr;
 }
 
-// rebuilds the active set from host_inst (done flags) and the lookup tables
+// rebuilds the active set from host_inst (done and paused flags) and the lookup tables
 static int rebuild_tables(hgrp1d *c)
 {
     std::vector<int> act, pre_if(1, 0), pre_c(1, 0);
     for (int i = 0; i < c->n_inst; ++i)
-        if (!c->host_inst[i].done) {
+        if (!c->host_inst[i].done && !c->host_inst[i].paused) {
             act.push_back(i);
             pre_if.push_back(pre_if.back() + c->m[i] + 1);
             pre_c.push_back(pre_c.back() + c->m[i]);
@@ -420,8 +427,9 @@ static int reset_instances(hgrp1d *c)
 {
     for (int i = 0; i < c->n_inst; ++i) {
         Inst &I = c->host_inst[i];
-        I.k = 1; I.done = 0; I.status = 0; I.found = 0; I.flags = 0u;
+        I.k = 1; I.done = 0; I.status = 0; I.found = 0; I.flags = 0u; I.paused = 0;
         I.time_c = 0.0;
+        I.t_pause = INFINITY;
         I.tau = c->P.tau_fixed;
         I.hsm_bits = 0x7ff0000000000000ull;
     }
@@ -571,7 +579,7 @@ static int run_all(hgrp1d *c, int use_graph, int max_steps, double *ms_out)
         long live_c = 0;
         int live = 0;
         for (int i = 0; i < c->n_inst; ++i)
-            if (!c->host_inst[i].done) { live_c += c->m[i]; ++live; }
+            if (!c->host_inst[i].done && !c->host_inst[i].paused) { live_c += c->m[i]; ++live; }
         // shrink the launches once a tenth of the covered cells has finished (or everything has)
         if (live == 0 || live_c < 0.9 * since_rebuild_act) {
             int rc = rebuild_tables(c);
@@ -589,10 +597,26 @@ static int run_all(hgrp1d *c, int use_graph, int max_steps, double *ms_out)
     return 0;
 }
 
+// Sets every instance's pause time (NULL: none), clears the paused flags and
+// rebuilds the active set.
+static int arm(hgrp1d *c, const double *t_pause)
+{
+    ECHECK(hipMemcpy(c->host_inst.data(), c->inst, c->n_inst * sizeof(Inst), hipMemcpyDeviceToHost));
+    for (int i = 0; i < c->n_inst; ++i) {
+        c->host_inst[i].paused = 0;
+        c->host_inst[i].t_pause = t_pause ? t_pause[i] : INFINITY;
+    }
+    ECHECK(hipMemcpyAsync(c->inst, c->host_inst.data(), c->n_inst * sizeof(Inst), hipMemcpyHostToDevice, c->stream));
+    ECHECK(hipStreamSynchronize(c->stream));
+    return rebuild_tables(c);
+}
+
 extern "C" int hgrp1d_run(hgrp1d *c, int use_graph, double *ms, long long *cell_updates)
 {
     if (!c || !c->uploaded) return -1001;
-    int rc = run_all(c, use_graph, 0, ms);
+    int rc = arm(c, nullptr);
+    if (rc) return rc;
+    rc = run_all(c, use_graph, 0, ms);
     if (rc) return rc;
     ECHECK(hipMemcpy(c->host_inst.data(), c->inst, c->n_inst * sizeof(Inst), hipMemcpyDeviceToHost));
     if (cell_updates) {
@@ -605,6 +629,19 @@ extern "C" int hgrp1d_run(hgrp1d *c, int use_graph, double *ms, long long *cell_
         }
         *cell_updates = s;
     }
+    return 0;
+}
+
+extern "C" int hgrp1d_run_until(hgrp1d *c, int use_graph, const double *t_pause, int *paused)
+{
+    if (!c || !c->uploaded || !t_pause) return -1001;
+    int rc = arm(c, t_pause);
+    if (rc) return rc;
+    rc = run_all(c, use_graph, 0, nullptr);
+    if (rc) return rc;
+    ECHECK(hipMemcpy(c->host_inst.data(), c->inst, c->n_inst * sizeof(Inst), hipMemcpyDeviceToHost));
+    if (paused)
+        for (int i = 0; i < c->n_inst; ++i) paused[i] = c->host_inst[i].paused && !c->host_inst[i].done;
     return 0;
 }
 

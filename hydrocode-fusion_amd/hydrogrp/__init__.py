@@ -71,6 +71,7 @@ def lib():
     L.hgrp1d_download.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP]
     L.hgrp1d_run.argtypes = [C.c_void_p, C.c_int, _DP, C.POINTER(C.c_longlong)]
     L.hgrp1d_results.argtypes = [C.c_void_p, _IP, _IP, _DP]
+    L.hgrp1d_run_until.argtypes = [C.c_void_p, C.c_int, _DP, _IP]
     L.hgrp1d_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP]
     _lib = L
     return L
@@ -277,6 +278,14 @@ class Ensemble1D:
         ms, cu = _D(), C.c_longlong()
         _check(lib().hgrp1d_run(self._h, int(graph), C.byref(ms), C.byref(cu)), "hgrp1d_run")
         return ms.value, cu.value
+
+    def run_until(self, t_pause, graph=True):
+        """Advances every unfinished instance until it finishes or, at the start
+        of a step, its time has reached t_pause[i]; returns the paused flags."""
+        t = np.ascontiguousarray(np.broadcast_to(t_pause, (len(self.cells),)), dtype=np.float64)
+        paused = np.zeros(len(self.cells), np.int32)
+        _check(lib().hgrp1d_run_until(self._h, int(graph), _p(t), paused.ctypes.data_as(_IP)), "hgrp1d_run_until")
+        return paused.astype(bool)
 
     def results(self):
         n = len(self.cells)

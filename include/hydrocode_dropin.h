@@ -33,6 +33,11 @@ void GRP_solver_2D_split_EUL_source(const int m, const int n, struct cell_var_st
 void GRP_solver_2D_EUL_source(const int m, const int n, struct cell_var_stru *CV, double **X, double **Y,
                               double *cpu_time, const char *problem, int N_T, int *N_plot, double time_plot[]);
 
+/* replaces GRP_solver_EUL_source, src/include/finite_volume.h:37
+ * (src/finite_volume/grp_solver_EUL_source.c:27-334): the 1D scheme; CV.X[k]
+ * are the plot frames of m cells (host/lib_grp_solver_1D.c) */
+void GRP_solver_EUL_source(const int m, struct cell_var_stru CV, double *cpu_time, int *N_plot, double time_plot[]);
+
 /* The per-interface solvers, one GPU thread per call (host/lib_units.c).
  * Single-fluid members only (the reference's program build has no
  * MULTIFLUID_BASICS): D and U of the Q1D solver carry rho u v p.  A failed

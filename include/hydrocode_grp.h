@@ -200,6 +200,13 @@ int hgrp1d_download(hgrp1d *ctx, double *rho, double *u, double *p, double *e);
  * hipGraphs (1) or one by one (0).  ms: device time; cell_updates: cells times
  * steps that updated them, summed over instances (either may be NULL). */
 int hgrp1d_run(hgrp1d *ctx, int use_graph, double *ms, long long *cell_updates);
+/* As hgrp1d_run, but instance i also stops at the start of a step once its
+ * time has reached t_pause[i] (INFINITY: never), the test the reference makes
+ * before it copies a plot frame (grp_solver_EUL_source.c:116-125); the test
+ * is made after every step of this call, so a call advances each unfinished
+ * instance by at least one step.  paused[i] (may be NULL): 1 where instance i
+ * stopped that way (a later call continues it), 0 where it has finished. */
+int hgrp1d_run_until(hgrp1d *ctx, int use_graph, const double *t_pause, int *paused);
 /* Per instance: the reference's step counter k (config[5] on return), status
  * (0 normal end, 1 aborted, 2 stopped by a check), and the time reached. */
 int hgrp1d_results(hgrp1d *ctx, int *steps, int *status, double *time_end);

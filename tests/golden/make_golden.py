@@ -267,6 +267,19 @@ def grp1d_cases(ref):
         metas.append([bc, c[5], npl, tp[npl - 1]])
     out["bc_final"] = np.stack(finals)
     out["bc_meta"] = np.array(metas)
+    # plot frames: Sod-400 at five plot times, and a first plot time after t = 0
+    # (the loop runs in frame 0 until then)
+    tp = np.array([0.0, 0.05, 0.1, 0.15, 0.2])
+    cfg = make_config(t_all=0.2, h_x=1 / 400, bc_x=-2, CFL=0.45)
+    fr, npl, tp, c = ref.grp1d(400, cases.sod_1d(400), cfg, N_T=5, time_plot=tp)
+    out["frames_all"] = fr[:, :npl]
+    out["frames_meta"] = np.r_[c[5], npl, tp[:npl]]
+    init = {k: v[:, 0] for k, v in cases.smooth_random(150, 1, 5).items() if k != "v"}
+    out["late_init"] = np.stack([init[k] for k in ("rho", "u", "p")])
+    cfg = make_config(t_all=0.16, h_x=1 / 150, bc_x=-7, CFL=0.45)
+    fr, npl, tp, c = ref.grp1d(150, init, cfg, N_T=3, time_plot=np.array([0.04, 0.1, 0.16]))
+    out["late_frames"] = fr[:, :npl]
+    out["late_meta"] = np.r_[c[5], npl, tp[:npl]]
     np.savez_compressed(os.path.join(HERE, "grp1d_cases.npz"), **out)
 
 

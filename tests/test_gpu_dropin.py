@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 import cases
-from _libs import CVFrames, ROOT
+from _libs import CVFrames, ROOT, make_config
 from conftest import rel_linf
 
 pytestmark = pytest.mark.gpu
@@ -71,6 +71,76 @@ def test_dropin_entry_point(host, oracle, tmp_path, split):
     tec = sorted(p.name for p in (tmp_path / "data_out" / "two-dim" / "EUL_2_order" / "dropin").iterdir())
     want = sorted("FLU_VAR_%.8g.tec" % (t + 1e-9) for t in time_plot[:-1])   # frames written at plot times
     assert tec == want, tec
+
+
+def _grp1d_dropin(host, m, init, cfg, time_plot):
+    """Calls GRP_solver_EUL_source with CV.X[k] frames of m cells (CV passed by value)."""
+    from _libs import CellVarStru, DP, DPP, initial_energy
+    N_T = len(time_plot)
+    data = np.zeros((4, N_T, m))
+    r, u, p = (np.asarray(init[k], dtype=np.float64) for k in ("rho", "u", "p"))
+    data[0, 0], data[1, 0], data[2, 0] = r, u, p
+    data[3, 0] = initial_energy(r, u, 0.0, p, cfg[6])
+    cv = CellVarStru()
+    keep = []
+    for f, name in enumerate(("RHO", "U", "P", "E")):
+        rows = (DP * N_T)(*[data[f, k].ctypes.data_as(DP) for k in range(N_T)])
+        keep.append(rows)
+        setattr(cv, name, C.cast(rows, DPP))
+    for k in range(400):
+        host.cfg[k] = float(cfg[k])
+    host.cfg[0] = 1.0
+    N_plot = C.c_int(N_T)
+    tp = (C.c_double * N_T)(*time_plot)
+    cpu = (C.c_double * N_T)()
+    print(f"1D drop-in {m} cells, {N_T} plot times", flush=True)
+    host.GRP_solver_EUL_source(m, cv, cpu, C.byref(N_plot), tp)
+    return data, N_plot.value, np.array(tp[:]), host.cfg[5]
+
+
+def test_dropin_grp1d_frames(host, golden_1d):
+    """GRP_solver_EUL_source: every plot frame against the reference's, also
+    when the first plot time is after t = 0 (the loop runs in frame 0 until then)."""
+    from _libs import CellVarStru
+    host.GRP_solver_EUL_source.restype = None
+    host.GRP_solver_EUL_source.argtypes = [C.c_int, CellVarStru, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double)]
+    g = golden_1d
+    cfg = make_config(t_all=0.2, h_x=1 / 400, bc_x=-2, CFL=0.45)
+    data, npl, tp, k = _grp1d_dropin(host, 400, cases.sod_1d(400), cfg, [0.0, 0.05, 0.1, 0.15, 0.2])
+    assert np.array_equal(np.r_[k, npl, tp[:npl]], g["frames_meta"])
+    for f in range(npl):
+        assert rel_linf(data[:, f], g["frames_all"][:, f]).max() <= 1e-10, f
+    init = dict(zip(("rho", "u", "p"), g["late_init"]))
+    cfg = make_config(t_all=0.16, h_x=1 / 150, bc_x=-7, CFL=0.45)
+    data, npl, tp, k = _grp1d_dropin(host, 150, init, cfg, [0.04, 0.1, 0.16])
+    assert np.array_equal(np.r_[k, npl], g["late_meta"][:2])
+    assert rel_linf(tp[:npl], g["late_meta"][2:]).max() <= 1e-13
+    for f in range(npl):
+        assert rel_linf(data[:, f], g["late_frames"][:, f]).max() <= 1e-10, f
+
+
+def test_dropin_grp1d_error_paths(host, oracle):
+    """Aborting reconstructions (negative pressure, vacuum density) and a
+    stopping check (a time step below eps) end with the oracle's step counter,
+    N_plot, time and state."""
+    from _libs import CellVarStru
+    host.GRP_solver_EUL_source.restype = None
+    host.GRP_solver_EUL_source.argtypes = [C.c_int, CellVarStru, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                           C.POINTER(C.c_double)]
+    m = 60
+    for where, val, cfl in ((("p", 17), -1.0, 0.45), (("rho", 30), 1e-12, 0.45), (("p", 0), 1.0, 1e-12)):
+        init = {k: v[:, 0].copy() for k, v in cases.smooth_random(m, 1, 9).items() if k != "v"}
+        init[where[0]][where[1]] = val
+        cfg = make_config(t_all=0.1, h_x=1 / m, bc_x=-4, CFL=cfl)
+        tpl = [0.0, 0.02, 0.1]
+        data, npl, tp, k = _grp1d_dropin(host, m, init, cfg, tpl)
+        fo, npo, tpo, co, st = oracle.grp1d(m, init, cfg, N_T=3, time_plot=np.array(tpl))
+        assert (k, npl) == (co[5], npo), where
+        assert tp[npl - 1] == tpo[npo - 1], where
+        ok = np.isfinite(fo[:, npo - 1])
+        assert np.array_equal(np.isfinite(data[:, npl - 1]), ok), where
+        assert np.max(np.abs(data[:, npl - 1][ok] - fo[:, npo - 1][ok])) <= 1e-9, where
 
 
 def _read_frames(path, n_x, n_y):

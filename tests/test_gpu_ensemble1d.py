@@ -161,3 +161,37 @@ def test_error_paths(hg, oracle):
         assert np.max(np.abs(fo[:, npo - 1][ok] - fin[ok])) <= 1e-9
     with pytest.raises(RuntimeError):
         hg.Ensemble1D([10], [0.1], make_config(t_all=0.1, bc_x=-3))
+
+
+def test_pauses_at_plot_times_leave_results_unchanged(hg):
+    """hgrp1d_run_until: instances stop at the start of the step where their
+    time first reaches their pause time, continue from there, and end bit for
+    bit where an uninterrupted run ends."""
+    cells, h, rho, u, p = cases.ensemble_1d(12, seed=77, n_lo=64, n_hi=900)
+    cfg = _cfg(0.15)
+    ref = hg.Ensemble1D(cells, h, cfg)
+    ens = hg.Ensemble1D(cells, h, cfg)
+    try:
+        ref.upload(rho, u, p)
+        ref.run()
+        want = ref.download()
+        want_res = ref.results()
+        ens.upload(rho, u, p)
+        pauses = np.linspace(0.01, 0.14, len(cells))
+        paused = ens.run_until(pauses)
+        assert paused.sum() >= len(cells) - 2        # the others ended (or stopped) before their pause time
+        steps, status, t = ens.results()
+        assert np.all(t[paused] >= pauses[paused]) and np.all(status[paused] == 0)
+        assert np.all(steps[paused] < want_res[0][paused])
+        for q in range(3):                            # the unpaused ones are done: their final results
+            assert np.array_equal((steps, status, t)[q][~paused], want_res[q][~paused])
+        paused2 = ens.run_until(np.full(len(cells), np.inf), graph=False)
+        assert not paused2.any()
+        got = ens.download()
+        got_res = ens.results()
+    finally:
+        ref.close()
+        ens.close()
+    assert np.array_equal(got, want)
+    for a, b in zip(got_res, want_res):
+        assert np.array_equal(a, b)

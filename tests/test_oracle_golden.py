@@ -254,6 +254,24 @@ def test_grp1d_sod_and_boundaries(oracle, golden_1d):
         assert _eq(fr[:, npl - 1], g["bc_final"][k]), bc
 
 
+def test_grp1d_plot_frames(oracle, golden_1d):
+    """Frames at plot times, and a first plot time after t = 0 (the loop runs in
+    frame 0 until then): every frame equals the reference's bit for bit."""
+    g = golden_1d
+    cfg = make_config(t_all=0.2, h_x=1 / 400, bc_x=-2, CFL=0.45)
+    fr, npl, tp, c, st = oracle.grp1d(400, cases.sod_1d(400), cfg, N_T=5,
+                                      time_plot=np.array([0.0, 0.05, 0.1, 0.15, 0.2]))
+    assert np.array_equal(np.r_[c[5], npl, tp[:npl]], g["frames_meta"])
+    assert _eq(fr[:, :npl], g["frames_all"])
+    init = dict(zip(("rho", "u", "p"), g["late_init"]))
+    cfg = make_config(t_all=0.16, h_x=1 / 150, bc_x=-7, CFL=0.45)
+    fr, npl, tp, c, st = oracle.grp1d(150, init, cfg, N_T=3, time_plot=np.array([0.04, 0.1, 0.16]))
+    assert np.array_equal(np.r_[c[5], npl, tp[:npl]], g["late_meta"])
+    assert _eq(fr[:, :npl], g["late_frames"])
+    # the frames differ from each other (each is a different time)
+    assert not np.array_equal(g["frames_all"][:, 1], g["frames_all"][:, 2])
+
+
 def test_against_live_reference(oracle, reflib):
     """Fresh random vectors and runs (not in the fixtures) against the reference build."""
     L, R = cases.interface_pairs(3000, seed=99)
