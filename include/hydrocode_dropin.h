@@ -38,6 +38,26 @@ void GRP_solver_2D_EUL_source(const int m, const int n, struct cell_var_stru *CV
  * are the plot frames of m cells (host/lib_grp_solver_1D.c) */
 void GRP_solver_EUL_source(const int m, struct cell_var_stru CV, double *cpu_time, int *N_plot, double time_plot[]);
 
+/* The 2D step's stages, over the whole grid on the GPU (host/lib_stages.c):
+ * the same CV members, b_f_var arrays and return values as the reference's;
+ * at an interface whose solver check fails the fluxes stored are NaN. */
+/* replaces bound_cond_slope_limiter_x, src/include/inter_process.h:51-52
+ * (src/inter_process/bound_cond_slope_limiter_x.c:27-129) */
+hgrp_bool bound_cond_slope_limiter_x(const int m, const int n, const int nt, struct cell_var_stru *CV,
+                                 struct b_f_var *bfv_L, struct b_f_var *bfv_R, struct b_f_var *bfv_D,
+                                 struct b_f_var *bfv_U, hgrp_bool find_bound_x, const hgrp_bool Slope, const double t_c);
+/* replaces bound_cond_slope_limiter_y, src/include/inter_process.h:56-57
+ * (src/inter_process/bound_cond_slope_limiter_y.c:27-129) */
+hgrp_bool bound_cond_slope_limiter_y(const int m, const int n, const int nt, struct cell_var_stru *CV,
+                                 struct b_f_var *bfv_L, struct b_f_var *bfv_R, struct b_f_var *bfv_D,
+                                 struct b_f_var *bfv_U, hgrp_bool find_bound_y, const hgrp_bool Slope, const double t_c);
+/* replaces flux_generator_x, src/include/flux_calc.h:16-17 (src/flux_calc/flux_generator_x.c:31-165) */
+int flux_generator_x(const int m, const int n, const int nt, const double tau, struct cell_var_stru *CV,
+                     struct b_f_var *bfv_L, struct b_f_var *bfv_R, const hgrp_bool Transversal);
+/* replaces flux_generator_y, src/include/flux_calc.h:21-22 (src/flux_calc/flux_generator_y.c:31-165) */
+int flux_generator_y(const int m, const int n, const int nt, const double tau, struct cell_var_stru *CV,
+                     struct b_f_var *bfv_D, struct b_f_var *bfv_U, const hgrp_bool Transversal);
+
 /* The per-interface solvers, one GPU thread per call (host/lib_units.c).
  * Single-fluid members only (the reference's program build has no
  * MULTIFLUID_BASICS): D and U of the Q1D solver carry rho u v p.  A failed

@@ -169,6 +169,38 @@ int hgrp_grp2d_flux_general_batch(const double *L, const double *R, int count, d
  * d_u d_p of both sides.  out: count x 6 = D[3], U[3] (rho u p). */
 int hgrp_edir_batch(const double *L, const double *R, int count, double eps, double atc, double *out);
 
+/* ------------------------------------------------------------------------- */
+/* Stages of the 2D step over a whole grid, for callers of the reference's    */
+/* per-stage functions (host arrays in/out; the time step fuses them).        */
+/* Fields are 4 arrays of m*n doubles in the order rho u v p, cell (j, i) at  */
+/* j*n + i (j the x index).                                                   */
+/* ------------------------------------------------------------------------- */
+
+/* The minmod slopes of bound_cond_slope_limiter_x (dir 0,
+ * src/inter_process/bound_cond_slope_limiter_x.c:83-92 with minmod_limiter_2D_x,
+ * slope_limiter_2D_x.c:32-84) or _y (dir 1, with minmod_limiter, slope_limiter.c:31-83):
+ * ghost_lo / ghost_hi hold the ghost values rho u v p of each line (dir 0: n
+ * lines along x, ghosts left/right; dir 1: m lines along y, ghosts down/up),
+ * 4 doubles per line; found != 0 selects minmod3(alpha s_L, alpha s_R, s_prev)
+ * (s_prev: the previous slopes, same layout as s_out), else minmod2(s_L, s_R). */
+int hgrp2d_slopes(int dir, int m, int n, const double *state, const double *ghost_lo, const double *ghost_hi,
+                  const double *s_prev, int found, double alpha, double h, double *s_out);
+
+/* The interface fluxes of flux_generator_x (dir 0, src/flux_calc/flux_generator_x.c:31-165)
+ * or flux_generator_y (dir 1, flux_generator_y.c:31-165): GRP_2D_flux with atc = eps
+ * at every interface.  slope_n: the slopes along dir (x-slopes for dir 0,
+ * y-slopes for dir 1); slope_t: the other direction's, used when transversal
+ * != 0 (negated for dir 1, as the reference does; may be NULL otherwise).
+ * bfv_lo / bfv_hi: per line 12 doubles = the ghost's rho u v p, its slopes
+ * along dir, its transversal slopes (the b_f_var members RHO U V P, S*, T* for
+ * dir 0; RHO U V P, T*, S* for dir 1).  Out, nif = (m+1)*n (dir 0, interface
+ * (j, i) at j*n + i) or m*(n+1) (dir 1, at j*(n+1) + i): F[4*nif] = F_rho F_u
+ * F_v F_e, I[4*nif] = rho u v p at t + tau, codes[nif] = 0, 1 (ifvar_check
+ * failed), 2 (the solver's check failed; F and I are NaN there). */
+int hgrp2d_fluxes(int dir, int m, int n, const double *state, const double *slope_n, const double *slope_t,
+                  const double *bfv_lo, const double *bfv_hi, double tau, double h, double gamma, double eps,
+                  int transversal, double *F, double *I, int *codes);
+
 /* pow(x[k], y[k]) as evaluated on the device (precision check of the GPU's FP64 pow). */
 int hgrp_pow_batch(const double *x, const double *y, int count, double *out);
 

@@ -18,6 +18,7 @@
 #include "var_struc.h"
 #include "riemann_solver.h"
 #include "flux_calc.h"
+#include "inter_process.h"
 #include "finite_volume.h"
 
 double config[N_CONF];
@@ -190,4 +191,93 @@ void ref_grp1d(int m, double *fields, int N_T, int *N_plot, double *time_plot)
     *N_plot = N_T;
     GRP_solver_EUL_source(m, CV, cpu_time, N_plot, time_plot);
     free(cpu_time); free(rows);
+}
+
+/* ---- the 2D step's stages on their own: CV[0] (frame 0 + scratch), CV[1] (frame nt = 1) ---- */
+
+static double **rows_of(double *block, int m, int n)
+{
+    double **r = malloc((size_t)m * sizeof(double *));
+    for (int j = 0; j < m; ++j) r[j] = block + (size_t)j * n;
+    return r;
+}
+
+/* fields f: 4*m*n doubles rho u v p -> frame k of CV */
+static void set_frame(struct cell_var_stru *cv, double *f, int m, int n)
+{
+    const size_t mn = (size_t)m * n;
+    cv->RHO = rows_of(f, m, n); cv->U = rows_of(f + mn, m, n);
+    cv->V = rows_of(f + 2 * mn, m, n); cv->P = rows_of(f + 3 * mn, m, n);
+}
+
+static void free_frame(struct cell_var_stru *cv)
+{
+    free(cv->RHO); free(cv->U); free(cv->V); free(cv->P);
+}
+
+/* bound_cond_slope_limiter_{x,y} (dir 0/1) with nt = 1.  f0, f1: frames 0 and 1;
+ * sl: the direction's slopes of CV[0] in (previous values) and out, 4*m*n;
+ * cross: the other direction's slopes (untouched, 4*m*n); bfv*: b_f_var arrays
+ * (13 doubles each) in and out.  config[] must be set. */
+int ref_limiter(int dir, int m, int n, double *f0, double *f1, double *sl, double *cross, double *bfvL,
+                double *bfvR, double *bfvD, double *bfvU, int found, int slope)
+{
+    struct cell_var_stru CV[2];
+    memset(CV, 0, sizeof CV);
+    set_frame(&CV[0], f0, m, n);
+    set_frame(&CV[1], f1, m, n);
+    const size_t mn = (size_t)m * n;
+    double *sx = dir ? cross : sl, *sy = dir ? sl : cross;
+    CV[0].s_rho = rows_of(sx, m, n); CV[0].s_u = rows_of(sx + mn, m, n);
+    CV[0].s_v = rows_of(sx + 2 * mn, m, n); CV[0].s_p = rows_of(sx + 3 * mn, m, n);
+    CV[0].t_rho = rows_of(sy, m, n); CV[0].t_u = rows_of(sy + mn, m, n);
+    CV[0].t_v = rows_of(sy + 2 * mn, m, n); CV[0].t_p = rows_of(sy + 3 * mn, m, n);
+    _Bool r = dir ? bound_cond_slope_limiter_y(m, n, 1, CV, (struct b_f_var *)bfvL, (struct b_f_var *)bfvR,
+                                               (struct b_f_var *)bfvD, (struct b_f_var *)bfvU, found, slope, 0.0)
+                  : bound_cond_slope_limiter_x(m, n, 1, CV, (struct b_f_var *)bfvL, (struct b_f_var *)bfvR,
+                                               (struct b_f_var *)bfvD, (struct b_f_var *)bfvU, found, slope, 0.0);
+    free(CV[0].s_rho); free(CV[0].s_u); free(CV[0].s_v); free(CV[0].s_p);
+    free(CV[0].t_rho); free(CV[0].t_u); free(CV[0].t_v); free(CV[0].t_p);
+    free_frame(&CV[0]);
+    free_frame(&CV[1]);
+    return r;
+}
+
+/* flux_generator_{x,y} (dir 0/1) with nt = 1 on frame f1, x-slopes s and
+ * y-slopes t of CV[0]; bfv_lo / bfv_hi: b_f_var arrays (13 doubles each).
+ * F, I: 4 * nif doubles (F_rho F_u F_v F_e / rho u v p at t + tau), interfaces
+ * at [j][i] of (m+1) x n (x) or m x (n+1) (y).  Returns the function's value. */
+int ref_flux_gen(int dir, int m, int n, double *f1, double *s, double *t, double *bfv_lo, double *bfv_hi,
+                 double tau, int transversal, double *F, double *I)
+{
+    struct cell_var_stru CV[2];
+    memset(CV, 0, sizeof CV);
+    set_frame(&CV[1], f1, m, n);
+    const size_t mn = (size_t)m * n;
+    CV[0].s_rho = rows_of(s, m, n); CV[0].s_u = rows_of(s + mn, m, n);
+    CV[0].s_v = rows_of(s + 2 * mn, m, n); CV[0].s_p = rows_of(s + 3 * mn, m, n);
+    CV[0].t_rho = rows_of(t, m, n); CV[0].t_u = rows_of(t + mn, m, n);
+    CV[0].t_v = rows_of(t + 2 * mn, m, n); CV[0].t_p = rows_of(t + 3 * mn, m, n);
+    const int rows = dir ? m : m + 1, cols = dir ? n + 1 : n;
+    const size_t nif = (size_t)rows * cols;
+    double **fr[4], **ir[4];
+    for (int q = 0; q < 4; ++q) {
+        fr[q] = rows_of(F + q * nif, rows, cols);
+        ir[q] = rows_of(I + q * nif, rows, cols);
+    }
+    int rc;
+    if (dir) {
+        CV[0].G_rho = fr[0]; CV[0].G_u = fr[1]; CV[0].G_v = fr[2]; CV[0].G_e = fr[3];
+        CV[0].rhoIy = ir[0]; CV[0].uIy = ir[1]; CV[0].vIy = ir[2]; CV[0].pIy = ir[3];
+        rc = flux_generator_y(m, n, 1, tau, CV, (struct b_f_var *)bfv_lo, (struct b_f_var *)bfv_hi, transversal);
+    } else {
+        CV[0].F_rho = fr[0]; CV[0].F_u = fr[1]; CV[0].F_v = fr[2]; CV[0].F_e = fr[3];
+        CV[0].rhoIx = ir[0]; CV[0].uIx = ir[1]; CV[0].vIx = ir[2]; CV[0].pIx = ir[3];
+        rc = flux_generator_x(m, n, 1, tau, CV, (struct b_f_var *)bfv_lo, (struct b_f_var *)bfv_hi, transversal);
+    }
+    for (int q = 0; q < 4; ++q) { free(fr[q]); free(ir[q]); }
+    free(CV[0].s_rho); free(CV[0].s_u); free(CV[0].s_v); free(CV[0].s_p);
+    free(CV[0].t_rho); free(CV[0].t_u); free(CV[0].t_v); free(CV[0].t_p);
+    free_frame(&CV[1]);
+    return rc;
 }

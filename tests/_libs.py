@@ -219,6 +219,10 @@ class RefLib:
         L.ref_grp_edir.argtypes = [DP, DP, D, D, DP]
         L.ref_grp_2d_flux_struct.argtypes = [DP, DP, D, DP, DP]
         L.ref_grp_2d_flux_struct.restype = C.c_int
+        L.ref_limiter.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, DP, DP, DP, DP, DP, DP, C.c_int, C.c_int]
+        L.ref_limiter.restype = C.c_int
+        L.ref_flux_gen.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, DP, DP, DP, D, C.c_int, DP, DP]
+        L.ref_flux_gen.restype = C.c_int
         L.ref_split2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
         L.ref_unsplit2d.argtypes = [C.c_int, C.c_int, DP, C.c_int, IP, DP]
         L.ref_grp1d.argtypes = [C.c_int, DP, C.c_int, IP, DP]
@@ -271,6 +275,27 @@ class RefLib:
         Lo, Ro = np.zeros(34), np.zeros(34)
         rc = self.lib.ref_grp_2d_flux_struct(dptr(Lm), dptr(Rm), tau, dptr(Lo), dptr(Ro))
         return rc, Lo, Ro
+
+    def limiter(self, direction, m, n, f0, f1, slopes, cross, bfv, found, slope=1):
+        """bound_cond_slope_limiter_{x,y} with nt = 1: f0, f1 (4, m, n) frames; slopes
+        (4, m, n) of the direction (previous values in); cross: the other direction's;
+        bfv: dict L, R (n, 13), D, U (m, 13).  Returns (value, slopes, bfv) after the call."""
+        sl = np.ascontiguousarray(slopes, dtype=np.float64).copy()
+        cr = np.ascontiguousarray(cross, dtype=np.float64).copy()
+        b = {k: np.ascontiguousarray(v, dtype=np.float64).copy() for k, v in bfv.items()}
+        f0 = np.ascontiguousarray(f0, dtype=np.float64).copy()
+        f1 = np.ascontiguousarray(f1, dtype=np.float64).copy()
+        r = self.lib.ref_limiter(direction, m, n, dptr(f0), dptr(f1), dptr(sl), dptr(cr), dptr(b["L"]), dptr(b["R"]),
+                                 dptr(b["D"]), dptr(b["U"]), int(found), int(slope))
+        return r, sl, b
+
+    def flux_gen(self, direction, m, n, f1, s, t, bfv_lo, bfv_hi, tau, transversal):
+        """flux_generator_{x,y} with nt = 1: returns (value, F (4, rows, cols), I (4, rows, cols))."""
+        rows, cols = (m, n + 1) if direction else (m + 1, n)
+        F, I = np.zeros((4, rows, cols)), np.zeros((4, rows, cols))
+        arrs = [np.ascontiguousarray(a, dtype=np.float64).copy() for a in (f1, s, t, bfv_lo, bfv_hi)]
+        r = self.lib.ref_flux_gen(direction, m, n, *(dptr(a) for a in arrs), tau, int(transversal), dptr(F), dptr(I))
+        return r, F, I
 
     def split2d(self, m, n, init, config, time_plot=None, N_T=None):
         return self._run2d(self.lib.ref_split2d, m, n, init, config, time_plot, N_T)

@@ -51,6 +51,11 @@ def golden_units():
 
 
 @pytest.fixture(scope="session")
+def golden_stages():
+    return load_golden("stage_cases.npz")
+
+
+@pytest.fixture(scope="session")
 def golden_2d():
     return load_golden("split2d_cases.npz")
 

@@ -113,6 +113,68 @@ def interface_units(ref):
                         struct_out_L=sout_L, struct_out_R=sout_R, struct_rc=src)
 
 
+STAGE_BCS = ((-4, -4), (-2, -7), (-7, -24), (-24, -2), (-1, -1))
+
+
+def stage_fields(m, n, seed):
+    """Two frames (4, m, n) of smooth positive states, slope sets and ghost
+    arrays with every member set (b_f_var: H RHO P U V SRHO SP SU SV TRHO TP TU TV)."""
+    rng = np.random.default_rng(seed)
+    f = []
+    for k in range(2):
+        init = cases.smooth_random(m, n, seed + k)
+        f.append(np.stack([init["rho"], init["u"], init["v"], init["p"]]))
+    s, t = rng.normal(0, 2, (4, m, n)), rng.normal(0, 2, (4, m, n))
+    bfv = {"L": rng.uniform(0.5, 1.5, (n, 13)), "R": rng.uniform(0.5, 1.5, (n, 13)),
+           "D": rng.uniform(0.5, 1.5, (m, 13)), "U": rng.uniform(0.5, 1.5, (m, 13))}
+    return f[0], f[1], s, t, bfv
+
+
+def stage_cases(ref):
+    """bound_cond_slope_limiter_{x,y} and flux_generator_{x,y} on a 9 x 7 grid
+    (nt = 1), every boundary condition pair, both limiter forms, with and without
+    transversal derivatives, and a flux run with a failing cell."""
+    m, n = 9, 7
+    out = {}
+    f0, f1, s, t, bfv = stage_fields(m, n, 31)
+    out.update(f0=f0, f1=f1, s=s, t=t, **{"bfv_" + k: v for k, v in bfv.items()})
+    lim_meta, lim_s, lim_bfv = [], [], []
+    for bx, by in STAGE_BCS:
+        for direction in (0, 1):
+            for found, slope in ((0, 1), (1, 1), (0, 0)):
+                ref.set_config(make_config(h_x=1 / m, h_y=1 / n, bc_x=bx, bc_y=by))
+                r, sl, b = ref.limiter(direction, m, n, f0, f1, t if direction else s, s if direction else t, bfv,
+                                       found, slope)
+                lim_meta.append([bx, by, direction, found, slope, r])
+                lim_s.append(sl)
+                lim_bfv.append(np.concatenate([b[k] for k in ("L", "R", "D", "U")]))
+    out.update(lim_meta=np.array(lim_meta), lim_s=np.array(lim_s), lim_bfv=np.array(lim_bfv))
+    flux_meta, flux_F, flux_I = [], [], []
+    tau = 2e-3
+    for direction in (0, 1):
+        for transversal in (0, 1):
+            ref.set_config(make_config(h_x=1 / m, h_y=1 / n))
+            lo, hi = (bfv["L"], bfv["R"]) if direction == 0 else (bfv["D"], bfv["U"])
+            r, F, I = ref.flux_gen(direction, m, n, f1, s * 0.1, t * 0.1, lo, hi, tau, transversal)
+            flux_meta.append([direction, transversal, r])
+            flux_F.append(F.reshape(4, -1))
+            flux_I.append(I.reshape(4, -1))
+    out.update(flux_meta=np.array(flux_meta), flux_tau=tau)
+    out["flux_F_x"], out["flux_I_x"] = np.array(flux_F[:2]), np.array(flux_I[:2])
+    out["flux_F_y"], out["flux_I_y"] = np.array(flux_F[2:]), np.array(flux_I[2:])
+    # a cell with negative pressure: its interfaces fail (return value only; the
+    # reference's fluxes there depend on its thread schedule)
+    fbad = f1.copy()
+    fbad[3, 4, 3] = -0.5
+    bad = []
+    for direction in (0, 1):
+        lo, hi = (bfv["L"], bfv["R"]) if direction == 0 else (bfv["D"], bfv["U"])
+        r, F, I = ref.flux_gen(direction, m, n, fbad, s * 0.1, t * 0.1, lo, hi, tau, 1)
+        bad.append(r)
+    out["fail_f1"], out["fail_rc"] = fbad, np.array(bad)
+    np.savez_compressed(os.path.join(HERE, "stage_cases.npz"), **out)
+
+
 def split2d_cases(ref):
     out = {}
     # four-quadrant 64^2 to t = 0.3 (127 steps), full final state
@@ -286,7 +348,8 @@ def grp1d_cases(ref):
 def main():
     ref = RefLib()
     only = sys.argv[1:]
-    for name, fn in (("interface", interface_vectors), ("units", interface_units), ("split2d", split2d_cases),
+    for name, fn in (("interface", interface_vectors), ("units", interface_units), ("stages", stage_cases),
+                     ("split2d", split2d_cases),
                      ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
         if not only or name in only:
             fn(ref)

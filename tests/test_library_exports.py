@@ -12,7 +12,7 @@ LIBDIR = os.path.join(ROOT, "hydrocode-fusion_amd", "lib")
 # which shared library implements each header
 HEADER_LIB = {"hydrocode_grp.h": "libhydrogrp.so", "hydrocode_dropin.h": "libhydrocode2d.so"}
 
-DECL = re.compile(r"^\s*(?:extern\s+)?(?:const\s+)?(?:int|void|double|_Bool|struct\s+\w+)\s*\*?\s*(\w+)\s*\(",
+DECL = re.compile(r"^\s*(?:extern\s+)?(?:const\s+)?(?:int|void|double|_Bool|hgrp_bool|struct\s+\w+)\s*\*?\s*(\w+)\s*\(",
                   re.MULTILINE)
 
 
@@ -29,8 +29,11 @@ def declared_functions():
 
 def test_headers_declare_the_boundary():
     names = declared_functions()
-    for required in ("GRP_solver_2D_split_EUL_source", "hgrp2d_create", "hgrp2d_solve", "hgrp2d_bench",
-                     "hgrp_grp2d_flux_batch"):
+    for required in ("GRP_solver_2D_split_EUL_source", "GRP_solver_2D_EUL_source", "GRP_solver_EUL_source",
+                     "GRP_2D_flux", "linear_GRP_solver_Edir_Q1D", "linear_GRP_solver_Edir", "Riemann_solver_exact",
+                     "bound_cond_slope_limiter_x", "bound_cond_slope_limiter_y", "flux_generator_x",
+                     "flux_generator_y", "hgrp2d_create", "hgrp2d_solve", "hgrp2d_bench", "hgrp_grp2d_flux_batch",
+                     "hgrp2d_slopes", "hgrp2d_fluxes", "hgrp1d_run_until"):
         assert required in names, required
 
 
