@@ -22,9 +22,11 @@
 #if defined(__HIP__) || defined(__HIPCC__)
 #include <hip/hip_runtime.h>
 #define HGRP_HD __host__ __device__ __forceinline__
+#define HGRP_COLD static __host__ __device__ __attribute__((noinline))
 #define HGRP_POW_TABLE __device__ __constant__ static
 #else
 #define HGRP_HD static inline
+#define HGRP_COLD static __attribute__((noinline))
 #define HGRP_POW_TABLE static
 #endif
 
@@ -100,6 +102,12 @@ HGRP_HD double exp_dd(DD p)
     return ldexp(s.hi + tail, k2 >> 7);
 }
 
+// The platform pow for what the fast path leaves (bases that are not positive
+// normal numbers, results near overflow or underflow).  Out of line: pow_with
+// is inlined at dozens of sites of the sweep kernels, and a copy of the
+// library pow at each would double their code size for a path they rarely take.
+HGRP_COLD double pow_special(double x, double y) { return pow(x, y); }
+
 HGRP_HD bool pow_base_ok(double x) { return x >= 2.2250738585072014e-308 && x < INFINITY; }
 
 // A base whose log is computed once and raised to several exponents.
@@ -128,7 +136,7 @@ HGRP_HD double pow_with(const PowBase &b, double y)
             return exp_dd(two_sum(p_hi, p_lo));
         }
     }
-    return pow(b.x, y);
+    return pow_special(b.x, y);
 }
 
 HGRP_HD double fast_pow(double x, double y) { return pow_with(pow_base(x), y); }
