@@ -11,6 +11,7 @@
 //
 // Method (textbook range reduction, e.g. Muller, "Elementary Functions", ch. 6-7):
 //   log: x = 2^k m, m in [1,2); r = m*INVC[j] - 1 exact (|r| < 2^-7);
+//        (the tables are one array, pow_tables.hpp, so a kernel keeps one address)
 //        log x = k ln2 + LOGC[j] + log1p(r), log1p by its Taylor series to r^10.
 //   exp: y log x = (k2/128) ln2 + z, |z| <= ln2/256;
 //        exp = 2^(k2 >> 7) * EXP2[k2 & 127] * (1 + z + ... + z^7/7!).
@@ -53,7 +54,8 @@ HGRP_HD DD log_dd(double x)
     m *= 2.0;
     k -= 1;                            // x = 2^k m, m in [1, 2)
     const int j = (int)((m - 1.0) * 128.0);
-    const double invc = powtab::INVC[j];
+    const double *lt = powtab::TAB + powtab::LOG_ROW * j;
+    const double invc = lt[0];
     const double r = fma(m, invc, -1.0);                  // exact
     const double q_hi = r * r;
     const double q_lo = fma(r, r, -q_hi);                  // r^2 = q_hi + q_lo exactly
@@ -68,10 +70,10 @@ HGRP_HD DD log_dd(double x)
     t = fma(t, r, 1.0 / 3.0);
     const double tail = t * (q_hi * r);
     const double kd = (double)k;
-    const DD s1 = two_sum(kd * powtab::LN2_HI, powtab::LOGC_HI[j]);
+    const DD s1 = two_sum(kd * powtab::LN2_HI, lt[1]);
     const DD s2 = two_sum(s1.hi, r);
     const DD s3 = two_sum(s2.hi, -0.5 * q_hi);
-    const double lo = s1.lo + s2.lo + s3.lo + kd * powtab::LN2_LO + powtab::LOGC_LO[j] - 0.5 * q_lo + tail;
+    const double lo = s1.lo + s2.lo + s3.lo + kd * powtab::LN2_LO + lt[2] - 0.5 * q_lo + tail;
     const double hi = s3.hi + lo;
     return {hi, lo - (hi - s3.hi)};
 }
@@ -94,7 +96,8 @@ HGRP_HD double exp_dd(DD p)
     t = fma(t, z_hi, 0.5);
     const double small = z_lo + z_hi * z_lo + (z_hi * z_hi) * t;
     const int j = k2 & 127;
-    const double T_hi = powtab::EXP2_HI[j], T_lo = powtab::EXP2_LO[j];
+    const double *et = powtab::TAB + powtab::EXP_BASE + powtab::EXP_ROW * j;
+    const double T_hi = et[0], T_lo = et[1];
     const double prod_hi = T_hi * z_hi;
     const double prod_lo = fma(T_hi, z_hi, -prod_hi);
     const DD s = two_sum(T_hi, prod_hi);

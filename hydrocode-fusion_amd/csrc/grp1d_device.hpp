@@ -44,7 +44,7 @@ __device__ inline void grp_edir(double D[3], double U[3], const Side1 &L, const 
 {
     const double rho_L = L.rho, rho_R = R.rho, u_L = L.u, u_R = R.u, p_L = L.p, p_R = R.p;
     const double s_rho_L = L.s_rho, s_rho_R = R.s_rho, s_u_L = L.s_u, s_u_R = R.s_u, s_p_L = L.s_p, s_p_R = R.s_p;
-    const double gamma = G.g, zeta = G.z, zts = G.zz;
+    const double gamma = G.g, zeta = G.z, zts = G.zz();
 
     int crw0 = 0, crw1 = 0;
     double u_star, p_star, rs_L, rs_R, cs_L, cs_R;

@@ -336,13 +336,13 @@ struct SweepRec {
 static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double nu, int slot, int prev_slot)
 {
     const int in = c->cs, out = 1 - c->cs;
-    a.rho = c->st[in][0]; a.u = c->st[in][1]; a.v = c->st[in][2]; a.p = c->st[in][3]; a.e = c->st[in][4];
-    a.rho_o = c->st[out][0]; a.u_o = c->st[out][1]; a.v_o = c->st[out][2]; a.p_o = c->st[out][3]; a.e_o = c->st[out][4];
     const int sin = dir == 0 ? c->cx : c->cy;
-    for (int q = 0; q < 4; ++q) {
-        a.s[q] = dir == 0 ? c->sx[sin][q] : c->sy[sin][q];
-        a.s_o[q] = dir == 0 ? c->sx[1 - sin][q] : c->sy[1 - sin][q];
-    }
+    // the planes of each group are consecutive in c->block (hgrp2d_create)
+    a.st = c->st[in][0];
+    a.st_o = c->st[out][0];
+    a.s = dir == 0 ? c->sx[sin][0] : c->sy[sin][0];
+    a.s_o = dir == 0 ? c->sx[1 - sin][0] : c->sy[1 - sin][0];
+    a.plane = (size_t)c->m * c->ld;
     a.bnd_lo = c->bnd[dir == 0 ? 0 : 2];
     a.bnd_hi = c->bnd[dir == 0 ? 1 : 3];
     a.m = c->m; a.n = c->n; a.ld = c->ld;
@@ -719,13 +719,12 @@ static constexpr int UNSPLIT_SLOTS = STATUS_SLOTS / 2;
 static void fill_unsplit_args(hgrp2d *c, UnsplitArgs &a, double tau, double nu, double mu, int slot, int prev_slot)
 {
     const int in = c->cs, out = 1 - c->cs;
-    a.rho = c->st[in][0]; a.u = c->st[in][1]; a.v = c->st[in][2]; a.p = c->st[in][3]; a.e = c->st[in][4];
-    a.rho_o = c->st[out][0]; a.u_o = c->st[out][1]; a.v_o = c->st[out][2]; a.p_o = c->st[out][3];
-    a.e_o = c->st[out][4];
-    for (int q = 0; q < 4; ++q) {
-        a.s[q] = c->sx[c->cx][q]; a.s_o[q] = c->sx[1 - c->cx][q];
-        a.t[q] = c->sy[c->cy][q]; a.t_o[q] = c->sy[1 - c->cy][q];
-    }
+    // the planes of each group are consecutive in c->block (hgrp2d_create)
+    a.st = c->st[in][0];
+    a.st_o = c->st[out][0];
+    a.s = c->sx[c->cx][0]; a.s_o = c->sx[1 - c->cx][0];
+    a.t = c->sy[c->cy][0]; a.t_o = c->sy[1 - c->cy][0];
+    a.plane = (size_t)c->m * c->ld;
     a.bx_lo = c->bnd[0]; a.bx_hi = c->bnd[1]; a.by_lo = c->bnd[2]; a.by_hi = c->bnd[3];
     a.m = c->m; a.n = c->n; a.ld = c->ld;
     a.bc_x = (int)c->cfg[17]; a.bc_y = (int)c->cfg[18];

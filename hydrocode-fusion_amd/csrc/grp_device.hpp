@@ -32,33 +32,35 @@ struct Side {
 
 // Constants of one gamma-law gas, formed once (on the host) with exactly the
 // expressions the reference writes inline, so every use sees the same double.
-// Passed by value in kernel arguments, they stay in scalar registers.
+// Passed by value in kernel arguments, they live in scalar registers; the ones
+// that are a single operation on g or zeta are formed where they are used
+// (the same operation, so the same double), which keeps the struct at ten
+// doubles: the sweeps run short of scalar registers.
 struct Gas {
-    double g, gm1, gp1;            // gamma, gamma - 1, gamma + 1
+    double g;                      // gamma
     double mu, nu;                 // (g-1)/(2g), (g+1)/(2g)       wave curves
-    double z, opz, op2z, omz;      // zeta = (g-1)/(g+1), 1+zeta, 1+2 zeta, 1-zeta
-    double tpz, zz, omzz, zzm1;    // 2+zeta, zeta^2, 1-zeta^2, zeta^2-1
+    double z;                      // zeta = (g-1)/(g+1)
     double inv_g, e_cs;            // 1/g, 0.5 (g-1)/g            star density / sound speed
     double e_sonic, e_a, e_b;      // 2g/(g-1), 0.5/zeta, (1+zeta)/zeta
     double g_gm1;                  // g/(g-1)                      energy flux
+    __host__ __device__ double gm1() const { return g - 1.0; }
+    __host__ __device__ double gp1() const { return g + 1.0; }
+    __host__ __device__ double opz() const { return 1.0 + z; }          // 1 + zeta
+    __host__ __device__ double op2z() const { return 1.0 + 2.0 * z; }   // 1 + 2 zeta
+    __host__ __device__ double omz() const { return 1 - z; }            // 1 - zeta
+    __host__ __device__ double tpz() const { return 2.0 + z; }          // 2 + zeta
+    __host__ __device__ double zz() const { return z * z; }             // zeta^2
+    __host__ __device__ double omzz() const { return 1.0 - z * z; }     // 1 - zeta^2
+    __host__ __device__ double zzm1() const { return z * z - 1.0; }     // zeta^2 - 1
 };
 
 __host__ __device__ inline Gas make_gas(double g)
 {
     Gas s;
     s.g = g;
-    s.gm1 = g - 1.0;
-    s.gp1 = g + 1.0;
     s.mu = (g - 1.0) / (2.0 * g);
     s.nu = (g + 1.0) / (2.0 * g);
     s.z = (g - 1.0) / (g + 1.0);
-    s.opz = 1.0 + s.z;
-    s.op2z = 1.0 + 2.0 * s.z;
-    s.omz = 1 - s.z;
-    s.tpz = 2.0 + s.z;
-    s.zz = s.z * s.z;
-    s.omzz = 1.0 - s.z * s.z;
-    s.zzm1 = s.z * s.z - 1.0;
     s.inv_g = 1.0 / g;
     s.e_cs = 0.5 * (g - 1.0) / g;
     s.e_sonic = 2.0 * g / (g - 1.0);
@@ -137,7 +139,7 @@ __device__ __forceinline__ double wave_curve(double p, double u_s, double p_s, d
         const PowBase b = pow_base(p / p_s);
         *logx = b.L;
         w = pow_with(b, G.mu) - 1.0;
-        w = 2.0 * c_s * w / G.gm1;
+        w = 2.0 * c_s * w / G.gm1();
     }
     return SIGN < 0 ? u_s - w : u_s + w;
 }
@@ -343,7 +345,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         } else {
             rs_L = rho_L * (p_star + zL * p_L) / (p_L + zL * p_star);
             cs_L = sqrt(gL * p_star / rs_L);
-            speed_L = u_L - c_L * sqrt(0.5 * (GL.gp1 * (p_star / p_L) + GL.gm1) / gL);
+            speed_L = u_L - c_L * sqrt(0.5 * (GL.gp1() * (p_star / p_L) + GL.gm1()) / gL);
         }
         if (crw1) {
             const PowBase b = pow_base(p_star / p_R);
@@ -353,7 +355,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         } else {
             rs_R = rho_R * (p_star + zR * p_R) / (p_R + zR * p_star);
             cs_R = sqrt(gR * p_star / rs_R);
-            speed_R = u_R + c_R * sqrt(0.5 * (GR.gp1 * (p_star / p_R) + GR.gm1) / gR);
+            speed_R = u_R + c_R * sqrt(0.5 * (GR.gp1() * (p_star / p_R) + GR.gm1()) / gR);
         }
     }
 
@@ -379,13 +381,13 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         // larger atc (INFINITY: acoustic only) takes the exact star state,
         // whose rarefaction fans the t-axis can lie in (:173-192).
         if (X && crw0 && ((u_star - cs_L) > lu)) {
-            U[1] = zL * (u_L + 2.0 * (c_L + lu) / GL.gm1);
+            U[1] = zL * (u_L + 2.0 * (c_L + lu) / GL.gm1());
             C = U[1] - lu;
             U[3] = pow_with(pow_base(C / c_L), GL.e_sonic) * p_L;
             U[0] = gL * U[3] / C / C;
             U[2] = v_L;
         } else if (X && crw1 && ((u_star + cs_R) < lu)) {
-            U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1);
+            U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1());
             C = lu - U[1];
             U[3] = pow_with(pow_base(C / c_R), GR.e_sonic) * p_R;
             U[0] = gR * U[3] / C / C;
@@ -422,38 +424,38 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
     }
 
     if (crw0 && ((u_star - cs_L) > lu)) {                 // t-axis inside the 1-CRW
-        U[1] = zL * (u_L + 2.0 * (c_L + lu) / GL.gm1);
+        U[1] = zL * (u_L + 2.0 * (c_L + lu) / GL.gm1());
         C = U[1] - lu;
         const PowBase b = pow_base(C / c_L);             // c_frac = C / c_L below
         U[3] = pow_with(b, GL.e_sonic) * p_L;
         U[0] = gL * U[3] / C / C;
         U[2] = v_L;
-        const double TdS = (d_p_L - d_rho_L * c_L * c_L) / GL.gm1 / rho_L;
-        const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / GL.gm1 / rho_L;
+        const double TdS = (d_p_L - d_rho_L * c_L * c_L) / GL.gm1() / rho_L;
+        const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / GL.gm1() / rho_L;
         const double pw_a = pow_with(b, GL.e_a), pw_b = pow_with(b, GL.e_b);
-        D[1] = (GL.opz * pw_a + zL * pw_b);
-        D[1] = D[1] / GL.op2z * TdS;
+        D[1] = (GL.opz() * pw_a + zL * pw_b);
+        D[1] = D[1] / GL.op2z() * TdS;
         D[1] = D[1] - c_L * pw_a * d_Psi;
         D[3] = U[0] * (U[1] - lu) * D[1];
-        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GL.gm1;
+        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GL.gm1();
         D[0] = (D[0] + D[3]) / C / C;
         D[2] = -(U[1] - lu) * d_v_L * U[0] / rho_L;
         if constexpr (TF) { FO[0] = FA[0]; FO[1] = -(U[1] - lu) * FA[1] * U[0] / rho_L; }
     } else if (crw1 && ((u_star + cs_R) < lu)) {          // t-axis inside the 3-CRW
-        U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1);
+        U[1] = zR * (u_R - 2.0 * (c_R - lu) / GR.gm1());
         C = lu - U[1];
         const PowBase b = pow_base(C / c_R);
         U[3] = pow_with(b, GR.e_sonic) * p_R;
         U[0] = gR * U[3] / C / C;
         U[2] = v_R;
-        const double TdS = (d_p_R - d_rho_R * c_R * c_R) / GR.gm1 / rho_R;
-        const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / GR.gm1 / rho_R;
+        const double TdS = (d_p_R - d_rho_R * c_R * c_R) / GR.gm1() / rho_R;
+        const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / GR.gm1() / rho_R;
         const double pw_a = pow_with(b, GR.e_a), pw_b = pow_with(b, GR.e_b);
-        D[1] = (GR.opz * pw_a + zR * pw_b);
-        D[1] = D[1] / GR.op2z * TdS;
+        D[1] = (GR.opz() * pw_a + zR * pw_b);
+        D[1] = D[1] / GR.op2z() * TdS;
         D[1] = D[1] + c_R * pw_a * d_Phi;
         D[3] = U[0] * (U[1] - lu) * D[1];
-        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GR.gm1;
+        D[0] = U[0] * (U[1] - lu) * pw_b * TdS * GR.gm1();
         D[0] = (D[0] + D[3]) / C / C;
         D[2] = -(U[1] - lu) * d_v_R * U[0] / rho_R;
         if constexpr (TF) { FO[0] = FA[2]; FO[1] = -(U[1] - lu) * FA[3] * U[0] / rho_R; }
@@ -470,19 +472,19 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
             a_L = 1.0;
             b_L = 1.0 / rs_L / cs_L;
             const PowBase b = pow_base(cs_L / c_L);
-            const double TdS = (d_p_L - d_rho_L * c_L * c_L) / GL.gm1 / rho_L;
-            const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / GL.gm1 / rho_L;
+            const double TdS = (d_p_L - d_rho_L * c_L * c_L) / GL.gm1() / rho_L;
+            const double d_Psi = d_u_L + (gL * d_p_L / c_L - c_L * d_rho_L) / GL.gm1() / rho_L;
             const double pw_a = pow_with(b, GL.e_a);
             pwL_b = pow_with(b, GL.e_b);
-            d_L = (GL.opz * pw_a + zL * pwL_b);
-            d_L = d_L / GL.op2z * TdS;
+            d_L = (GL.opz() * pw_a + zL * pwL_b);
+            d_L = d_L / GL.op2z() * TdS;
             d_L = d_L - c_L * pw_a * d_Psi;
         } else {
-            const double SmUs = -sqrt(0.5 * (GL.gp1 * p_L + GL.gm1 * p_star) / rs_L);
-            const double SmUL = -sqrt(0.5 * (GL.gp1 * p_star + GL.gm1 * p_L) / rho_L);
-            const double VAR = sqrt(GL.omz / (rho_L * (p_star + zL * p_L)));
-            const double H1 = 0.5 * VAR * (p_star + GL.op2z * p_L) / (p_star + zL * p_L);
-            const double H2 = -0.5 * VAR * (GL.tpz * p_star + zL * p_L) / (p_star + zL * p_L);
+            const double SmUs = -sqrt(0.5 * (GL.gp1() * p_L + GL.gm1() * p_star) / rs_L);
+            const double SmUL = -sqrt(0.5 * (GL.gp1() * p_star + GL.gm1() * p_L) / rho_L);
+            const double VAR = sqrt(GL.omz() / (rho_L * (p_star + zL * p_L)));
+            const double H1 = 0.5 * VAR * (p_star + GL.op2z() * p_L) / (p_star + zL * p_L);
+            const double H2 = -0.5 * VAR * (GL.tpz() * p_star + zL * p_L) / (p_star + zL * p_L);
             const double H3 = -0.5 * VAR * (p_star - p_L) / rho_L;
             const double L_p = -1.0 / rho_L - SmUL * H2;
             const double L_u = SmUL + rho_L * (c_L * c_L * H2 + H3);
@@ -495,19 +497,19 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
             a_R = 1.0;
             b_R = -1.0 / rs_R / cs_R;
             const PowBase b = pow_base(cs_R / c_R);
-            const double TdS = (d_p_R - d_rho_R * c_R * c_R) / GR.gm1 / rho_R;
-            const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / GR.gm1 / rho_R;
+            const double TdS = (d_p_R - d_rho_R * c_R * c_R) / GR.gm1() / rho_R;
+            const double d_Phi = d_u_R - (gR * d_p_R / c_R - c_R * d_rho_R) / GR.gm1() / rho_R;
             const double pw_a = pow_with(b, GR.e_a);
             pwR_b = pow_with(b, GR.e_b);
-            d_R = (GR.opz * pw_a + zR * pwR_b);
-            d_R = d_R / GR.op2z * TdS;
+            d_R = (GR.opz() * pw_a + zR * pwR_b);
+            d_R = d_R / GR.op2z() * TdS;
             d_R = d_R + c_R * pw_a * d_Phi;
         } else {
-            const double SmUs = sqrt(0.5 * (GR.gp1 * p_R + GR.gm1 * p_star) / rs_R);
-            const double SmUR = sqrt(0.5 * (GR.gp1 * p_star + GR.gm1 * p_R) / rho_R);
-            const double VAR = sqrt(GR.omz / (rho_R * (p_star + zR * p_R)));
-            const double H1 = 0.5 * VAR * (p_star + GR.op2z * p_R) / (p_star + zR * p_R);
-            const double H2 = -0.5 * VAR * (GR.tpz * p_star + zR * p_R) / (p_star + zR * p_R);
+            const double SmUs = sqrt(0.5 * (GR.gp1() * p_R + GR.gm1() * p_star) / rs_R);
+            const double SmUR = sqrt(0.5 * (GR.gp1() * p_star + GR.gm1() * p_R) / rho_R);
+            const double VAR = sqrt(GR.omz() / (rho_R * (p_star + zR * p_R)));
+            const double H1 = 0.5 * VAR * (p_star + GR.op2z() * p_R) / (p_star + zR * p_R);
+            const double H2 = -0.5 * VAR * (GR.tpz() * p_star + zR * p_R) / (p_star + zR * p_R);
             const double H3 = -0.5 * VAR * (p_star - p_R) / rho_R;
             const double L_p = -1.0 / rho_R + SmUR * H2;
             const double L_u = SmUR - rho_R * (c_R * c_R * H2 + H3);
@@ -530,11 +532,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[2] = D[2] + lu * d_v_R;
                 if constexpr (TF) { FO[0] = FA[2]; FO[1] = -U[1] * FA[3] * U[0] / rho_R + lu * FA[3]; }
             } else {
-                const double SmUs = sqrt(0.5 * (GR.gp1 * p_R + GR.gm1 * p_star) / rs_R);
-                const double SmUR = sqrt(0.5 * (GR.gp1 * p_star + GR.gm1 * p_R) / rho_R);
+                const double SmUs = sqrt(0.5 * (GR.gp1() * p_R + GR.gm1() * p_star) / rs_R);
+                const double SmUR = sqrt(0.5 * (GR.gp1() * p_star + GR.gm1() * p_R) / rho_R);
                 const double VAR = p_R + zR * p_star;
-                const double H1 = rho_R * p_R * GR.omzz / VAR / VAR;
-                const double H2 = rho_R * p_star * GR.zzm1 / VAR / VAR;
+                const double H1 = rho_R * p_R * GR.omzz() / VAR / VAR;
+                const double H2 = rho_R * p_star * GR.zzm1() / VAR / VAR;
                 const double H3 = (p_star + zR * p_R) / VAR;
                 const double L_rho = SmUR * H3 * d_rho_R;
                 const double L_u = -rho_R * (H2 * c_R * c_R + H3) * d_u_R;
@@ -556,11 +558,11 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
                 D[2] = D[2] + lu * d_v_L;
                 if constexpr (TF) { FO[0] = FA[0]; FO[1] = -U[1] * FA[1] * U[0] / rho_L + lu * FA[1]; }
             } else {
-                const double SmUs = -sqrt(0.5 * (GL.gp1 * p_L + GL.gm1 * p_star) / rs_L);
-                const double SmUL = -sqrt(0.5 * (GL.gp1 * p_star + GL.gm1 * p_L) / rho_L);
+                const double SmUs = -sqrt(0.5 * (GL.gp1() * p_L + GL.gm1() * p_star) / rs_L);
+                const double SmUL = -sqrt(0.5 * (GL.gp1() * p_star + GL.gm1() * p_L) / rho_L);
                 const double VAR = p_L + zL * p_star;
-                const double H1 = rho_L * p_L * GL.omzz / VAR / VAR;
-                const double H2 = rho_L * p_star * GL.zzm1 / VAR / VAR;
+                const double H1 = rho_L * p_L * GL.omzz() / VAR / VAR;
+                const double H2 = rho_L * p_star * GL.zzm1() / VAR / VAR;
                 const double H3 = (p_star + zL * p_L) / VAR;
                 const double L_rho = SmUL * H3 * d_rho_L;
                 const double L_u = -rho_L * (H2 * c_L * c_L + H3) * d_u_L;

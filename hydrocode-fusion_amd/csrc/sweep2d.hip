@@ -51,7 +51,8 @@ struct Line {
     __device__ void cell(int k, double w[4]) const
     {
         const size_t o = at(k);
-        w[0] = a.rho[o]; w[1] = a.u[o]; w[2] = a.v[o]; w[3] = a.p[o];
+        const double *f = a.st + o;
+        w[0] = f[0]; w[1] = f[a.plane]; w[2] = f[2 * a.plane]; w[3] = f[3 * a.plane];
     }
     // value of cell k, ghost values at k == -1 / k == cnt (bound_cond_slope_limiter_x.c:34-82)
     __device__ void value(int k, double w[4]) const
@@ -85,7 +86,7 @@ struct Line {
     {
         const size_t o = at(k);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) sp[q] = a.limiter3 ? a.s[q][o] : 0.0;
+        for (int q = 0; q < 4; ++q) sp[q] = a.limiter3 ? a.s[o + q * a.plane] : 0.0;
     }
     // limited slope from the neighbours and the previous slope sp (slope_limiter_2D_x.c:177-211)
     __device__ void limited_sp(const double vm[4], const double v0[4], const double vp[4], const double sp[4],
@@ -134,7 +135,7 @@ struct Line {
     __device__ void update(int k, const double w[4], const double Fl[4], const double Fr[4],
                            const double Il[4], const double Ir[4]) const
     {
-        update_e(k, w, a.e[at(k)], Fl, Fr, Il, Ir);
+        update_e(k, w, a.st[at(k) + 4 * a.plane], Fl, Fr, Il, Ir);
     }
     // the same with E of the cell already loaded
     __device__ void update_e(int k, const double w[4], const double e, const double Fl[4], const double Fr[4],
@@ -148,11 +149,13 @@ struct Line {
         const double rho = w[0] + -nu * (Fr[0] - Fl[0]);
         const double u = mom_x / rho;
         const double v = mom_y / rho;
-        const double p = (ene - 0.5 * mom_x * u - 0.5 * mom_y * v) * a.gas.gm1;
-        a.rho_o[o] = rho; a.u_o[o] = u; a.v_o[o] = v; a.e_o[o] = ene / rho; a.p_o[o] = p;
+        const double p = (ene - 0.5 * mom_x * u - 0.5 * mom_y * v) * a.gas.gm1();
+        double *f = a.st_o + o;
+        const size_t P = a.plane;
+        f[0] = rho; f[P] = u; f[2 * P] = v; f[4 * P] = ene / rho; f[3 * P] = p;
         if (p < a.eps || rho < a.eps) atomicOr(&a.status->flags, 1u);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a.s_o[q][o] = div_by(Ir[q] - Il[q], a.div_h);
+        for (int q = 0; q < 4; ++q) a.s_o[o + q * P] = div_by(Ir[q] - Il[q], a.div_h);
     }
 };
 
@@ -256,7 +259,7 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][6
     if (col && c <= m) L.value(c, v);
     if (own) {
         L.prev_slopes(c, sp);
-        e_c = a.e[L.at(c)];
+        e_c = a.st[L.at(c) + 4 * a.plane];
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
@@ -373,7 +376,7 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     if (rowv && c <= n) L.value(c, w);
     if (own) {
         L.prev_slopes(c, sp);
-        e_c = a.e[L.at(c)];
+        e_c = a.st[L.at(c) + 4 * a.plane];
     }
     const bool first = rowv && k == 0 && c >= 0, last = rowv && k == G - 1 && c + 1 <= n;
     const bool last_own = rowv && k == G - 1 && c + 1 < n;

@@ -30,10 +30,13 @@ struct SweepStatus {
 };
 
 struct SweepArgs {
-    const double *rho, *u, *v, *p, *e;        // state in
-    double *rho_o, *u_o, *v_o, *p_o, *e_o;    // state out
-    const double *s[4];                       // slopes of this direction (rho u v p), in
-    double *s_o[4];                           // refreshed slopes, out
+    // Field planes lie `plane` doubles apart, so one address per group reaches
+    // every field (fewer scalar registers held across the solve)
+    const double *st;                         // state in: rho u v p e
+    double *st_o;                             // state out, same layout
+    const double *s;                          // slopes of this direction (rho u v p), in
+    double *s_o;                              // refreshed slopes, out
+    size_t plane;                             // doubles from one field plane to the next
     const double *bnd_lo, *bnd_hi;            // BC -1 ghost values from the initial data: [line*4 + q]
     int m, n, ld;                             // cells in x, in y, row pitch
     int bc;                                   // boundary condition along the sweep

@@ -35,7 +35,8 @@ struct LineTF {
     __device__ void cell(int k, double w[NQ]) const
     {
         const size_t o = at(k);
-        w[0] = a.rho[o]; w[1] = a.u[o]; w[2] = a.v[o]; w[3] = a.p[o]; w[4] = a.fa[o];
+        const double *f = a.st + o;
+        w[0] = f[0]; w[1] = f[a.plane]; w[2] = f[2 * a.plane]; w[3] = f[3 * a.plane]; w[4] = a.fa[o];
     }
     __device__ void value(int k, double w[NQ]) const
     {
@@ -61,7 +62,7 @@ struct LineTF {
     __device__ void prev_slopes(int k, double sp[NQ]) const
     {
         const size_t o = at(k);
-        for (int q = 0; q < 4; ++q) sp[q] = a.limiter3 ? a.s[q][o] : 0.0;
+        for (int q = 0; q < 4; ++q) sp[q] = a.limiter3 ? a.s[o + q * a.plane] : 0.0;
         sp[4] = a.limiter3 ? a.sa[o] : 0.0;
     }
     __device__ void limited_sp(const double vm[NQ], const double v0[NQ], const double vp[NQ], const double sp[NQ],
@@ -118,13 +119,15 @@ struct LineTF {
         fa = fmin(fmax(fa, 0.0), 1.0);
         const double X = ene - 0.5 * mom_x * u - 0.5 * mom_y * v;
         double p;
-        if (fa == 1.0) p = X * a.two.a.gm1;
-        else if (fa == 0.0) p = X * a.two.b.gm1;
+        if (fa == 1.0) p = X * a.two.a.gm1();
+        else if (fa == 0.0) p = X * a.two.b.gm1();
         else p = X / (fa / (a.two.a.g - 1.0) + (1.0 - fa) / (a.two.b.g - 1.0));
-        a.rho_o[o] = rho; a.u_o[o] = u; a.v_o[o] = v; a.e_o[o] = ene / rho; a.p_o[o] = p; a.fa_o[o] = fa;
+        double *f = a.st_o + o;
+        const size_t P = a.plane;
+        f[0] = rho; f[P] = u; f[2 * P] = v; f[4 * P] = ene / rho; f[3 * P] = p; a.fa_o[o] = fa;
         if (p < a.eps || rho < a.eps) atomicOr(&a.status->flags, 1u);
 #pragma unroll
-        for (int q = 0; q < 4; ++q) a.s_o[q][o] = div_by(Ir[q] - Il[q], a.div_h);
+        for (int q = 0; q < 4; ++q) a.s_o[o + q * P] = div_by(Ir[q] - Il[q], a.div_h);
         a.sa_o[o] = div_by(Zr[2] - Zl[2], a.div_h);
     }
 };
@@ -214,7 +217,7 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
     if (col && c <= m) L.value(c, v);
     if (own) {
         L.prev_slopes(c, sp);
-        e_c = a.e[L.at(c)];
+        e_c = a.st[L.at(c) + 4 * a.plane];
     }
 #pragma unroll
     for (int q = 0; q < NQ; ++q) xa[w][q][lane] = v[q];
@@ -303,7 +306,7 @@ __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chu
     if (rowv && c <= n) L.value(c, w);
     if (own) {
         L.prev_slopes(c, sp);
-        e_c = a.e[L.at(c)];
+        e_c = a.st[L.at(c) + 4 * a.plane];
     }
     const bool first = rowv && k == 0 && c >= 0, last = rowv && k == G - 1 && c + 1 <= n;
     const bool last_own = rowv && k == G - 1 && c + 1 < n;

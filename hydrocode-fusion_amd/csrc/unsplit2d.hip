@@ -39,7 +39,8 @@ struct Grid {
     __device__ __forceinline__ void cell(int c, int ci, double w[4]) const
     {
         const size_t o = at(c, ci);
-        w[0] = a.rho[o]; w[1] = a.u[o]; w[2] = a.v[o]; w[3] = a.p[o];
+        const double *f = a.st + o;
+        w[0] = f[0]; w[1] = f[a.plane]; w[2] = f[2 * a.plane]; w[3] = f[3 * a.plane];
     }
     // value of row c in [-1, m] of column ci in [0, n): x-ghosts per config[17]
     // (bound_cond_slope_limiter_x.c:34-82)
@@ -78,14 +79,14 @@ struct Grid {
         }
     }
     // minmod slope from the neighbours (slope_limiter_2D_x.c:32-84, slope_limiter.c:31-83)
-    __device__ __forceinline__ void limit(const double vm[4], const double v0[4], const double vp[4], const double *const prev[4],
+    __device__ __forceinline__ void limit(const double vm[4], const double v0[4], const double vp[4], const double *prev,
                           size_t o, int lim3, const RecipDiv &d, double s[4]) const
     {
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const double s_L = div_by(v0[q] - vm[q], d);
             const double s_R = div_by(vp[q] - v0[q], d);
-            s[q] = lim3 ? minmod3(a.alpha * s_L, a.alpha * s_R, prev[q][o]) : minmod2(s_L, s_R);
+            s[q] = lim3 ? minmod3(a.alpha * s_L, a.alpha * s_R, prev[o + q * a.plane]) : minmod2(s_L, s_R);
         }
     }
     __device__ __forceinline__ void lim_x(int c, int ci, const double v0[4], double s[4]) const
@@ -268,7 +269,7 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
     // combined update and slope refresh (grp_solver_2D_EUL_source.c:231-262)
     if (w >= 1 && w <= W - 2 && xin && yin && lane >= 1 && lane <= 62) {
         const size_t o = G.at(c, ci);
-        const double e_c = a.e[o];
+        const double e_c = a.st[o + 4 * a.plane];
         double vc[4], Fl[4], Ixl[4], Fr[4], Ixr[4], Gy[4], Iy[4], Gl[4], Iyl[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
@@ -285,13 +286,15 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
         const double rho = vc[0] + (-nu * (Fr[0] - Fl[0]) - mu * (Gy[0] - Gl[0]));
         const double uu = mom_x / rho;
         const double vv = mom_y / rho;
-        const double p = (ene - 0.5 * mom_x * uu - 0.5 * mom_y * vv) * a.gas.gm1;
-        a.rho_o[o] = rho; a.u_o[o] = uu; a.v_o[o] = vv; a.e_o[o] = ene / rho; a.p_o[o] = p;
+        const double p = (ene - 0.5 * mom_x * uu - 0.5 * mom_y * vv) * a.gas.gm1();
+        double *fo = a.st_o + o;
+        const size_t P = a.plane;
+        fo[0] = rho; fo[P] = uu; fo[2 * P] = vv; fo[4 * P] = ene / rho; fo[3 * P] = p;
         if (p < a.eps || rho < a.eps) atomicOr(&a.status[0].flags, 1u);
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            a.s_o[q][o] = div_by(Ixr[q] - Ixl[q], a.div_hx);
-            a.t_o[q][o] = div_by(Iy[q] - Iyl[q], a.div_hy);
+            a.s_o[o + q * P] = div_by(Ixr[q] - Ixl[q], a.div_hx);
+            a.t_o[o + q * P] = div_by(Iy[q] - Iyl[q], a.div_hy);
         }
     }
 }
@@ -383,7 +386,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
         __syncthreads();
         if (xin && yin && lane >= 1 && lane <= 62) {
             const size_t o = G.at(r, ci);
-            const double e_c = a.e[o];
+            const double e_c = a.st[o + 4 * a.plane];
             double vc[4], Fl[4], Ixl[4], Gy[4], Gl[4], Iy[4], Iyl[4];
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
@@ -399,13 +402,15 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
             const double rho = vc[0] + (-nu * (F[0] - Fl[0]) - mu * (Gy[0] - Gl[0]));
             const double uu = mom_x / rho;
             const double vv = mom_y / rho;
-            const double p = (ene - 0.5 * mom_x * uu - 0.5 * mom_y * vv) * a.gas.gm1;
-            a.rho_o[o] = rho; a.u_o[o] = uu; a.v_o[o] = vv; a.e_o[o] = ene / rho; a.p_o[o] = p;
+            const double p = (ene - 0.5 * mom_x * uu - 0.5 * mom_y * vv) * a.gas.gm1();
+            double *fo = a.st_o + o;
+            const size_t P = a.plane;
+            fo[0] = rho; fo[P] = uu; fo[2 * P] = vv; fo[4 * P] = ene / rho; fo[3 * P] = p;
             if (p < a.eps || rho < a.eps) atomicOr(&a.status[0].flags, 1u);
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                a.s_o[q][o] = div_by(I[q] - Ixl[q], a.div_hx);
-                a.t_o[q][o] = div_by(Iy[q] - Iyl[q], a.div_hy);
+                a.s_o[o + q * P] = div_by(I[q] - Ixl[q], a.div_hx);
+                a.t_o[o + q * P] = div_by(Iy[q] - Iyl[q], a.div_hy);
             }
         }
         // x-interface r + 1 is the next row's lower interface (each lane keeps its own slot)

@@ -17,10 +17,13 @@
 namespace hgrp {
 
 struct UnsplitArgs {
-    const double *rho, *u, *v, *p, *e;        // state in
-    double *rho_o, *u_o, *v_o, *p_o, *e_o;    // state out
-    const double *s[4], *t[4];                // x- and y-slopes (rho u v p) of the previous step
-    double *s_o[4], *t_o[4];                  // refreshed slopes, out
+    // Field planes lie `plane` doubles apart, so one address per group reaches
+    // every field (fewer scalar registers held across the solves)
+    const double *st;                         // state in: rho u v p e
+    double *st_o;                             // state out, same layout
+    const double *s, *t;                      // x- and y-slopes (rho u v p) of the previous step
+    double *s_o, *t_o;                        // refreshed slopes, out
+    size_t plane;                             // doubles from one field plane to the next
     const double *bx_lo, *bx_hi;              // BC -1 ghosts of the x-ends from the initial data: [i*4 + q]
     const double *by_lo, *by_hi;              // ... of the y-ends: [j*4 + q]
     int m, n, ld;                             // cells in x, in y, row pitch
