@@ -56,6 +56,24 @@ struct SweepArgs {
     const SweepStatus *prev;                  // the sweep before (nullptr for the first)
 };
 
+// The kernel's argument block (its first parameter) read again at the point of
+// the call: the copy's fields are loaded there from kernel-argument memory
+// (scalar-cache loads) through an address the compiler cannot trace back to
+// the loads at the top of the kernel, so values used only after the solves
+// are not held in scalar registers across them.
+template <class Args>
+__device__ __forceinline__ Args reload_args()
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    using P = __attribute__((address_space(4))) const Args *;
+    P p = (P)__builtin_amdgcn_kernarg_segment_ptr();
+    __asm__ volatile("" : "+s"(p));
+    return *p;
+#else
+    return Args{};   // device code only
+#endif
+}
+
 // Interface state from a cell value w and its slope s along the interface
 // normal: w + h/2 s on the left (lower) side, w - h/2 s on the right (upper)
 // side, with the slope as the normal derivative (flux_generator_x.c:45-86).

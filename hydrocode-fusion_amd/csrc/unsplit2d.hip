@@ -268,6 +268,7 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
 
     // combined update and slope refresh (grp_solver_2D_EUL_source.c:231-262)
     if (w >= 1 && w <= W - 2 && xin && yin && lane >= 1 && lane <= 62) {
+        const UnsplitArgs a = reload_args<UnsplitArgs>();
         const size_t o = G.at(c, ci);
         const double e_c = a.st[o + 4 * a.plane];
         double vc[4], Fl[4], Ixl[4], Fr[4], Ixr[4], Gy[4], Iy[4], Gl[4], Iyl[4];
@@ -328,7 +329,8 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
 
     auto put_side = [&](int buf, int c) {
         double v[4] = {1.0, 0.0, 0.0, 1.0}, s[4] = {0.0, 0.0, 0.0, 0.0}, t[4] = {0.0, 0.0, 0.0, 0.0};
-        if (colv && c <= m && ((c >= 0 && c < m) || yin)) G.side(c, ci, v, s, t);
+        const UnsplitArgs ar = reload_args<UnsplitArgs>();   // nothing of the loads is held across the solves
+        if (colv && c <= m && ((c >= 0 && c < m) || yin)) Grid(ar).side(c, ci, v, s, t);
 #pragma unroll
         for (int q = 0; q < 4; ++q) { sd[buf][q][lane] = v[q]; sd[buf][4 + q][lane] = s[q]; sd[buf][8 + q][lane] = t[q]; }
     };
@@ -385,6 +387,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
         }
         __syncthreads();
         if (xin && yin && lane >= 1 && lane <= 62) {
+            const UnsplitArgs a = reload_args<UnsplitArgs>();
             const size_t o = G.at(r, ci);
             const double e_c = a.st[o + 4 * a.plane];
             double vc[4], Fl[4], Ixl[4], Gy[4], Gl[4], Iy[4], Iyl[4];
