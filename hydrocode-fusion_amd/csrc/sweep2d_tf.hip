@@ -276,7 +276,8 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
         for (int q = 0; q < 3; ++q) Zl[q] = xz[w - 1][q][lane];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) vc[q] = xa[w][q][lane];
-        L.update(c, vc, xe[w][lane], Fl, F, Zl, Z, Il, I);
+        const SweepArgs ar = reload_args<SweepArgs>();   // not held across the solve
+        LineTF<0>(ar, i).update(c, vc, xe[w][lane], Fl, F, Zl, Z, Il, I);
     }
 }
 
@@ -360,7 +361,8 @@ __global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chu
         double wc[NQ];
 #pragma unroll
         for (int q = 0; q < NQ; ++q) wc[q] = st[q][lane];
-        L.update(c, wc, st[NQ][lane], Fl, F, Zl, Z, Il, I);
+        const SweepArgs ar = reload_args<SweepArgs>();   // not held across the solve
+        LineTF<1>(ar, j).update(c, wc, st[NQ][lane], Fl, F, Zl, Z, Il, I);
     }
 }
 
