@@ -202,6 +202,27 @@ __device__ __forceinline__ void newton_step(const Gas &GL, const Gas &GR, double
     v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
 }
 
+// GuessP (src/riemann_solver/riemann_solver_starPU.c:20-58), the reference's
+// PVRS / two-shock starting pressure: PVRS where the pressure ratio is at most
+// 2 and the estimate lies between the two pressures, or lies below both;
+// otherwise the two-shock estimate; clipped below at eps.  The 2D path never
+// calls it (riemann_exact starts from the halved tangent intersection); it is
+// the named alternative starting guess (hgrp_riemann_pvrs_batch).
+__device__ __forceinline__ double guess_pvrs(double DL, double DR, double UL, double UR, double PL, double PR,
+                                             double CL, double CR, double GammaL, double GammaR, double eps)
+{
+    const double cup = 0.25 * (DL + DR) * (CL + CR);
+    double ppv = 0.5 * (PL + PR) + 0.5 * (UL - UR) * cup;
+    ppv = eps > ppv ? eps : ppv;
+    const double pmin = PL < PR ? PL : PR, pmax = PL > PR ? PL : PR;
+    const double qmax = pmax / pmin;
+    if ((qmax <= 2.0 && pmin <= ppv && ppv <= pmax) || ppv < pmin) return ppv;
+    const double gel = sqrt((2. / ((GammaL + 1.) * DL)) / ((GammaL - 1.) / (GammaL + 1.) * PL + ppv));
+    const double ger = sqrt((2. / ((GammaR + 1.) * DR)) / ((GammaR - 1.) / (GammaR + 1.) * PR + ppv));
+    const double pm = (gel * PL + ger * PR - (UR - UL)) / (gel + ger);
+    return eps > pm ? eps : pm;
+}
+
 // Riemann_solver_exact: wave types, the halved acoustic-tangent guess, Newton
 // on the velocity gap, u* from the last tangents.
 // p_guess > 0 (not in the reference): first up to GUESS_ITERS Newton steps

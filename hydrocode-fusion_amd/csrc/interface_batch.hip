@@ -39,6 +39,31 @@ __global__ void riemann_batch_kernel(const double *in, int count, int N, double 
     o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
 }
 
+// GuessP per row: DL DR UL UR PL PR CL CR GammaL GammaR eps
+__global__ void guessp_batch_kernel(const double *in, int count, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *a = in + (size_t)k * 11;
+    out[k] = guess_pvrs(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]);
+}
+
+// The exact solve started from GuessP (at most GUESS_ITERS Newton steps from
+// it, then the reference's own guess).  in: rho_L u_L p_L rho_R u_R p_R gamma_L
+// gamma_R eps tol; out: u* p* CRW0 CRW1 iterations guess.
+__global__ void riemann_pvrs_batch_kernel(const double *in, int count, int N, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *a = in + (size_t)k * 10;
+    const double rho_L = a[0], u_L = a[1], p_L = a[2], rho_R = a[3], u_R = a[4], p_R = a[5], gL = a[6], gR = a[7];
+    const double c_L = sqrt(gL * p_L / rho_L), c_R = sqrt(gR * p_R / rho_R);
+    const double pg = guess_pvrs(rho_L, rho_R, u_L, u_R, p_L, p_R, c_L, c_R, gL, gR, a[8]);
+    const StarResult r = riemann_exact(make_gas(gL), make_gas(gR), u_L, u_R, p_L, p_R, c_L, c_R, a[8], a[9], N, pg);
+    double *o = out + (size_t)k * 6;
+    o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters; o[5] = pg;
+}
+
 // Interface sides of the per-function entry points: 15 doubles in the order
 // gamma, lambda_u, lambda_v, rho, u, v, p, d_rho, d_u, d_v, d_p, t_rho, t_u, t_v, t_p.
 __device__ __forceinline__ Side side_of(const double *a)
@@ -211,6 +236,38 @@ extern "C" int hgrp_edir_batch(const double *L, const double *R, int count, doub
         hipLaunchKernelGGL(edir_batch_kernel, dim3((c->count + 127) / 128), dim3(128), 0, 0, (const double *)d[0],
                            (const double *)d[1], c->count, c->a, c->b, (double *)o);
     });
+}
+
+extern "C" int hgrp_guessp_batch(const double *in, int count, double *out)
+{
+    if (count < 0 || (count && (!in || !out))) return -1001;
+    if (count == 0) return 0;
+    const void *hin[1] = {in};
+    const size_t nb[1] = {(size_t)count * 11 * 8};
+    int cnt = count;
+    return run_batch(hin, nb, 1, out, (size_t)count * 8,
+                     [](void *const *d, void *o, void *cx) {
+                         const int c = *(const int *)cx;
+                         hipLaunchKernelGGL(guessp_batch_kernel, dim3((c + 127) / 128), dim3(128), 0, 0,
+                                            (const double *)d[0], c, (double *)o);
+                     },
+                     &cnt);
+}
+
+extern "C" int hgrp_riemann_pvrs_batch(const double *in, int count, int N, double *out)
+{
+    if (count < 0 || (count && (!in || !out))) return -1001;
+    if (count == 0) return 0;
+    const void *hin[1] = {in};
+    const size_t nb[1] = {(size_t)count * 10 * 8};
+    RiemCtx rc = {count, N};
+    return run_batch(hin, nb, 1, out, (size_t)count * 6 * 8,
+                     [](void *const *d, void *o, void *cx) {
+                         const RiemCtx *r = (const RiemCtx *)cx;
+                         hipLaunchKernelGGL(riemann_pvrs_batch_kernel, dim3((r->count + 127) / 128), dim3(128), 0, 0,
+                                            (const double *)d[0], r->count, r->N, (double *)o);
+                     },
+                     &rc);
 }
 
 extern "C" int hgrp_pow_batch(const double *x, const double *y, int count, double *out)

@@ -47,6 +47,8 @@ def lib():
     L.hgrp_riemann_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
     L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
     L.hgrp_q1d_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
+    L.hgrp_guessp_batch.argtypes = [_DP, C.c_int, _DP]
+    L.hgrp_riemann_pvrs_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
     L.hgrp_grp2d_flux_general_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _D, _D, _DP]
     L.hgrp_edir_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
     L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
@@ -330,6 +332,24 @@ def riemann_batch(inp, N):
     inp = np.ascontiguousarray(inp, dtype=np.float64)
     out = np.zeros((len(inp), 6))
     _check(lib().hgrp_riemann_batch(_p(inp), len(inp), N, _p(out)), "riemann batch")
+    return out
+
+
+def guessp_batch(rows):
+    """GuessP (the reference's PVRS / two-shock starting pressure) per row of
+    DL DR UL UR PL PR CL CR GammaL GammaR eps."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    out = np.zeros(len(rows))
+    _check(lib().hgrp_guessp_batch(_p(rows), len(rows), _p(out)), "guessp batch")
+    return out
+
+
+def riemann_pvrs_batch(rows, N=500):
+    """The exact solve from GuessP per row of rho_L u_L p_L rho_R u_R p_R gamma_L
+    gamma_R eps tol: rows of u* p* CRW0 CRW1 iterations guess."""
+    rows = np.ascontiguousarray(rows, dtype=np.float64)
+    out = np.zeros((len(rows), 6))
+    _check(lib().hgrp_riemann_pvrs_batch(_p(rows), len(rows), N, _p(out)), "riemann pvrs batch")
     return out
 
 

@@ -147,6 +147,19 @@ int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double ta
  * the reference's return value (the velocity gap left at the end). */
 int hgrp_riemann_batch(const double *in, int count, int N, double *out);
 
+/* GuessP (src/riemann_solver/riemann_solver_starPU.c:20-58), the reference's
+ * PVRS / two-shock starting pressure, per row of in[count*11] = DL DR UL UR PL
+ * PR CL CR GammaL GammaR eps; out[count]. */
+int hgrp_guessp_batch(const double *in, int count, double *out);
+
+/* The exact Riemann solve (Riemann_solver_exact) started from GuessP instead of
+ * the halved tangent intersection: at most 3 Newton steps from it, then the
+ * reference's own guess, so the answer is the exact solver's to its tolerance.
+ * in[count*10] = rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R eps tol (as
+ * hgrp_riemann_guess_batch); out[count*6] = u* p* CRW0 CRW1 Newton-iterations
+ * starting-guess. */
+int hgrp_riemann_pvrs_batch(const double *in, int count, int N, double *out);
+
 /* linear_GRP_solver_Edir_Q1D (src/riemann_solver/linear_grp_solver_Edir_Q1D.c:38-554,
  * src/include/riemann_solver.h:66) with all of its arguments: atc (eps: the
  * quasi-1D GRP with its acoustic case; INFINITY: acoustic only; -0.0:

@@ -66,6 +66,11 @@ double orc_riemann_exact_ben(double *U_star, double *P_star, double gamma,
                              double c_L, double c_R, int CRW[2],
                              double eps, double tol, int N, int *n_iter);
 
+/* GuessP (src/riemann_solver/riemann_solver_starPU.c:20-58): PVRS / two-shock
+ * starting pressure (the named fallback guess; the 2D path never calls it). */
+double orc_guessp(double DL, double DR, double UL, double UR, double PL, double PR,
+                  double CL, double CR, double GammaL, double GammaR, double eps);
+
 /* linear_GRP_solver_Edir_Q1D (src/riemann_solver/linear_grp_solver_Edir_Q1D.c:38-554),
  * single-fluid members: D[0..3], U[0..3] = (rho, u, v, p); U_star[6]. */
 void orc_grp_q1d(double wave_speed[2], double D[4], double U[4], double U_star[6],

@@ -5,6 +5,8 @@
  * Follows src/riemann_solver/riemann_solver_exact_Ben.c:
  *   orc_riemann_exact     <- Riemann_solver_exact      (:30-211)
  *   orc_riemann_exact_ben <- Riemann_solver_exact_Ben  (:230-408)
+ * and src/riemann_solver/riemann_solver_starPU.c:
+ *   orc_guessp            <- GuessP                    (:20-58)
  * Both evaluate the two wave curves through the left/right states
  * (shock branch above the state pressure, isentrope below), classify the two
  * waves with an eps margin, start Newton from half the intersection of the two
@@ -119,6 +121,28 @@ double orc_riemann_exact_ben(double *U_star, double *P_star, double gamma,
     const double mu = (gamma - 1.0) / (2.0 * gamma), nu = (gamma + 1.0) / (2.0 * gamma);
     return riemann_core(U_star, P_star, gamma, gamma, mu, nu, mu, nu,
                         u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter);
+}
+
+/* GuessP (riemann_solver_starPU.c:20-58): the PVRS pressure, kept where the
+ * pressure ratio is at most 2 and it lies between the two pressures, or where
+ * it lies below both (the two-rarefaction branch is commented out there);
+ * otherwise the two-shock estimate; both clipped below at eps. */
+double orc_guessp(double DL, double DR, double UL, double UR, double PL, double PR,
+                  double CL, double CR, double GammaL, double GammaR, double eps)
+{
+    const double quser = 2.0;
+    const double cup = 0.25 * (DL + DR) * (CL + CR);
+    double ppv = 0.5 * (PL + PR) + 0.5 * (UL - UR) * cup;
+    ppv = eps > ppv ? eps : ppv;
+    const double pmin = PL < PR ? PL : PR;
+    const double pmax = PL > PR ? PL : PR;
+    const double qmax = pmax / pmin;
+    if ((qmax <= quser && pmin <= ppv && ppv <= pmax) || ppv < pmin)
+        return ppv;
+    const double gel = sqrt((2. / ((GammaL + 1.) * DL)) / ((GammaL - 1.) / (GammaL + 1.) * PL + ppv));
+    const double ger = sqrt((2. / ((GammaR + 1.) * DR)) / ((GammaR - 1.) / (GammaR + 1.) * PR + ppv));
+    const double pm = (gel * PL + ger * PR - (UR - UL)) / (gel + ger);
+    return eps > pm ? eps : pm;
 }
 
 /* libm's pow over arrays: the pow the reference's arithmetic uses (glibc), for

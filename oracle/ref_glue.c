@@ -17,6 +17,9 @@
 
 #include "var_struc.h"
 #include "riemann_solver.h"
+/* The reference's Toro solver file, compiled in this unit so that its static
+ * GuessP can be called (it is no part of the 2D path otherwise). */
+#include "riemann_solver_starPU.c"
 #include "flux_calc.h"
 #include "inter_process.h"
 #include "finite_volume.h"
@@ -119,6 +122,12 @@ int ref_grp_2d_flux_struct(const double *Lm, const double *Rm, double tau, doubl
     ifv_to_members(Lout, &L);
     ifv_to_members(Rout, &R);
     return rc;
+}
+
+/* GuessP of riemann_solver_starPU.c:20-58.  in: DL DR UL UR PL PR CL CR GammaL GammaR eps */
+double ref_guessp(const double *in)
+{
+    return GuessP(in[0], in[1], in[2], in[3], in[4], in[5], in[6], in[7], in[8], in[9], in[10]);
 }
 
 /* out: D[3], U[3] */

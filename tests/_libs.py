@@ -74,6 +74,8 @@ class Oracle:
         L.orc_grp_q1d.argtypes = [DP, DP, DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), D, D]
         L.orc_grp_2d_flux.restype = C.c_int
         L.orc_grp_2d_flux.argtypes = [OrcIfv, OrcIfv, D, D, D, D, DP, DP]
+        L.orc_guessp.restype = D
+        L.orc_guessp.argtypes = [D] * 11
         L.orc_grp_edir.restype = None
         L.orc_grp_edir.argtypes = [DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), D, D]
         PP = C.POINTER(DP)
@@ -107,6 +109,9 @@ class Oracle:
         gap = self.lib.orc_riemann_exact_ben(C.byref(us), C.byref(ps), g, uL, uR, pL, pR, cL, cR,
                                              crw, eps, tol, N, C.byref(nit))
         return us.value, ps.value, crw[0], crw[1], gap, nit.value
+
+    def guessp(self, row):
+        return self.lib.orc_guessp(*[float(x) for x in row])
 
     def q1d(self, Lv, Rv, eps, atc):
         ws, Dd, U, Us = (D * 2)(), (D * 4)(), (D * 4)(), (D * 6)()
@@ -219,6 +224,8 @@ class RefLib:
         L.ref_grp_edir.argtypes = [DP, DP, D, D, DP]
         L.ref_grp_2d_flux_struct.argtypes = [DP, DP, D, DP, DP]
         L.ref_grp_2d_flux_struct.restype = C.c_int
+        L.ref_guessp.argtypes = [DP]
+        L.ref_guessp.restype = D
         L.ref_limiter.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, DP, DP, DP, DP, DP, DP, C.c_int, C.c_int]
         L.ref_limiter.restype = C.c_int
         L.ref_flux_gen.argtypes = [C.c_int, C.c_int, C.c_int, DP, DP, DP, DP, DP, D, C.c_int, DP, DP]
@@ -275,6 +282,11 @@ class RefLib:
         Lo, Ro = np.zeros(34), np.zeros(34)
         rc = self.lib.ref_grp_2d_flux_struct(dptr(Lm), dptr(Rm), tau, dptr(Lo), dptr(Ro))
         return rc, Lo, Ro
+
+    def guessp(self, row):
+        """GuessP (riemann_solver_starPU.c:20-58): row = DL DR UL UR PL PR CL CR GammaL GammaR eps."""
+        row = np.ascontiguousarray(row, dtype=np.float64)
+        return self.lib.ref_guessp(dptr(row))
 
     def limiter(self, direction, m, n, f0, f1, slopes, cross, bfv, found, slope=1):
         """bound_cond_slope_limiter_{x,y} with nt = 1: f0, f1 (4, m, n) frames; slopes

@@ -107,7 +107,12 @@ def interface_units(ref):
             if name == "d_rho":
                 sin_R[k, IFV_MEMBERS.index(name)] = v
         src[k], sout_L[k], sout_R[k] = ref.flux2d_struct(sin_L[k], sin_R[k], tau, EPS)
+    # GuessP on the same pairs: DL DR UL UR PL PR CL CR GammaL GammaR eps
+    gp_in = np.stack([L[:, 3], R[:, 3], L[:, 4], R[:, 4], L[:, 6], R[:, 6], cL, cR, L[:, 0], R[:, 0],
+                      np.full(n, EPS)], axis=1)
+    gp = np.array([ref.guessp(gp_in[k]) for k in range(n)])
     np.savez_compressed(os.path.join(HERE, "interface_units.npz"), L=L, R=R, tau=tau, eps=EPS,
+                        guessp_in=gp_in, guessp=gp,
                         riemann_in=riem_in, riemann=riem, atc=atcs, q1d=q1d, edir_inf=edir_inf,
                         normals=normals, flux=flux, struct_in_L=sin_L, struct_in_R=sin_R,
                         struct_out_L=sout_L, struct_out_R=sout_R, struct_rc=src)

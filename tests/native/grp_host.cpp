@@ -104,3 +104,12 @@ extern "C" void hg_riemann_many(const double *in, int n, int N, double *out)
         o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
     }
 }
+
+// GuessP per row: DL DR UL UR PL PR CL CR GammaL GammaR eps
+extern "C" void hg_guessp_many(const double *in, int n, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *a = in + 11 * k;
+        out[k] = guess_pvrs(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]);
+    }
+}

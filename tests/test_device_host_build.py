@@ -140,3 +140,11 @@ def test_interface_units_against_reference_and_devpow_oracle(dev, golden_units, 
     same = np.array([oracle_devpow.edir(L[k], R[k], EPS, np.inf) for k in range(n)])
     assert np.array_equal(out, same)
     assert _rel(out, g["edir_inf"]).max() <= 1e-12
+
+
+def test_guessp_against_reference(dev, golden_units):
+    """The device GuessP (no pow: sqrt and divisions only) equals the reference's bit for bit."""
+    gi = np.ascontiguousarray(golden_units["guessp_in"])
+    out = np.zeros(len(gi))
+    dev.hg_guessp_many(_dp(gi), len(gi), _dp(out))
+    assert np.array_equal(out, golden_units["guessp"])

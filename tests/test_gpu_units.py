@@ -149,3 +149,30 @@ def test_dropin_grp_2d_flux_structs(host, golden_units):
         else:
             assert _rel(outL[results], g["struct_out_L"][k, results]).max() <= TOL, k
     assert set(codes) == {0, 1, 2, 3}
+
+
+def test_guessp_batch(hg, golden_units):
+    """GuessP on the GPU equals the reference's bit for bit."""
+    out = hg.guessp_batch(golden_units["guessp_in"])
+    assert np.array_equal(out, golden_units["guessp"])
+
+
+def test_riemann_from_guessp(hg, golden_units):
+    """The exact solve started from GuessP ends at the reference's answer (to
+    the Newton tolerance) with the same wave types; the Newton iterations it
+    takes against the reference's halved-tangent guess are reported."""
+    from hydrogrp import mlp
+    g = golden_units
+    Lv, Rv = g["L"], g["R"]
+    rows = np.stack([Lv[:, 3], Lv[:, 4], Lv[:, 6], Rv[:, 3], Rv[:, 4], Rv[:, 6], Lv[:, 0], Rv[:, 0],
+                     np.full(len(Lv), EPS), np.full(len(Lv), EPS)], axis=1)
+    out = hg.riemann_pvrs_batch(rows, 500)
+    ref = g["riemann"]
+    assert np.array_equal(out[:, 2:4], ref[:, 2:4])
+    scale = np.maximum(np.abs(ref[:, :2]), 1.0)
+    assert (np.abs(out[:, :2] - ref[:, :2]) / scale).max() <= 1e-8
+    assert np.all(out[:, 5] >= EPS)
+    plain = mlp.riemann_guess_batch(rows[:, :8], eps=EPS, tol=EPS)
+    nz = plain[:, 4] > 0
+    print(f"Newton iterations over {nz.sum()} solved interfaces: halved tangents {plain[nz, 4].mean():.2f}, "
+          f"GuessP {out[nz, 4].mean():.2f}", flush=True)

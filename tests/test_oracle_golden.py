@@ -66,6 +66,29 @@ def test_interface_units_bitexact(oracle, golden_units):
     assert not bad, bad[:10]
 
 
+def test_guessp_bitexact_and_branches(oracle, golden_units):
+    """GuessP (the PVRS / two-shock starting pressure) of the oracle equals the
+    reference's on 1000 pairs that take each of its three branches; and it is
+    what its formulas say: the PVRS value where that branch applies, the
+    two-shock root where the collision is strong (checked against the textbook
+    two-shock approximation, Toro eq. 9.42, written independently below)."""
+    g = golden_units
+    gi, want = g["guessp_in"], g["guessp"]
+    got = np.array([oracle.guessp(r) for r in gi])
+    assert np.array_equal(got, want)
+    DL, DR, UL, UR, PL, PR, CL, CR, gL, gR, eps = gi.T
+    ppv = np.maximum(0.5 * (PL + PR) + 0.5 * (UL - UR) * 0.25 * (DL + DR) * (CL + CR), eps)
+    pmin, pmax = np.minimum(PL, PR), np.maximum(PL, PR)
+    pv = ((pmax / pmin <= 2) & (pmin <= ppv) & (ppv <= pmax)) | (ppv < pmin)
+    assert pv.sum() > 200 and (~pv).sum() > 200
+    assert np.allclose(got[pv], ppv[pv], rtol=1e-14, atol=0)
+    AL, AR = 2.0 / ((gL + 1) * DL), 2.0 / ((gR + 1) * DR)
+    BL, BR = (gL - 1) / (gL + 1) * PL, (gR - 1) / (gR + 1) * PR
+    gel, ger = np.sqrt(AL / (BL + ppv)), np.sqrt(AR / (BR + ppv))
+    ts = np.maximum((gel * PL + ger * PR - (UR - UL)) / (gel + ger), eps)
+    assert np.allclose(got[~pv], ts[~pv], rtol=1e-12, atol=1e-300)
+
+
 def test_interface_units_cover_the_arguments(golden_units):
     """atc changes the result (acoustic-only and nonlinear-only differ from the
     default), the grid velocity and the second gas are in play."""
