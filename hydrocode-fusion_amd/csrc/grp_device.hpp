@@ -165,12 +165,12 @@ struct StarResult {
     int iters;     // Newton iterations taken (-1: equal-state early exit)
 };
 
-// One Newton step of Riemann_solver_exact (riemann_solver_exact_Ben.c:133-203):
-// tangents k1, k3 at p, the update (halved when negative), the curves at the
-// new p.
-__device__ __forceinline__ void newton_step(const Gas &GL, const Gas &GR, double u_L, double u_R, double p_L,
-                                            double p_R, double c_L, double c_R, double &p_INT, double &v_L,
-                                            double &v_R, double &k1, double &k3, DD &bL, DD &bR)
+// The update of one Newton step of Riemann_solver_exact
+// (riemann_solver_exact_Ben.c:133-203): tangents k1, k3 at p, the new p
+// (halved when negative).  The wave curves at the new p follow in the caller.
+__device__ __forceinline__ void newton_update(const Gas &GL, const Gas &GR, double p_L, double p_R, double c_L,
+                                              double c_R, double &p_INT, double v_L, double v_R, double &k1,
+                                              double &k3, const DD &bL, const DD &bR)
 {
     const double gL = GL.g, gR = GR.g;
     double t1, t2, t3, dp;
@@ -198,8 +198,6 @@ __device__ __forceinline__ void newton_step(const Gas &GL, const Gas &GR, double
     }
     const double p_new = p_INT + (v_R - v_L) / (k1 - k3);
     p_INT = (p_new < 0.0) ? 0.5 * p_INT : p_new;
-    v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
-    v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
 }
 
 // GuessP (src/riemann_solver/riemann_solver_starPU.c:20-58), the reference's
@@ -256,38 +254,41 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
         r.iters = -1;
         return r;
     }
-    int done = 0;
-    if (p_guess > 0.0) {
-        double p = p_guess;
-        double v_L = wave_curve<-1>(p, u_L, p_L, c_L, GL, &bL);
-        double v_R = wave_curve<+1>(p, u_R, p_R, c_R, GR, &bR);
-        double gap = fabs(v_L - v_R);
-        while ((gap > tol) && (done != GUESS_ITERS)) {
-            newton_step(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, p, v_L, v_R, k1, k3, bL, bR);
-            ++done;
-            gap = fabs(v_L - v_R);
+    // One loop for both phases, so the Newton code is laid out once: st 0
+    // evaluates the curves at the learned guess, 1 steps from it, 2 evaluates
+    // them at the reference's guess p_INT (the restart, tangents reset), 3 steps
+    // from it exactly as the reference's loop does.
+    const bool guessed = p_guess > 0.0;
+    int st = guessed ? 0 : 2, n = 0, done = 0;
+    double p = 0.0, v_L = 0.0, v_R = 0.0, gap = 0.0;
+    for (;;) {
+        if (st == 1 || st == 3) {
+            newton_update(GL, GR, p_L, p_R, c_L, c_R, p, v_L, v_R, k1, k3, bL, bR);
+        } else {
+            p = st == 0 ? p_guess : p_INT;
+            k1 = k1_0;
+            k3 = k3_0;
         }
-        if (gap <= tol) {
-            r.p_star = p;
-            r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
-            r.gap = gap;
-            r.iters = done;
-            return r;
-        }
-        k1 = k1_0;
-        k3 = k3_0;
-    }
-    double v_L = wave_curve<-1>(p_INT, u_L, p_L, c_L, GL, &bL);
-    double v_R = wave_curve<+1>(p_INT, u_R, p_R, c_R, GR, &bR);
-    double gap = fabs(v_L - v_R);
-
-    int n = 0;
-    while ((gap > tol) && (n != N)) {
-        newton_step(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, p_INT, v_L, v_R, k1, k3, bL, bR);
-        ++n;
+        v_L = wave_curve<-1>(p, u_L, p_L, c_L, GL, &bL);
+        v_R = wave_curve<+1>(p, u_R, p_R, c_R, GR, &bR);
         gap = fabs(v_L - v_R);
+        if (st == 1) ++done;
+        if (st == 3) ++n;
+        if (st <= 1) {
+            if (gap <= tol) {
+                r.p_star = p;
+                r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
+                r.gap = gap;
+                r.iters = done;
+                return r;
+            }
+            st = (gap > tol && done != GUESS_ITERS) ? 1 : 2;
+        } else {
+            if (!((gap > tol) && (n != N))) break;
+            st = 3;
+        }
     }
-    r.p_star = p_INT;
+    r.p_star = p;
     r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
     r.gap = gap;
     r.iters = n + done;

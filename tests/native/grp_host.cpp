@@ -113,3 +113,15 @@ extern "C" void hg_guessp_many(const double *in, int n, double *out)
         out[k] = guess_pvrs(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], a[10]);
     }
 }
+
+// riemann_exact from a given starting pressure (guess[k] <= 0: the reference's guess)
+extern "C" void hg_riemann_guess_many(const double *in, int n, int N, const double *guess, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *a = in + 10 * k;
+        const StarResult r = riemann_exact(make_gas(a[0]), make_gas(a[1]), a[2], a[3], a[4], a[5], a[6], a[7], a[8],
+                                           a[9], N, guess[k]);
+        double *o = out + 6 * k;
+        o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
+    }
+}

@@ -148,3 +148,25 @@ def test_guessp_against_reference(dev, golden_units):
     out = np.zeros(len(gi))
     dev.hg_guessp_many(_dp(gi), len(gi), _dp(out))
     assert np.array_equal(out, golden_units["guessp"])
+
+
+def test_riemann_starting_guess_paths(dev, golden_units):
+    """The exact solve from a supplied starting pressure: from p* itself it
+    stops at once with p*; from a guess 1e8 times too high it restarts after
+    three Newton steps and then ends bit for bit where the reference's path
+    ends (three more iterations counted)."""
+    rin = np.ascontiguousarray(golden_units["riemann_in"])
+    n = len(rin)
+    plain = np.zeros((n, 6))
+    dev.hg_riemann_many(_dp(rin), n, 500, _dp(plain))
+    solved = plain[:, 5] > 0
+    out = np.zeros((n, 6))
+    g = np.ascontiguousarray(np.where(solved, plain[:, 1], -1.0))
+    dev.hg_riemann_guess_many(_dp(rin), n, 500, _dp(g), _dp(out))
+    assert np.array_equal(out[solved, 1], plain[solved, 1]) and np.all(out[solved, 5] == 0)
+    assert np.array_equal(out[~solved], plain[~solved])
+    g = np.ascontiguousarray(np.where(solved, 1e8 * plain[:, 1], -1.0))
+    dev.hg_riemann_guess_many(_dp(rin), n, 500, _dp(g), _dp(out))
+    restarted = solved & (out[:, 5] == plain[:, 5] + 3)
+    assert restarted.sum() >= 0.9 * solved.sum()
+    assert np.array_equal(out[restarted, :5], plain[restarted, :5])
