@@ -90,6 +90,10 @@ __host__ __device__ inline RecipDiv recip_div(double b)
     d.ok = ab >= 0x1p-100 && ab <= 0x1p100;
     return d;
 }
+// the division itself, out of line: div_by is inlined at dozens of sites and
+// this path (zeros, infinities, NaN, extreme quotients) is rarely taken
+__device__ __attribute__((noinline)) static double div_slow(double a, double b) { return a / b; }
+
 __device__ __forceinline__ double div_by(double a, const RecipDiv &d)
 {
     const double q = a * d.y;
@@ -98,7 +102,7 @@ __device__ __forceinline__ double div_by(double a, const RecipDiv &d)
     const double aq = fabs(q);
     if (__builtin_expect(d.ok && aq >= 0x1p-800 && aq <= 0x1p800, 1)) return q2;
     if (a == 0.0 && d.ok) return q;
-    return a / d.b;
+    return div_slow(a, d.b);
 }
 
 __device__ __forceinline__ double minmod2(double s_L, double s_R)
