@@ -99,12 +99,34 @@ def learned_leg(size=4096, steps=200):
                      "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
     a, b = states["learned"], states["plain"]
     rel_l1 = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
+    # the developed flow: the whole run to t = 0.3 at dev^2 (plain, tol 1e-12), then the same
+    # steps_dev steps from its final state with either guess (engine checkpoint between them)
+    dev, steps_dev = size // 2, 60
+    cfg_d = cases.four_quadrant_config(dev, dev)
+    init_d = cases.four_quadrant(dev, dev)
+    s = hydrogrp.Solver2D(dev, dev, cfg_d)
+    s.set_newton(1e-12, None)
+    s.upload(init_d["rho"], init_d["u"], init_d["v"], init_d["p"])
+    _quiet(s.solve, np.array([0.0, cfg_d[1]]), 2)      # the solver prints its end-of-run lines
+    blob = s.checkpoint()
+    developed = {}
+    for name, p in (("plain", None), ("learned", params)):
+        s.restore(blob)
+        s.set_newton(1e-12, p)
+        ms, ms_x, ms_y = s.bench(steps_dev)
+        developed[name] = {"value": dev * dev * steps_dev / (ms / 1000.0), "ms_per_step": ms / steps_dev,
+                           "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
+    s.close()
+    developed["workload"] = (f"2D four-quadrant {dev}x{dev}, Newton tol 1e-12, {steps_dev} steps from the "
+                             "state at t = 0.3 (the whole run done first, plain)")
+    developed["speedup"] = developed["learned"]["value"] / developed["plain"]["value"]
     _, _, st, ok = mlp.data(777, 1 << 16, 1e-12)
     it_p = mlp.riemann_guess_batch(st[ok], tol=1e-12)[:, 4]
     it_l = mlp.riemann_guess_batch(st[ok], tol=1e-12, params=params)[:, 4]
     return {"workload": f"2D four-quadrant {size}x{size}, Newton tol 1e-12, {steps} steps from t=0",
             "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
             "speedup": out["learned"]["value"] / out["plain"]["value"], "rel_l1_difference": rel_l1,
+            "developed_flow": developed,
             "newton_iterations_held_out": {"states": int(ok.sum()), "plain": float(it_p.mean()),
                                            "learned": float(it_l.mean()),
                                            "fallback_share": float(np.mean(it_l > 3))}}
