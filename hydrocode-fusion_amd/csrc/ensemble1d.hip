@@ -139,8 +139,10 @@ __device__ __forceinline__ double hs_min_wave(double v, int slot)
     return r;
 }
 
+// four waves per SIMD: 128 VGPRs and a 4-register spill (unbounded it takes 130, three waves);
+// measured 5 % faster on the 512-instance ensemble
 template <int BS>
-__global__ void __launch_bounds__(BS) iface_kernel(Tables T, int n_if, Inst *inst, Params P, const double *rho,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) iface_kernel(Tables T, int n_if, Inst *inst, Params P, const double *rho,
                                                    const double *u, const double *p, const double *s_rho,
                                                    const double *s_u, const double *s_p, double *mid, double *dire,
                                                    size_t n_if_all)
