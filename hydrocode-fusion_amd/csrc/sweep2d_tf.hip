@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 
 // Y sweep: a wave owns 64 consecutive cells of a y-line (as sweep_y_kernel)
 template <bool LEARNED, int G = 16>
-__global__ void __launch_bounds__(256, 2) sweep_y_tf_kernel(SweepArgs a, int chunks)
+__global__ void __launch_bounds__(256, 3) sweep_y_tf_kernel(SweepArgs a, int chunks)
 {
     // 64/G y-lines of G lanes per wave, as in sweep_y_kernel: lane k of a row
     // owns cell c0-1+k and updates it for k >= 1
