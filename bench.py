@@ -297,8 +297,24 @@ def cpu_leg(target_seconds=12.0):
     t1 = run(1)
     k = max(2, int(target_seconds / max(t1, 1e-3)))
     t = run(k)
-    return {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
-            "sample": f"{k} steps of the same {M}x{N} four-quadrant workload ({t:.1f} s), {label}, {threads} threads"}
+    out = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+           "sample": f"{k} steps of the same {M}x{N} four-quadrant workload ({t:.1f} s), {label}, {threads} threads"}
+    # the same on one thread (OpenMP's thread count set through its runtime, then restored)
+    single = None
+    try:
+        import ctypes
+        gomp = ctypes.CDLL("libgomp.so.1")
+        gomp.omp_set_num_threads(1)
+        try:
+            k1 = max(2, int(0.5 * target_seconds / max(run(1), 1e-3)))
+            t = run(k1)
+        finally:
+            gomp.omp_set_num_threads(threads)
+        single = {"value": M * N * k1 / t, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+                  "sample": f"{k1} steps of the same workload ({t:.1f} s), {label}, 1 thread"}
+    except OSError:
+        pass
+    return out, single
 
 
 def main():
@@ -365,7 +381,9 @@ def main():
         line["unsplit"] = unsplit_leg(with_cpu=not args.no_cpu)
         note("unsplit leg")
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_leg()
+        line["cpu_baseline"], single = cpu_leg()
+        if single:
+            line["cpu_baseline_1thread"] = single
         note("CPU baseline")
     print(json.dumps(line), flush=True)
 
