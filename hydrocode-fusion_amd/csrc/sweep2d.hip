@@ -243,7 +243,6 @@ template <int DIR, int W, bool LEARNED>
 __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][64], double (*xb)[4][64],
                                            double (*xf)[4][64], double (*xi)[4][64], double (*xe)[64])
 {
-    if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int line = blockIdx.x * 64 + lane;
     const bool col = line < (DIR == 0 ? a.n : a.m);
@@ -273,6 +272,9 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][6
         L.value(c + 2, vrr);
         L.prev_slopes(c + 1, sp1);
     }
+    // the predecessor's status is read while the loads above are in flight
+    // (they stay in bounds either way); the whole block returns together
+    if (sweep_halted(a)) return;
     __syncthreads();
     if (w > 0)
 #pragma unroll
@@ -353,7 +355,6 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     constexpr int ROWS = 64 / G;
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
     __shared__ double stash[4][5][64];   // this lane's cell values and E, read back for the update
-    if (sweep_halted(a)) return;
     const int lane = threadIdx.x & 63, k = lane % G;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int jg = (int)(wave / chunks);
@@ -387,6 +388,8 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
         L.value(c + 2, wrr);
         L.prev_slopes(c + 1, sp1);
     }
+    // the predecessor's status is read while the loads above are in flight
+    if (sweep_halted(a)) return;
     // each lane reads back only its own slots: no barrier needed
     double(*st)[64] = stash[(threadIdx.x >> 6) & 3];
 #pragma unroll
