@@ -203,7 +203,6 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
     __shared__ double xz[W][3][64];    // u_n a, u_n at t + tau/2; a at t + tau
     __shared__ double xi[W][4][64];    // interface values at t + tau
     __shared__ double xe[W][64];       // E of the owned row
-    if (tf_halted(a)) return;
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int i = blockIdx.x * 64 + lane;
     const bool col = i < a.n;
@@ -230,6 +229,9 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
         L.value(c + 2, vrr);
         L.prev_slopes(c + 1, sp1);
     }
+    // the predecessor's status is read while the loads above are in flight
+    // (they stay in bounds either way); the whole block returns together
+    if (tf_halted(a)) return;
     __syncthreads();
     if (w > 0)
 #pragma unroll
@@ -290,7 +292,6 @@ __global__ void __launch_bounds__(256, 3) sweep_y_tf_kernel(SweepArgs a, int chu
     constexpr int ROWS = 64 / G;
     __shared__ double stash[4][NQ + 1][64];
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
-    if (tf_halted(a)) return;
     const int lane = threadIdx.x & 63, k = lane % G;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
     const int jg = (int)(wave / chunks);
@@ -318,6 +319,8 @@ __global__ void __launch_bounds__(256, 3) sweep_y_tf_kernel(SweepArgs a, int chu
         L.value(c + 2, wrr);
         L.prev_slopes(c + 1, sp1);
     }
+    // the predecessor's status is read while the loads above are in flight
+    if (tf_halted(a)) return;
     double(*st)[64] = stash[(threadIdx.x >> 6) & 3];
 #pragma unroll
     for (int q = 0; q < NQ; ++q) st[q][lane] = w[q];

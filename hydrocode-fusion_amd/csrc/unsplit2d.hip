@@ -187,13 +187,6 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
     __shared__ double sd[W][12][64];     // side data of row c: value, x-slope, y-slope
     __shared__ double fx[W - 1][8][64];  // x-interface c+1: flux, values at t + tau
     __shared__ double gy[W - 2][8][64];  // y-interfaces of the updated rows: flux, values at t + tau
-    if (a.prev != nullptr) {           // a failed step ends the run: later steps do nothing
-        const SweepStatus p0 = a.prev[0], p1 = a.prev[1];
-        if (p0.fail_key != 0ull || p0.flags != 0u || p1.fail_key != 0ull) {
-            if (threadIdx.x == 0) atomicOr(&a.status[0].flags, 2u);
-            return;
-        }
-    }
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m = a.m, n = a.n;
     const int i0 = blockIdx.x * 62, ci = i0 - 1 + lane;
@@ -205,6 +198,16 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
         if (c <= m && ci <= n && (xin || yin)) G.side(c, ci, v, s, t);   // a cell or a ghost, not a corner
 #pragma unroll
         for (int q = 0; q < 4; ++q) { sd[w][q][lane] = v[q]; sd[w][4 + q][lane] = s[q]; sd[w][8 + q][lane] = t[q]; }
+    }
+    // a failed step ends the run: later steps do nothing.  The status is read
+    // after the side loads went out (they stay in bounds either way); the
+    // whole block returns together.
+    if (a.prev != nullptr) {
+        const SweepStatus p0 = a.prev[0], p1 = a.prev[1];
+        if (p0.fail_key != 0ull || p0.flags != 0u || p1.fail_key != 0ull) {
+            if (threadIdx.x == 0) atomicOr(&a.status[0].flags, 2u);
+            return;
+        }
     }
     __syncthreads();
 
