@@ -163,6 +163,53 @@ def test_four_quadrant_256(hg, golden_2d):
     assert np.allclose(fin.reshape(5, -1).sum(axis=1), g["fq256_sum"], rtol=1e-12, atol=1e-9)
 
 
+def test_full_size_developed_state_vs_oracle(hg, oracle):
+    """At the benchmark's size, 1024 x 1024: the four-quadrant state at t = 0.3
+    (the whole run, done on the device) is the input of 4 more steps on the
+    device and in the oracle, from the same values (E included).  By then most
+    interfaces take the exact Riemann solver and the nonlinear GRP branches."""
+    m = n = 1024
+    fr, npl, tp, c, res = _run(hg, m, n, cases.four_quadrant(m, n), cases.four_quadrant_config(m, n))
+    assert res.status == 0 and npl == 2
+    dev = {k: fr[q, 1].copy() for q, k in enumerate(("rho", "u", "v", "p", "e"))}
+    cfg = cases.four_quadrant_config(m, n, N=4)
+    cfg[1] = np.inf
+    fo, npo, tpo, co, st = oracle.split2d(m, n, dev, cfg)
+    fg, npg, tpg, cg, res = _run(hg, m, n, dev, cfg)
+    assert st == 0 and res.status == 0
+    assert (co[5], npo, tpo[npo - 1]) == (cg[5], npg, tpg[npg - 1])
+    d = rel_linf(fg[:, npg - 1], fo[:, npo - 1])
+    print("developed 1024^2, 4 steps: rel Linf per field", d, flush=True)
+    assert d.max() <= TOL
+
+
+def test_conservation_full_size(hg):
+    """Periodic in both directions: mass, momentum and energy sums are conserved
+    (to rounding) over 24 split steps of a 4096 x 4096 four-quadrant state, the
+    largest size the benchmark quotes."""
+    m = n = 4096
+    cfg = cases.four_quadrant_config(m, n, N=24)
+    cfg[17] = cfg[18] = -7
+    cfg[1] = np.inf
+    init = cases.four_quadrant(m, n)
+    s = hg.Solver2D(m, n, cfg)
+    try:
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        r0, u0, v0, p0, e0 = s.download()
+        res = s.solve(np.array([0.0, np.inf]), 2)
+        r1, u1, v1, p1, e1 = s.download()
+    finally:
+        s.close()
+    assert res.status == 0 and res.k == 25
+    for name, a, b in (("mass", r0, r1), ("x-momentum", r0 * u0, r1 * u1), ("y-momentum", r0 * v0, r1 * v1),
+                       ("energy", r0 * e0, r1 * e1)):
+        s0, s1 = a.sum(), b.sum()
+        scale = np.abs(a).sum()
+        print(f"{name}: relative drift {(s1 - s0) / scale:.2e}", flush=True)
+        assert abs(s1 - s0) <= 1e-11 * scale, name
+    assert np.all(np.isfinite(r1)) and r1.min() > 0 and p1.min() > 0
+
+
 @pytest.mark.parametrize("m,n,bx,by", [(1, 50, -4, -7), (50, 1, -7, -2), (63, 64, -2, -24),
                                        (64, 63, -24, -1), (127, 129, -7, -7), (65, 200, -1, -4)])
 def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by):
