@@ -34,6 +34,7 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 SWEEP_BYTES_PER_CELL = 128.0   # 384 B per cell-update / 3 sweeps
 M = N = 1024
+WHOLE_RUN_STEPS = 2017     # the whole four-quadrant run to t = 0.3 at 1024^2 (frozen tau)
 WORKLOAD = "2D four-quadrant Riemann problem {m}x{n}, dimension-split GRP (X/2 Y X/2), FP64"
 
 
@@ -120,13 +121,33 @@ def learned_leg(size=4096, steps=200):
     developed["workload"] = (f"2D four-quadrant {dev}x{dev}, Newton tol 1e-12, {steps_dev} steps from the "
                              "state at t = 0.3 (the whole run done first, plain)")
     developed["speedup"] = developed["learned"]["value"] / developed["plain"]["value"]
+    # the whole run at the benchmark's 1024^2 with either guess
+    whole, finals = {}, {}
+    cfg_w = cases.four_quadrant_config(M, N)
+    init_w = cases.four_quadrant(M, N)
+    for name, p in (("plain", None), ("learned", params)):
+        s = hydrogrp.Solver2D(M, N, cfg_w)
+        s.set_newton(1e-12, p)
+        s.upload(init_w["rho"], init_w["u"], init_w["v"], init_w["p"])
+        s.bench(3)
+        s.upload(init_w["rho"], init_w["u"], init_w["v"], init_w["p"])
+        ms, ms_x, ms_y = s.bench(WHOLE_RUN_STEPS)
+        finals[name] = s.download()
+        s.close()
+        whole[name] = {"value": M * N * WHOLE_RUN_STEPS / (ms / 1000.0), "ms_per_step": ms / WHOLE_RUN_STEPS,
+                       "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
+    a, b = finals["learned"], finals["plain"]
+    whole["workload"] = (f"2D four-quadrant {M}x{N}, Newton tol 1e-12, the whole run to t = 0.3 "
+                         f"({WHOLE_RUN_STEPS} steps from t=0)")
+    whole["speedup"] = whole["learned"]["value"] / whole["plain"]["value"]
+    whole["rel_l1_difference"] = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
     _, _, st, ok = mlp.data(777, 1 << 16, 1e-12)
     it_p = mlp.riemann_guess_batch(st[ok], tol=1e-12)[:, 4]
     it_l = mlp.riemann_guess_batch(st[ok], tol=1e-12, params=params)[:, 4]
     return {"workload": f"2D four-quadrant {size}x{size}, Newton tol 1e-12, {steps} steps from t=0",
             "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
             "speedup": out["learned"]["value"] / out["plain"]["value"], "rel_l1_difference": rel_l1,
-            "developed_flow": developed,
+            "developed_flow": developed, "whole_run": whole,
             "newton_iterations_held_out": {"states": int(ok.sum()), "plain": float(it_p.mean()),
                                            "learned": float(it_l.mean()),
                                            "fallback_share": float(np.mean(it_l > 3))}}
