@@ -257,7 +257,8 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
             }
         }
         double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
-        if (active) solve_t(a, pass, SL, SR, TL, TR, idx, F, I);
+        // the argument block is read again for the solve, so no scalar register holds it across
+        if (active) solve_t(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I);
         if (pass == 0 && w <= W - 2) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
@@ -380,7 +381,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
                 }
             }
             double Fo[4] = {0.0, 0.0, 0.0, 0.0}, Io[4] = {0.0, 0.0, 0.0, 0.0};
-            if (active) solve_t(a, 1 - pass, SL, SR, TL, TR, idx, Fo, Io);
+            if (active) solve_t(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io);
             if (pass == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { gb[q][lane] = Fo[q]; gb[4 + q][lane] = Io[q]; }
