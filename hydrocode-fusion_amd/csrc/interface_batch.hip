@@ -118,12 +118,22 @@ __global__ void edir_batch_kernel(const double *L, const double *R, int count, d
     for (int q = 0; q < 3; ++q) { o[q] = D[q]; o[3 + q] = U[q]; }
 }
 
-// plain streaming copy: the calibration kernel for HBM byte counters
+// plain streaming copy: the calibration kernel for HBM byte counters.  Each
+// thread has four 16-byte loads in flight before it stores (grid-stride, every
+// index checked against n2), so the copy is limited by HBM, not by latency.
 __global__ void __launch_bounds__(256) copy_probe_kernel(const double2 *__restrict__ src, double2 *__restrict__ dst,
                                                          size_t n2)
 {
-    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += (size_t)gridDim.x * blockDim.x)
-        dst[k] = src[k];
+    const size_t S = (size_t)gridDim.x * blockDim.x;
+    for (size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x; k < n2; k += 4 * S) {
+        double2 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u * S < n2) v[u] = src[k + u * S];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+            if (k + u * S < n2) dst[k + u * S] = v[u];
+    }
 }
 
 __global__ void pow_batch_kernel(const double *x, const double *y, int count, double *out)
