@@ -176,3 +176,18 @@ def test_riemann_from_guessp(hg, golden_units):
     nz = plain[:, 4] > 0
     print(f"Newton iterations over {nz.sum()} solved interfaces: halved tangents {plain[nz, 4].mean():.2f}, "
           f"GuessP {out[nz, 4].mean():.2f}", flush=True)
+
+
+def test_copy_probe(hg):
+    """The calibration copy: sizes that are not a multiple of the grid's
+    stride run, the argument checks reject a zero size, and the rate is that
+    of a streaming kernel (well above 1 TB/s for 256 MiB read + written)."""
+    for nbytes in (16, 4096 + 48, (1 << 20) + 16 * 7):
+        assert hg.copy_probe(nbytes, 1) > 0.0
+    with pytest.raises(RuntimeError):
+        hg.copy_probe(0, 1)
+    nbytes = 256 << 20
+    ms = hg.copy_probe(nbytes, 5)
+    gbs = 2.0 * nbytes / (ms * 1e-3) / 1e9
+    print(f"copy probe: {ms:.4f} ms per 256 MiB copy, {gbs:.0f} GB/s", flush=True)
+    assert gbs > 1000.0
