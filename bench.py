@@ -166,7 +166,7 @@ def ensemble_leg(n_inst=512, seed=2024):
     cfg = make_config(t_all=0.15, bc_x=-4, CFL=0.45)
     ens = hydrogrp.Ensemble1D(cells, h, cfg)
     ens.upload(rho, u, p)
-    bs = ens.tune(4)
+    bs = ens.tune(32)
     ms, cu = ens.run()
     steps, status, _ = ens.results()
     ens.close()
