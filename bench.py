@@ -1,29 +1,39 @@
-"""Benchmark: 2D dimension-split Eulerian GRP, FP64, cell-updates per second.
+"""Benchmark: 2D dimension-split Eulerian GRP, FP64, cell-updates per second,
+on the learned-guess configuration: the four-quadrant Riemann problem
+(data_in/two-dim/RP2D_Positive/Config3 states) at 4096 x 4096, gamma 1.4, CFL
+0.45, free boundaries, Newton tolerance 1e-12 with the converge rule, started
+from the learned p* guess; beside it the same with the reference's starting
+guess ("plain").  One step is one full X(tau/2) Y(tau) X(tau/2) step over the
+whole grid with the frozen tau = CFL * h / S_max of the reference; a
+cell-update is one cell advanced by one step.  The state is resident in HBM
+before the timed region; device time comes from HIP events on the solver's
+stream (hgrp2d_bench).
 
-Workload: the 2D four-quadrant Riemann problem (data_in/two-dim/RP2D_Positive/Config3
-states) at 1024 x 1024, gamma 1.4, CFL 0.45, free boundaries.  One step is one
-full X(tau/2) Y(tau) X(tau/2) step over the whole grid with the frozen
-tau = CFL * h / S_max of the reference; a cell-update is one cell advanced by
-one step.  The state is resident in HBM before the timed region; device time
-comes from HIP events on the solver's stream (hgrp2d_bench).
+Window: with --steps K, K steps from the state at t = 0.15 (4034 steps of the
+reference's own solve, untimed, then checkpointed), each form restarting from
+that checkpoint; without --steps, the whole run, 8068 steps from t = 0 to
+t = 0.3.  Before timing, the sweep autotuner (hgrp2d_tune) picks each form's
+launch variants on the t = 0.15 state.
 
-Prints one JSON line (rank 0).  `roofline` reports the dominant kernel (the
-x-sweep, two launches per step) against HBM peak with the algorithmic bytes of
-a sweep (128 B per cell: read and write rho u v p and their four slopes);
+Prints one JSON line.  `roofline` reports the dominant kernel (the learned
+form's X sweep, two launches per step) against HBM peak with the algorithmic
+bytes of a sweep (128 B per cell: read and write rho u v p and their four
+slopes), its per-launch time from events around every launch of a second run
+over the same window (the run that gives `value` carries no such events);
+`traffic` is read from the committed rocprofv3 counter summary of that window.
 `cpu_baseline` times the reference's own split solver (its sources built in
 oracle/_ref, OpenMP; the C99 restatement in oracle/ where that build is absent)
-on a bounded number of steps of the same workload on the host's cores.
-`learned_guess` compares, at 4096^2 with Newton tolerance 1e-12, the plain
-solve with the one started from the learned p* guess (rate, and the relative L1
-difference of the two final states).  `ensemble_1d` runs 512 independent 1D
-GRP problems of uneven sizes through the batched engine; `two_fluid` runs the
-shock-bubble interaction (4096 x 1024) with and without the learned guess;
-`unsplit` runs the same four-quadrant workload through the unsplit scheme
-(config[33] = 0) with its own roofline and CPU baseline.
+on a few steps of the same 4096^2 workload from the same t = 0.15 state.
+`ensemble_1d` runs 512 independent 1D GRP problems of uneven sizes through the
+batched engine; `two_fluid` runs the shock-bubble interaction (4096 x 1024)
+with and without the learned guess; `unsplit` runs the four-quadrant workload
+at 1024^2 through the unsplit scheme (config[33] = 0) with its own roofline and
+CPU baseline.
 """
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -33,18 +43,34 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 SWEEP_BYTES_PER_CELL = 128.0   # 384 B per cell-update / 3 sweeps
-M = N = 1024
-WHOLE_RUN_STEPS = 2017     # the whole four-quadrant run to t = 0.3 at 1024^2 (frozen tau)
-WORKLOAD = "2D four-quadrant Riemann problem {m}x{n}, dimension-split GRP (X/2 Y X/2), FP64"
+M = N = 4096
+T_MID = 0.15                   # start of the --steps window
+WHOLE_RUN_STEPS = 8068         # the whole four-quadrant run to t = 0.3 at 4096^2 (frozen tau)
+NEWTON_TOL = 1e-12
+WORKLOAD = ("2D four-quadrant Riemann problem {m}x{n}, dimension-split GRP (X/2 Y X/2), FP64, learned p* guess, "
+            "Newton tol 1e-12 with the converge rule")
 
 
-def measured_traffic(kernel, m, n):
+def cpu_model():
+    """The host CPU's model name and thread count from lscpu."""
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+    except (OSError, subprocess.SubprocessError):
+        return None
+    info = {}
+    for line in out.splitlines():
+        if ":" in line:
+            k, v = line.split(":", 1)
+            info[k.strip()] = v.strip()
+    return {k: info.get(k) for k in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)")}
+
+
+def measured_traffic(kernel, workload_tag):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
     WRITE_SIZE summary of this workload (profiles/*_hbm_traffic.md, a line
-    `traffic_bytes: {...}`), or None where no such profile exists for this
-    kernel and size."""
-    path = os.path.join(ROOT, "profiles", f"sweep_kernels_fourquadrant{m}_hbm_traffic.md")
-    if m != n or not os.path.exists(path):
+    `traffic_bytes: {...}`), or None where no such profile exists."""
+    path = os.path.join(ROOT, "profiles", f"sweep_kernels_{workload_tag}_hbm_traffic.md")
+    if not os.path.exists(path):
         return None
     for line in open(path):
         if line.startswith("traffic_bytes:"):
@@ -52,105 +78,70 @@ def measured_traffic(kernel, m, n):
     return None
 
 
-def gpu_leg(steps, warmup):
-    import numpy as np
-    import hydrogrp
-    import cases
-    cfg = cases.four_quadrant_config(M, N)
-    s = hydrogrp.Solver2D(M, N, cfg)
-    init = cases.four_quadrant(M, N)
-    s.upload(init["rho"], init["u"], init["v"], init["p"])
-    if warmup:
-        s.bench(warmup)
-        s.upload(init["rho"], init["u"], init["v"], init["p"])    # timed run starts from t = 0
-    ms_total, ms_x, ms_y = s.bench(steps)
-    kernels = (s.sweep_kernel(0), s.sweep_kernel(1))
-    state = s.download()
-    s.close()
-    assert np.all(np.isfinite(state)), "non-finite state after the benchmark"
-    # a plain copy of 1 GiB: the measured HBM ceiling for streaming kernels
-    nbytes = 1 << 30
-    copy_gbs = 2.0 * nbytes / (hydrogrp.copy_probe(nbytes, 10) / 1000.0) / 1e9
-    return ms_total, ms_x, ms_y, kernels, hydrogrp.device_name(), copy_gbs
-
-
-def learned_leg(size=4096, steps=200):
-    """The learned-guess workload: the same four-quadrant problem at size^2 with
-    Newton tolerance 1e-12, plain (the reference's starting guess) against the
-    learned guess, from t = 0 for `steps` steps each; plus Newton iteration
-    counts of both on held-out states of the training distribution."""
+def c3_leg(steps, warmup):
+    """Both forms (learned, plain) on the 4096^2 workload: the window of
+    `steps` steps from the t = 0.15 checkpoint, or the whole run (steps None)."""
     import numpy as np
     import hydrogrp
     from hydrogrp import mlp
     import cases
-    params = mlp.load()
-    cfg = cases.four_quadrant_config(size, size)
-    init = cases.four_quadrant(size, size)
-    out, states = {}, {}
-    for name, p in (("plain", None), ("learned", params)):
-        s = hydrogrp.Solver2D(size, size, cfg)
-        s.set_newton(1e-12, p)
+    cfg = cases.four_quadrant_config(M, N)
+    init = cases.four_quadrant(M, N)
+    s = hydrogrp.Solver2D(M, N, cfg)
+    try:
         s.upload(init["rho"], init["u"], init["v"], init["p"])
-        s.bench(3)
-        s.upload(init["rho"], init["u"], init["v"], init["p"])
-        ms, ms_x, ms_y = s.bench(steps)
-        states[name] = s.download()
+        tau = cfg[7] * s.h_over_smax()
+        k_mid = int(np.floor(T_MID / tau))
+        s.bench(k_mid)                                   # the reference's solve to t = 0.15, untimed
+        blob = s.checkpoint()
+        mid_state = s.download()
+        out, finals = {}, {}
+        for name, params in (("learned", mlp.load()), ("plain", None)):
+            s.set_newton(NEWTON_TOL, params, converge=True)
+            s.restore(blob)
+            xv, yv, _, _ = s.tune(4)
+            if steps:
+                s.restore(blob)
+                if warmup:
+                    s.bench(warmup)
+                s.restore(blob)
+                ms, _, _ = s.bench(steps)
+                finals[name] = s.download()
+                s.restore(blob)
+                _, ms_x, ms_y = s.bench(steps, 1)
+                n_steps = steps
+            else:
+                s.upload(init["rho"], init["u"], init["v"], init["p"])
+                if warmup:
+                    s.bench(warmup)
+                    s.upload(init["rho"], init["u"], init["v"], init["p"])
+                ms, ms_x, ms_y = s.bench(WHOLE_RUN_STEPS, 8)
+                finals[name] = s.download()
+                n_steps = WHOLE_RUN_STEPS
+            out[name] = {"value": M * N * n_steps / (ms / 1000.0), "ms_per_step": ms / n_steps,
+                         "sweeps": {"x": s.sweep_kernel(0), "y": s.sweep_kernel(1), "variants": [xv, yv]},
+                         "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
+    finally:
         s.close()
-        out[name] = {"value": size * size * steps / (ms / 1000.0), "ms_per_step": ms / steps,
-                     "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
-    a, b = states["learned"], states["plain"]
-    rel_l1 = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
-    # the developed flow: the whole run to t = 0.3 at dev^2 (plain, tol 1e-12), then the same
-    # steps_dev steps from its final state with either guess (engine checkpoint between them)
-    dev, steps_dev = size // 2, 60
-    cfg_d = cases.four_quadrant_config(dev, dev)
-    init_d = cases.four_quadrant(dev, dev)
-    s = hydrogrp.Solver2D(dev, dev, cfg_d)
-    s.set_newton(1e-12, None)
-    s.upload(init_d["rho"], init_d["u"], init_d["v"], init_d["p"])
-    _quiet(s.solve, np.array([0.0, cfg_d[1]]), 2)      # the solver prints its end-of-run lines
-    blob = s.checkpoint()
-    developed = {}
-    for name, p in (("plain", None), ("learned", params)):
-        s.restore(blob)
-        s.set_newton(1e-12, p)
-        ms, ms_x, ms_y = s.bench(steps_dev)
-        developed[name] = {"value": dev * dev * steps_dev / (ms / 1000.0), "ms_per_step": ms / steps_dev,
-                           "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
-    s.close()
-    developed["workload"] = (f"2D four-quadrant {dev}x{dev}, Newton tol 1e-12, {steps_dev} steps from the "
-                             "state at t = 0.3 (the whole run done first, plain)")
-    developed["speedup"] = developed["learned"]["value"] / developed["plain"]["value"]
-    # the whole run at the benchmark's 1024^2 with either guess
-    whole, finals = {}, {}
-    cfg_w = cases.four_quadrant_config(M, N)
-    init_w = cases.four_quadrant(M, N)
-    for name, p in (("plain", None), ("learned", params)):
-        s = hydrogrp.Solver2D(M, N, cfg_w)
-        s.set_newton(1e-12, p)
-        s.upload(init_w["rho"], init_w["u"], init_w["v"], init_w["p"])
-        s.bench(3)
-        s.upload(init_w["rho"], init_w["u"], init_w["v"], init_w["p"])
-        ms, ms_x, ms_y = s.bench(WHOLE_RUN_STEPS)
-        finals[name] = s.download()
-        s.close()
-        whole[name] = {"value": M * N * WHOLE_RUN_STEPS / (ms / 1000.0), "ms_per_step": ms / WHOLE_RUN_STEPS,
-                       "ms_per_launch": {"sweep_x": ms_x, "sweep_y": ms_y}}
     a, b = finals["learned"], finals["plain"]
-    whole["workload"] = (f"2D four-quadrant {M}x{N}, Newton tol 1e-12, the whole run to t = 0.3 "
-                         f"({WHOLE_RUN_STEPS} steps from t=0)")
-    whole["speedup"] = whole["learned"]["value"] / whole["plain"]["value"]
-    whole["rel_l1_difference"] = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
-    _, _, st, ok = mlp.data(777, 1 << 16, 1e-12)
-    it_p = mlp.riemann_guess_batch(st[ok], tol=1e-12)[:, 4]
-    it_l = mlp.riemann_guess_batch(st[ok], tol=1e-12, params=params)[:, 4]
-    return {"workload": f"2D four-quadrant {size}x{size}, Newton tol 1e-12, {steps} steps from t=0",
-            "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
-            "speedup": out["learned"]["value"] / out["plain"]["value"], "rel_l1_difference": rel_l1,
-            "developed_flow": developed, "whole_run": whole,
-            "newton_iterations_held_out": {"states": int(ok.sum()), "plain": float(it_p.mean()),
-                                           "learned": float(it_l.mean()),
-                                           "fallback_share": float(np.mean(it_l > 3))}}
+    assert np.all(np.isfinite(a)) and np.all(np.isfinite(b)), "non-finite state after the benchmark"
+    rel_l1 = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
+    window = (f"{steps} steps from t = {k_mid * tau:.4f} ({k_mid} steps of the reference's solve, untimed)"
+              if steps else f"the whole run to t = 0.3, {WHOLE_RUN_STEPS} steps from t = 0")
+    return out, rel_l1, window, (k_mid, mid_state, tau)
+
+
+def newton_iterations():
+    """Newton iterations of both guesses under the converge rule on held-out
+    states of the training distribution, and the largest p* difference."""
+    import numpy as np
+    from hydrogrp import mlp
+    _, _, st, ok = mlp.data(777, 1 << 16, NEWTON_TOL)
+    plain = mlp.riemann_guess_batch(st[ok], tol=NEWTON_TOL, converge=True)
+    learned = mlp.riemann_guess_batch(st[ok], tol=NEWTON_TOL, params=mlp.load(), converge=True)
+    dp = np.abs(learned[:, 1] - plain[:, 1]) / plain[:, 1]
+    return {"states": int(ok.sum()), "plain": float(plain[:, 4].mean()), "learned": float(learned[:, 4].mean()),
+            "p_star_max_rel_difference": float(dp.max())}
 
 
 def ensemble_leg(n_inst=512, seed=2024):
@@ -192,7 +183,7 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
     out = {}
     for name, p in (("plain", None), ("learned", mlp.load())):
         s = hydrogrp.Solver2D(m, n, cfg)
-        s.set_newton(1e-12, p)
+        s.set_newton(1e-12, p, converge=True)
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         s.set_fluids(1.67, frac)
         s.bench(2)
@@ -212,39 +203,40 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
 UNSPLIT_BYTES_PER_CELL = 208.0   # read and write rho u v p E and both slope sets (26 doubles)
 
 
-def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
-    """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0) on the same
-    four-quadrant workload: `steps` steps of one fused launch each with
-    tau = CFL * h / S_max from t = 0 (2017 steps = the whole run to t = 0.3),
-    device time from HIP events; roofline of the step kernel against HBM peak
-    with 208 B per cell (rho u v p E and the x- and y-slopes, read and
-    written); and the oracle's unsplit loop on the host cores on a bounded
-    number of steps."""
+def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
+    """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0) on the
+    four-quadrant workload at size^2: `steps` steps of one fused launch each
+    with tau = CFL * h / S_max from t = 0 (2017 steps = the whole run to t = 0.3
+    at 1024^2), device time from HIP events; roofline of the step kernel
+    against HBM peak with 208 B per cell (rho u v p E and the x- and y-slopes,
+    read and written); and the reference's unsplit loop on the host cores on a
+    bounded number of steps."""
     import numpy as np
     import hydrogrp
     import cases
     from _libs import make_config
-    cfg = cases.four_quadrant_config(M, N, split=0)
-    init = cases.four_quadrant(M, N)
-    s = hydrogrp.Solver2D(M, N, cfg)
+    m = n = size
+    cfg = cases.four_quadrant_config(m, n, split=0)
+    init = cases.four_quadrant(m, n)
+    s = hydrogrp.Solver2D(m, n, cfg)
     s.upload(init["rho"], init["u"], init["v"], init["p"])
     s.bench_unsplit(5)
     s.upload(init["rho"], init["u"], init["v"], init["p"])
-    ms, ms_step = s.bench_unsplit(steps)
+    ms, ms_step = s.bench_unsplit(steps, 8)
     kernel = s.sweep_kernel(2)
     state = s.download()
     s.close()
     assert np.all(np.isfinite(state)), "non-finite state after the unsplit benchmark"
-    achieved = UNSPLIT_BYTES_PER_CELL * M * N / (ms_step / 1000.0) / 1e9
-    out = {"workload": f"2D four-quadrant Riemann problem {M}x{N}, unsplit GRP (one step per tau), FP64, "
+    achieved = UNSPLIT_BYTES_PER_CELL * m * n / (ms_step / 1000.0) / 1e9
+    out = {"workload": f"2D four-quadrant Riemann problem {m}x{n}, unsplit GRP (one step per tau), FP64, "
                        f"{steps} steps from t=0",
-           "value": M * N * steps / (ms / 1000.0), "unit": "cell-updates/s", "ms_per_step": ms / steps,
+           "value": m * n * steps / (ms / 1000.0), "unit": "cell-updates/s", "ms_per_step": ms / steps,
            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, M, N),
-                        "ms_per_launch": ms_step}}
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": measured_traffic(kernel, f"fourquadrant{m}"), "ms_per_launch": ms_step}}
     if with_cpu:
         run2d, kind, label, threads = _cpu_solver(split=False)
-        h = 1.0 / M
+        h = 1.0 / m
         c = np.sqrt(1.4 * init["p"] / init["rho"])
         tau = 0.45 * np.min(h / (c + np.abs(init["u"]) + np.abs(init["v"])))
 
@@ -252,13 +244,13 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True):
             cc = make_config(N=k, tau=tau, h_x=h, h_y=h, bc_x=-4, bc_y=-4, split=0)
             cc[1] = np.inf
             t0 = time.perf_counter()
-            run2d(M, N, init, cc, time_plot=np.array([0.0, np.inf]))
+            run2d(m, n, init, cc, time_plot=np.array([0.0, np.inf]))
             return time.perf_counter() - t0
 
         k = max(2, int(cpu_seconds / max(run(1), 1e-3)))
         t = run(k)
-        out["cpu_baseline"] = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
-                               "sample": f"{k} unsplit steps of the same {M}x{N} workload ({t:.1f} s), {label}, "
+        out["cpu_baseline"] = {"value": m * n * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
+                               "sample": f"{k} unsplit steps of the same {m}x{n} workload ({t:.1f} s), {label}, "
                                          f"{threads} threads"}
     return out
 
@@ -296,17 +288,16 @@ def _cpu_solver(split):
         return fn, "port", "oracle/liboracle.so (C99 restatement, gcc -O2 -fopenmp)", threads
 
 
-def cpu_leg(target_seconds=12.0):
-    """The reference's CPU solver on the host cores, fixed-tau mode (exactly the
-    frozen tau), bounded sample."""
+def cpu_leg(mid, target_seconds=12.0):
+    """The reference's CPU split solver on the host cores, fixed-tau mode
+    (exactly the frozen tau), from the same t = 0.15 state of the same 4096^2
+    workload, a bounded number of steps; and the same on one thread."""
     import numpy as np
-    import cases
     from _libs import make_config
+    k_mid, state, tau = mid
     run2d, kind, label, threads = _cpu_solver(split=True)
-    init = cases.four_quadrant(M, N)
+    init = {k: state[q] for q, k in enumerate(("rho", "u", "v", "p"))}
     h = 1.0 / M
-    c = np.sqrt(1.4 * init["p"] / init["rho"])
-    tau = 0.45 * np.min(h / (c + np.abs(init["u"]) + np.abs(init["v"])))
 
     def run(k):
         cfg = make_config(N=k, tau=tau, h_x=h, h_y=h, bc_x=-4, bc_y=-4, split=1)
@@ -318,8 +309,11 @@ def cpu_leg(target_seconds=12.0):
     t1 = run(1)
     k = max(2, int(target_seconds / max(t1, 1e-3)))
     t = run(k)
-    out = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind,
-           "sample": f"{k} steps of the same {M}x{N} four-quadrant workload ({t:.1f} s), {label}, {threads} threads"}
+    cpu = cpu_model()
+    where = f"{cpu['Model name']}, {cpu['CPU(s)']} logical CPUs" if cpu and cpu.get("Model name") else "host CPU"
+    out = {"value": M * N * k / t, "unit": "cell-updates/s", "cores": threads, "kind": kind, "cpu": cpu,
+           "sample": f"{k} steps of the same {M}x{N} four-quadrant workload from t = {k_mid * tau:.4f} ({t:.1f} s), "
+                     f"{label}, {threads} threads on {where}"}
     # the same on one thread (OpenMP's thread count set through its runtime, then restored)
     single = None
     try:
@@ -327,12 +321,11 @@ def cpu_leg(target_seconds=12.0):
         gomp = ctypes.CDLL("libgomp.so.1")
         gomp.omp_set_num_threads(1)
         try:
-            k1 = max(2, int(0.5 * target_seconds / max(run(1), 1e-3)))
-            t = run(k1)
+            t = run(1)
         finally:
             gomp.omp_set_num_threads(threads)
-        single = {"value": M * N * k1 / t, "unit": "cell-updates/s", "cores": 1, "kind": kind,
-                  "sample": f"{k1} steps of the same workload ({t:.1f} s), {label}, 1 thread"}
+        single = {"value": M * N / t, "unit": "cell-updates/s", "cores": 1, "kind": kind,
+                  "sample": f"1 step of the same workload ({t:.1f} s), {label}, 1 thread"}
     except OSError:
         pass
     return out, single
@@ -341,57 +334,63 @@ def cpu_leg(target_seconds=12.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    # 2017 steps = the whole run to t = 0.3 at the frozen tau (tau = 0.45 h / 2.9544)
-    ap.add_argument("--steps", type=int, default=2017)
-    ap.add_argument("--warmup", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--size", type=int, default=1024, help="grid cells per side (default: the 1024^2 workload)")
-    ap.add_argument("--no-learned", action="store_true", help="skip the learned-guess comparison at 4096^2")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="time this many steps from the t = 0.15 state (default: the whole run, 8068 steps)")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline legs")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched 1D ensemble")
     ap.add_argument("--no-two-fluid", action="store_true", help="skip the two-fluid shock-bubble")
     ap.add_argument("--no-unsplit", action="store_true", help="skip the unsplit scheme")
     args = ap.parse_args()
-    global M, N
-    M = N = args.size
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
         sys.exit(2)
+    if args.steps is not None and args.steps < 1:
+        ap.error("--steps must be positive")
 
+    import hydrogrp
     t0 = time.perf_counter()
-    ms_total, ms_x, ms_y, kernels, device, copy_gbs = gpu_leg(args.steps, args.warmup)
-    wall = time.perf_counter() - t0
-    value = M * N * args.steps / (ms_total / 1000.0)
+
+    def note(what):
+        print(f"bench.py: {what} done ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
+
+    forms, rel_l1, window, mid = c3_leg(args.steps, args.warmup)
+    note("learned / plain 4096^2 leg")
+    learned, plain = forms["learned"], forms["plain"]
+    steps = args.steps or WHOLE_RUN_STEPS
+    kernel = learned["sweeps"]["x"]
+    ms_x = learned["ms_per_launch"]["sweep_x"]
     achieved = SWEEP_BYTES_PER_CELL * M * N / (ms_x / 1000.0) / 1e9
+    nbytes = 1 << 30
+    copy_gbs = 2.0 * nbytes / (hydrogrp.copy_probe(nbytes, 10) / 1000.0) / 1e9
+    tag = f"fourquadrant{M}_t015" if args.steps else None
     line = {
         "metric": "2D GRP FP64 cell-updates/s",
-        "value": value,
+        "value": learned["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_total / args.steps,
+        "ms_per_step": learned["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (four-quadrant initial data generated in-process)",
-        "config": {"workload": WORKLOAD.format(m=M, n=N), "n_x": M, "n_y": N, "gamma": 1.4, "cfl": 0.45,
-                   "boundary": "free (-4, -4)", "device": device},
-        "roofline": {"bound": "hbm", "kernel": kernels[0], "achieved": achieved, "peak": HBM_PEAK_GBS,
+        "config": {"workload": WORKLOAD.format(m=M, n=N) + "; " + window, "n_x": M, "n_y": N, "gamma": 1.4,
+                   "cfl": 0.45, "boundary": "free (-4, -4)", "newton": "tol 1e-12, converge rule, learned guess",
+                   "device": hydrogrp.device_name()},
+        "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(kernels[0], M, N),
+                     "traffic": measured_traffic(kernel, tag) if tag else None,
                      "copy_kernel_gbs": copy_gbs,
-                     "ms_per_launch": {kernels[0]: ms_x, kernels[1]: ms_y}},
-        "wall_s": wall,
+                     "ms_per_launch": {kernel: ms_x, learned["sweeps"]["y"]: learned["ms_per_launch"]["sweep_y"]}},
+        "learned_vs_plain": {"window": window, "learned": learned, "plain": plain,
+                             "speedup": learned["value"] / plain["value"], "rel_l1_difference": rel_l1,
+                             "newton_iterations_held_out": newton_iterations()},
     }
-    def note(what):
-        print(f"bench.py: {what} done ({time.perf_counter() - t0:.0f} s)", file=sys.stderr, flush=True)
-
-    note("main leg")
-    if not args.no_learned:
-        line["learned_guess"] = learned_leg()
-        note("learned-guess leg")
+    note("held-out Newton iterations")
     if not args.no_ensemble:
         line["ensemble_1d"] = ensemble_leg()
         note("1D ensemble leg")
@@ -402,10 +401,11 @@ def main():
         line["unsplit"] = unsplit_leg(with_cpu=not args.no_cpu)
         note("unsplit leg")
     if not args.no_cpu:
-        line["cpu_baseline"], single = cpu_leg()
+        line["cpu_baseline"], single = cpu_leg(mid)
         if single:
             line["cpu_baseline_1thread"] = single
         note("CPU baseline")
+    line["wall_s"] = time.perf_counter() - t0
     print(json.dumps(line), flush=True)
 
 

@@ -551,9 +551,26 @@ extern "C" int hgrp1d_download(hgrp1d *c, double *rho, double *u, double *p, dou
     return 0;
 }
 
-// Runs every instance to its end.  use_graph 0: plain launches (for checks).
+// Captures STEPS_PER_GRAPH steps over the current active set into a graph
+// (once per set: rebuild_tables drops it).
+static int ensure_graph(hgrp1d *c)
+{
+    if (c->gexec) return 0;
+    ECHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    for (int s = 0; s < STEPS_PER_GRAPH; ++s) enqueue_step(c);
+    ECHECK(hipStreamEndCapture(c->stream, &c->graph));
+    ECHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
+    return 0;
+}
+
+// Runs every instance to its end (or max_steps steps).  use_graph 0: plain
+// launches (for checks).  The first graph is captured before the timed region.
 static int run_all(hgrp1d *c, int use_graph, int max_steps, double *ms_out)
 {
+    if (use_graph && c->n_act > 0 && (max_steps <= 0 || max_steps >= STEPS_PER_GRAPH)) {
+        const int rc = ensure_graph(c);
+        if (rc) return rc;
+    }
     hipEvent_t e0, e1;
     ECHECK(hipEventCreate(&e0));
     ECHECK(hipEventCreate(&e1));
@@ -563,12 +580,8 @@ static int run_all(hgrp1d *c, int use_graph, int max_steps, double *ms_out)
         int chunk = STEPS_PER_GRAPH;
         if (max_steps > 0) chunk = std::min(chunk, max_steps - steps);
         if (use_graph && chunk == STEPS_PER_GRAPH) {
-            if (!c->gexec) {
-                ECHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-                for (int s = 0; s < STEPS_PER_GRAPH; ++s) enqueue_step(c);
-                ECHECK(hipStreamEndCapture(c->stream, &c->graph));
-                ECHECK(hipGraphInstantiate(&c->gexec, c->graph, nullptr, nullptr, 0));
-            }
+            const int rc = ensure_graph(c);
+            if (rc) return rc;
             ECHECK(hipGraphLaunch(c->gexec, c->stream));
         } else {
             for (int s = 0; s < chunk; ++s) enqueue_step(c);
@@ -662,10 +675,21 @@ extern "C" int hgrp1d_results(hgrp1d *c, int *steps, int *status, double *time_e
 
 // Autotuner: times `probe_steps` steps from the uploaded state for every pair
 // of block sizes, twice (the faster of the two counts, so one slow sample does
-// not decide), keeps the fastest pair, and restores the uploaded state.
+// not decide), keeps the fastest pair, and restores the uploaded state.  A
+// multiple of STEPS_PER_GRAPH steps is timed through captured graphs, the way
+// hgrp1d_run launches them.  On an error the block sizes and the instances are
+// put back as they were.
 extern "C" int hgrp1d_tune(hgrp1d *c, int probe_steps, int *bs_iface, int *bs_update)
 {
     if (!c || !c->uploaded || probe_steps < 1) return -1001;
+    const int use_graph = probe_steps % STEPS_PER_GRAPH == 0 ? 1 : 0;
+    const int bi0 = c->bs_if, bu0 = c->bs_up;
+    auto fail = [&](int rc) {
+        c->bs_if = bi0;
+        c->bs_up = bu0;
+        (void)reset_instances(c);
+        return rc;
+    };
     float best = 1e30f;
     int bi = c->bs_if, bu = c->bs_up;
     for (int a : BS_CHOICES)
@@ -675,11 +699,11 @@ extern "C" int hgrp1d_tune(hgrp1d *c, int probe_steps, int *bs_iface, int *bs_up
             double fastest = 1e30;
             for (int trial = 0; trial < 2; ++trial) {
                 int rc = reset_instances(c);
-                if (rc) return rc;
+                if (rc) return fail(rc);
                 double ms = 0.0;
                 enqueue_step(c);          // warm the code
-                rc = run_all(c, 0, probe_steps, &ms);
-                if (rc) return rc;
+                rc = run_all(c, use_graph, probe_steps, &ms);
+                if (rc) return fail(rc);
                 if (ms < fastest) fastest = ms;
             }
             if (fastest < best) { best = (float)fastest; bi = a; bu = b; }

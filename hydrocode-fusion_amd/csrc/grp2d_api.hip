@@ -23,9 +23,10 @@
 #include "unsplit2d.hpp"
 
 namespace hgrp {
-hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st);
-hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st);
-const char *sweep_kernel_name(int dir, int m, int n, bool learned);
+hipError_t launch_sweep_x(const SweepArgs &a, int variant, hipStream_t st);
+hipError_t launch_sweep_y(const SweepArgs &a, int variant, hipStream_t st);
+const char *sweep_kernel_name(int dir, int variant, int nm);
+int sweep_variants(int dir);
 hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st);
 hipError_t launch_sweep_y_tf(const SweepArgs &a, hipStream_t st);
 hipError_t launch_cfl_tf(const double *rho, const double *u, const double *v, const double *p, const double *fa,
@@ -35,7 +36,7 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st);
-const char *unsplit_kernel_name(int m, int n);
+const char *unsplit_kernel_name(int m, int n, int nm);
 int unsplit_max_rows();
 }
 
@@ -78,6 +79,8 @@ struct hgrp2d {
     LoopPos at;                 // inside a frame callback: where the loop goes on after it
     LoopPos resume;             // set by hgrp2d_restore: the next solve starts here
     double newton_tol;          // <= 0: the reference's (eps)
+    int newton_rule;            // HGRP_NEWTON_REFERENCE or HGRP_NEWTON_CONVERGE
+    int xv, yv;                 // launch variants of the X and Y sweeps (hgrp2d_tune)
     MlpPacked *mlp;             // device copy of the learned-guess weights
     bool use_mlp;
     bool two_fluid;             // volume fraction of fluid a carried (hgrp2d_set_fluids)
@@ -234,8 +237,10 @@ extern "C" int hgrp2d_h_over_smax(hgrp2d *c, double *h)
 extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
 {
     if (!c || dir < 0 || dir > 2) return "";
-    if (dir == 2) return unsplit_kernel_name(c->m, c->n);
-    return sweep_kernel_name(dir, c->m, c->n, c->use_mlp);
+    const int nm = c->use_mlp ? NM_LEARNED : c->newton_rule == HGRP_NEWTON_CONVERGE ? NM_CONV : NM_REF;
+    if (dir == 2) return unsplit_kernel_name(c->m, c->n, nm);
+    if (c->two_fluid) return "";
+    return sweep_kernel_name(dir, dir == 0 ? c->xv : c->yv, nm);
 }
 
 // h/S_max of the current state: the reference's CFL loop reads CV[0], which is
@@ -308,11 +313,15 @@ extern "C" int hgrp2d_download_fraction(hgrp2d *c, double *frac_a)
     return 0;
 }
 
-extern "C" int hgrp2d_set_newton(hgrp2d *c, double tol, const float *params)
+extern "C" int hgrp2d_set_newton(hgrp2d *c, double tol, int rule, const float *params)
 {
-    if (!c || !(tol == tol)) return ARG_ERROR - 1;
+    if (!c || !(tol == tol) || (rule != HGRP_NEWTON_REFERENCE && rule != HGRP_NEWTON_CONVERGE)) return ARG_ERROR - 1;
+    // the learned guess runs under the converge rule: under the reference's a
+    // solve can stop on a different iterate than from the reference's guess
+    if (params && rule != HGRP_NEWTON_CONVERGE) return ARG_ERROR - 2;
     HCHECK(hipStreamSynchronize(c->stream));
     c->newton_tol = tol;
+    c->newton_rule = rule;
     c->use_mlp = false;
     if (params) {
         MlpPacked h;
@@ -357,6 +366,7 @@ static void fill_args(hgrp2d *c, SweepArgs &a, int dir, double tau_flux, double 
     a.eps = c->cfg[4];
     a.alpha = c->cfg[41];
     a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
+    a.conv = c->newton_rule == HGRP_NEWTON_CONVERGE;
     if (c->two_fluid) {
         a.fa = c->fa[in]; a.fa_o = c->fa[out];
         a.sa = dir == 0 ? c->sax[sin] : c->say[sin];
@@ -375,7 +385,7 @@ static int enqueue_sweep(hgrp2d *c, int dir, double tau_flux, double nu, int slo
     if (c->two_fluid)
         HCHECK(dir == 0 ? launch_sweep_x_tf(a, c->stream) : launch_sweep_y_tf(a, c->stream));
     else
-        HCHECK(dir == 0 ? launch_sweep_x(a, c->stream) : launch_sweep_y(a, c->stream));
+        HCHECK(dir == 0 ? launch_sweep_x(a, c->xv, c->stream) : launch_sweep_y(a, c->yv, c->stream));
     c->cs = 1 - c->cs;
     if (dir == 0) { c->cx = 1 - c->cx; c->found_x = 1; }
     else          { c->cy = 1 - c->cy; c->found_y = 1; }
@@ -384,10 +394,7 @@ static int enqueue_sweep(hgrp2d *c, int dir, double tau_flux, double nu, int slo
 
 static bool valid_bc(int b) { return b == -1 || b == -2 || b == -4 || b == -7 || b == -24; }
 
-// Steps between per-launch event samples in the benchmark entry points.
-static int bench_sample_every(int steps) { return steps >= 64 ? 8 : 1; }
-
-extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x, double *ms_y)
+extern "C" int hgrp2d_bench(hgrp2d *c, int steps, int sample_every, double *ms_total, double *ms_x, double *ms_y)
 {
     if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
     if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
@@ -395,10 +402,10 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x
     const double half_tau = tau * 0.5, half_nu = half_tau / c->cfg[10], mu = tau / c->cfg[11];
     HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
     // per-sweep events for the average launch duration of each sweep kind, on
-    // every EVERY-th step only (a uniform sample of the run), so that the timed
-    // stream carries few event packets besides the sweeps themselves
-    const int every = bench_sample_every(steps);
-    const int ns = (steps + every - 1) / every;
+    // every EVERY-th step only (a uniform sample of the run); none with
+    // sample_every <= 0, so that the timed stream carries the sweeps alone
+    const int every = sample_every > 0 ? sample_every : steps + 1;
+    const int ns = sample_every > 0 ? (steps + every - 1) / every : 0;
     std::vector<hipEvent_t> evs((size_t)6 * ns);
     for (auto &e : evs) HCHECK(hipEventCreate(&e));
     HCHECK(hipEventRecord(c->ev[0], c->stream));
@@ -429,9 +436,105 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, double *ms_total, double *ms_x
             HCHECK(hipEventElapsedTime(&t, evs[(size_t)6 * k + 2 * w], evs[(size_t)6 * k + 2 * w + 1]));
             if (w == 1) sy += t; else sx += t;
         }
-    if (ms_x) *ms_x = sx / (2.0 * ns);
-    if (ms_y) *ms_y = sy / ns;
+    if (ms_x) *ms_x = ns ? sx / (2.0 * ns) : NAN;
+    if (ms_y) *ms_y = ns ? sy / ns : NAN;
     for (auto &e : evs) (void)hipEventDestroy(e);
+    return 0;
+}
+
+extern "C" int hgrp2d_set_sweeps(hgrp2d *c, int xv, int yv)
+{
+    if (!c || xv < 0 || xv >= sweep_variants(0) || yv < 0 || yv >= sweep_variants(1)) return ARG_ERROR - 1;
+    c->xv = xv;
+    c->yv = yv;
+    return 0;
+}
+
+// The sweep autotuner.  Times every launch variant of the X sweep, then of the
+// Y sweep (with the X sweep's best), over `steps` steps from the context's
+// current state, two trials each, and keeps the fastest (per-launch device
+// time from HIP events around each launch of the kind being tuned).  The
+// state, slopes and limiter switches are copied aside on the device first and
+// put back after every trial and at the end, so tuning changes no result.
+extern "C" int hgrp2d_tune(hgrp2d *c, int steps, int *xv_out, int *yv_out, double *ms_out)
+{
+    if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
+    if (c->two_fluid) return ARG_ERROR - 4;      // the two-fluid sweeps have one launch form
+    if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
+    const size_t cells = (size_t)c->m * c->ld;
+    const size_t nb = (size_t)(2 * 5 + 2 * 4 + 2 * 4) * cells;     // state and slope planes
+    double *save = nullptr;
+    HCHECK(hipMalloc((void **)&save, nb * sizeof(double)));
+    if (hipMemcpyAsync(save, c->block, nb * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+        (void)hipFree(save);
+        return ARG_ERROR - 5;
+    }
+    const int cs = c->cs, cx = c->cx, cy = c->cy, fx = c->found_x, fy = c->found_y;
+    const int xv0 = c->xv, yv0 = c->yv;
+    int rc = 0;
+    auto put_back = [&]() -> int {
+        HCHECK(hipMemcpyAsync(c->block, save, nb * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        c->cs = cs; c->cx = cx; c->cy = cy; c->found_x = fx; c->found_y = fy;
+        return 0;
+    };
+    const double tau = c->cfg[7] * c->h_S_max;
+    const double half_tau = tau * 0.5, half_nu = half_tau / c->cfg[10], mu = tau / c->cfg[11];
+    std::vector<hipEvent_t> evs((size_t)4 * steps);
+    for (auto &e : evs)
+        if (hipEventCreate(&e) != hipSuccess) { (void)hipFree(save); return ARG_ERROR - 5; }
+    // average device time of one launch of the kind `dir` over `steps` steps
+    auto trial = [&](int dir, double *ms) -> int {
+        if (int r = put_back()) return r;
+        HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
+        int slot = 0, ne = 0;
+        for (int s = 0; s < steps; ++s) {
+            const int dirs[3] = {0, 1, 0};
+            for (int w = 0; w < 3; ++w) {
+                const int d = dirs[w];
+                if (d == dir) HCHECK(hipEventRecord(evs[(size_t)ne++], c->stream));
+                if (int r = enqueue_sweep(c, d, d == 0 ? half_tau : tau, d == 0 ? half_nu : mu, slot++ % STATUS_SLOTS, -1))
+                    return r;
+                if (d == dir) HCHECK(hipEventRecord(evs[(size_t)ne++], c->stream));
+            }
+        }
+        HCHECK(hipEventSynchronize(evs[(size_t)ne - 1]));
+        double sum = 0.0;
+        for (int k = 0; k < ne; k += 2) {
+            float t = 0.f;
+            HCHECK(hipEventElapsedTime(&t, evs[(size_t)k], evs[(size_t)k + 1]));
+            sum += t;
+        }
+        *ms = sum / (ne / 2);
+        return 0;
+    };
+    double best[2] = {0.0, 0.0};
+    for (int dir = 0; dir < 2 && !rc; ++dir) {
+        int pick = 0;
+        for (int v = 0; v < sweep_variants(dir) && !rc; ++v) {
+            if (dir == 0) c->xv = v; else c->yv = v;
+            double t = 1e300;
+            for (int k = 0; k < 2 && !rc; ++k) {
+                double ms = 0.0;
+                rc = trial(dir, &ms);
+                if (!rc && ms < t) t = ms;
+            }
+            if (!rc && (v == 0 || t < best[dir])) { best[dir] = t; pick = v; }
+        }
+        if (dir == 0) c->xv = pick; else c->yv = pick;
+    }
+    if (rc) { c->xv = xv0; c->yv = yv0; }
+    const int rb = put_back();
+    if (!rc) rc = rb;
+    if (!rc) {
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = -(int)e - 1;
+    }
+    for (auto &e : evs) (void)hipEventDestroy(e);
+    (void)hipFree(save);
+    if (rc) return rc;
+    if (xv_out) *xv_out = c->xv;
+    if (yv_out) *yv_out = c->yv;
+    if (ms_out) { ms_out[0] = best[0]; ms_out[1] = best[1]; }
     return 0;
 }
 
@@ -735,6 +838,7 @@ static void fill_unsplit_args(hgrp2d *c, UnsplitArgs &a, double tau, double nu, 
     a.eps = c->cfg[4];
     a.alpha = c->cfg[41];
     a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
+    a.conv = c->newton_rule == HGRP_NEWTON_CONVERGE;
     a.gas = make_gas(c->cfg[6]);
     a.status = c->status + 2 * slot;
     a.prev = prev_slot >= 0 ? c->status + 2 * prev_slot : nullptr;
@@ -754,6 +858,10 @@ static int enqueue_unsplit(hgrp2d *c, double tau, double nu, double mu, int slot
 
 static bool unsplit_supported(hgrp2d *c)
 {
+    if (c->use_mlp) {
+        printf("The unsplit scheme (config[33] = 0) has no learned starting guess.\n");
+        return false;
+    }
     if (c->two_fluid) {
         printf("The unsplit scheme (config[33] = 0) runs one gamma-law gas only.\n");
         return false;
@@ -912,7 +1020,7 @@ static int solve_unsplit(hgrp2d *c, const double *time_plot, int N_plot, int N_T
     return 0;
 }
 
-extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, double *ms_total, double *ms_step)
+extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, int sample_every, double *ms_total, double *ms_step)
 {
     if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
     if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
@@ -920,8 +1028,8 @@ extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, double *ms_total, doub
     const double tau = c->cfg[7] * c->h_S_max;
     const double nu = tau / c->cfg[10], mu = tau / c->cfg[11];
     HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
-    const int every = bench_sample_every(steps);
-    const int ns = (steps + every - 1) / every;
+    const int every = sample_every > 0 ? sample_every : steps + 1;
+    const int ns = sample_every > 0 ? (steps + every - 1) / every : 0;
     std::vector<hipEvent_t> evs((size_t)2 * ns);
     for (auto &e : evs) HCHECK(hipEventCreate(&e));
     HCHECK(hipEventRecord(c->ev[0], c->stream));
@@ -944,7 +1052,7 @@ extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, double *ms_total, doub
         HCHECK(hipEventElapsedTime(&t, evs[(size_t)2 * k], evs[(size_t)2 * k + 1]));
         sum += t;
     }
-    if (ms_step) *ms_step = sum / ns;
+    if (ms_step) *ms_step = ns ? sum / ns : NAN;
     for (auto &e : evs) (void)hipEventDestroy(e);
     return 0;
 }

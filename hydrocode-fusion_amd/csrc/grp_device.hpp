@@ -162,6 +162,19 @@ __device__ __forceinline__ double pow_from_log(double x, DD L, double y)
 // from the reference's guess.
 constexpr int GUESS_ITERS = 3;
 
+// The converge rule (not in the reference; its loop stops at the first iterate
+// whose velocity gap is within tol).  An absolute gap of tol leaves p up to
+// tol / (p |dv/dp|) relative off the root, which near vacuum (p |dv/dp| down
+// to ~1e-3) is 1e-10 at tol = 1e-12: where the stop falls then depends on the
+// starting guess.  Under this rule, once the gap is within tol, one more Newton
+// step is taken if none has been taken yet, or if the error that the last
+// tangents give, gap / |k1 - k3|, exceeds CONV_REL * p; an iterate reached by
+// that extra step ends the loop if it meets tol (so the count stays bounded
+// where p* is ill-conditioned), else the ordinary steps go on.  Quadratic
+// convergence takes an error of e to ~e^2, so either starting guess ends at
+// the root to within rounding.
+constexpr double CONV_REL = 1e-13;
+
 struct StarResult {
     double u_star, p_star;
     double gap;    // the solver's return value: |v_L - v_R| at the end (|u_L - u_R| on the early exit)
@@ -230,7 +243,8 @@ __device__ __forceinline__ double guess_pvrs(double DL, double DR, double UL, do
 // p_guess > 0 (not in the reference): first up to GUESS_ITERS Newton steps
 // from p_guess; if they meet tol that is the answer, otherwise the solve
 // restarts from the reference's guess exactly as the reference does.  iters
-// counts both phases.
+// counts both phases.  CONV: the converge rule (CONV_REL) in both phases.
+template <bool CONV = false>
 __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR, double u_L, double u_R,
                                                     double p_L, double p_R, double c_L, double c_R,
                                                     double eps, double tol, int N, double p_guess = -1.0)
@@ -261,34 +275,46 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
     // One loop for both phases, so the Newton code is laid out once: st 0
     // evaluates the curves at the learned guess, 1 steps from it, 2 evaluates
     // them at the reference's guess p_INT (the restart, tangents reset), 3 steps
-    // from it exactly as the reference's loop does.
+    // from it exactly as the reference's loop does; 4 and 5 are the converge
+    // rule's extra step in the learned phase and in the reference's phase.
     const bool guessed = p_guess > 0.0;
     int st = guessed ? 0 : 2, n = 0, done = 0;
     double p = 0.0, v_L = 0.0, v_R = 0.0, gap = 0.0;
     for (;;) {
-        if (st == 1 || st == 3) {
-            newton_update(GL, GR, p_L, p_R, c_L, c_R, p, v_L, v_R, k1, k3, bL, bR);
-        } else {
+        if (st == 0 || st == 2) {
             p = st == 0 ? p_guess : p_INT;
             k1 = k1_0;
             k3 = k3_0;
+        } else {
+            newton_update(GL, GR, p_L, p_R, c_L, c_R, p, v_L, v_R, k1, k3, bL, bR);
         }
         v_L = wave_curve<-1>(p, u_L, p_L, c_L, GL, &bL);
         v_R = wave_curve<+1>(p, u_R, p_R, c_R, GR, &bR);
         gap = fabs(v_L - v_R);
-        if (st == 1) ++done;
-        if (st == 3) ++n;
-        if (st <= 1) {
-            if (gap <= tol) {
+        const bool learned_phase = st <= 1 || st == 4;
+        if (st == 1 || st == 4) ++done;
+        if (st == 3 || st == 5) ++n;
+        if (gap <= tol) {
+            // converge rule: an iterate that meets tol, reached from a starting value or by an
+            // ordinary step, gets one more step if none was taken or if the error is above CONV_REL
+            if (CONV && st < 4 && (learned_phase || n != N) &&
+                (st == 0 || st == 2 || gap > CONV_REL * p * fabs(k1 - k3))) {
+                st = learned_phase ? 4 : 5;
+                continue;
+            }
+            if (learned_phase) {
                 r.p_star = p;
                 r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
                 r.gap = gap;
                 r.iters = done;
                 return r;
             }
-            st = (gap > tol && done != GUESS_ITERS) ? 1 : 2;
+            break;
+        }
+        if (learned_phase) {
+            st = done < GUESS_ITERS ? 1 : 2;
         } else {
-            if (!((gap > tol) && (n != N))) break;
+            if (n == N) break;
             st = 3;
         }
     }
@@ -323,13 +349,14 @@ __device__ __forceinline__ void side_trivial(double D[4], double U[4], double rh
 // The rest of linear_GRP_solver_Edir_Q1D's interface, for callers that use it
 // as a function on its own (the solver's C-ABI entry points): atc and the
 // grid velocity (lambda_u, lambda_v), which the flux path fixes at eps and 0;
-// wave_speed[2] and U_star[6] out.  The sweeps pass none.
+// wave_speed[2] and U_star[6] out.  The sweeps pass none.  CONV: the
+// converge rule of the exact solve (riemann_exact).
 struct Q1DExtra {
     double atc, lu, lv;
     double *ws, *us;
 };
 
-template <bool TF = false>
+template <bool TF = false, bool CONV = false>
 __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L, const Side &R,
                                         const double TL[4], const double TR[4],
                                         const Gas &GL, const Gas &GR, double eps, double tol,
@@ -361,7 +388,7 @@ __device__ __forceinline__ void grp_q1d(double D[4], double U[4], const Side &L,
         rs_L = rho_L; cs_L = c_L; speed_L = u_star - cs_L;
         rs_R = rho_R; cs_R = c_R; speed_R = u_star + cs_R;
     } else {
-        const StarResult st = riemann_exact(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, 500, p_guess);
+        const StarResult st = riemann_exact<CONV>(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, 500, p_guess);
         u_star = st.u_star; p_star = st.p_star; crw0 = st.crw0; crw1 = st.crw1;
         if (crw0) {
             const PowBase b = pow_base(p_star / p_L);
@@ -670,9 +697,10 @@ __device__ __forceinline__ int flux_from_mid(const double mid[4], const double d
 // the split path's zero tangential derivatives (+0.0 in x, -0.0 in y).
 // On success F = (F_rho, F_u, F_v, F_e), I = (rho, u, v, p) at t + tau; on a
 // star/derivative failure F and I are zero and the check code is returned.
-// tol, p_guess: Newton tolerance and learned starting pressure (riemann_exact);
-// the reference's path is tol = eps, p_guess <= 0.
-template <int DIR>
+// tol, p_guess, CONV: Newton tolerance, learned starting pressure and the
+// converge rule (riemann_exact); the reference's path is tol = eps,
+// p_guess <= 0, CONV false.
+template <int DIR, bool CONV = false>
 __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double tau, double eps,
                                            double F[4], double I[4], double tol, double p_guess = -1.0)
 {
@@ -693,7 +721,7 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double 
     const double T[4] = {tz, t_u, t_v, tz};
 
     double dire[4], mid[4];
-    grp_q1d(dire, mid, L, R, T, T, G, G, eps, tol, p_guess);
+    grp_q1d<false, CONV>(dire, mid, L, R, T, T, G, G, eps, tol, p_guess);
     return flux_from_mid<DIR>(mid, dire, G, tau, eps, F, I);
 }
 
@@ -701,9 +729,10 @@ __device__ __forceinline__ int grp_2d_flux(Side L, Side R, const Gas &G, double 
 // side (GRP_2D_flux itself, flux_solver.c:158-237: any unit normal (n_x, n_y),
 // the energy flux with the left side's gamma).  TL, TR hold (t_rho, t_u, t_v,
 // t_p) in grid axes; they are rotated with the velocities (:167-185).
+template <bool CONV = false>
 __device__ __forceinline__ int grp_2d_flux_g(Side L, Side R, const double TL[4], const double TR[4], const Gas &GL,
                                              const Gas &GR, double n_x, double n_y, double tau, double eps,
-                                             double F[4], double I[4], double tol)
+                                             double F[4], double I[4], double tol, double p_guess = -1.0)
 {
     const double u = L.u * n_x + L.v * n_y;
     const double u_R = R.u * n_x + R.v * n_y;
@@ -717,17 +746,18 @@ __device__ __forceinline__ int grp_2d_flux_g(Side L, Side R, const double TL[4],
     R.d_v = -R.d_u * n_y + R.d_v * n_x;
     L.u = u; R.u = u_R; L.d_u = d_u; R.d_u = d_u_R;
     double dire[4], mid[4];
-    grp_q1d(dire, mid, L, R, TLr, TRr, GL, GR, eps, tol, -1.0);
+    grp_q1d<false, CONV>(dire, mid, L, R, TLr, TRr, GL, GR, eps, tol, p_guess);
     return flux_from_mid_n(mid, dire, GL, n_x, n_y, tau, eps, F, I);
 }
 
 // The unsplit step's form: one gas, normal (1, 0) or (0, 1)
 // (flux_generator_{x,y}.c:96-121 with Transversal = true).
+template <bool CONV = false>
 __device__ __forceinline__ int grp_2d_flux_n(Side L, Side R, const double TL[4], const double TR[4], const Gas &G,
                                              double n_x, double n_y, double tau, double eps, double F[4], double I[4],
-                                             double tol)
+                                             double tol, double p_guess = -1.0)
 {
-    return grp_2d_flux_g(L, R, TL, TR, G, G, n_x, n_y, tau, eps, F, I, tol);
+    return grp_2d_flux_g<CONV>(L, R, TL, TR, G, G, n_x, n_y, tau, eps, F, I, tol, p_guess);
 }
 
 // Two gamma-law fluids a and b mixed by the volume fraction a of fluid a:
@@ -752,7 +782,7 @@ __device__ __forceinline__ Gas gas_of(double fa, const TwoGas &T)
 // :203-235): gamma per side from the reconstructed volume fractions, gamma at
 // t + tau/2 from a_mid for the energy flux.  FA = {a_L, d_a_L, a_R, d_a_R};
 // on success FZ = {u_n a at t + tau/2, u_n at t + tau/2, a at t + tau}.
-template <int DIR>
+template <int DIR, bool CONV = false>
 __device__ __forceinline__ int grp_2d_flux_tf(Side L, Side R, const TwoGas &T, const double FA[4], double tau,
                                               double eps, double F[4], double I[4], double FZ[3], double tol,
                                               double p_guess = -1.0)
@@ -775,7 +805,7 @@ __device__ __forceinline__ int grp_2d_flux_tf(Side L, Side R, const TwoGas &T, c
     const double Tz[4] = {tz, t_u, t_v, tz};
 
     double dire[4], mid[4], fo[2];
-    grp_q1d<true>(dire, mid, L, R, Tz, Tz, GL, GR, eps, tol, p_guess, FA, fo);
+    grp_q1d<true, CONV>(dire, mid, L, R, Tz, Tz, GL, GR, eps, tol, p_guess, FA, fo);
     const int bad = star_dire_bad(mid, dire, eps);
     if (bad) {
         F[0] = F[1] = F[2] = F[3] = 0.0;

@@ -39,6 +39,33 @@ __global__ void riemann_batch_kernel(const double *in, int count, int N, double 
     o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
 }
 
+// GRP_2D_flux with the reference's multi-fluid members for the split sweeps'
+// normals: sides of 21 doubles (gamma, lambda_u, lambda_v, rho u v p, d_rho
+// d_u d_v d_p, t_rho t_u t_v t_p, z_a d_z_a t_z_a, phi d_phi t_phi), of which
+// the tangential derivatives and the mass fraction are not read (zero in the
+// split path) and gamma follows from z_a (fluids a: ga, b: gb).
+// out: code, F_rho F_u F_v F_e, RHO_int U_int V_int P_int, z_a at t + tau.
+__global__ void flux_tf_batch_kernel(const double *L, const double *R, int count, double tau, double eps, int dir,
+                                     double ga, double gb, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *l = L + (size_t)k * 21, *r = R + (size_t)k * 21;
+    Side SL = {l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]};
+    Side SR = {r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+    TwoGas T;
+    T.a = make_gas(ga);
+    T.b = make_gas(gb);
+    const double FA[4] = {l[15], l[16], r[15], r[16]};
+    double F[4], I[4], Z[3];
+    const int code = dir == 0 ? grp_2d_flux_tf<0>(SL, SR, T, FA, tau, eps, F, I, Z, eps)
+                              : grp_2d_flux_tf<1>(SL, SR, T, FA, tau, eps, F, I, Z, eps);
+    double *o = out + (size_t)k * 10;
+    o[0] = code;
+    for (int q = 0; q < 4; ++q) { o[1 + q] = F[q]; o[5 + q] = I[q]; }
+    o[9] = Z[2];
+}
+
 // GuessP per row: DL DR UL UR PL PR CL CR GammaL GammaR eps
 __global__ void guessp_batch_kernel(const double *in, int count, double *out)
 {
@@ -200,6 +227,26 @@ extern "C" int hgrp_grp2d_flux_batch(const double *L, const double *R, int count
                          hipLaunchKernelGGL(flux_batch_kernel, dim3((f->count + 127) / 128), dim3(128), 0, 0,
                                             (const double *)d[0], (const double *)d[1], f->count, f->tau, f->eps,
                                             f->dir, (double *)o);
+                     },
+                     &fc);
+}
+
+struct FluxTfCtx { int count; double tau, eps; int dir; double ga, gb; };
+
+extern "C" int hgrp_grp2d_flux_tf_batch(const double *L, const double *R, int count, double tau, double eps, int dir,
+                                        double gamma_a, double gamma_b, double *out)
+{
+    if (count < 0 || (count && (!L || !R || !out)) || (dir != 0 && dir != 1)) return -1001;
+    if (count == 0) return 0;
+    const void *in[2] = {L, R};
+    const size_t nb[2] = {(size_t)count * 21 * 8, (size_t)count * 21 * 8};
+    FluxTfCtx fc = {count, tau, eps, dir, gamma_a, gamma_b};
+    return run_batch(in, nb, 2, out, (size_t)count * 10 * 8,
+                     [](void *const *d, void *o, void *cx) {
+                         const FluxTfCtx *f = (const FluxTfCtx *)cx;
+                         hipLaunchKernelGGL(flux_tf_batch_kernel, dim3((f->count + 127) / 128), dim3(128), 0, 0,
+                                            (const double *)d[0], (const double *)d[1], f->count, f->tau, f->eps,
+                                            f->dir, f->ga, f->gb, (double *)o);
                      },
                      &fc);
 }

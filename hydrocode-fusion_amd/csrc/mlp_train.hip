@@ -93,7 +93,7 @@ __global__ void mlp_data_kernel(uint64_t seed, int count, double tol, float *fea
     const double p_R = p_L / pow(10.0, -4.0 + 8.0 * unif(seed, (uint64_t)k, 1));
     const double c_L = sqrt(gL * p_L / rho_L), c_R = sqrt(gR * p_R / rho_R);
     const double u_R = u_L + (-5.0 + 10.0 * unif(seed, (uint64_t)k, 2)) * c_L;
-    const StarResult r = riemann_exact(make_gas(gL), make_gas(gR), u_L, u_R, p_L, p_R, c_L, c_R, 1e-9, tol, 500);
+    const StarResult r = riemann_exact<true>(make_gas(gL), make_gas(gR), u_L, u_R, p_L, p_R, c_L, c_R, 1e-9, tol, 500);
     float x[MLP_IN], p_tr;
     const bool good = mlp_features(rho_L, u_L, p_L, rho_R, u_R, p_R, gL, gR, x, &p_tr) && r.p_star > 0.0 &&
                       isfinite(r.p_star) && r.iters < 500;
@@ -244,9 +244,10 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(const MlpPacked *W, cons
 }
 
 // in: rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R eps tol per interface;
-// out: u* p* crw0 crw1 iterations starting-guess (-1: the reference's guess)
-__global__ void __launch_bounds__(256) riemann_guess_kernel(const double *in, int count, int N, const MlpPacked *W,
-                                                            double *out)
+// out: u* p* crw0 crw1 iterations starting-guess (-1: the reference's guess);
+// conv: the converge rule (riemann_exact)
+__global__ void __launch_bounds__(256) riemann_guess_kernel(const double *in, int count, int N, int conv,
+                                                            const MlpPacked *W, double *out)
 {
     __shared__ uint32_t lds[4][64 * 8];
     const int k = blockIdx.x * 256 + threadIdx.x;
@@ -258,7 +259,9 @@ __global__ void __launch_bounds__(256) riemann_guess_kernel(const double *in, in
     if (W) pg = learned_guess_wave(W, act, rho_L, u_L, p_L, rho_R, u_R, p_R, gL, gR, lds[threadIdx.x >> 6]);
     if (!act) return;
     const double c_L = sqrt(gL * p_L / rho_L), c_R = sqrt(gR * p_R / rho_R);
-    const StarResult r = riemann_exact(make_gas(gL), make_gas(gR), u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, N, pg);
+    const Gas GL = make_gas(gL), GR = make_gas(gR);
+    const StarResult r = conv ? riemann_exact<true>(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, N, pg)
+                              : riemann_exact<false>(GL, GR, u_L, u_R, p_L, p_R, c_L, c_R, eps, tol, N, pg);
     double *o = out + (size_t)k * 6;
     o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.iters; o[5] = pg;
 }
@@ -344,9 +347,11 @@ extern "C" int hgrp_mlp_infer(const float *params, const float *feat, int count,
     return 0;
 }
 
-extern "C" int hgrp_riemann_guess_batch(const double *in, int count, int N, const float *params, double *out)
+extern "C" int hgrp_riemann_guess_batch(const double *in, int count, int N, int rule, const float *params,
+                                        double *out)
 {
-    if (count < 1 || !in || !out || N < 0) return -1000;
+    if (count < 1 || !in || !out || N < 0 || (rule != HGRP_NEWTON_REFERENCE && rule != HGRP_NEWTON_CONVERGE))
+        return -1000;
     Dev<MlpPacked> dw;
     if (params) {
         MlpPacked h;
@@ -358,7 +363,7 @@ extern "C" int hgrp_riemann_guess_batch(const double *in, int count, int N, cons
     MCHECK(din.alloc((size_t)count * 10));
     MCHECK(dout.alloc((size_t)count * 6));
     MCHECK(hipMemcpy(din.p, in, (size_t)count * 10 * 8, hipMemcpyHostToDevice));
-    hipLaunchKernelGGL(riemann_guess_kernel, dim3((count + 255) / 256), dim3(256), 0, 0, din.p, count, N,
+    hipLaunchKernelGGL(riemann_guess_kernel, dim3((count + 255) / 256), dim3(256), 0, 0, din.p, count, N, rule,
                        params ? dw.p : (const MlpPacked *)nullptr, dout.p);
     MCHECK(hipGetLastError());
     MCHECK(hipDeviceSynchronize());

@@ -3,11 +3,10 @@
 //   sweep_x_rows_kernel  one X(tau_s) sweep: a block of W waves covers 64
 //                    consecutive i (one coalesced 512-byte row segment per wave)
 //                    and W cell rows; x-neighbours are exchanged through LDS.
-//   sweep_x_kernel   the same sweep with threads walking down x-lines in a
-//                    sliding register window (kept as a tuning alternative).
-//   sweep_y_kernel   one Y(tau) sweep: a wave owns 64 consecutive cells of a y-line
-//                    (contiguous in memory) and exchanges neighbour values,
-//                    slopes and fluxes with lane shuffles; 63 cells per wave.
+//   sweep_y_kernel   one Y(tau) sweep: a wave holds 64/G rows of G consecutive
+//                    cells of a y-line (contiguous in memory) and exchanges
+//                    neighbour values, slopes and fluxes with lane shuffles.
+// Each kernel takes the Newton mode (NM_*) as a template argument.
 //   cfl_kernel       h/S_max over the frozen initial data (order-independent min).
 //
 // Semantics follow src/finite_volume/grp_solver_2D_split_EUL_source.c:182-303,
@@ -159,13 +158,13 @@ struct Line {
     }
 };
 
-template <int DIR>
+template <int DIR, int NM>
 __device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &SL, const Side &SR,
                                                 unsigned long long loop_index, double F[4], double I[4],
                                                 double p_guess = -1.0)
 {
     int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
-    if (grp_2d_flux<DIR>(SL, SR, a.gas, a.tau, a.eps, F, I, a.tol, p_guess)) code = 2;
+    if (grp_2d_flux<DIR, NM != NM_REF>(SL, SR, a.gas, a.tau, a.eps, F, I, a.tol, p_guess)) code = 2;
     if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
 }
 
@@ -185,48 +184,6 @@ __device__ __forceinline__ double interface_guess(const SweepArgs &a, bool activ
     return learned_guess_wave(a.mlp, want, SL.rho, uL, SL.p, SR.rho, uR, SR.p, a.gas.g, a.gas.g, lds);
 }
 
-// X sweep: thread = x-line i, segment of TJ cells in j.
-template <int TJ>
-__global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
-{
-    if (sweep_halted(a)) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.n) return;
-    const int m = a.m;
-    const int j0 = blockIdx.y * TJ;
-    const int j1 = min(j0 + TJ, m);
-    const Line<0> L(a, i);
-    const double hh = a.hh;
-
-    double vm[4], v0[4], vp[4], sm[4], s0[4], Fp[4], Ip[4];
-    L.value(j0 - 1, vm);
-    L.cell(j0, v0);
-    if (j0 == 0) {
-        L.ghost_slope(true, sm);
-    } else {
-        double vmm[4];
-        L.value(j0 - 2, vmm);
-        L.limited(j0 - 1, vmm, vm, v0, sm);
-    }
-    for (int jf = j0; jf <= j1; ++jf) {
-        if (jf < m) {
-            L.value(jf + 1, vp);
-            L.limited(jf, vm, v0, vp, s0);
-        } else {
-            L.ghost_slope(false, s0);
-        }
-        const Side SL = recon(vm, sm, hh, true);
-        const Side SR = recon(v0, s0, hh, false);
-        double F[4], I[4];
-        solve_interface<0>(a, SL, SR, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)jf, F, I);
-        if (jf > j0) L.update(jf - 1, vm, Fp, F, Ip, I);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            vm[q] = v0[q]; v0[q] = vp[q]; sm[q] = s0[q]; Fp[q] = F[q]; Ip[q] = I[q];
-        }
-    }
-}
-
 // Line-parallel sweep along DIR: a block of W waves covers 64 consecutive
 // grid lines of the sweep direction (lane: line; DIR 0: x-line i, DIR 1:
 // y-line j) and the cells c0-1 .. c0+W-2 along them (wave w: cell c = c0-1+w).
@@ -239,7 +196,7 @@ __global__ void __launch_bounds__(256) sweep_x_kernel(SweepArgs a)
 // lanes read consecutive doubles.  (For DIR 1 each lane would read its own row,
 // stride ld: on MI355X that form ran as fast as sweep_y_kernel at 1024^2 and
 // 3x slower at 4096^2, where the rows no longer stay in cache.)
-template <int DIR, int W, bool LEARNED>
+template <int DIR, int W, int NM>
 __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][64], double (*xb)[4][64],
                                            double (*xf)[4][64], double (*xi)[4][64], double (*xe)[64])
 {
@@ -306,7 +263,7 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][6
     double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
     const int f = c + 1;
     double pg = -1.0;
-    if constexpr (LEARNED) {
+    if constexpr (NM == NM_LEARNED) {
         // xf[w] is not written before the solve: it holds this wave's network inputs
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
@@ -315,7 +272,7 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][6
     if (col && f <= m) {
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        solve_interface<DIR>(a, SL, SR, (unsigned long long)line * (unsigned long long)(m + 1) + (unsigned long long)f,
+        solve_interface<DIR, NM>(a, SL, SR, (unsigned long long)line * (unsigned long long)(m + 1) + (unsigned long long)f,
                              F, I, pg);
     }
 #pragma unroll
@@ -332,7 +289,7 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][6
 
 // X sweep, row-parallel form (line_sweep<0>): lanes are 64 consecutive i,
 // waves the cell rows j0-1 .. j0+W-2.
-template <int W, int MINW, bool LEARNED>
+template <int W, int MINW, int NM>
 __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 {
     __shared__ double xa[W][4][64];   // values
@@ -340,7 +297,7 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     __shared__ double xf[W][4][64];   // fluxes
     __shared__ double xi[W][4][64];   // interface values at t + tau
     __shared__ double xe[W][64];      // E of the owned cell (read back for the update)
-    line_sweep<0, W, LEARNED>(a, xa, xb, xf, xi, xe);
+    line_sweep<0, W, NM>(a, xa, xb, xf, xi, xe);
 }
 
 // Y sweep: a wave holds 64/G y-lines (rows j) of G lanes each; lane k of a
@@ -348,11 +305,12 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
 // k >= 1 (G-1 cells per row).  With G = 64 a wave is one stretch of one line;
 // smaller G stacks rows, so that a front lying along y (constant i) meets
 // fewer waves, each with more lanes on it.
-template <int MINW, bool LEARNED, int G = 64>
+template <int MINW, int NM, int G = 64>
 __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chunks)
 {
     static_assert(G == 64 || G == 32 || G == 16 || G == 8, "lanes per row");
     constexpr int ROWS = 64 / G;
+    constexpr bool LEARNED = NM == NM_LEARNED;
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
     __shared__ double stash[4][5][64];   // this lane's cell values and E, read back for the update
     const int lane = threadIdx.x & 63, k = lane % G;
@@ -431,7 +389,7 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     if (face) {
         const Side SL = recon(w, s, hh, true);
         const Side SR = recon(wr, sr, hh, false);
-        solve_interface<1>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I,
+        solve_interface<1, NM>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I,
                            pg);
     }
     double Fl[4], Il[4];
@@ -470,125 +428,91 @@ __global__ void __launch_bounds__(256) cfl_kernel(const double *rho, const doubl
 }
 
 // ---- launchers -----------------------------------------------------------------
-static constexpr int TJ_X = 32;
+// Launch variants of each sweep (the tuner, hgrp2d_tune, times them on the
+// context's own data; the defaults, variant 0, were the fastest on four-quadrant
+// runs at 1024^2 and 4096^2 on MI355X):
+//   X  0: 8 rows per block, register budget for 4 waves/SIMD (128 VGPRs, a few spills)
+//      1: 8 rows per block, the compiler's budget (~150 VGPRs, 3 waves/SIMD)
+//      2: 16 rows per block, 4 waves/SIMD
+//   Y  0: 4 rows of 16 lanes per wave, 4 waves/SIMD (a front lying along y meets
+//         fewer waves, with more lanes on it: 12 % faster than variant 2 at 1024^2)
+//      1: 2 rows of 32 lanes;  2: one 64-lane row;  3: 8 rows of 8 lanes (all 4 waves/SIMD)
+//      4: one 64-lane row, the compiler's budget
+int sweep_variants(int dir) { return dir == 0 ? 3 : 5; }
 
-// Launch variants.  The defaults were picked by timing four-quadrant runs at
-// 1024^2 (whole run) and 4096^2 (200 steps) on MI355X: the X sweep with 8 rows
-// per block and the register budget for 4 waves/SIMD (128 VGPRs, a few
-// spills); the Y sweep with the same budget and 4 rows of 16 lanes per wave
-// (12 % faster than one 64-lane stretch per wave at 1024^2, 6 % at 4096^2;
-// 2 rows of 32 and 8 rows of 8 lie between).
-// HGRP_SWEEP_X / HGRP_SWEEP_Y override them (read once, for tuning runs):
-//   HGRP_SWEEP_X  1 = 8 rows, free budget;  2 = 16 rows, 4 waves/SIMD;
-//                 3 = sliding register window;  4 = 8 rows, 4 waves/SIMD
-//   HGRP_SWEEP_Y  1 = one 64-lane row per wave, free budget;  2 = the same, 3 waves/SIMD;
-//                 3 = the same, 4 waves/SIMD;  4, 5, 6 = 2, 4, 8 rows of 32, 16, 8 lanes, 4 waves/SIMD
-static int env_variant(const char *name, int hi)
+template <int NM>
+static hipError_t launch_x(const SweepArgs &a, int v, hipStream_t st)
 {
-    const char *e = getenv(name);
-    int v = e ? atoi(e) : 0;
-    return (v < 0 || v > hi) ? 0 : v;
-}
-static int sweep_x_variant(const SweepArgs &a)
-{
-    static int v = -1;
-    if (v < 0) v = env_variant("HGRP_SWEEP_X", 4);
-    (void)a;
-    return v != 0 ? v : 4;
-}
-static int sweep_y_variant()
-{
-    static int v = -1;
-    if (v < 0) v = env_variant("HGRP_SWEEP_Y", 6);
-    return v != 0 ? v : 5;
-}
-
-hipError_t launch_sweep_x(const SweepArgs &a, hipStream_t st)
-{
-    if (a.mlp) {
-        dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-        if (sweep_x_variant(a) == 4)
-            hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4, true>), grid, dim3(64 * 8), 0, st, a);
-        else
-            hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1, true>), grid, dim3(64 * 8), 0, st, a);
-        return hipGetLastError();
-    }
-    switch (sweep_x_variant(a)) {
-    case 3: {
-        const int bx = a.n >= 256 ? 256 : (a.n >= 128 ? 128 : 64);
-        dim3 grid((a.n + bx - 1) / bx, (a.m + TJ_X - 1) / TJ_X);
-        hipLaunchKernelGGL(sweep_x_kernel<TJ_X>, grid, dim3(bx), 0, st, a);
+    switch (v) {
+    case 1:
+        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1, NM>), dim3((a.n + 63) / 64, (a.m + 6) / 7), dim3(64 * 8), 0,
+                           st, a);
         break;
-    }
-    case 2: {
-        dim3 grid((a.n + 63) / 64, (a.m + 14) / 15);
-        hipLaunchKernelGGL((sweep_x_rows_kernel<16, 4, false>), grid, dim3(64 * 16), 0, st, a);
+    case 2:
+        hipLaunchKernelGGL((sweep_x_rows_kernel<16, 4, NM>), dim3((a.n + 63) / 64, (a.m + 14) / 15), dim3(64 * 16),
+                           0, st, a);
         break;
-    }
-    case 4: {
-        dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4, false>), grid, dim3(64 * 8), 0, st, a);
+    default:
+        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 4, NM>), dim3((a.n + 63) / 64, (a.m + 6) / 7), dim3(64 * 8), 0,
+                           st, a);
         break;
-    }
-    default: {
-        dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-        hipLaunchKernelGGL((sweep_x_rows_kernel<8, 1, false>), grid, dim3(64 * 8), 0, st, a);
-        break;
-    }
     }
     return hipGetLastError();
 }
 
-hipError_t launch_sweep_y(const SweepArgs &a, hipStream_t st)
+static int y_lanes(int v) { return v == 1 ? 32 : (v == 2 || v == 4) ? 64 : v == 3 ? 8 : 16; }
+
+template <int NM>
+static hipError_t launch_y(const SweepArgs &a, int v, hipStream_t st)
 {
-    const int yv = sweep_y_variant();
-    const int G = yv == 4 ? 32 : yv == 5 ? 16 : yv == 6 ? 8 : 64, rows = 64 / G;
+    const int G = y_lanes(v), rows = 64 / G;
     const int chunks = (a.n + G - 2) / (G - 1);
     const long waves = (long)((a.m + rows - 1) / rows) * chunks;
     const dim3 grid((unsigned)((waves + 3) / 4)), block(256);
-    if (a.mlp) {
-        switch (yv) {
-        case 1: hipLaunchKernelGGL((sweep_y_kernel<1, true>), grid, block, 0, st, a, chunks); break;
-        case 2: case 3: hipLaunchKernelGGL((sweep_y_kernel<4, true>), grid, block, 0, st, a, chunks); break;
-        default: hipLaunchKernelGGL((sweep_y_kernel<4, true, 16>), grid, block, 0, st, a, chunks); break;
-        }
-        return hipGetLastError();
-    }
-    switch (yv) {
-    case 1: hipLaunchKernelGGL((sweep_y_kernel<1, false>), grid, block, 0, st, a, chunks); break;
-    case 2: hipLaunchKernelGGL((sweep_y_kernel<3, false>), grid, block, 0, st, a, chunks); break;
-    case 3: hipLaunchKernelGGL((sweep_y_kernel<4, false>), grid, block, 0, st, a, chunks); break;
-    case 4: hipLaunchKernelGGL((sweep_y_kernel<4, false, 32>), grid, block, 0, st, a, chunks); break;
-    case 6: hipLaunchKernelGGL((sweep_y_kernel<4, false, 8>), grid, block, 0, st, a, chunks); break;
-    default: hipLaunchKernelGGL((sweep_y_kernel<4, false, 16>), grid, block, 0, st, a, chunks); break;
+    switch (v) {
+    case 1: hipLaunchKernelGGL((sweep_y_kernel<4, NM, 32>), grid, block, 0, st, a, chunks); break;
+    case 2: hipLaunchKernelGGL((sweep_y_kernel<4, NM, 64>), grid, block, 0, st, a, chunks); break;
+    case 3: hipLaunchKernelGGL((sweep_y_kernel<4, NM, 8>), grid, block, 0, st, a, chunks); break;
+    case 4: hipLaunchKernelGGL((sweep_y_kernel<1, NM, 64>), grid, block, 0, st, a, chunks); break;
+    default: hipLaunchKernelGGL((sweep_y_kernel<4, NM, 16>), grid, block, 0, st, a, chunks); break;
     }
     return hipGetLastError();
 }
 
-const char *sweep_kernel_name(int dir, int m, int n, bool learned)
+hipError_t launch_sweep_x(const SweepArgs &a, int variant, hipStream_t st)
 {
-    SweepArgs a{};
-    a.m = m; a.n = n;
-    if (dir == 1) {
-        const int yv = sweep_y_variant();
-        if (learned) return yv == 1 ? "sweep_y_kernel<1, true>" : yv <= 3 ? "sweep_y_kernel<4, true>"
-                                                                         : "sweep_y_kernel<4, true, 16>";
-        switch (yv) {
-        case 1: return "sweep_y_kernel<1, false>";
-        case 2: return "sweep_y_kernel<3, false>";
-        case 3: return "sweep_y_kernel<4, false>";
-        case 4: return "sweep_y_kernel<4, false, 32>";
-        case 6: return "sweep_y_kernel<4, false, 8>";
-        default: return "sweep_y_kernel<4, false, 16>";
-        }
+    switch (newton_mode(a)) {
+    case NM_LEARNED: return launch_x<NM_LEARNED>(a, variant, st);
+    case NM_CONV: return launch_x<NM_CONV>(a, variant, st);
+    default: return launch_x<NM_REF>(a, variant, st);
     }
-    if (learned) return sweep_x_variant(a) == 4 ? "sweep_x_rows_kernel<8, 4, true>" : "sweep_x_rows_kernel<8, 1, true>";
-    switch (sweep_x_variant(a)) {
-    case 3: return "sweep_x_kernel<32>";
-    case 2: return "sweep_x_rows_kernel<16, 4, false>";
-    case 4: return "sweep_x_rows_kernel<8, 4, false>";
-    default: return "sweep_x_rows_kernel<8, 1, false>";
+}
+
+hipError_t launch_sweep_y(const SweepArgs &a, int variant, hipStream_t st)
+{
+    switch (newton_mode(a)) {
+    case NM_LEARNED: return launch_y<NM_LEARNED>(a, variant, st);
+    case NM_CONV: return launch_y<NM_CONV>(a, variant, st);
+    default: return launch_y<NM_REF>(a, variant, st);
     }
+}
+
+// the kernel's name as profilers print it
+const char *sweep_kernel_name(int dir, int variant, int nm)
+{
+    static const char *xn[3][3] = {
+        {"sweep_x_rows_kernel<8, 4, 0>", "sweep_x_rows_kernel<8, 1, 0>", "sweep_x_rows_kernel<16, 4, 0>"},
+        {"sweep_x_rows_kernel<8, 4, 1>", "sweep_x_rows_kernel<8, 1, 1>", "sweep_x_rows_kernel<16, 4, 1>"},
+        {"sweep_x_rows_kernel<8, 4, 2>", "sweep_x_rows_kernel<8, 1, 2>", "sweep_x_rows_kernel<16, 4, 2>"}};
+    static const char *yn[3][5] = {
+        {"sweep_y_kernel<4, 0, 16>", "sweep_y_kernel<4, 0, 32>", "sweep_y_kernel<4, 0, 64>", "sweep_y_kernel<4, 0, 8>",
+         "sweep_y_kernel<1, 0, 64>"},
+        {"sweep_y_kernel<4, 1, 16>", "sweep_y_kernel<4, 1, 32>", "sweep_y_kernel<4, 1, 64>", "sweep_y_kernel<4, 1, 8>",
+         "sweep_y_kernel<1, 1, 64>"},
+        {"sweep_y_kernel<4, 2, 16>", "sweep_y_kernel<4, 2, 32>", "sweep_y_kernel<4, 2, 64>", "sweep_y_kernel<4, 2, 8>",
+         "sweep_y_kernel<1, 2, 64>"}};
+    if (nm < 0 || nm > 2 || variant < 0 || variant >= sweep_variants(dir)) return "";
+    return dir == 0 ? xn[nm][variant] : yn[nm][variant];
 }
 
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,

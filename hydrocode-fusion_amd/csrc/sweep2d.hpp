@@ -41,6 +41,8 @@ struct SweepArgs {
     int m, n, ld;                             // cells in x, in y, row pitch
     int bc;                                   // boundary condition along the sweep
     int limiter3;                             // 0: minmod2 (first sweep of this direction), 1: minmod3
+    int conv;                                 // Newton stop: 0 the reference's (gap <= tol), 1 the converge rule
+                                              // (the launchers pick the kernel's Newton mode from it and mlp)
     double h, hh, tau, nu, eps, alpha;        // h, h/2, flux time, update factor, ...
     double tol;                               // Newton tolerance (the reference: eps)
     RecipDiv div_h;                           // division by h
@@ -55,6 +57,13 @@ struct SweepArgs {
     SweepStatus *status;                      // this sweep
     const SweepStatus *prev;                  // the sweep before (nullptr for the first)
 };
+
+// Newton modes of the sweep kernels, a template argument so that each mode's
+// solve is compiled on its own (a run-time switch costs registers in all of
+// them): the reference's solve; the converge rule (HGRP_NEWTON_CONVERGE); the
+// converge rule started from the learned guess.
+enum { NM_REF = 0, NM_CONV = 1, NM_LEARNED = 2 };
+inline int newton_mode(const SweepArgs &a) { return a.mlp ? NM_LEARNED : (a.conv ? NM_CONV : NM_REF); }
 
 // The kernel's argument block (its first parameter) read again at the point of
 // the call: the copy's fields are loaded there from kernel-argument memory

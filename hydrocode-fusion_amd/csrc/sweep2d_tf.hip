@@ -146,7 +146,7 @@ __device__ __forceinline__ void recon_tf(const double w[NQ], const double s[NQ],
     d_fa = s[4];
 }
 
-template <int DIR>
+template <int DIR, int NM>
 __device__ __forceinline__ void solve_interface_tf(const SweepArgs &a, const double wl[NQ], const double sl[NQ],
                                                    const double wr[NQ], const double sr[NQ],
                                                    unsigned long long loop_index, double F[4], double I[4],
@@ -157,7 +157,7 @@ __device__ __forceinline__ void solve_interface_tf(const SweepArgs &a, const dou
     recon_tf(wl, sl, a.hh, true, SL, FA[0], FA[1]);
     recon_tf(wr, sr, a.hh, false, SR, FA[2], FA[3]);
     int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
-    if (grp_2d_flux_tf<DIR>(SL, SR, a.two, FA, a.tau, a.eps, F, I, Z, a.tol, p_guess)) code = 2;
+    if (grp_2d_flux_tf<DIR, NM != NM_REF>(SL, SR, a.two, FA, a.tau, a.eps, F, I, Z, a.tol, p_guess)) code = 2;
     if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
 }
 
@@ -194,9 +194,10 @@ __device__ __forceinline__ bool tf_halted(const SweepArgs &a)
 }
 
 // X sweep, row-parallel (the layout of sweep_x_rows_kernel with five quantities)
-template <int W, bool LEARNED>
+template <int W, int NM>
 __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 {
+    constexpr bool LEARNED = NM == NM_LEARNED;
     __shared__ double xa[W][NQ][64];   // values
     __shared__ double xb[W][NQ][64];   // slopes
     __shared__ double xf[W][4][64];    // fluxes
@@ -263,7 +264,7 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
     if constexpr (LEARNED)   // xf[w] is not written before the solve: it holds the network inputs
         pg = interface_guess_tf<0>(a, col && f <= m, v, s, vr, sr, (uint32_t *)&xf[w][0][0]);
     if (col && f <= m)
-        solve_interface_tf<0>(a, v, s, vr, sr, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f,
+        solve_interface_tf<0, NM>(a, v, s, vr, sr, (unsigned long long)i * (unsigned long long)(m + 1) + (unsigned long long)f,
                               F, I, Z, pg);
 #pragma unroll
     for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
@@ -284,9 +285,10 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 }
 
 // Y sweep: a wave owns 64 consecutive cells of a y-line (as sweep_y_kernel)
-template <bool LEARNED, int G = 16>
+template <int NM, int G = 16>
 __global__ void __launch_bounds__(256, 3) sweep_y_tf_kernel(SweepArgs a, int chunks)
 {
+    constexpr bool LEARNED = NM == NM_LEARNED;
     // 64/G y-lines of G lanes per wave, as in sweep_y_kernel: lane k of a row
     // owns cell c0-1+k and updates it for k >= 1
     constexpr int ROWS = 64 / G;
@@ -353,7 +355,7 @@ __global__ void __launch_bounds__(256, 3) sweep_y_tf_kernel(SweepArgs a, int chu
     double pg = -1.0;
     if constexpr (LEARNED) pg = interface_guess_tf<1>(a, face, w, s, wr, sr, mlp_lds[(threadIdx.x >> 6) & 3]);
     if (face)
-        solve_interface_tf<1>(a, w, s, wr, sr, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f,
+        solve_interface_tf<1, NM>(a, w, s, wr, sr, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f,
                               F, I, Z, pg);
     double Fl[4], Il[4], Zl[3];
 #pragma unroll
@@ -429,10 +431,11 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st)
 {
     dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
-    if (a.mlp)
-        hipLaunchKernelGGL((sweep_x_tf_kernel<8, true>), grid, dim3(64 * 8), 0, st, a);
-    else
-        hipLaunchKernelGGL((sweep_x_tf_kernel<8, false>), grid, dim3(64 * 8), 0, st, a);
+    switch (newton_mode(a)) {
+    case NM_LEARNED: hipLaunchKernelGGL((sweep_x_tf_kernel<8, NM_LEARNED>), grid, dim3(64 * 8), 0, st, a); break;
+    case NM_CONV: hipLaunchKernelGGL((sweep_x_tf_kernel<8, NM_CONV>), grid, dim3(64 * 8), 0, st, a); break;
+    default: hipLaunchKernelGGL((sweep_x_tf_kernel<8, NM_REF>), grid, dim3(64 * 8), 0, st, a); break;
+    }
     return hipGetLastError();
 }
 
@@ -442,10 +445,11 @@ hipError_t launch_sweep_y_tf(const SweepArgs &a, hipStream_t st)
     const int chunks = (a.n + 14) / 15;
     const long waves = (long)((a.m + 3) / 4) * chunks;
     const long blocks = (waves + 3) / 4;
-    if (a.mlp)
-        hipLaunchKernelGGL(sweep_y_tf_kernel<true>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
-    else
-        hipLaunchKernelGGL(sweep_y_tf_kernel<false>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks);
+    switch (newton_mode(a)) {
+    case NM_LEARNED: hipLaunchKernelGGL(sweep_y_tf_kernel<NM_LEARNED>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    case NM_CONV: hipLaunchKernelGGL(sweep_y_tf_kernel<NM_CONV>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    default: hipLaunchKernelGGL(sweep_y_tf_kernel<NM_REF>, dim3((unsigned)blocks), dim3(256), 0, st, a, chunks); break;
+    }
     return hipGetLastError();
 }
 

@@ -170,18 +170,20 @@ struct Grid {
 // GRP solve of one interface with transversal derivatives, normal x (dir 0)
 // or y (dir 1); failures are recorded in status[dir] as
 // ((loop index << 2) | code) + 1, the reference's loop order
+template <int NM>
 __device__ __forceinline__ void solve_t(const UnsplitArgs &a, int dir, const Side &SL, const Side &SR,
                                         const double TL[4], const double TR[4], unsigned long long loop_index,
                                         double F[4], double I[4])
 {
     int code = ifvar_bad_t(SL, SR, TL, TR, a.eps) ? 1 : 0;
-    if (grp_2d_flux_n(SL, SR, TL, TR, a.gas, dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, a.tau, a.eps, F, I, a.tol)) code = 2;
+    if (grp_2d_flux_n<NM != NM_REF>(SL, SR, TL, TR, a.gas, dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, a.tau, a.eps, F, I, a.tol))
+        code = 2;
     if (code) atomicMax(&a.status[dir].fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
 }
 
 }  // namespace
 
-template <int W, int MINW>
+template <int W, int MINW, int NM>
 __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs a)
 {
     __shared__ double sd[W][12][64];     // side data of row c: value, x-slope, y-slope
@@ -258,7 +260,7 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
         }
         double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
         // the argument block is read again for the solve, so no scalar register holds it across
-        if (active) solve_t(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I);
+        if (active) solve_t<NM>(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I);
         if (pass == 0 && w <= W - 2) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
@@ -311,6 +313,7 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
 // kept from the iteration before.  No wave waits for another: the block is
 // one wave, its LDS holds two rows of side data and two rows of interface
 // results (16 KiB).  One extra x-interface per R rows.
+template <int NM>
 __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int R)
 {
     __shared__ double sd[2][12][64];   // side data of rows r and r + 1 (alternating)
@@ -381,7 +384,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
                 }
             }
             double Fo[4] = {0.0, 0.0, 0.0, 0.0}, Io[4] = {0.0, 0.0, 0.0, 0.0};
-            if (active) solve_t(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io);
+            if (active) solve_t<NM>(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io);
             if (pass == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { gb[q][lane] = Fo[q]; gb[4 + q][lane] = Io[q]; }
@@ -470,33 +473,41 @@ static int unsplit_rows(int v) { return v == 2 ? 2 : 6; }
 
 int unsplit_max_rows() { return 65535 * 4; }
 
-hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
+template <int NM>
+static hipError_t launch_unsplit(const UnsplitArgs &a, hipStream_t st)
 {
     const int v = unsplit_variant(a.m, a.n);
     if (v == 4) {
         const int R = strip_rows(a.m, a.n);
-        hipLaunchKernelGGL(unsplit_strip_kernel, dim3((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R)),
+        hipLaunchKernelGGL(unsplit_strip_kernel<NM>, dim3((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R)),
                            dim3(64), 0, st, a, R);
         return hipGetLastError();
     }
     const int rows = unsplit_rows(v);
     const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + rows - 1) / rows));
     switch (v) {
-    case 2: hipLaunchKernelGGL((unsplit_step_kernel<4, 4>), grid, dim3(256), 0, st, a); break;
-    case 3: hipLaunchKernelGGL((unsplit_step_kernel<8, 4>), grid, dim3(512), 0, st, a); break;
-    default: hipLaunchKernelGGL((unsplit_step_kernel<8, 1>), grid, dim3(512), 0, st, a); break;
+    case 2: hipLaunchKernelGGL((unsplit_step_kernel<4, 4, NM>), grid, dim3(256), 0, st, a); break;
+    case 3: hipLaunchKernelGGL((unsplit_step_kernel<8, 4, NM>), grid, dim3(512), 0, st, a); break;
+    default: hipLaunchKernelGGL((unsplit_step_kernel<8, 1, NM>), grid, dim3(512), 0, st, a); break;
     }
     return hipGetLastError();
 }
 
-const char *unsplit_kernel_name(int m, int n)
+hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
 {
-    switch (unsplit_variant(m, n)) {
-    case 2: return "unsplit_step_kernel<4, 4>";
-    case 3: return "unsplit_step_kernel<8, 4>";
-    case 4: return "unsplit_strip_kernel";
-    default: return "unsplit_step_kernel<8, 1>";
-    }
+    return a.conv ? launch_unsplit<NM_CONV>(a, st) : launch_unsplit<NM_REF>(a, st);
+}
+
+const char *unsplit_kernel_name(int m, int n, int nm)
+{
+    static const char *names[2][4] = {
+        {"unsplit_step_kernel<8, 1, 0>", "unsplit_step_kernel<4, 4, 0>", "unsplit_step_kernel<8, 4, 0>",
+         "unsplit_strip_kernel<0>"},
+        {"unsplit_step_kernel<8, 1, 1>", "unsplit_step_kernel<4, 4, 1>", "unsplit_step_kernel<8, 4, 1>",
+         "unsplit_strip_kernel<1>"}};
+    if (nm < 0 || nm > 1) return "";
+    const int v = unsplit_variant(m, n);
+    return names[nm][v >= 1 && v <= 4 ? v - 1 : 0];
 }
 
 }  // namespace hgrp

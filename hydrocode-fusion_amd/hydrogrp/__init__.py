@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libhydrogrp.so")
 N_CONF = 400
 
 _D, _DP, _IP = C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)
+# stopping rules of the Newton solve for p* (HGRP_NEWTON_* in include/hydrocode_grp.h)
+NEWTON_REFERENCE, NEWTON_CONVERGE = 0, 1
 
 
 class Result(C.Structure):
@@ -40,11 +42,14 @@ def lib():
     L.hgrp2d_upload.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
     L.hgrp2d_download.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
     L.hgrp2d_solve.argtypes = [C.c_void_p, _DP, C.c_int, C.c_int, FRAME_FN, C.c_void_p, C.POINTER(Result)]
-    L.hgrp2d_bench.argtypes = [C.c_void_p, C.c_int, _DP, _DP, _DP]
-    L.hgrp2d_bench_unsplit.argtypes = [C.c_void_p, C.c_int, _DP, _DP]
+    L.hgrp2d_bench.argtypes = [C.c_void_p, C.c_int, C.c_int, _DP, _DP, _DP]
+    L.hgrp2d_bench_unsplit.argtypes = [C.c_void_p, C.c_int, C.c_int, _DP, _DP]
+    L.hgrp2d_set_sweeps.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.hgrp2d_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP, _DP]
     L.hgrp2d_h_over_smax.argtypes = [C.c_void_p, _DP]
     L.hgrp_grp2d_flux_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _DP]
     L.hgrp_riemann_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
+    L.hgrp_grp2d_flux_tf_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _D, _D, _DP]
     L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
     L.hgrp_q1d_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
     L.hgrp_guessp_batch.argtypes = [_DP, C.c_int, _DP]
@@ -54,7 +59,7 @@ def lib():
     L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
     L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
     _FP = C.POINTER(C.c_float)
-    L.hgrp2d_set_newton.argtypes = [C.c_void_p, _D, _FP]
+    L.hgrp2d_set_newton.argtypes = [C.c_void_p, _D, C.c_int, _FP]
     L.hgrp2d_set_fluids.argtypes = [C.c_void_p, _D, _DP]
     L.hgrp2d_download_fraction.argtypes = [C.c_void_p, _DP]
     L.hgrp2d_checkpoint_size.argtypes = [C.c_void_p, C.POINTER(C.c_size_t)]
@@ -66,7 +71,7 @@ def lib():
     L.hgrp_mlp_loss_grad.argtypes = [_FP, _FP, _FP, C.c_int, _DP, _FP]
     L.hgrp_mlp_infer.argtypes = [_FP, _FP, C.c_int, _FP]
     L.hgrp_mlp_train.argtypes = [C.c_ulonglong, C.c_int, C.c_int, C.c_int, _D, _D, _FP, _FP]
-    L.hgrp_riemann_guess_batch.argtypes = [_DP, C.c_int, C.c_int, _FP, _DP]
+    L.hgrp_riemann_guess_batch.argtypes = [_DP, C.c_int, C.c_int, C.c_int, _FP, _DP]
     L.hgrp1d_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _IP, _DP, _DP]
     L.hgrp1d_destroy.argtypes = [C.c_void_p]
     L.hgrp1d_upload.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP]
@@ -154,14 +159,16 @@ class Solver2D:
         """Name of the kernel launched for an x (0) or y (1) sweep of this grid."""
         return lib().hgrp2d_sweep_kernel(self._h, direction).decode()
 
-    def set_newton(self, tol=0.0, params=None):
-        """Newton tolerance for p* (<= 0: eps, the reference's) and the learned
-        starting guess (params: master vector from hydrogrp.mlp, None: off)."""
+    def set_newton(self, tol=0.0, params=None, converge=False):
+        """Newton tolerance for p* (<= 0: eps, the reference's), the learned
+        starting guess (params: master vector from hydrogrp.mlp, None: off) and
+        the stopping rule (converge: NEWTON_CONVERGE, else the reference's)."""
         fp = None
         if params is not None:
             params = np.ascontiguousarray(params, dtype=np.float32)
             fp = params.ctypes.data_as(C.POINTER(C.c_float))
-        _check(lib().hgrp2d_set_newton(self._h, tol, fp), "hgrp2d_set_newton")
+        rule = NEWTON_CONVERGE if converge else NEWTON_REFERENCE
+        _check(lib().hgrp2d_set_newton(self._h, tol, rule, fp), "hgrp2d_set_newton")
 
     def set_fluids(self, gamma_b, frac_a):
         """Two fluids: volume fraction of fluid a (gamma = config[6]); fluid b has gamma_b."""
@@ -186,35 +193,51 @@ class Solver2D:
         buf = C.create_string_buffer(blob, len(blob))
         _check(lib().hgrp2d_restore(self._h, buf, len(blob)), "hgrp2d_restore")
 
-    def bench(self, steps):
+    def bench(self, steps, sample_every=0):
+        """(ms of all steps, average ms of one X and of one Y sweep launch, from
+        events around every sample_every-th step; NaN for sample_every 0)."""
         t, x, y = _D(), _D(), _D()
-        _check(lib().hgrp2d_bench(self._h, steps, C.byref(t), C.byref(x), C.byref(y)), "hgrp2d_bench")
+        _check(lib().hgrp2d_bench(self._h, steps, sample_every, C.byref(t), C.byref(x), C.byref(y)), "hgrp2d_bench")
         return t.value, x.value, y.value
 
-    def bench_unsplit(self, steps):
+    def bench_unsplit(self, steps, sample_every=0):
         """(ms of all steps, average ms of one unsplit step launch)."""
         t, s = _D(), _D()
-        _check(lib().hgrp2d_bench_unsplit(self._h, steps, C.byref(t), C.byref(s)), "hgrp2d_bench_unsplit")
+        _check(lib().hgrp2d_bench_unsplit(self._h, steps, sample_every, C.byref(t), C.byref(s)),
+               "hgrp2d_bench_unsplit")
         return t.value, s.value
 
+    def set_sweeps(self, x_variant, y_variant):
+        _check(lib().hgrp2d_set_sweeps(self._h, x_variant, y_variant), "hgrp2d_set_sweeps")
 
-def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None):
+    def tune(self, steps=4):
+        """Times the sweep launch variants on the current state (which it
+        restores); returns (x_variant, y_variant, ms per X launch, ms per Y launch)."""
+        xv, yv, ms = C.c_int(), C.c_int(), (C.c_double * 2)()
+        _check(lib().hgrp2d_tune(self._h, steps, C.byref(xv), C.byref(yv), ms), "hgrp2d_tune")
+        return xv.value, yv.value, ms[0], ms[1]
+
+
+def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None,
+                converge=False):
     """Whole run like the oracle's split2d(): returns (frames, N_plot, time_plot, config, status).
-    newton_tol / guess_params: see Solver2D.set_newton (defaults: the reference's solve)."""
+    newton_tol / guess_params / converge: see Solver2D.set_newton (defaults: the
+    reference's solve)."""
     config = np.array(config, dtype=np.float64)
     config[33] = 1.0
-    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params)
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge)
 
 
-def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0):
+def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None,
+                  converge=False):
     """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0), like the
     oracle's unsplit2d(); results as run_split2d."""
     config = np.array(config, dtype=np.float64)
     config[33] = 0.0
-    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, None)
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge)
 
 
-def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params):
+def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False):
     if time_plot is None:
         time_plot = np.array([0.0, config[1]])
     time_plot = np.array(time_plot, dtype=np.float64)
@@ -222,8 +245,8 @@ def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params):
         N_T = min(5, len(time_plot))
     s = Solver2D(m, n, config)
     try:
-        if newton_tol or guess_params is not None:
-            s.set_newton(newton_tol, guess_params)
+        if newton_tol or guess_params is not None or converge:
+            s.set_newton(newton_tol, guess_params, converge)
         s.upload(init["rho"], init["u"], init["v"], init["p"], init.get("e"))
         frames = np.zeros((5, N_T, m, n))
         frames[:, 0] = s.download()
@@ -323,6 +346,18 @@ def flux_batch(L, R, tau, eps, direction):
     R = np.ascontiguousarray(R, dtype=np.float64)
     out = np.zeros((len(L), 9))
     _check(lib().hgrp_grp2d_flux_batch(_p(L), _p(R), len(L), tau, eps, direction, _p(out)), "flux batch")
+    return out
+
+
+def flux_tf_batch(L, R, tau, eps, direction, gamma_a, gamma_b):
+    """GRP_2D_flux with the two-fluid members (hgrp_grp2d_flux_tf_batch): rows of
+    21-double sides; returns rows (code, F x4, interface values x4, z_a at t + tau)."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    assert L.shape == R.shape and L.shape[1] == 21
+    out = np.zeros((len(L), 10))
+    _check(lib().hgrp_grp2d_flux_tf_batch(_p(L), _p(R), len(L), tau, eps, direction, gamma_a, gamma_b, _p(out)),
+           "hgrp_grp2d_flux_tf_batch")
     return out
 
 

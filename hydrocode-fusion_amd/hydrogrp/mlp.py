@@ -77,9 +77,10 @@ def train(params, seed=1, n_samples=1 << 20, epochs=4, batch=8192, lr=3e-3, tol=
     return params, losses
 
 
-def riemann_guess_batch(states, gamma=1.4, eps=1e-9, tol=1e-12, params=None, N=500):
+def riemann_guess_batch(states, gamma=1.4, eps=1e-9, tol=1e-12, params=None, N=500, converge=False):
     """Exact solves of (rho_L u_L p_L rho_R u_R p_R [gamma_L gamma_R ...]) rows (without
-    the gamma columns every row has `gamma`); params None: the reference's guess.
+    the gamma columns every row has `gamma`); params None: the reference's guess;
+    converge: the converge stopping rule (else the reference's).
     Returns rows (u*, p*, CRW0, CRW1, iterations, starting guess or -1)."""
     states = np.asarray(states, np.float64)
     inp = np.zeros((len(states), 10))
@@ -91,7 +92,8 @@ def riemann_guess_batch(states, gamma=1.4, eps=1e-9, tol=1e-12, params=None, N=5
     inp[:, 8], inp[:, 9] = eps, tol
     out = np.zeros((len(states), 6))
     fp = None if params is None else _f(np.ascontiguousarray(params, np.float32))
-    _check(lib().hgrp_riemann_guess_batch(_p(inp), len(inp), N, fp, _p(out)), "hgrp_riemann_guess_batch")
+    rule = 1 if converge else 0
+    _check(lib().hgrp_riemann_guess_batch(_p(inp), len(inp), N, rule, fp, _p(out)), "hgrp_riemann_guess_batch")
     return out
 
 

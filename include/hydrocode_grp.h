@@ -93,13 +93,14 @@ int hgrp2d_restore(hgrp2d *ctx, const void *buf, size_t bytes);
  * tau = CFL * h/S_max, no time clipping, plot frames or host syncs inside.
  * ms_total: device time of all steps (HIP events on the solver's stream);
  * ms_x, ms_y (may be NULL): average device time of one X / one Y sweep launch,
- * from events around the launches of every 8th step (every step below 64). */
-int hgrp2d_bench(hgrp2d *ctx, int steps, double *ms_total, double *ms_x, double *ms_y);
+ * from events around the launches of every sample_every-th step (NaN when
+ * sample_every <= 0: then the stream holds no event but the two that time it). */
+int hgrp2d_bench(hgrp2d *ctx, int steps, int sample_every, double *ms_total, double *ms_x, double *ms_y);
 
 /* Benchmark form of the unsplit scheme: `steps` steps with tau = CFL * h/S_max.
  * ms_total: device time of all steps; ms_step (may be NULL): average device
  * time of one step launch (unsplit_step_kernel), sampled as in hgrp2d_bench. */
-int hgrp2d_bench_unsplit(hgrp2d *ctx, int steps, double *ms_total, double *ms_step);
+int hgrp2d_bench_unsplit(hgrp2d *ctx, int steps, int sample_every, double *ms_total, double *ms_step);
 
 /* h/S_max of the uploaded initial data (the frozen CFL reduction). */
 int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
@@ -116,12 +117,41 @@ int hgrp2d_h_over_smax(hgrp2d *ctx, double *h_S_max);
 int hgrp2d_set_fluids(hgrp2d *ctx, double gamma_b, const double *frac_a);
 int hgrp2d_download_fraction(hgrp2d *ctx, double *frac_a);
 
+/* Stopping rules of the Newton solve for p*.
+ * HGRP_NEWTON_REFERENCE: the reference's loop (riemann_solver_exact_Ben.c:133-203),
+ *   which ends at the first iterate whose velocity gap is within tol.
+ * HGRP_NEWTON_CONVERGE: the same loop, but an iterate whose gap is within tol
+ *   gets one more Newton step if none has been taken from the starting value
+ *   or if the error estimated from the last tangents, gap / |k1 - k3|, exceeds
+ *   1e-13 p; an iterate that such an extra step gave ends the loop if its gap
+ *   is within tol (else ordinary steps go on).  It ends at the root to within
+ *   rounding whatever the starting guess; the reference's rule leaves up to
+ *   tol / (p |dv/dp|) relative, ~1e-10 near vacuum at tol 1e-12. */
+#define HGRP_NEWTON_REFERENCE 0
+#define HGRP_NEWTON_CONVERGE 1
+
 /* Newton solve for p* (no counterpart in the reference, which always uses
- * tol = eps and its own starting guess): tol > 0 sets the Newton tolerance on
- * the velocity gap (<= 0: eps, the reference's); params (HGRP_MLP_NP floats,
- * the master vector of hgrp_mlp_train) switches on the learned starting guess,
- * NULL switches it off.  The other uses of eps are unchanged. */
-int hgrp2d_set_newton(hgrp2d *ctx, double tol, const float *params);
+ * tol = eps, its own stopping rule and its own starting guess): tol > 0 sets
+ * the Newton tolerance on the velocity gap (<= 0: eps, the reference's); rule
+ * is one of the HGRP_NEWTON_* rules; params (HGRP_MLP_NP floats, the master
+ * vector of hgrp_mlp_train) switches on the learned starting guess, NULL
+ * switches it off.  The other uses of eps are unchanged. */
+int hgrp2d_set_newton(hgrp2d *ctx, double tol, int rule, const float *params);
+
+/* Launch variants of the X sweep (0..2) and the Y sweep (0..4) (no counterpart
+ * in the reference).  X: 0 = 8 cell rows per block at 4 waves/SIMD (the
+ * default), 1 = 8 rows at the compiler's register budget, 2 = 16 rows.  Y: 0 =
+ * 4 rows of 16 lanes per wave (the default), 1 = 2 x 32, 2 = 1 x 64, 3 = 8 x 8,
+ * 4 = 1 x 64 at the compiler's register budget.  Results do not depend on them. */
+int hgrp2d_set_sweeps(hgrp2d *ctx, int x_variant, int y_variant);
+
+/* Sweep autotuner: times every X variant, then every Y variant, over `steps`
+ * full steps from the context's current state (two trials each, per-launch
+ * device time), sets the fastest with hgrp2d_set_sweeps and restores the state,
+ * so that the run goes on exactly as without it.  x_variant, y_variant, ms[2]
+ * (average ms per X and per Y launch of the winners) may be NULL.  Not for
+ * two-fluid contexts. */
+int hgrp2d_tune(hgrp2d *ctx, int steps, int *x_variant, int *y_variant, double *ms);
 
 /* Name of the kernel the context launches for an x (dir 0) or y (dir 1) sweep
  * (it depends on the grid size), or for an unsplit step (dir 2); "" for a bad
@@ -139,6 +169,19 @@ const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
  * out: count x 9 = check code, F_rho F_u F_v F_e, RHO_int U_int V_int P_int. */
 int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double tau,
                           double eps, int dir, double *out);
+
+/* GRP_2D_flux with the reference's multi-fluid members
+ * (src/flux_calc/flux_solver.c:158-237 with -DMULTIFLUID_BASICS, its branch at
+ * :203-235; Q1D's z_a at linear_grp_solver_Edir_Q1D.c:55-61) as the two-fluid
+ * split sweeps solve it: normal x (dir 0) or y (dir 1), tangential
+ * derivatives zero, each side's gamma from its volume fraction z_a (fluid a:
+ * gamma_a, the reference's config[6]; fluid b: gamma_b, its config[106]), the
+ * energy flux with gamma at t + tau/2 from z_a there.  Sides are 21 doubles:
+ * the 15 above, then z_a d_z_a t_z_a phi d_phi t_phi (t_z_a and phi unread).
+ * out: count x 10 = check code, F_rho F_u F_v F_e, RHO_int U_int V_int P_int,
+ * z_a at t + tau. */
+int hgrp_grp2d_flux_tf_batch(const double *L, const double *R, int count, double tau, double eps, int dir,
+                             double gamma_a, double gamma_b, double *out);
 
 /* Riemann_solver_exact (src/riemann_solver/riemann_solver_exact_Ben.c:30-211,
  * declared at src/include/riemann_solver.h:24-27).
@@ -275,7 +318,7 @@ int hgrp1d_tune(hgrp1d *ctx, int probe_steps, int *bs_iface, int *bs_update);
  * log-uniform 1e-3..1e3, p_L/p_R log-uniform 1e-4..1e4, (u_R-u_L)/c_L uniform
  * -5..5; half with gamma 1.4 on both sides, half with gamma_L, gamma_R uniform
  * in 1.2..1.8; counter-based from seed), their p* from the exact solver at
- * tolerance tol: feat[count*6], target[count] = log(p* / p_TR), ok[count];
+ * tolerance tol with the converge rule (HGRP_NEWTON_CONVERGE): feat[count*6], target[count] = log(p* / p_TR), ok[count];
  * states[count*9] (may be NULL) = rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R p*. */
 int hgrp_mlp_data(unsigned long long seed, int count, double tol, float *feat, float *target,
                   double *states, int *ok);
@@ -296,10 +339,11 @@ int hgrp_mlp_train(unsigned long long seed, int n_samples, int epochs, int batch
                    float *params, float *epoch_loss);
 
 /* Exact Riemann solve per interface with an optional learned guess (params
- * NULL: the reference's guess).  in[count*10]: rho_L u_L p_L rho_R u_R p_R
- * gamma_L gamma_R eps tol; out[count*6]: u* p* CRW[0] CRW[1] Newton-iterations
- * starting-guess (-1 where the reference's guess was used). */
-int hgrp_riemann_guess_batch(const double *in, int count, int N, const float *params, double *out);
+ * NULL: the reference's guess) and a stopping rule (HGRP_NEWTON_*).
+ * in[count*10]: rho_L u_L p_L rho_R u_R p_R gamma_L gamma_R eps tol;
+ * out[count*6]: u* p* CRW[0] CRW[1] Newton-iterations starting-guess (-1 where
+ * the reference's guess was used). */
+int hgrp_riemann_guess_batch(const double *in, int count, int N, int rule, const float *params, double *out);
 
 /* Name of the device the library runs on (empty if none). */
 int hgrp_device_name(char *buf, int len);

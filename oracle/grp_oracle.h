@@ -60,6 +60,21 @@ double orc_riemann_exact(double *U_star, double *P_star, double gammaL, double g
                          double c_L, double c_R, int CRW[2],
                          double eps, double tol, int N, int *n_iter);
 
+/* The same with a stopping rule: 0 the reference's (the loop above), 1 the
+ * converge rule (an iterate within tol gets one more Newton step when none has
+ * been taken or when gap / |k1 - k3| > 1e-13 p, unless an extra step gave it;
+ * no reference counterpart). */
+double orc_riemann_exact_rule(double *U_star, double *P_star, double gammaL, double gammaR,
+                              double u_L, double u_R, double p_L, double p_R,
+                              double c_L, double c_R, int CRW[2],
+                              double eps, double tol, int N, int *n_iter, int rule);
+
+/* Newton tolerance (<= 0: eps, the reference's) and stopping rule of the
+ * exact solves inside orc_grp_q1d, so of every 2D run of the oracle. */
+void orc_set_newton(double tol, int rule);
+extern double orc_newton_tol;
+extern int orc_newton_rule;
+
 /* Riemann_solver_exact_Ben (same file :230-408): single-gamma twin (1D path). */
 double orc_riemann_exact_ben(double *U_star, double *P_star, double gamma,
                              double u_L, double u_R, double p_L, double p_R,

@@ -62,8 +62,8 @@ void orc_grp_q1d(double wave_speed[2], double D[4], double U[4], double U_star[6
         rs_L = rho_L; cs_L = c_L; speed_L = u_star - cs_L;
         rs_R = rho_R; cs_R = c_R; speed_R = u_star + cs_R;
     } else {
-        orc_riemann_exact(&u_star, &p_star, gL, gR, u_L, u_R, p_L, p_R, c_L, c_R, CRW,
-                          eps, eps, 500, 0);
+        orc_riemann_exact_rule(&u_star, &p_star, gL, gR, u_L, u_R, p_L, p_R, c_L, c_R, CRW,
+                               eps, orc_newton_tol > 0.0 ? orc_newton_tol : eps, 500, 0, orc_newton_rule);
         if (CRW[0]) {
             rs_L = rho_L * pow(p_star / p_L, 1.0 / gL);
             cs_L = c_L * pow(p_star / p_L, 0.5 * (gL - 1.0) / gL);

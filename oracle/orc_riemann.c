@@ -11,6 +11,13 @@
  * (shock branch above the state pressure, isentrope below), classify the two
  * waves with an eps margin, start Newton from half the intersection of the two
  * acoustic tangents and iterate on the absolute velocity gap.
+ *
+ * orc_riemann_exact_rule adds the converge stopping rule (no counterpart in the
+ * reference): an iterate within tol that the starting value or an ordinary
+ * step gave gets one more Newton step when none has been taken yet or when
+ * gap / |k1 - k3| (the last tangents) exceeds 1e-13 p; an iterate that such an
+ * extra step gave ends the loop if it is within tol.  Pinned by the root of the
+ * wave-curve equation computed in 40-digit arithmetic (tests/test_oracle_math.py).
  */
 #include <math.h>
 #include "grp_oracle.h"
@@ -36,7 +43,7 @@ static double riemann_core(double *U_star, double *P_star,
                            double gL, double gR, double muL, double nuL, double muR, double nuR,
                            double u_L, double u_R, double p_L, double p_R,
                            double c_L, double c_R, int CRW[2],
-                           double eps, double tol, int N, int *n_iter)
+                           double eps, double tol, int N, int *n_iter, int conv)
 {
     /* wave types (:50-84): (u_LR, p_R) on the left curve, (u_RL, p_L) on the right */
     const double u_LR = wave_curve(p_R, u_L, p_L, c_L, gL, muL, nuL, -1.0);
@@ -63,8 +70,19 @@ static double riemann_core(double *U_star, double *P_star,
         return fabs(u_L - u_R);
     }
 
-    int n = 0;
-    while ((gap > tol) && (n != N)) {                       /* :133-203 */
+    int n = 0, extra = 0;   /* extra: the last step was the converge rule's extra step */
+    for (;;) {                                              /* :133-203 */
+        if (!((gap > tol) && (n != N))) {
+            /* the converge rule: an iterate within tol that a starting value or an
+             * ordinary step gave gets one more step when none was taken or when
+             * the error the last tangents give exceeds 1e-13 p */
+            if (conv && !extra && n != N && (n == 0 || gap > 1e-13 * p_INT * fabs(k1 - k3)))
+                extra = 1;
+            else
+                break;
+        } else {
+            extra = 0;
+        }
         double t1, t2, t3, dp;
         if (p_INT > p_L) {
             dp = p_INT - p_L;
@@ -110,7 +128,28 @@ double orc_riemann_exact(double *U_star, double *P_star, double gammaL, double g
     const double muL = (gammaL - 1.0) / (2.0 * gammaL), nuL = (gammaL + 1.0) / (2.0 * gammaL);
     const double muR = (gammaR - 1.0) / (2.0 * gammaR), nuR = (gammaR + 1.0) / (2.0 * gammaR);
     return riemann_core(U_star, P_star, gammaL, gammaR, muL, nuL, muR, nuR,
-                        u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter);
+                        u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter, 0);
+}
+
+double orc_riemann_exact_rule(double *U_star, double *P_star, double gammaL, double gammaR,
+                              double u_L, double u_R, double p_L, double p_R,
+                              double c_L, double c_R, int CRW[2],
+                              double eps, double tol, int N, int *n_iter, int rule)
+{
+    const double muL = (gammaL - 1.0) / (2.0 * gammaL), nuL = (gammaL + 1.0) / (2.0 * gammaL);
+    const double muR = (gammaR - 1.0) / (2.0 * gammaR), nuR = (gammaR + 1.0) / (2.0 * gammaR);
+    return riemann_core(U_star, P_star, gammaL, gammaR, muL, nuL, muR, nuR,
+                        u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter, rule == 1);
+}
+
+/* Newton settings of the interface solves inside the oracle's Q1D (and so its
+ * 2D runs): tol <= 0 keeps the reference's tol = eps, rule 0 its stopping rule. */
+double orc_newton_tol = 0.0;
+int orc_newton_rule = 0;
+void orc_set_newton(double tol, int rule)
+{
+    orc_newton_tol = tol;
+    orc_newton_rule = rule;
 }
 
 double orc_riemann_exact_ben(double *U_star, double *P_star, double gamma,
@@ -120,7 +159,7 @@ double orc_riemann_exact_ben(double *U_star, double *P_star, double gamma,
 {
     const double mu = (gamma - 1.0) / (2.0 * gamma), nu = (gamma + 1.0) / (2.0 * gamma);
     return riemann_core(U_star, P_star, gamma, gamma, mu, nu, mu, nu,
-                        u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter);
+                        u_L, u_R, p_L, p_R, c_L, c_R, CRW, eps, tol, N, n_iter, 0);
 }
 
 /* GuessP (riemann_solver_starPU.c:20-58): the PVRS pressure, kept where the

@@ -16,6 +16,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 ORACLE_DEVPOW_SO = os.path.join(ROOT, "oracle", "liboracle_devpow.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref.so")
+REF_MF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref_mf.so")
 PKG_DIR = os.path.join(ROOT, "hydrocode-fusion_amd")
 if PKG_DIR not in sys.path:
     sys.path.insert(0, PKG_DIR)
@@ -68,6 +69,9 @@ class Oracle:
         L = self.lib = C.CDLL(path)
         L.orc_riemann_exact.restype = D
         L.orc_riemann_exact.argtypes = [DP, DP, D, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP]
+        L.orc_riemann_exact_rule.restype = D
+        L.orc_riemann_exact_rule.argtypes = [DP, DP, D, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP, C.c_int]
+        L.orc_set_newton.argtypes = [D, C.c_int]
         L.orc_riemann_exact_ben.restype = D
         L.orc_riemann_exact_ben.argtypes = [DP, DP, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP]
         L.orc_grp_q1d.restype = None
@@ -102,6 +106,15 @@ class Oracle:
                                          crw, eps, tol, N, C.byref(nit))
         return us.value, ps.value, crw[0], crw[1], gap, nit.value
 
+    def riemann_rule(self, gL, gR, uL, uR, pL, pR, cL, cR, eps, tol, N, rule):
+        """riemann() with a stopping rule (0 the reference's, 1 the converge rule)."""
+        us, ps = D(), D()
+        crw = (C.c_int * 2)()
+        nit = C.c_int()
+        gap = self.lib.orc_riemann_exact_rule(C.byref(us), C.byref(ps), gL, gR, uL, uR, pL, pR, cL, cR,
+                                              crw, eps, tol, N, C.byref(nit), rule)
+        return us.value, ps.value, crw[0], crw[1], gap, nit.value
+
     def riemann_ben(self, g, uL, uR, pL, pR, cL, cR, eps, tol, N):
         us, ps = D(), D()
         crw = (C.c_int * 2)()
@@ -128,16 +141,18 @@ class Oracle:
         self.lib.orc_grp_edir(Dd, U, C.byref(ifv_from(Lv)), C.byref(ifv_from(Rv)), eps, atc)
         return np.array(list(Dd) + list(U))
 
-    def split2d(self, m, n, init, config, time_plot=None, N_T=None):
+    def split2d(self, m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, converge=False):
         """init: dict rho,u,v,p (m x n arrays, [j, i]), optionally e (else formed as the
-        reference's main does); returns (frames, N_plot, time_plot, config, status)."""
-        return self._run2d(self.lib.orc_split2d, m, n, init, config, time_plot, N_T)
+        reference's main does); returns (frames, N_plot, time_plot, config, status).
+        newton_tol (<= 0: eps) / converge: the Newton settings of the solves (the
+        defaults are the reference's)."""
+        return self._run2d(self.lib.orc_split2d, m, n, init, config, time_plot, N_T, newton_tol, converge)
 
-    def unsplit2d(self, m, n, init, config, time_plot=None, N_T=None):
+    def unsplit2d(self, m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, converge=False):
         """GRP_solver_2D_EUL_source (the unsplit step); arguments and results as split2d."""
-        return self._run2d(self.lib.orc_unsplit2d, m, n, init, config, time_plot, N_T)
+        return self._run2d(self.lib.orc_unsplit2d, m, n, init, config, time_plot, N_T, newton_tol, converge)
 
-    def _run2d(self, fn, m, n, init, config, time_plot, N_T):
+    def _run2d(self, fn, m, n, init, config, time_plot, N_T, newton_tol=0.0, converge=False):
         if time_plot is None:
             time_plot = np.array([0.0, config[1]])
         time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
@@ -153,7 +168,11 @@ class Oracle:
         cfg = np.array(config, dtype=np.float64)
         npl = C.c_int(N_plot)
         cpu = np.zeros(N_T)
-        st = fn(m, n, *ptrs, N_T, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
+        self.lib.orc_set_newton(newton_tol, 1 if converge else 0)
+        try:
+            st = fn(m, n, *ptrs, N_T, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
+        finally:
+            self.lib.orc_set_newton(0.0, 0)
         return frames, npl.value, time_plot, cfg, st
 
     def grp1d(self, m, init, config, N_T=2, time_plot=None):
@@ -349,6 +368,38 @@ class RefLib:
         self.lib.ref_grp1d(m, dptr(frames), N_T, C.byref(npl), dptr(time_plot))
         cfg = np.array([self.lib.ref_config_get(k) for k in range(N_CONF)])
         return frames, npl.value, time_plot, cfg
+
+
+class RefLibMF:
+    """The reference's interface solvers built with -DMULTIFLUID_BASICS in
+    oracle/_ref/libhydroref_mf.so (checker only); sides of 21 doubles
+    (oracle/ref_glue_mf.c)."""
+
+    def __init__(self, path=REF_MF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = C.CDLL(path)
+        L.refmf_config_reset.restype = None
+        L.refmf_config_set.argtypes = [C.c_int, D]
+        L.refmf_q1d.argtypes = [DP, DP, D, D, DP]
+        L.refmf_grp_2d_flux.argtypes = [DP, DP, D, D, D, DP]
+
+    def set_config(self, values):
+        self.lib.refmf_config_reset()
+        for k, v in values.items():
+            self.lib.refmf_config_set(k, float(v))
+
+    def q1d(self, Lv, Rv, eps, atc):
+        out = np.zeros(20)
+        self.lib.refmf_q1d(dptr(np.ascontiguousarray(Lv, dtype=np.float64)),
+                           dptr(np.ascontiguousarray(Rv, dtype=np.float64)), eps, atc, dptr(out))
+        return out
+
+    def grp_2d_flux(self, Lv, Rv, n_x, n_y, tau):
+        out = np.zeros(13)
+        self.lib.refmf_grp_2d_flux(dptr(np.ascontiguousarray(Lv, dtype=np.float64)),
+                                   dptr(np.ascontiguousarray(Rv, dtype=np.float64)), n_x, n_y, tau, dptr(out))
+        return out
 
 
 # ---- the reference's struct layouts through ctypes (include/var_struc.h) -----------

@@ -187,3 +187,29 @@ def material_interface(m, n, u0=0.5, v0=0.25):
     frac_a = 1.0 - inside
     rho = frac_a * 1.0 + inside * 0.138
     return ({"rho": rho, "u": np.full((m, n), u0), "v": np.full((m, n), v0), "p": np.ones((m, n))}, frac_a)
+
+
+def c3_states(count, seed, near_vacuum=0):
+    """Interface states of the learned-guess distribution at gamma 1.4: left
+    (1, 0, 1); rho_L/rho_R log-uniform in [1e-3, 1e3], p_L/p_R in [1e-4, 1e4],
+    (u_R - u_L)/c_L uniform in [-5, 5]; states that would open a vacuum are
+    dropped.  near_vacuum: that many more with (u_R - u_L)/c_L in [4.5, 4.98]
+    and p_L/p_R in [1e2, 1e4] (double rarefactions with p* down to ~1e-16).
+    Rows: rho_L u_L p_L rho_R u_R p_R."""
+    g = 1.4
+    rng = np.random.default_rng(seed)
+    rows = []
+    for k in range(count + near_vacuum):
+        rr = 1.0 / 10 ** rng.uniform(-3, 3)
+        if k < count:
+            pr = 1.0 / 10 ** rng.uniform(-4, 4)
+            du = rng.uniform(-5, 5)
+        else:
+            pr = 1.0 / 10 ** rng.uniform(2, 4)
+            du = rng.uniform(4.5, 4.98)
+        cl, cr = np.sqrt(g), np.sqrt(g * pr / rr)
+        ur = du * cl
+        if 2 * (cl + cr) / (g - 1) <= ur:
+            continue
+        rows.append((1.0, 0.0, 1.0, rr, ur, pr))
+    return np.array(rows)

@@ -66,6 +66,11 @@ def golden_unsplit():
 
 
 @pytest.fixture(scope="session")
+def golden_twofluid():
+    return load_golden("twofluid_vectors.npz")
+
+
+@pytest.fixture(scope="session")
 def golden_1d():
     return load_golden("grp1d_cases.npz")
 
@@ -79,4 +84,23 @@ def rel_linf(a, b):
         scale = np.max(np.abs(b[f]))
         d = np.max(np.abs(a[f] - b[f])) if a[f].size else 0.0
         out.append(d / scale if scale > 0 else d)
+    return np.array(out)
+
+
+def parity(a, b):
+    """Relative L-infinity difference per field (fields on axis 0: rho u v p E):
+    per cell for the positive fields rho, p, E (max |a - b| / |b|), against the
+    field's largest magnitude for the velocities, which pass through zero."""
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    out = []
+    for f in range(a.shape[0]):
+        if a[f].size == 0:
+            out.append(0.0)
+        elif f in (0, 3, 4):
+            out.append(float(np.max(np.abs(a[f] - b[f]) / np.abs(b[f]))))
+        else:
+            scale = np.max(np.abs(b[f]))
+            d = np.max(np.abs(a[f] - b[f]))
+            out.append(d / scale if scale > 0 else d)
     return np.array(out)

@@ -115,3 +115,34 @@ def sod_l1(rho, u, p, t=0.2, x0=0.5):
     er, eu, ep = sample(1.0, 0.0, 1.0, 0.125, 0.0, 0.1, (x - x0) / t)
     h = 1.0 / m
     return np.array([np.sum(np.abs(rho - er)) * h, np.sum(np.abs(u - eu)) * h, np.sum(np.abs(p - ep)) * h])
+
+
+def star_pressure_mp(rhoL, uL, pL, rhoR, uR, pR, g=1.4, gR=None, p0=None, dps=40):
+    """The star pressure to `dps` digits (mpmath, Toro's f functions; bracketed
+    from the double-precision root or p0), and p |f'(p)| there: the factor that
+    turns a velocity error into a relative pressure error at the root."""
+    import mpmath as mp
+    gL, gR = g, (g if gR is None else gR)
+    with mp.workdps(dps):
+        rL, rR, PL, PR = mp.mpf(rhoL), mp.mpf(rhoR), mp.mpf(pL), mp.mpf(pR)
+        cL, cR = mp.sqrt(gL * PL / rL), mp.sqrt(gR * PR / rR)
+        du = mp.mpf(uR) - mp.mpf(uL)
+
+        def f(p, rho, pk, c, gg):
+            if p > pk:
+                A = 2 / ((gg + 1) * rho)
+                B = mp.mpf(gg - 1) / (gg + 1) * pk
+                return (p - pk) * mp.sqrt(A / (p + B))
+            return 2 * c / (gg - 1) * ((p / pk) ** (mp.mpf(gg - 1) / (2 * gg)) - 1)
+
+        F = lambda p: f(p, rL, PL, cL, gL) + f(p, rR, PR, cR, gR) + du
+        if p0 is None:
+            p0 = star_state(rhoL, uL, pL, rhoR, uR, pR, g, gR)[0]
+        lo, hi = mp.mpf(p0) * (1 - mp.mpf("1e-6")), mp.mpf(p0) * (1 + mp.mpf("1e-6"))
+        while F(lo) > 0:
+            lo /= 2
+        while F(hi) < 0:
+            hi *= 2
+        p = mp.findroot(F, (lo, hi), solver="anderson")
+        slope = mp.diff(F, p)
+        return float(p), float(p * abs(slope))

@@ -250,6 +250,64 @@ def split2d_cases(ref):
     np.savez_compressed(os.path.join(HERE, "split2d_cases.npz"), **out)
 
 
+def split2d_fq1024(ref):
+    """The four-quadrant problem at 1024^2 to t = 0.3 (2017 steps), the
+    benchmark's size, through the reference's own split solver: per-field sums
+    and L1 norms, probe cells and every 97th x-line in full."""
+    m = n = 1024
+    cfg = cases.four_quadrant_config(m, n)
+    fr, npl, tp, c = ref.split2d(m, n, cases.four_quadrant(m, n), cfg)
+    fin = fr[:, npl - 1]
+    rng = np.random.default_rng(1024)
+    probes = np.concatenate([np.array([[0, 0], [511, 511], [512, 512], [1023, 1023], [300, 700], [700, 300]]),
+                             rng.integers(0, 1024, (26, 2))])
+    out = {"sum": fin.reshape(5, -1).sum(axis=1), "l1": np.abs(fin).reshape(5, -1).sum(axis=1),
+           "probes": probes.astype(np.float64), "probe_vals": fin[:, probes[:, 0], probes[:, 1]],
+           "rows": fin[:, ::97, :], "meta": np.array([c[5], npl, tp[npl - 1]])}
+    np.savez_compressed(os.path.join(HERE, "split2d_fq1024.npz"), **out)
+
+
+def twofluid_vectors(_ref):
+    """Two-fluid interfaces through the reference's GRP_2D_flux and Q1D built
+    with -DMULTIFLUID_BASICS (oracle/_ref/libhydroref_mf.so): fluid a gamma 1.4
+    (config[6]), fluid b gamma 1.67 (config[106]); each side pure a, pure b or
+    mixed (volume fraction z_a uniform in (0, 1), gamma from it by the
+    reference's mixture rule, src/inter_process_unstruct/cons_qty_calc.c:70),
+    weak and strong waves, near-equal states; normals x and y with the split
+    path's zero tangential derivatives (+0 in x, -0 in y)."""
+    from _libs import RefLibMF
+    ref = RefLibMF()
+    ga, gb = 1.4, 1.67
+    ref.set_config({4: EPS, 6: ga, 106: gb})
+    rng = np.random.default_rng(1067)
+    n = 1200
+
+    def side(k):
+        kind = rng.integers(0, 3)
+        z = 1.0 if kind == 0 else (0.0 if kind == 1 else rng.uniform(0.02, 0.98))
+        g = ga if z == 1.0 else (gb if z == 0.0 else 1.0 / (z / (ga - 1.0) + (1.0 - z) / (gb - 1.0)) + 1.0)
+        rho, p = 10 ** rng.uniform(-1, 1), 10 ** rng.uniform(-1, 1)
+        u, v = rng.uniform(-1, 1), rng.uniform(-1, 1)
+        d = rng.normal(0, 1, 4)
+        dz = rng.normal(0, 1) if 0.0 < z < 1.0 else 0.0
+        return [g, 0.0, 0.0, rho, u, v, p, *d, 0.0, 0.0, 0.0, 0.0, z, dz, 0.0, 0.0, 0.0, 0.0]
+
+    L = np.array([side(k) for k in range(n)])
+    R = np.array([side(k) for k in range(n)])
+    # near-equal states (the acoustic case) for a tenth of the pairs
+    near = rng.uniform(size=n) < 0.1
+    R[near, 3:7] = L[near, 3:7] * (1 + 1e-11 * rng.normal(size=(near.sum(), 4)))
+    tau = 2e-3
+    out = {"L": L, "R": R, "tau": tau, "eps": EPS, "gamma_a": ga, "gamma_b": gb}
+    for direction, (nx, ny), tz in ((0, (1.0, 0.0), 0.0), (1, (0.0, 1.0), -0.0)):
+        Ld, Rd = L.copy(), R.copy()
+        Ld[:, 11:15] = tz
+        Rd[:, 11:15] = tz
+        out[f"flux_{direction}"] = np.array([ref.grp_2d_flux(Ld[k], Rd[k], nx, ny, tau) for k in range(n)])
+    out["q1d"] = np.array([ref.q1d(L[k], R[k], EPS, EPS) for k in range(n)])
+    np.savez_compressed(os.path.join(HERE, "twofluid_vectors.npz"), **out)
+
+
 def unsplit2d_cases(ref):
     """GRP_solver_2D_EUL_source (the unsplit step, config[33] = 0)."""
     out = {}
@@ -354,7 +412,7 @@ def main():
     ref = RefLib()
     only = sys.argv[1:]
     for name, fn in (("interface", interface_vectors), ("units", interface_units), ("stages", stage_cases),
-                     ("split2d", split2d_cases),
+                     ("split2d", split2d_cases), ("split2d_fq1024", split2d_fq1024), ("twofluid", twofluid_vectors),
                      ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
         if not only or name in only:
             fn(ref)

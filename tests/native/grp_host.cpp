@@ -125,3 +125,40 @@ extern "C" void hg_riemann_guess_many(const double *in, int n, int N, const doub
         o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
     }
 }
+
+// riemann_exact with a stopping rule (0 the reference's, 1 the converge rule)
+// from a given starting pressure (guess[k] <= 0: the reference's guess)
+extern "C" void hg_riemann_rule_many(const double *in, int n, int N, int rule, const double *guess, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *a = in + 10 * k;
+        const Gas GL = make_gas(a[0]), GR = make_gas(a[1]);
+        const StarResult r =
+            rule ? riemann_exact<true>(GL, GR, a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], N, guess[k])
+                 : riemann_exact<false>(GL, GR, a[2], a[3], a[4], a[5], a[6], a[7], a[8], a[9], N, guess[k]);
+        double *o = out + 6 * k;
+        o[0] = r.u_star; o[1] = r.p_star; o[2] = r.crw0; o[3] = r.crw1; o[4] = r.gap; o[5] = r.iters;
+    }
+}
+
+// GRP_2D_flux with the two-fluid members, as flux_tf_batch_kernel (21-double sides)
+extern "C" void hg_flux_tf_many(const double *L, const double *R, int n, double tau, double eps, int dir, double ga,
+                                double gb, double *out)
+{
+    TwoGas T;
+    T.a = make_gas(ga);
+    T.b = make_gas(gb);
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 21 * k, *r = R + 21 * k;
+        Side SL = {l[3], l[4], l[5], l[6], l[7], l[8], l[9], l[10]};
+        Side SR = {r[3], r[4], r[5], r[6], r[7], r[8], r[9], r[10]};
+        const double FA[4] = {l[15], l[16], r[15], r[16]};
+        double F[4], I[4], Z[3];
+        const int code = dir == 0 ? grp_2d_flux_tf<0>(SL, SR, T, FA, tau, eps, F, I, Z, eps)
+                                  : grp_2d_flux_tf<1>(SL, SR, T, FA, tau, eps, F, I, Z, eps);
+        double *o = out + 10 * k;
+        o[0] = code;
+        for (int q = 0; q < 4; ++q) { o[1 + q] = F[q]; o[5 + q] = I[q]; }
+        o[9] = Z[2];
+    }
+}

@@ -147,7 +147,7 @@ def test_conservation_full_size(hg):
     s = hg.Solver2D(m, n, cfg)
     try:
         if not os.environ.get("HGRP_UNSPLIT"):          # the launcher's own choice at this size
-            assert s.sweep_kernel(2) == "unsplit_strip_kernel"
+            assert s.sweep_kernel(2) == "unsplit_strip_kernel<0>"
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         r0, u0, v0, p0, e0 = s.download()
         res = s.solve(np.array([0.0, np.inf]), 2)

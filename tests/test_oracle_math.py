@@ -291,3 +291,38 @@ def test_two_gamma_star_state_against_independent_solver(oracle):
         p_ex, u_ex = star_state(rL, 0.0, pL, rR, uR, pR, gL, gR)
         assert abs(ps - p_ex) <= 1e-10 * p_ex
         assert abs(us - u_ex) <= 1e-9 * (cL + cR)
+
+
+def _c3_inputs(states, g=1.4, tol=1e-12):
+    cl = np.sqrt(g * states[:, 2] / states[:, 0])
+    cr = np.sqrt(g * states[:, 5] / states[:, 3])
+    return [(g, g, s[1], s[4], s[2], s[5], cl[k], cr[k], 1e-9, tol) for k, s in enumerate(states)]
+
+
+def test_converge_rule_reaches_the_root(oracle):
+    """The converge stopping rule (no counterpart in the reference) ends at the
+    root of the wave-curve equation, computed here to 40 digits, to within
+    1e-12 relative or the rounding floor of the root itself (16 ulp of the
+    velocities over p |f'(p)|); the reference's rule at the same tolerance does
+    not, on near-vacuum double rarefactions, so this pin tells the two apart."""
+    states = cases.c3_states(200, seed=41, near_vacuum=60)
+    worst_conv, worst_ref, ref_over = 0.0, 0.0, 0
+    for args, s in zip(_c3_inputs(states), states):
+        p_true, pf = ex.star_pressure_mp(*s)
+        floor = 16 * 2.0 ** -52 * (abs(s[1]) + abs(s[4]) + args[6] + args[7]) / pf
+        bound = max(1e-12, floor)
+        ref = oracle.riemann_rule(*args, 500, 0)
+        conv = oracle.riemann_rule(*args, 500, 1)
+        e_ref = abs(ref[1] - p_true) / p_true
+        e_conv = abs(conv[1] - p_true) / p_true
+        assert e_conv <= bound, (s, e_conv, bound)
+        assert conv[5] <= ref[5] + 2 and conv[4] <= 1e-12         # at most a step or two more; gap within tol
+        assert conv[2:4] == ref[2:4]                                # wave types do not depend on the stop
+        worst_conv, worst_ref = max(worst_conv, e_conv / bound), max(worst_ref, e_ref)
+        ref_over += e_ref > bound
+    print(f"converge rule: worst error / bound {worst_conv:.2f}; reference rule: worst {worst_ref:.1e}, "
+          f"{ref_over} of {len(states)} states beyond the bound")
+    assert ref_over >= 3
+    # rule 0 is the reference's loop itself
+    for args in _c3_inputs(states[:50]):
+        assert oracle.riemann_rule(*args, 500, 0) == oracle.riemann(*args, 500)
