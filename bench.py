@@ -189,7 +189,7 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
         s.bench(2)
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         s.set_fluids(1.67, frac)
-        ms, ms_x, ms_y = s.bench(steps)
+        ms, ms_x, ms_y = s.bench(steps, 8)
         state = s.download()
         s.close()
         assert np.all(np.isfinite(state)), "non-finite two-fluid state"

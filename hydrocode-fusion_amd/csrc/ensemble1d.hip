@@ -673,6 +673,15 @@ extern "C" int hgrp1d_results(hgrp1d *c, int *steps, int *status, double *time_e
     return 0;
 }
 
+extern "C" int hgrp1d_set_blocks(hgrp1d *c, int bs_iface, int bs_update)
+{
+    if (!c || bs_index(bs_iface) < 0 || bs_index(bs_update) < 0) return -1001;
+    c->bs_if = bs_iface;
+    c->bs_up = bs_update;
+    drop_graph(c);              // captured launches carry the old sizes
+    return 0;
+}
+
 // Autotuner: times `probe_steps` steps from the uploaded state for every pair
 // of block sizes, twice (the faster of the two counts, so one slow sample does
 // not decide), keeps the fastest pair, and restores the uploaded state.  A

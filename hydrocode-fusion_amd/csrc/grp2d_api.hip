@@ -412,7 +412,7 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, int sample_every, double *ms_t
     int slot = 0;
     for (int s = 0; s < steps; ++s) {
         const int dirs[3] = {0, 1, 0};
-        const bool sampled = s % every == 0;
+        const bool sampled = ns > 0 && s % every == 0;
         const size_t e0 = (size_t)6 * (s / every);
         for (int w = 0; w < 3; ++w) {
             const int d = dirs[w];
@@ -839,6 +839,7 @@ static void fill_unsplit_args(hgrp2d *c, UnsplitArgs &a, double tau, double nu, 
     a.alpha = c->cfg[41];
     a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
     a.conv = c->newton_rule == HGRP_NEWTON_CONVERGE;
+    a.mlp = c->use_mlp ? c->mlp : nullptr;
     a.gas = make_gas(c->cfg[6]);
     a.status = c->status + 2 * slot;
     a.prev = prev_slot >= 0 ? c->status + 2 * prev_slot : nullptr;
@@ -858,10 +859,6 @@ static int enqueue_unsplit(hgrp2d *c, double tau, double nu, double mu, int slot
 
 static bool unsplit_supported(hgrp2d *c)
 {
-    if (c->use_mlp) {
-        printf("The unsplit scheme (config[33] = 0) has no learned starting guess.\n");
-        return false;
-    }
     if (c->two_fluid) {
         printf("The unsplit scheme (config[33] = 0) runs one gamma-law gas only.\n");
         return false;
@@ -1034,7 +1031,7 @@ extern "C" int hgrp2d_bench_unsplit(hgrp2d *c, int steps, int sample_every, doub
     for (auto &e : evs) HCHECK(hipEventCreate(&e));
     HCHECK(hipEventRecord(c->ev[0], c->stream));
     for (int s = 0; s < steps; ++s) {
-        const bool sampled = s % every == 0;
+        const bool sampled = ns > 0 && s % every == 0;
         if (sampled) HCHECK(hipEventRecord(evs[(size_t)2 * (s / every)], c->stream));
         // status slots are not chained here: the benchmark must run every step
         const int rc = enqueue_unsplit(c, tau, nu, mu, s % UNSPLIT_SLOTS, -1);

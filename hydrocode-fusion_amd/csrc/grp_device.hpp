@@ -165,15 +165,18 @@ constexpr int GUESS_ITERS = 3;
 // The converge rule (not in the reference; its loop stops at the first iterate
 // whose velocity gap is within tol).  An absolute gap of tol leaves p up to
 // tol / (p |dv/dp|) relative off the root, which near vacuum (p |dv/dp| down
-// to ~1e-3) is 1e-10 at tol = 1e-12: where the stop falls then depends on the
-// starting guess.  Under this rule, once the gap is within tol, one more Newton
-// step is taken if none has been taken yet, or if the error that the last
-// tangents give, gap / |k1 - k3|, exceeds CONV_REL * p; an iterate reached by
-// that extra step ends the loop if it meets tol (so the count stays bounded
-// where p* is ill-conditioned), else the ordinary steps go on.  Quadratic
-// convergence takes an error of e to ~e^2, so either starting guess ends at
-// the root to within rounding.
-constexpr double CONV_REL = 1e-13;
+// to ~1e-3) is 1e-10 at tol = 1e-12, and even on weak waves leaves ~1e-13:
+// where the stop falls then depends on the starting guess.  Under this rule
+// an iterate within tol gets one more Newton step if none has been taken yet,
+// or if its gap is above the rounding floor of the curves' velocities,
+// CONV_ULP (|v_L| + |v_R| + c_L + c_R) (8 ulp), or above CONV_REL p |k1 - k3|
+// (the last tangents put p more than CONV_REL off the root; the smaller
+// threshold decides, so that near vacuum, where p |dv/dp| is small, p itself is
+// converged); an iterate reached by that extra step ends the loop if it meets
+// tol (so the count stays bounded where p* is ill-conditioned), else the
+// ordinary steps go on.  Quadratic convergence takes an error of e to ~e^2, so
+// either starting guess ends at the root to within rounding.
+constexpr double CONV_ULP = 0x1p-49, CONV_REL = 1e-13;
 
 struct StarResult {
     double u_star, p_star;
@@ -294,28 +297,34 @@ __device__ __forceinline__ StarResult riemann_exact(const Gas &GL, const Gas &GR
         const bool learned_phase = st <= 1 || st == 4;
         if (st == 1 || st == 4) ++done;
         if (st == 3 || st == 5) ++n;
-        if (gap <= tol) {
-            // converge rule: an iterate that meets tol, reached from a starting value or by an
-            // ordinary step, gets one more step if none was taken or if the error is above CONV_REL
-            if (CONV && st < 4 && (learned_phase || n != N) &&
-                (st == 0 || st == 2 || gap > CONV_REL * p * fabs(k1 - k3))) {
-                st = learned_phase ? 4 : 5;
-                continue;
-            }
-            if (learned_phase) {
+        // converge rule: an iterate within tol, reached from a starting value or by an ordinary
+        // step, gets one more step if none was taken or if the error is above CONV_REL
+        const bool extra = CONV && st != 4 && st != 5 &&
+                           (st == 0 || st == 2 ||
+                            gap > fmin(CONV_ULP * (fabs(v_L) + fabs(v_R) + c_L + c_R), CONV_REL * p * fabs(k1 - k3)));
+        if (learned_phase) {
+            if (gap <= tol) {
+                if (extra) {
+                    st = 4;
+                    continue;
+                }
                 r.p_star = p;
                 r.u_star = k1 * (v_R - v_L) / (k1 - k3) + v_L;
                 r.gap = gap;
                 r.iters = done;
                 return r;
             }
-            break;
-        }
-        if (learned_phase) {
-            st = done < GUESS_ITERS ? 1 : 2;
+            // a NaN gap restarts from the reference's guess, as an unmet tolerance does
+            st = (gap > tol && done < GUESS_ITERS) ? 1 : 2;
         } else {
-            if (n == N) break;
-            st = 3;
+            // the reference's loop condition: a NaN gap ends the loop
+            if (gap > tol && n != N) {
+                st = 3;
+            } else if (extra && n != N) {
+                st = 5;
+            } else {
+                break;
+            }
         }
     }
     r.p_star = p;

@@ -267,23 +267,27 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][6
         // xf[w] is not written before the solve: it holds this wave's network inputs
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        pg = interface_guess<DIR>(a, col && f <= m, SL, SR, (uint32_t *)&xf[w][0][0]);
+        pg = interface_guess<DIR>(reload_args<SweepArgs>(), col && f <= m, SL, SR, (uint32_t *)&xf[w][0][0]);
     }
     if (col && f <= m) {
+        // the argument block is read again here and after the solve, so that
+        // none of it is held across the solve
         const Side SL = recon(v, s, hh, true);
         const Side SR = recon(vr, sr, hh, false);
-        solve_interface<DIR, NM>(a, SL, SR, (unsigned long long)line * (unsigned long long)(m + 1) + (unsigned long long)f,
-                             F, I, pg);
+        solve_interface<DIR, NM>(reload_args<SweepArgs>(), SL, SR,
+                                 (unsigned long long)line * (unsigned long long)(m + 1) + (unsigned long long)f, F, I, pg);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
     __syncthreads();
     if (own && w >= 1) {
         // the cell's values and E come back from LDS, so no register holds them across the solve
+        const SweepArgs ar = reload_args<SweepArgs>();
+        const Line<DIR> Lr(ar, line);
         double Fl[4], Il[4], vc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; vc[q] = xa[w][q][lane]; }
-        L.update_e(c, vc, xe[w][lane], Fl, F, Il, I);
+        Lr.update_e(c, vc, xe[w][lane], Fl, F, Il, I);
     }
 }
 
@@ -384,22 +388,26 @@ __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chu
     if constexpr (LEARNED) {
         const Side SL = recon(w, s, hh, true);
         const Side SR = recon(wr, sr, hh, false);
-        pg = interface_guess<1>(a, face, SL, SR, mlp_lds[(threadIdx.x >> 6) & 3]);
+        pg = interface_guess<1>(reload_args<SweepArgs>(), face, SL, SR, mlp_lds[(threadIdx.x >> 6) & 3]);
     }
     if (face) {
+        // the argument block is read again here and after the solve, so that
+        // none of it is held across the solve
         const Side SL = recon(w, s, hh, true);
         const Side SR = recon(wr, sr, hh, false);
-        solve_interface<1, NM>(a, SL, SR, (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I,
-                           pg);
+        solve_interface<1, NM>(reload_args<SweepArgs>(), SL, SR,
+                               (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I, pg);
     }
     double Fl[4], Il[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
     if (k >= 1 && own) {
+        const SweepArgs ar = reload_args<SweepArgs>();
+        const Line<1> Lr(ar, j);
         double wc[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) wc[q] = st[q][lane];
-        L.update_e(c, wc, st[4][lane], Fl, F, Il, I);
+        Lr.update_e(c, wc, st[4][lane], Fl, F, Il, I);
     }
 }
 

@@ -173,12 +173,33 @@ struct Grid {
 template <int NM>
 __device__ __forceinline__ void solve_t(const UnsplitArgs &a, int dir, const Side &SL, const Side &SR,
                                         const double TL[4], const double TR[4], unsigned long long loop_index,
-                                        double F[4], double I[4])
+                                        double F[4], double I[4], double p_guess)
 {
     int code = ifvar_bad_t(SL, SR, TL, TR, a.eps) ? 1 : 0;
-    if (grp_2d_flux_n<NM != NM_REF>(SL, SR, TL, TR, a.gas, dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, a.tau, a.eps, F, I, a.tol))
+    if (grp_2d_flux_n<NM != NM_REF>(SL, SR, TL, TR, a.gas, dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, a.tau, a.eps, F, I, a.tol,
+                                    p_guess))
         code = 2;
     if (code) atomicMax(&a.status[dir].fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
+}
+
+// Learned starting pressure for the interface (SL | SR) with normal x (dir 0)
+// or y (dir 1), every lane of the wave (uniform call); -1 outside NM_LEARNED,
+// where Newton will not run (the acoustic shortcut, the equal-state exit) or
+// where the lane has no interface.  The states are those normal to the
+// interface (GRP_2D_flux's rotation keeps rho and p; u_n is u or v).
+template <int NM>
+__device__ __forceinline__ double unsplit_guess(const UnsplitArgs &a, bool active, int dir, const Side &SL,
+                                                const Side &SR, uint32_t *lds)
+{
+    if constexpr (NM != NM_LEARNED) {
+        return -1.0;
+    } else {
+        const double uL = dir ? SL.v : SL.u, uR = dir ? SR.v : SR.u;
+        const double dist = sqrt((SL.rho - SR.rho) * (SL.rho - SR.rho) + (uL - uR) * (uL - uR)
+                                 + (SL.p - SR.p) * (SL.p - SR.p));
+        const bool want = active && !(dist < a.eps) && !(fabs(uL - uR) < a.tol && fabs(SL.p - SR.p) < a.tol);
+        return learned_guess_wave(a.mlp, want, SL.rho, uL, SL.p, SR.rho, uR, SR.p, a.gas.g, a.gas.g, lds);
+    }
 }
 
 }  // namespace
@@ -189,6 +210,7 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
     __shared__ double sd[W][12][64];     // side data of row c: value, x-slope, y-slope
     __shared__ double fx[W - 1][8][64];  // x-interface c+1: flux, values at t + tau
     __shared__ double gy[W - 2][8][64];  // y-interfaces of the updated rows: flux, values at t + tau
+    __shared__ uint32_t mlp_lds[NM == NM_LEARNED ? W : 1][NM == NM_LEARNED ? 64 * 8 : 1];   // network inputs
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m = a.m, n = a.n;
     const int i0 = blockIdx.x * 62, ci = i0 - 1 + lane;
@@ -259,8 +281,9 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
             }
         }
         double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
+        const double pg = unsplit_guess<NM>(a, active, pass, SL, SR, mlp_lds[NM == NM_LEARNED ? w : 0]);
         // the argument block is read again for the solve, so no scalar register holds it across
-        if (active) solve_t<NM>(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I);
+        if (active) solve_t<NM>(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I, pg);
         if (pass == 0 && w <= W - 2) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
@@ -319,6 +342,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
     __shared__ double sd[2][12][64];   // side data of rows r and r + 1 (alternating)
     __shared__ double fl[8][64];       // x-interface r: flux, values at t + tau
     __shared__ double gb[8][64];       // y-interfaces of row r: flux, values at t + tau
+    __shared__ uint32_t mlp_lds[NM == NM_LEARNED ? 64 * 8 : 1];   // network inputs
     if (a.prev != nullptr) {
         const SweepStatus p0 = a.prev[0], p1 = a.prev[1];
         if (p0.fail_key != 0ull || p0.flags != 0u || p1.fail_key != 0ull) {
@@ -384,7 +408,8 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
                 }
             }
             double Fo[4] = {0.0, 0.0, 0.0, 0.0}, Io[4] = {0.0, 0.0, 0.0, 0.0};
-            if (active) solve_t<NM>(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io);
+            const double pg = unsplit_guess<NM>(a, active, 1 - pass, SL, SR, mlp_lds);
+            if (active) solve_t<NM>(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io, pg);
             if (pass == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { gb[q][lane] = Fo[q]; gb[4 + q][lane] = Io[q]; }
@@ -431,20 +456,18 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
 }
 
 // ---- launchers -----------------------------------------------------------------
-// HGRP_UNSPLIT picks the block shape (read once, for tuning runs):
-//   1 = 8 waves (6 rows) per block, free register budget (no spills, 2
-//       waves/SIMD; 7% faster than the 4-waves/SIMD form with its spills at
-//       1024^2 and 4096^2 on MI355X);
-//   2 = 4 waves (2 rows), 4 waves/SIMD;  3 = 8 waves, 4 waves/SIMD;
-//   4 = the strip form (one wave per block walking down its rows).
-// Without it, unsplit_variant picks 1 or 4 by grid size.
+// Two forms: the tile (8 waves, 6 rows per block, the compiler's register
+// budget: no spills, 2 waves/SIMD, which its 100 KiB of LDS per block sets
+// anyway) and the strip (one wave per block walking down its rows).
+// HGRP_UNSPLIT = 1 (tile) or 4 (strip) forces one (read once, for tuning
+// runs); otherwise unsplit_variant picks by grid size.
 static int unsplit_env()
 {
     static int v = -1;
     if (v < 0) {
         const char *e = getenv("HGRP_UNSPLIT");
         v = e ? atoi(e) : 0;
-        if (v < 1 || v > 4) v = 0;
+        if (v != 1 && v != 4) v = 0;
     }
     return v;
 }
@@ -468,46 +491,35 @@ static int unsplit_variant(int m, int n)
     return strip_rows(m, n) >= 16 ? 4 : 1;
 }
 
-// rows of cells one block of the tile forms updates
-static int unsplit_rows(int v) { return v == 2 ? 2 : 6; }
-
-int unsplit_max_rows() { return 65535 * 4; }
+int unsplit_max_rows() { return 65535 * 4; }   // grid.y of the strip form at R = 4
 
 template <int NM>
 static hipError_t launch_unsplit(const UnsplitArgs &a, hipStream_t st)
 {
-    const int v = unsplit_variant(a.m, a.n);
-    if (v == 4) {
+    if (unsplit_variant(a.m, a.n) == 4) {
         const int R = strip_rows(a.m, a.n);
         hipLaunchKernelGGL(unsplit_strip_kernel<NM>, dim3((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R)),
                            dim3(64), 0, st, a, R);
-        return hipGetLastError();
-    }
-    const int rows = unsplit_rows(v);
-    const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + rows - 1) / rows));
-    switch (v) {
-    case 2: hipLaunchKernelGGL((unsplit_step_kernel<4, 4, NM>), grid, dim3(256), 0, st, a); break;
-    case 3: hipLaunchKernelGGL((unsplit_step_kernel<8, 4, NM>), grid, dim3(512), 0, st, a); break;
-    default: hipLaunchKernelGGL((unsplit_step_kernel<8, 1, NM>), grid, dim3(512), 0, st, a); break;
+    } else {
+        const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + 5) / 6));
+        hipLaunchKernelGGL((unsplit_step_kernel<8, 1, NM>), grid, dim3(512), 0, st, a);
     }
     return hipGetLastError();
 }
 
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
 {
+    if (a.mlp) return launch_unsplit<NM_LEARNED>(a, st);
     return a.conv ? launch_unsplit<NM_CONV>(a, st) : launch_unsplit<NM_REF>(a, st);
 }
 
 const char *unsplit_kernel_name(int m, int n, int nm)
 {
-    static const char *names[2][4] = {
-        {"unsplit_step_kernel<8, 1, 0>", "unsplit_step_kernel<4, 4, 0>", "unsplit_step_kernel<8, 4, 0>",
-         "unsplit_strip_kernel<0>"},
-        {"unsplit_step_kernel<8, 1, 1>", "unsplit_step_kernel<4, 4, 1>", "unsplit_step_kernel<8, 4, 1>",
-         "unsplit_strip_kernel<1>"}};
-    if (nm < 0 || nm > 1) return "";
-    const int v = unsplit_variant(m, n);
-    return names[nm][v >= 1 && v <= 4 ? v - 1 : 0];
+    static const char *names[3][2] = {{"unsplit_step_kernel<8, 1, 0>", "unsplit_strip_kernel<0>"},
+                                      {"unsplit_step_kernel<8, 1, 1>", "unsplit_strip_kernel<1>"},
+                                      {"unsplit_step_kernel<8, 1, 2>", "unsplit_strip_kernel<2>"}};
+    if (nm < 0 || nm > 2) return "";
+    return names[nm][unsplit_variant(m, n) == 4 ? 1 : 0];
 }
 
 }  // namespace hgrp

@@ -34,6 +34,7 @@ struct UnsplitArgs {
     double tau, nu, mu, eps, alpha, tol;      // tau, tau/h_x, tau/h_y, ...
     RecipDiv div_hx, div_hy;                  // division by h_x, h_y
     Gas gas;
+    const MlpPacked *mlp;                     // learned p* guess, or nullptr (the reference's guess)
     SweepStatus *status;                      // [0]: x-interfaces and the update, [1]: y-interfaces
     const SweepStatus *prev;                  // the previous step's pair (nullptr for the first)
 };

@@ -59,6 +59,7 @@ int main(int argc, char **argv)
     const int N = atoi(argv[1]), steps = atoi(argv[3]);
     const double t_mid = atof(argv[2]);
     const char *mode = argv[4];
+    setvbuf(stdout, NULL, _IOLBF, 0);   // progress lines reach a log file as they are printed
     const int xv = argc > 6 ? atoi(argv[5]) : 0, yv = argc > 6 ? atoi(argv[6]) : 0;
     if (N < 2 || N > 65536 || steps < 1 || !(t_mid >= 0.0) ||
         (strcmp(mode, "ref") && strcmp(mode, "conv") && strcmp(mode, "learned"))) {
@@ -88,7 +89,14 @@ int main(int argc, char **argv)
     if (!rc) rc = hgrp2d_h_over_smax(ctx, &h_s);
     const int k_mid = (int)floor(t_mid / (cfg[7] * h_s));
     double ms = 0.0;
-    if (!rc && k_mid > 0) rc = hgrp2d_bench(ctx, k_mid, 0, &ms, NULL, NULL);
+    for (int done = 0; !rc && done < k_mid;) {      // in chunks, with a progress line after each
+        const int chunk = k_mid - done < 1000 ? k_mid - done : 1000;
+        double part = 0.0;
+        rc = hgrp2d_bench(ctx, chunk, 0, &part, NULL, NULL);
+        done += chunk;
+        ms += part;
+        if (!rc) printf("  %d of %d steps to t = %g\n", done, k_mid, t_mid);
+    }
     if (rc) {
         fprintf(stderr, "grp2d_bench: setup failed (%d)\n", rc);
         return 1;

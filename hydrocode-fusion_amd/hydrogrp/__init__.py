@@ -219,13 +219,13 @@ class Solver2D:
 
 
 def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None,
-                converge=False):
+                converge=False, sweeps=None):
     """Whole run like the oracle's split2d(): returns (frames, N_plot, time_plot, config, status).
     newton_tol / guess_params / converge: see Solver2D.set_newton (defaults: the
-    reference's solve)."""
+    reference's solve); sweeps: (x_variant, y_variant) for Solver2D.set_sweeps."""
     config = np.array(config, dtype=np.float64)
     config[33] = 1.0
-    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge)
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge, sweeps)
 
 
 def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None,
@@ -237,7 +237,7 @@ def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, 
     return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge)
 
 
-def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False):
+def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False, sweeps=None):
     if time_plot is None:
         time_plot = np.array([0.0, config[1]])
     time_plot = np.array(time_plot, dtype=np.float64)
@@ -247,6 +247,8 @@ def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converg
     try:
         if newton_tol or guess_params is not None or converge:
             s.set_newton(newton_tol, guess_params, converge)
+        if sweeps is not None:
+            s.set_sweeps(*sweeps)
         s.upload(init["rho"], init["u"], init["v"], init["p"], init.get("e"))
         frames = np.zeros((5, N_T, m, n))
         frames[:, 0] = s.download()

@@ -122,11 +122,13 @@ int hgrp2d_download_fraction(hgrp2d *ctx, double *frac_a);
  *   which ends at the first iterate whose velocity gap is within tol.
  * HGRP_NEWTON_CONVERGE: the same loop, but an iterate whose gap is within tol
  *   gets one more Newton step if none has been taken from the starting value
- *   or if the error estimated from the last tangents, gap / |k1 - k3|, exceeds
- *   1e-13 p; an iterate that such an extra step gave ends the loop if its gap
- *   is within tol (else ordinary steps go on).  It ends at the root to within
- *   rounding whatever the starting guess; the reference's rule leaves up to
- *   tol / (p |dv/dp|) relative, ~1e-10 near vacuum at tol 1e-12. */
+ *   or if its gap is above min(2^-49 (|v_L| + |v_R| + c_L + c_R), 1e-13 p
+ *   |k1 - k3|): the rounding floor of the curves' velocities, or an error above
+ *   1e-13 p by the last tangents; an iterate that such an extra step gave ends
+ *   the loop if its gap is within tol (else ordinary steps go on).  It ends at
+ *   the root to within rounding whatever the starting guess; the reference's
+ *   rule leaves up to tol / (p |dv/dp|) relative: ~1e-13 on weak waves and
+ *   ~1e-10 near vacuum at tol 1e-12. */
 #define HGRP_NEWTON_REFERENCE 0
 #define HGRP_NEWTON_CONVERGE 1
 
@@ -302,6 +304,9 @@ int hgrp1d_results(hgrp1d *ctx, int *steps, int *status, double *time_end);
  * block sizes (64..512) of the interface and update kernels, keeps the
  * fastest, restores the uploaded data.  The chosen sizes are returned. */
 int hgrp1d_tune(hgrp1d *ctx, int probe_steps, int *bs_iface, int *bs_update);
+/* Sets the block sizes (each one of 64, 128, 256, 512) of the interface and
+ * update kernels, as hgrp1d_tune would; results do not depend on them. */
+int hgrp1d_set_blocks(hgrp1d *ctx, int bs_iface, int bs_update);
 
 /* ------------------------------------------------------------------------- */
 /* Learned starting guess for p* (no counterpart in the reference).  A 6-32-32-1 */

@@ -62,8 +62,8 @@ double orc_riemann_exact(double *U_star, double *P_star, double gammaL, double g
 
 /* The same with a stopping rule: 0 the reference's (the loop above), 1 the
  * converge rule (an iterate within tol gets one more Newton step when none has
- * been taken or when gap / |k1 - k3| > 1e-13 p, unless an extra step gave it;
- * no reference counterpart). */
+ * been taken or when its gap exceeds min(2^-49 (|v_L| + |v_R| + c_L + c_R),
+ * 1e-13 p |k1 - k3|), unless an extra step gave it; no reference counterpart). */
 double orc_riemann_exact_rule(double *U_star, double *P_star, double gammaL, double gammaR,
                               double u_L, double u_R, double p_L, double p_R,
                               double c_L, double c_R, int CRW[2],

@@ -15,7 +15,9 @@
  * orc_riemann_exact_rule adds the converge stopping rule (no counterpart in the
  * reference): an iterate within tol that the starting value or an ordinary
  * step gave gets one more Newton step when none has been taken yet or when
- * gap / |k1 - k3| (the last tangents) exceeds 1e-13 p; an iterate that such an
+ * its gap exceeds the smaller of 2^-49 (|v_L| + |v_R| + c_L + c_R), the
+ * rounding floor of the curves' velocities, and 1e-13 p |k1 - k3| (p more than
+ * 1e-13 off the root by the last tangents); an iterate that such an
  * extra step gave ends the loop if it is within tol.  Pinned by the root of the
  * wave-curve equation computed in 40-digit arithmetic (tests/test_oracle_math.py).
  */
@@ -75,8 +77,10 @@ static double riemann_core(double *U_star, double *P_star,
         if (!((gap > tol) && (n != N))) {
             /* the converge rule: an iterate within tol that a starting value or an
              * ordinary step gave gets one more step when none was taken or when
-             * the error the last tangents give exceeds 1e-13 p */
-            if (conv && !extra && n != N && (n == 0 || gap > 1e-13 * p_INT * fabs(k1 - k3)))
+             * its gap is above the rounding floor of the velocities or puts p
+             * more than 1e-13 off the root */
+            if (conv && !extra && n != N &&
+                (n == 0 || gap > fmin(0x1p-49 * (fabs(v_L) + fabs(v_R) + c_L + c_R), 1e-13 * p_INT * fabs(k1 - k3))))
                 extra = 1;
             else
                 break;

@@ -213,3 +213,14 @@ def c3_states(count, seed, near_vacuum=0):
             continue
         rows.append((1.0, 0.0, 1.0, rr, ur, pr))
     return np.array(rows)
+
+
+def write_ensemble_bin(path, n_inst=512, seed=2024):
+    """ensemble_1d(n_inst, seed) in the file format of grp1d_bench.out:
+    int32 n_inst, int32 cells[], then float64 rho[], u[], p[] (little-endian)."""
+    cells, _, rho, u, p = ensemble_1d(n_inst, seed)
+    with open(path, "wb") as f:
+        np.array([n_inst], "<i4").tofile(f)
+        cells.astype("<i4").tofile(f)
+        for a in (rho, u, p):
+            np.asarray(a, "<f8").tofile(f)

@@ -267,6 +267,23 @@ def split2d_fq1024(ref):
     np.savez_compressed(os.path.join(HERE, "split2d_fq1024.npz"), **out)
 
 
+def split2d_fq1024_devpow(_ref):
+    """The same 1024^2 whole run through the oracle built with the device's pow
+    (oracle/liboracle_devpow.so): the reference's algorithm with the one
+    operation the GPU computes differently taken from the GPU's library.  The
+    same rows, probes and sums as split2d_fq1024."""
+    from _libs import Oracle, ORACLE_DEVPOW_SO
+    m = n = 1024
+    cfg = cases.four_quadrant_config(m, n)
+    fr, npl, tp, c, st = Oracle(ORACLE_DEVPOW_SO).split2d(m, n, cases.four_quadrant(m, n), cfg, N_T=2)
+    fin = fr[:, npl - 1]
+    probes = np.load(os.path.join(HERE, "split2d_fq1024.npz"), allow_pickle=False)["probes"].astype(int)
+    out = {"sum": fin.reshape(5, -1).sum(axis=1), "l1": np.abs(fin).reshape(5, -1).sum(axis=1),
+           "probe_vals": fin[:, probes[:, 0], probes[:, 1]], "rows": fin[:, ::97, :],
+           "meta": np.array([c[5], npl, tp[npl - 1]])}
+    np.savez_compressed(os.path.join(HERE, "split2d_fq1024_devpow.npz"), **out)
+
+
 def twofluid_vectors(_ref):
     """Two-fluid interfaces through the reference's GRP_2D_flux and Q1D built
     with -DMULTIFLUID_BASICS (oracle/_ref/libhydroref_mf.so): fluid a gamma 1.4
@@ -412,7 +429,8 @@ def main():
     ref = RefLib()
     only = sys.argv[1:]
     for name, fn in (("interface", interface_vectors), ("units", interface_units), ("stages", stage_cases),
-                     ("split2d", split2d_cases), ("split2d_fq1024", split2d_fq1024), ("twofluid", twofluid_vectors),
+                     ("split2d", split2d_cases), ("split2d_fq1024", split2d_fq1024),
+                     ("split2d_fq1024_devpow", split2d_fq1024_devpow), ("twofluid", twofluid_vectors),
                      ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
         if not only or name in only:
             fn(ref)

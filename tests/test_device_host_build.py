@@ -234,3 +234,31 @@ def test_two_fluid_flux_against_multifluid_reference(dev):
         print(f"two-fluid flux dir {direction}: {np.mean(np.all(got == want, axis=1)) * 100:.1f}% rows bit-identical, "
               f"max rel {d.max():.2e} (mixed sides: {d[mixed].max():.2e})", flush=True)
         assert d.max() <= 1e-12
+
+
+def test_flux_check_codes_on_invalid_states(dev, golden_units):
+    """GRP_2D_flux on the reference's struct cases, among them negative and
+    non-finite states: the device code built for the host returns the
+    reference's check code on every case (a NaN velocity gap ends the Newton
+    loop, as the reference's `while (gap > tol && n != N)` does), and the
+    reference's results where the code is 0."""
+    from _libs import IFV_MEMBERS, SIDE15_MEMBERS
+    g = golden_units
+    idx = [IFV_MEMBERS.index(name) for name in SIDE15_MEMBERS]
+    L = np.ascontiguousarray(g["struct_in_L"][:, idx])
+    R = np.ascontiguousarray(g["struct_in_R"][:, idx])
+    nx, ny = g["struct_in_L"][:, 0], g["struct_in_L"][:, 1]
+    tau = float(g["tau"])
+    res = ("F_rho", "F_u", "F_v", "F_e", "RHO_int", "U_int", "V_int", "P_int")
+    want_cols = [IFV_MEMBERS.index(name) for name in res]
+    bad = 0
+    for k in range(len(L)):
+        out = np.zeros(9)
+        dev.hg_flux_g_many(_dp(L[k:k + 1].copy()), _dp(R[k:k + 1].copy()), 1, C.c_double(tau), C.c_double(EPS),
+                           C.c_double(nx[k]), C.c_double(ny[k]), _dp(out))
+        assert out[0] == g["struct_rc"][k], k
+        bad += out[0] != 0
+        if out[0] == 0:
+            want = g["struct_out_L"][k, want_cols]
+            assert np.max(np.abs(out[1:] - want) / np.maximum(np.abs(want), 1.0)) <= 1e-10, k
+    assert bad > 0
