@@ -5,6 +5,8 @@
 #include <stdio.h>
 #include <string.h>
 
+#include <mutex>
+
 #include "../../include/hydrocode_grp.h"
 #include "grp_device.hpp"
 #include "grp1d_device.hpp"
@@ -229,6 +231,75 @@ extern "C" int hgrp_grp2d_flux_batch(const double *L, const double *R, int count
                                             f->dir, (double *)o);
                      },
                      &fc);
+}
+
+// ---- hgrp_grp2d_flux_batch without allocations ----------------------------------
+// The staging of the sides and of the results (what hgrp_grp2d_flux_batch
+// allocates per call) lives in one fixed device array, for callers that cannot
+// allocate device memory.  Each GPU the library runs on holds its own copy
+// until the process ends; every call on a GPU uses that GPU's copy, so calls
+// that overlap on one GPU race.  Nothing in it needs clearing: a call writes
+// the sides it reads and the results it copies back.
+constexpr int NOALLOC_MAX = 1 << 16;                                  // interfaces per call
+constexpr size_t NOALLOC_L = 0, NOALLOC_R = (size_t)NOALLOC_MAX * 15, NOALLOC_OUT = (size_t)NOALLOC_MAX * 30;
+constexpr size_t NOALLOC_DOUBLES = (size_t)NOALLOC_MAX * (15 + 15 + 9);   // 19.5 MiB
+static __device__ __align__(256) double scratch[NOALLOC_DOUBLES];
+
+namespace {
+constexpr int NOALLOC_DEVICES = 64;
+std::mutex scratch_mu;
+double *scratch_dev[NOALLOC_DEVICES];   // the address of `scratch` on each device, looked up once
+
+// The current device's copy of `scratch` (its device address: the array's name
+// is a host-side stand-in and must not reach a kernel or a copy).
+int scratch_here(double **out)
+{
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return -(int)e - 1;
+    if (dev < 0 || dev >= NOALLOC_DEVICES) return -1003;
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (!scratch_dev[dev]) {
+        void *p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "hydrogrp: no device address for the fixed staging array: %s\n", hipGetErrorString(e));
+            return e != hipSuccess ? -(int)e - 1 : -2;
+        }
+        scratch_dev[dev] = (double *)p;
+    }
+    *out = scratch_dev[dev];
+    return 0;
+}
+}  // namespace
+
+extern "C" int hgrp_grp2d_flux_batch_noalloc(const double *L, const double *R, int count, double tau, double eps,
+                                             int dir, double *out)
+{
+    if (count < 0 || (count && (!L || !R || !out)) || (dir != 0 && dir != 1)) return -1001;
+    if (count > NOALLOC_MAX) {
+        fprintf(stderr, "hydrogrp: hgrp_grp2d_flux_batch_noalloc: %d interfaces, at most %d per call\n", count,
+                NOALLOC_MAX);
+        return -1002;
+    }
+    if (count == 0) return 0;
+    double *base = nullptr;
+    const int rc = scratch_here(&base);
+    if (rc) return rc;
+    double *dL = base + NOALLOC_L, *dR = base + NOALLOC_R, *dO = base + NOALLOC_OUT;
+    const size_t nb = (size_t)count * 15 * 8;
+    hipError_t e = hipMemcpy(dL, L, nb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(dR, R, nb, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return -(int)e - 1;
+    hipLaunchKernelGGL(flux_batch_kernel, dim3((count + 127) / 128), dim3(128), 0, 0, dL, dR, count, tau, eps, dir, dO);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e == hipSuccess) e = hipMemcpy(out, dO, (size_t)count * 9 * 8, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) {
+        fprintf(stderr, "hydrogrp: hgrp_grp2d_flux_batch_noalloc: %s\n", hipGetErrorString(e));
+        return -(int)e - 1;
+    }
+    return 0;
 }
 
 struct FluxTfCtx { int count; double tau, eps; int dir; double ga, gb; };

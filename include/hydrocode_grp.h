@@ -172,6 +172,15 @@ const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
 int hgrp_grp2d_flux_batch(const double *L, const double *R, int count, double tau,
                           double eps, int dir, double *out);
 
+/* hgrp_grp2d_flux_batch without allocating device memory: the same arguments
+ * and results, the staging held in a fixed device array of 65536 interfaces
+ * (19.5 MiB per GPU, kept until the process ends).  No counterpart in the
+ * reference; for callers that cannot allocate (code under graph capture,
+ * serving loops).  count above 65536: -1002 and a message on stderr.  Calls on
+ * one GPU share its array: calls that overlap on one GPU race. */
+int hgrp_grp2d_flux_batch_noalloc(const double *L, const double *R, int count, double tau,
+                                  double eps, int dir, double *out);
+
 /* GRP_2D_flux with the reference's multi-fluid members
  * (src/flux_calc/flux_solver.c:158-237 with -DMULTIFLUID_BASICS, its branch at
  * :203-235; Q1D's z_a at linear_grp_solver_Edir_Q1D.c:55-61) as the two-fluid
