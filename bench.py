@@ -25,10 +25,11 @@ over the same window (the run that gives `value` carries no such events);
 oracle/_ref, OpenMP; the C99 restatement in oracle/ where that build is absent)
 on a few steps of the same 4096^2 workload from the same t = 0.15 state.
 `ensemble_1d` runs 512 independent 1D GRP problems of uneven sizes through the
-batched engine; `two_fluid` runs the shock-bubble interaction (4096 x 1024)
+batched engine; `batch_2d` 128 independent 2D problems of uneven sizes through
+the batched 2D engine, against the same grids solved one at a time; `two_fluid` runs the shock-bubble interaction (4096 x 1024)
 with and without the learned guess; `unsplit` runs the four-quadrant workload
 at 1024^2 through the unsplit scheme (config[33] = 0) with its own roofline and
-CPU baseline.
+CPU baseline, and its learned guess against the reference's at tol 1e-12.
 """
 import argparse
 import json
@@ -41,6 +42,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "hydrocode-fusion_amd"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
 
+METRIC = "2D GRP FP64 cell-updates/s; learned-p* vs plain GPU at equal L1 error"
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
 SWEEP_BYTES_PER_CELL = 128.0   # 384 B per cell-update / 3 sweeps
 M = N = 4096
@@ -144,6 +146,21 @@ def newton_iterations():
             "p_star_max_rel_difference": float(dp.max())}
 
 
+ENSEMBLE_BYTES_PER_CELL_UPDATE = 96.0   # 1D: rho u p and their slopes, read and written
+
+
+def measured_bytes_per_cell_update():
+    """HBM bytes per 1D cell-update of the ensemble's two wide kernels from the
+    committed rocprofv3 summary (profiles/grp1d_ensemble512_trace.md), or None."""
+    path = os.path.join(ROOT, "profiles", "grp1d_ensemble512_trace.md")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_cell_update:"):
+            return float(line.split(":", 1)[1])
+    return None
+
+
 def ensemble_leg(n_inst=512, seed=2024):
     """The batched 1D workload: n_inst independent Riemann problems, cell counts
     uniform in [256, 16384], states from the learned-guess training
@@ -161,11 +178,66 @@ def ensemble_leg(n_inst=512, seed=2024):
     ms, cu = ens.run()
     steps, status, _ = ens.results()
     ens.close()
+    achieved = ENSEMBLE_BYTES_PER_CELL_UPDATE * cu / (ms / 1000.0) / 1e9
     return {"workload": f"{n_inst} independent 1D GRP Riemann problems, {int(cells.sum())} cells "
                         f"(N uniform in [256, 16384], seed {seed}), FP64, t_end 0.15, per-instance CFL",
             "value": cu / (ms / 1000.0), "unit": "cell-updates/s", "cell_updates": int(cu), "ms": ms,
             "max_steps": int(steps.max()), "block_sizes": {"iface": bs[0], "update": bs[1]},
-            "ended_by_check": int(np.sum(status != 0))}
+            "ended_by_check": int(np.sum(status != 0)),
+            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell_update": ENSEMBLE_BYTES_PER_CELL_UPDATE,
+                         "traffic_bytes_per_cell_update": measured_bytes_per_cell_update()}}
+
+
+def batch2d_leg(n_grids=128, seed=7, t_all=0.1):
+    """Independent 2D problems of uneven sizes: n_grids four-quadrant grids
+    with m, n uniform in [64, 512] (seeded), each to t_all with its own frozen
+    time step, learned guess, Newton tolerance 1e-12 with the converge rule.
+    The batch (hgrp2db_*, one launch per sweep for every grid, launch variants
+    from its tuner) against the same grids solved one after another by the
+    one-grid solver (hgrp2d_solve); both rates count cells x completed steps
+    over the wall time of the step loops."""
+    import numpy as np
+    import hydrogrp
+    from hydrogrp import mlp
+    import cases
+    from _libs import make_config
+    rng = np.random.default_rng(seed)
+    sizes = [(int(a), int(b)) for a, b in rng.integers(64, 513, (n_grids, 2))]
+    inits = [cases.four_quadrant(m, n) for m, n in sizes]
+    params = mlp.load()
+    cfg = make_config(t_all=t_all, bc_x=-4, bc_y=-4, split=1)
+    b = hydrogrp.Batch2D([(m, n, 1.0 / m, 1.0 / n) for m, n in sizes], cfg)
+    try:
+        b.set_newton(NEWTON_TOL, params, converge=True)
+        b.upload(inits)
+        xv, yv, _, _ = b.tune(4)
+        t0 = time.perf_counter()
+        ms, cu = b.run()
+        wall = time.perf_counter() - t0
+        k, st, _, steps = b.results()
+    finally:
+        b.close()
+    cu_one, secs_one = 0, 0.0
+    for (m, n), init in zip(sizes, inits):
+        c1 = make_config(t_all=t_all, h_x=1.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-4, split=1)
+        s = hydrogrp.Solver2D(m, n, c1)
+        try:
+            s.set_newton(NEWTON_TOL, params, converge=True)
+            s.upload(init["rho"], init["u"], init["v"], init["p"])
+            t0 = time.perf_counter()
+            res = s.solve(np.array([0.0, t_all]), 2)
+            secs_one += time.perf_counter() - t0
+            cu_one += m * n * res.k                      # k = the steps taken (config[5])
+        finally:
+            s.close()
+    return {"workload": f"{n_grids} four-quadrant grids, m and n uniform in [64, 512] (seed {seed}), "
+                        f"{sum(m * n for m, n in sizes)} cells, to t = {t_all} each with its own tau, FP64, "
+                        "learned guess, Newton tol 1e-12 with the converge rule",
+            "unit": "cell-updates/s", "value": cu / wall, "device_ms": ms, "cell_updates": int(cu),
+            "variants": [xv, yv], "max_steps": int(steps.max()), "ended_by_check": int(np.sum(st != 0)),
+            "one_grid_at_a_time": {"value": cu_one / secs_one, "seconds": secs_one},
+            "speedup": (cu / wall) / (cu_one / secs_one)}
 
 
 def two_fluid_leg(m=4096, n=1024, steps=300):
@@ -209,8 +281,9 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
     with tau = CFL * h / S_max from t = 0 (2017 steps = the whole run to t = 0.3
     at 1024^2), device time from HIP events; roofline of the step kernel
     against HBM peak with 208 B per cell (rho u v p E and the x- and y-slopes,
-    read and written); and the reference's unsplit loop on the host cores on a
-    bounded number of steps."""
+    read and written); the learned guess against the reference's, both at Newton
+    tolerance 1e-12 with the converge rule, over the same steps; and the
+    reference's unsplit loop on the host cores on a bounded number of steps."""
     import numpy as np
     import hydrogrp
     import cases
@@ -225,15 +298,30 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
     ms, ms_step = s.bench_unsplit(steps, 8)
     kernel = s.sweep_kernel(2)
     state = s.download()
+    # the learned guess against the reference's guess, both at tol 1e-12 with the converge rule
+    from hydrogrp import mlp
+    lvp = {}
+    for name, params in (("learned", mlp.load()), ("plain", None)):
+        s.set_newton(NEWTON_TOL, params, converge=True)
+        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        ms_f, ms_step_f = s.bench_unsplit(steps, 8)
+        lvp[name] = {"value": m * n * steps / (ms_f / 1000.0), "ms_per_step": ms_f / steps,
+                     "ms_per_launch": ms_step_f, "kernel": s.sweep_kernel(2), "state": s.download()}
     s.close()
     assert np.all(np.isfinite(state)), "non-finite state after the unsplit benchmark"
+    a, b = lvp["learned"].pop("state"), lvp["plain"].pop("state")
+    learned_vs_plain = {"newton": "tol 1e-12, converge rule", "learned": lvp["learned"], "plain": lvp["plain"],
+                        "speedup": lvp["learned"]["value"] / lvp["plain"]["value"],
+                        "rel_l1_difference": float((np.abs(a - b).sum(axis=(1, 2))
+                                                    / np.abs(b).sum(axis=(1, 2))).max())}
     achieved = UNSPLIT_BYTES_PER_CELL * m * n / (ms_step / 1000.0) / 1e9
     out = {"workload": f"2D four-quadrant Riemann problem {m}x{n}, unsplit GRP (one step per tau), FP64, "
                        f"{steps} steps from t=0",
            "value": m * n * steps / (ms / 1000.0), "unit": "cell-updates/s", "ms_per_step": ms / steps,
            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": measured_traffic(kernel, f"fourquadrant{m}"), "ms_per_launch": ms_step}}
+                        "traffic": measured_traffic(kernel, f"fourquadrant{m}"), "ms_per_launch": ms_step},
+           "learned_vs_plain": learned_vs_plain}
     if with_cpu:
         run2d, kind, label, threads = _cpu_solver(split=False)
         h = 1.0 / m
@@ -366,7 +454,7 @@ def main():
     copy_gbs = 2.0 * nbytes / (hydrogrp.copy_probe(nbytes, 10) / 1000.0) / 1e9
     tag = f"fourquadrant{M}_t015" if args.steps else None
     line = {
-        "metric": "2D GRP FP64 cell-updates/s",
+        "metric": METRIC,
         "value": learned["value"],
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -394,6 +482,9 @@ def main():
     if not args.no_ensemble:
         line["ensemble_1d"] = ensemble_leg()
         note("1D ensemble leg")
+    if not args.no_ensemble:
+        line["batch_2d"] = batch2d_leg()
+        note("batched 2D leg")
     if not args.no_two_fluid:
         line["two_fluid"] = two_fluid_leg()
         note("two-fluid leg")

@@ -16,280 +16,9 @@
 #include <stdlib.h>
 #include "grp_device.hpp"
 #include "sweep2d.hpp"
+#include "sweep2d_body.hpp"
 
 namespace hgrp {
-
-__device__ __forceinline__ bool sweep_halted(const SweepArgs &a)
-{
-    if (a.prev == nullptr) return false;
-    const SweepStatus pv = *a.prev;
-    if (pv.fail_key != 0ull || pv.flags != 0u) {
-        if (threadIdx.x == 0) atomicOr(&a.status->flags, 2u);
-        return true;
-    }
-    return false;
-}
-
-// One grid line along the sweep direction.  DIR 0: x-line i (cells j = 0..m-1,
-// stride ld); DIR 1: y-line j (cells i = 0..n-1, stride 1).  Index -1 and cnt
-// are the ghost cells.
-template <int DIR>
-struct Line {
-    const SweepArgs &a;
-    size_t base, stride;
-    int cnt, line;
-    static constexpr int NV = DIR == 0 ? 1 : 2;   // normal velocity: u in x, v in y
-
-    __device__ Line(const SweepArgs &args, int ln) : a(args), line(ln)
-    {
-        if (DIR == 0) { base = (size_t)ln; stride = (size_t)args.ld; cnt = args.m; }
-        else          { base = (size_t)ln * (size_t)args.ld; stride = 1; cnt = args.n; }
-    }
-    __device__ size_t at(int k) const { return base + (size_t)k * stride; }
-
-    __device__ void cell(int k, double w[4]) const
-    {
-        const size_t o = at(k);
-        const double *f = a.st + o;
-        w[0] = f[0]; w[1] = f[a.plane]; w[2] = f[2 * a.plane]; w[3] = f[3 * a.plane];
-    }
-    // value of cell k, ghost values at k == -1 / k == cnt (bound_cond_slope_limiter_x.c:34-82)
-    __device__ void value(int k, double w[4]) const
-    {
-        if (k >= 0 && k < cnt) { cell(k, w); return; }
-        const bool lo = k < 0;
-        switch (a.bc) {
-        case BC_INITIAL: {
-            const double *b = (lo ? a.bnd_lo : a.bnd_hi) + (size_t)line * 4;
-            w[0] = b[0]; w[1] = b[1]; w[2] = b[2]; w[3] = b[3];
-            return;
-        }
-        case BC_PERIODIC:
-            cell(lo ? cnt - 1 : 0, w);
-            return;
-        case BC_REFLECT:
-            cell(lo ? 0 : cnt - 1, w);
-            w[NV] = -w[NV];
-            return;
-        case BC_REFLECT_FREE:
-            cell(lo ? 0 : cnt - 1, w);
-            if (lo) w[NV] = -w[NV];
-            return;
-        default:   // BC_FREE
-            cell(lo ? 0 : cnt - 1, w);
-            return;
-        }
-    }
-    // previous slopes of cell k (read by minmod3 only)
-    __device__ void prev_slopes(int k, double sp[4]) const
-    {
-        const size_t o = at(k);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sp[q] = a.limiter3 ? a.s[o + q * a.plane] : 0.0;
-    }
-    // limited slope from the neighbours and the previous slope sp (slope_limiter_2D_x.c:177-211)
-    __device__ void limited_sp(const double vm[4], const double v0[4], const double vp[4], const double sp[4],
-                               double s[4]) const
-    {
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const double s_L = div_by(v0[q] - vm[q], a.div_h);
-            const double s_R = div_by(vp[q] - v0[q], a.div_h);
-            s[q] = a.limiter3 ? minmod3(a.alpha * s_L, a.alpha * s_R, sp[q]) : minmod2(s_L, s_R);
-        }
-    }
-    __device__ void limited(int k, const double vm[4], const double v0[4], const double vp[4], double s[4]) const
-    {
-        double sp[4];
-        prev_slopes(k, sp);
-        limited_sp(vm, v0, vp, sp, s);
-    }
-    __device__ void limited_at(int k, double s[4]) const
-    {
-        double vm[4], v0[4], vp[4];
-        value(k - 1, vm); value(k, v0); value(k + 1, vp);
-        limited(k, vm, v0, vp, s);
-    }
-    // ghost slopes (bound_cond_slope_limiter_x.c:94-109); zero where the reference leaves them unset
-    __device__ void ghost_slope(bool lo, double s[4]) const
-    {
-        s[0] = s[1] = s[2] = s[3] = 0.0;
-        double t[4];
-        switch (a.bc) {
-        case BC_REFLECT:
-            limited_at(lo ? 0 : cnt - 1, t);
-            s[NV] = t[NV];
-            break;
-        case BC_PERIODIC:
-            limited_at(lo ? cnt - 1 : 0, s);
-            break;
-        case BC_REFLECT_FREE:
-            if (lo) { limited_at(0, t); s[NV] = t[NV]; }
-            break;
-        default:
-            break;
-        }
-    }
-    // conservative update of cell k and slope refresh (grp_solver_2D_split_EUL_source.c:236-254)
-    __device__ void update(int k, const double w[4], const double Fl[4], const double Fr[4],
-                           const double Il[4], const double Ir[4]) const
-    {
-        update_e(k, w, a.st[at(k) + 4 * a.plane], Fl, Fr, Il, Ir);
-    }
-    // the same with E of the cell already loaded
-    __device__ void update_e(int k, const double w[4], const double e, const double Fl[4], const double Fr[4],
-                             const double Il[4], const double Ir[4]) const
-    {
-        const size_t o = at(k);
-        const double nu = a.nu;
-        const double mom_x = w[0] * w[1] - nu * (Fr[1] - Fl[1]);
-        const double mom_y = w[0] * w[2] - nu * (Fr[2] - Fl[2]);
-        const double ene = w[0] * e - nu * (Fr[3] - Fl[3]);
-        const double rho = w[0] + -nu * (Fr[0] - Fl[0]);
-        const double u = mom_x / rho;
-        const double v = mom_y / rho;
-        const double p = (ene - 0.5 * mom_x * u - 0.5 * mom_y * v) * a.gas.gm1();
-        double *f = a.st_o + o;
-        const size_t P = a.plane;
-        f[0] = rho; f[P] = u; f[2 * P] = v; f[4 * P] = ene / rho; f[3 * P] = p;
-        if (p < a.eps || rho < a.eps) atomicOr(&a.status->flags, 1u);
-#pragma unroll
-        for (int q = 0; q < 4; ++q) a.s_o[o + q * P] = div_by(Ir[q] - Il[q], a.div_h);
-    }
-};
-
-template <int DIR, int NM>
-__device__ __forceinline__ void solve_interface(const SweepArgs &a, const Side &SL, const Side &SR,
-                                                unsigned long long loop_index, double F[4], double I[4],
-                                                double p_guess = -1.0)
-{
-    int code = ifvar_bad(SL, SR, a.eps) ? 1 : 0;
-    if (grp_2d_flux<DIR, NM != NM_REF>(SL, SR, a.gas, a.tau, a.eps, F, I, a.tol, p_guess)) code = 2;
-    if (code) atomicMax(&a.status->fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
-}
-
-// Learned starting pressure for the interface (SL | SR) normal to DIR, for
-// every lane of the wave (uniform call); -1 where Newton will not run (the
-// Q1D acoustic shortcut dist < eps, the solver's equal-state exit) or the lane
-// has no interface.
-template <int DIR>
-__device__ __forceinline__ double interface_guess(const SweepArgs &a, bool active, const Side &SL, const Side &SR,
-                                                  uint32_t *lds)
-{
-    const double uL = DIR == 0 ? SL.u : SL.v, uR = DIR == 0 ? SR.u : SR.v;
-    const double dist = sqrt((SL.rho - SR.rho) * (SL.rho - SR.rho) + (uL - uR) * (uL - uR)
-                             + (SL.p - SR.p) * (SL.p - SR.p));
-    // Newton runs only past the acoustic shortcut and the equal-state exit of the exact solver
-    const bool want = active && !(dist < a.eps) && !(fabs(uL - uR) < a.tol && fabs(SL.p - SR.p) < a.tol);
-    return learned_guess_wave(a.mlp, want, SL.rho, uL, SL.p, SR.rho, uR, SR.p, a.gas.g, a.gas.g, lds);
-}
-
-// Line-parallel sweep along DIR: a block of W waves covers 64 consecutive
-// grid lines of the sweep direction (lane: line; DIR 0: x-line i, DIR 1:
-// y-line j) and the cells c0-1 .. c0+W-2 along them (wave w: cell c = c0-1+w).
-// Neighbours along the sweep live in the next wave, so values, slopes and then
-// fluxes are exchanged through LDS (one array each: three barriers per block).
-// Wave w solves interface c+1 (between cells c and c+1); waves 1..W-1 update
-// cells c0 .. c0+W-2.  The 64 lanes of a wave lie across the sweep, so an
-// interface front (a shock or contact crossing the lines) is met by the whole
-// wave together and the GRP branches stay coherent.  Used for DIR 0, where the
-// lanes read consecutive doubles.  (For DIR 1 each lane would read its own row,
-// stride ld: on MI355X that form ran as fast as sweep_y_kernel at 1024^2 and
-// 3x slower at 4096^2, where the rows no longer stay in cache.)
-template <int DIR, int W, int NM>
-__device__ __forceinline__ void line_sweep(const SweepArgs &a, double (*xa)[4][64], double (*xb)[4][64],
-                                           double (*xf)[4][64], double (*xi)[4][64], double (*xe)[64])
-{
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int line = blockIdx.x * 64 + lane;
-    const bool col = line < (DIR == 0 ? a.n : a.m);
-    const Line<DIR> L(a, col ? line : 0);
-    const int m = L.cnt;                          // cells along the sweep
-    const int j0 = blockIdx.y * (W - 1);
-    const int c = j0 - 1 + w;                     // owned cell; -1 and m are ghosts
-    const double hh = a.hh;
-
-    // every global load of the owned cell up front: values, previous slopes, E
-    const bool own = col && c >= 0 && c < m;
-    double v[4] = {1.0, 0.0, 0.0, 1.0}, vl[4], vr[4], sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
-    if (col && c <= m) L.value(c, v);
-    if (own) {
-        L.prev_slopes(c, sp);
-        e_c = a.st[L.at(c) + 4 * a.plane];
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
-    xe[w][lane] = e_c;
-    if (col && w == 0 && c >= 0) L.value(c - 1, vl);
-    // the last wave also limits cell c + 1 (the right side of its interface): its loads go out now too
-    double vrr[4], sp1[4];
-    const bool next_own = col && w == W - 1 && c + 1 < m;
-    if (col && w == W - 1 && c + 1 <= m) L.value(c + 1, vr);
-    if (next_own) {
-        L.value(c + 2, vrr);
-        L.prev_slopes(c + 1, sp1);
-    }
-    // the predecessor's status is read while the loads above are in flight
-    // (they stay in bounds either way); the whole block returns together
-    if (sweep_halted(a)) return;
-    __syncthreads();
-    if (w > 0)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) vl[q] = xa[w - 1][q][lane];
-    if (w < W - 1)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) vr[q] = xa[w + 1][q][lane];
-
-    double s[4] = {0.0, 0.0, 0.0, 0.0}, sr[4] = {0.0, 0.0, 0.0, 0.0};
-    if (col) {
-        if (own) L.limited_sp(vl, v, vr, sp, s);
-        else if (c == -1) L.ghost_slope(true, s);
-        else if (c == m) L.ghost_slope(false, s);
-        if (w == W - 1) {
-            if (next_own) {
-                L.limited_sp(v, vr, vrr, sp1, sr);
-            } else if (c + 1 == m) {
-                L.ghost_slope(false, sr);
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) xb[w][q][lane] = s[q];
-    __syncthreads();
-    if (w < W - 1)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) sr[q] = xb[w + 1][q][lane];
-
-    double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
-    const int f = c + 1;
-    double pg = -1.0;
-    if constexpr (NM == NM_LEARNED) {
-        // xf[w] is not written before the solve: it holds this wave's network inputs
-        const Side SL = recon(v, s, hh, true);
-        const Side SR = recon(vr, sr, hh, false);
-        pg = interface_guess<DIR>(reload_args<SweepArgs>(), col && f <= m, SL, SR, (uint32_t *)&xf[w][0][0]);
-    }
-    if (col && f <= m) {
-        // the argument block is read again here and after the solve, so that
-        // none of it is held across the solve
-        const Side SL = recon(v, s, hh, true);
-        const Side SR = recon(vr, sr, hh, false);
-        solve_interface<DIR, NM>(reload_args<SweepArgs>(), SL, SR,
-                                 (unsigned long long)line * (unsigned long long)(m + 1) + (unsigned long long)f, F, I, pg);
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
-    __syncthreads();
-    if (own && w >= 1) {
-        // the cell's values and E come back from LDS, so no register holds them across the solve
-        const SweepArgs ar = reload_args<SweepArgs>();
-        const Line<DIR> Lr(ar, line);
-        double Fl[4], Il[4], vc[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; vc[q] = xa[w][q][lane]; }
-        Lr.update_e(c, vc, xe[w][lane], Fl, F, Il, I);
-    }
-}
 
 // X sweep, row-parallel form (line_sweep<0>): lanes are 64 consecutive i,
 // waves the cell rows j0-1 .. j0+W-2.
@@ -301,114 +30,19 @@ __global__ void __launch_bounds__(64 * W, MINW) sweep_x_rows_kernel(SweepArgs a)
     __shared__ double xf[W][4][64];   // fluxes
     __shared__ double xi[W][4][64];   // interface values at t + tau
     __shared__ double xe[W][64];      // E of the owned cell (read back for the update)
-    line_sweep<0, W, NM>(a, xa, xb, xf, xi, xe);
+    line_sweep<0, W, NM>(a, KernargArgs{}, blockIdx.x, blockIdx.y, xa, xb, xf, xi, xe);
 }
 
-// Y sweep: a wave holds 64/G y-lines (rows j) of G lanes each; lane k of a
-// row owns cell c0-1+k, computes interface c0+k and updates cell c0-1+k for
-// k >= 1 (G-1 cells per row).  With G = 64 a wave is one stretch of one line;
-// smaller G stacks rows, so that a front lying along y (constant i) meets
-// fewer waves, each with more lanes on it.
+// Y sweep (y_sweep): blocks of 4 waves, wave index = global wave number.
 template <int MINW, int NM, int G = 64>
 __global__ void __launch_bounds__(256, MINW) sweep_y_kernel(SweepArgs a, int chunks)
 {
-    static_assert(G == 64 || G == 32 || G == 16 || G == 8, "lanes per row");
-    constexpr int ROWS = 64 / G;
     constexpr bool LEARNED = NM == NM_LEARNED;
     __shared__ uint32_t mlp_lds[LEARNED ? 4 : 1][LEARNED ? 64 * 8 : 1];
-    __shared__ double stash[4][5][64];   // this lane's cell values and E, read back for the update
-    const int lane = threadIdx.x & 63, k = lane % G;
+    __shared__ double stash[4][5][64];   // each wave's cell values and E, read back for the update
+    const int wv = (threadIdx.x >> 6) & 3;
     const long wave = ((long)blockIdx.x * blockDim.x + threadIdx.x) >> 6;
-    const int jg = (int)(wave / chunks);
-    if (jg * ROWS >= a.m) return;                      // uniform per wave
-    const int j = jg * ROWS + lane / G;
-    const bool rowv = j < a.m;
-    const int c0 = (int)(wave % chunks) * (G - 1);
-    const int n = a.n;
-    const int c = c0 - 1 + k;
-    const Line<1> L(a, rowv ? j : 0);
-    const double hh = a.hh;
-
-    const bool own = rowv && c >= 0 && c < n;
-    // every global load of the wave up front, so that it pays one memory
-    // latency: the owned cell (values, previous slopes, E); the first lane of a
-    // row also the cell below it; the last lane the two cells above it and the
-    // previous slopes of the first (it limits that cell, the right side of its
-    // interface)
-    double w[4] = {1.0, 0.0, 0.0, 1.0}, sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
-    if (rowv && c <= n) L.value(c, w);
-    if (own) {
-        L.prev_slopes(c, sp);
-        e_c = a.st[L.at(c) + 4 * a.plane];
-    }
-    const bool first = rowv && k == 0 && c >= 0, last = rowv && k == G - 1 && c + 1 <= n;
-    const bool last_own = rowv && k == G - 1 && c + 1 < n;
-    double wl_e[4], wr_e[4], wrr[4], sp1[4];
-    if (first) L.value(c - 1, wl_e);
-    if (last) L.value(c + 1, wr_e);
-    if (last_own) {
-        L.value(c + 2, wrr);
-        L.prev_slopes(c + 1, sp1);
-    }
-    // the predecessor's status is read while the loads above are in flight
-    if (sweep_halted(a)) return;
-    // each lane reads back only its own slots: no barrier needed
-    double(*st)[64] = stash[(threadIdx.x >> 6) & 3];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) st[q][lane] = w[q];
-    st[4][lane] = e_c;
-    // neighbours along the row from the next lanes; a row's end lanes take the loaded ones
-    double wl[4], wr[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { wl[q] = __shfl_up(w[q], 1); wr[q] = __shfl_down(w[q], 1); }
-    if (first)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wl[q] = wl_e[q];
-    if (last)
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wr[q] = wr_e[q];
-
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    if (own) L.limited_sp(wl, w, wr, sp, s);
-    else if (rowv && c == -1) L.ghost_slope(true, s);
-    else if (rowv && c == n) L.ghost_slope(false, s);
-
-    double sr[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) sr[q] = __shfl_down(s[q], 1);
-    if (rowv && k == G - 1) {
-        if (last_own) L.limited_sp(w, wr, wrr, sp1, sr);
-        else if (c + 1 == n) L.ghost_slope(false, sr);
-    }
-
-    double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
-    const int f = c + 1;
-    const bool face = rowv && f <= n;
-    double pg = -1.0;
-    if constexpr (LEARNED) {
-        const Side SL = recon(w, s, hh, true);
-        const Side SR = recon(wr, sr, hh, false);
-        pg = interface_guess<1>(reload_args<SweepArgs>(), face, SL, SR, mlp_lds[(threadIdx.x >> 6) & 3]);
-    }
-    if (face) {
-        // the argument block is read again here and after the solve, so that
-        // none of it is held across the solve
-        const Side SL = recon(w, s, hh, true);
-        const Side SR = recon(wr, sr, hh, false);
-        solve_interface<1, NM>(reload_args<SweepArgs>(), SL, SR,
-                               (unsigned long long)j * (unsigned long long)(n + 1) + (unsigned long long)f, F, I, pg);
-    }
-    double Fl[4], Il[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) { Fl[q] = __shfl_up(F[q], 1); Il[q] = __shfl_up(I[q], 1); }
-    if (k >= 1 && own) {
-        const SweepArgs ar = reload_args<SweepArgs>();
-        const Line<1> Lr(ar, j);
-        double wc[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) wc[q] = st[q][lane];
-        Lr.update_e(c, wc, st[4][lane], Fl, F, Il, I);
-    }
+    y_sweep<NM, G>(a, KernargArgs{}, wave, chunks, stash[wv], mlp_lds[wv]);
 }
 
 // min over cells of min(h_x, h_y) / (|c| + |u| + |v|)  (grp_solver_2D_split_EUL_source.c:185-191).

@@ -56,6 +56,15 @@ def lib():
     L.hgrp_riemann_pvrs_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
     L.hgrp_grp2d_flux_general_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _D, _D, _DP]
     L.hgrp_edir_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
+    L.hgrp2db_create.argtypes = [C.POINTER(C.c_void_p), C.c_int, _IP, _IP, _DP, _DP, _DP]
+    L.hgrp2db_destroy.argtypes = [C.c_void_p]
+    L.hgrp2db_upload.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP]
+    L.hgrp2db_set_newton.argtypes = [C.c_void_p, _D, C.c_int, C.POINTER(C.c_float)]
+    L.hgrp2db_set_sweeps.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.hgrp2db_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP, _DP]
+    L.hgrp2db_run.argtypes = [C.c_void_p, _DP, C.POINTER(C.c_longlong)]
+    L.hgrp2db_results.argtypes = [C.c_void_p, _IP, _IP, _DP, _IP]
+    L.hgrp2db_download.argtypes = [C.c_void_p, _DP, _DP, _DP, _DP, _DP]
     L.hgrp_device_name.argtypes = [C.c_char_p, C.c_int]
     L.hgrp_copy_probe.argtypes = [C.c_size_t, C.c_int, _DP]
     _FP = C.POINTER(C.c_float)
@@ -264,6 +273,78 @@ def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converg
     config[5] = res.k
     time_plot[res.nt] = res.time_c
     return frames, res.nt + 1, time_plot, config, res
+
+
+class Batch2D:
+    """Independent 2D split problems of uneven sizes advanced together
+    (hgrp2db_* entry points).  grids: list of (m, n, h_x, h_y)."""
+
+    def __init__(self, grids, config):
+        self.m = np.ascontiguousarray([g[0] for g in grids], dtype=np.int32)
+        self.n = np.ascontiguousarray([g[1] for g in grids], dtype=np.int32)
+        hx = np.ascontiguousarray([g[2] for g in grids], dtype=np.float64)
+        hy = np.ascontiguousarray([g[3] for g in grids], dtype=np.float64)
+        self.config = np.ascontiguousarray(config, dtype=np.float64)
+        assert self.config.size == N_CONF
+        self.offsets = np.concatenate([[0], np.cumsum(self.m.astype(np.int64) * self.n)])
+        self._h = C.c_void_p()
+        _check(lib().hgrp2db_create(C.byref(self._h), len(self.m), self.m.ctypes.data_as(_IP),
+                                    self.n.ctypes.data_as(_IP), _p(hx), _p(hy), _p(self.config)), "hgrp2db_create")
+
+    def close(self):
+        if self._h:
+            lib().hgrp2db_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, states):
+        """states: one dict of rho, u, v, p (m x n arrays) per grid."""
+        cat = [np.ascontiguousarray(np.concatenate([np.asarray(s[f], dtype=np.float64).ravel() for s in states]))
+               for f in ("rho", "u", "v", "p")]
+        assert cat[0].size == self.offsets[-1]
+        _check(lib().hgrp2db_upload(self._h, *(_p(a) for a in cat)), "hgrp2db_upload")
+
+    def set_newton(self, tol, params=None, converge=False):
+        rule = NEWTON_CONVERGE if converge or params is not None else NEWTON_REFERENCE
+        fp = None
+        if params is not None:
+            pa = np.ascontiguousarray(params, dtype=np.float32)
+            self._params = pa
+            fp = pa.ctypes.data_as(C.POINTER(C.c_float))
+        _check(lib().hgrp2db_set_newton(self._h, tol, rule, fp), "hgrp2db_set_newton")
+
+    def set_sweeps(self, x_variant, y_variant):
+        _check(lib().hgrp2db_set_sweeps(self._h, x_variant, y_variant), "hgrp2db_set_sweeps")
+
+    def tune(self, steps=4):
+        xv, yv, ms = C.c_int(), C.c_int(), np.zeros(2)
+        _check(lib().hgrp2db_tune(self._h, steps, C.byref(xv), C.byref(yv), _p(ms)), "hgrp2db_tune")
+        return xv.value, yv.value, ms[0], ms[1]
+
+    def run(self):
+        ms, cu = _D(), C.c_longlong()
+        _check(lib().hgrp2db_run(self._h, C.byref(ms), C.byref(cu)), "hgrp2db_run")
+        return ms.value, cu.value
+
+    def results(self):
+        g = len(self.m)
+        k, st, steps, t = (np.zeros(g, np.int32), np.zeros(g, np.int32), np.zeros(g, np.int32), np.zeros(g))
+        _check(lib().hgrp2db_results(self._h, k.ctypes.data_as(_IP), st.ctypes.data_as(_IP), _p(t),
+                                     steps.ctypes.data_as(_IP)), "hgrp2db_results")
+        return k, st, t, steps
+
+    def download(self):
+        """The final states: a list of (5, m, n) arrays (rho u v p e)."""
+        tot = int(self.offsets[-1])
+        out = np.zeros((5, tot))
+        _check(lib().hgrp2db_download(self._h, *(_p(out[f]) for f in range(5))), "hgrp2db_download")
+        return [out[:, a:b].reshape(5, int(m), int(n))
+                for a, b, m, n in zip(self.offsets[:-1], self.offsets[1:], self.m, self.n)]
 
 
 class Ensemble1D:

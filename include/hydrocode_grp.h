@@ -161,6 +161,53 @@ int hgrp2d_tune(hgrp2d *ctx, int steps, int *x_variant, int *y_variant, double *
 const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
 
 /* ------------------------------------------------------------------------- */
+/* A batch of independent 2D split problems of uneven sizes, advanced        */
+/* together: each sweep of every grid of the batch is one launch (no         */
+/* counterpart in the reference).  Every grid runs                           */
+/* GRP_solver_2D_split_EUL_source (src/finite_volume/                        */
+/* grp_solver_2D_split_EUL_source.c:75-379) with plot times {0, t_all} and   */
+/* its own frozen tau = CFL h/S_max of its initial data: the same steps,     */
+/* state, step counter, time and status as hgrp2d_solve on that grid alone.  */
+/* ------------------------------------------------------------------------- */
+typedef struct hgrp2db hgrp2db;
+
+/* count grids; grid g has m[g] x n[g] cells of size h_x[g] x h_y[g].  config:
+ * the reference's config[] shared by all grids (indices read as for
+ * hgrp2d_create; 10 and 11 are taken from h_x, h_y; 17 and 18 must be valid
+ * boundary conditions, else ARG_ERROR - 4). */
+int hgrp2db_create(hgrp2db **ctx, int count, const int *m, const int *n, const double *h_x,
+                   const double *h_y, const double *config);
+int hgrp2db_destroy(hgrp2db *ctx);
+
+/* Initial data: the grids' m[g] x n[g] arrays (row j, column i at j*n + i)
+ * one after the other.  Computes each grid's h/S_max and its schedule. */
+int hgrp2db_upload(hgrp2db *ctx, const double *rho, const double *u, const double *v, const double *p);
+
+/* As hgrp2d_set_newton, for every grid. */
+int hgrp2db_set_newton(hgrp2db *ctx, double tol, int rule, const float *params);
+
+/* Launch variants as hgrp2d_set_sweeps (the same numbering). */
+int hgrp2db_set_sweeps(hgrp2db *ctx, int x_variant, int y_variant);
+
+/* Times every X variant, then every Y variant, over the first `steps` steps
+ * of the batch from the uploaded state (two trials each), keeps the fastest
+ * and restores the state.  ms[2]: average ms per X and per Y launch of the
+ * winners (any of the outputs may be NULL). */
+int hgrp2db_tune(hgrp2db *ctx, int steps, int *x_variant, int *y_variant, double *ms);
+
+/* Runs every grid to its end.  ms: device time of the run; cell_updates: the
+ * sum over grids of cells x completed steps.  Upload again before another run. */
+int hgrp2db_run(hgrp2db *ctx, double *ms, long long *cell_updates);
+
+/* Per grid: the step counter k (config[5] of the one-grid run), status (0, 1
+ * aborted, 2 stopped), the time reached and the completed steps.  Any output
+ * may be NULL. */
+int hgrp2db_results(hgrp2db *ctx, int *k, int *status, double *time_c, int *steps);
+
+/* Final states, laid out as for hgrp2db_upload; e (total energy per mass) may be NULL. */
+int hgrp2db_download(hgrp2db *ctx, double *rho, double *u, double *v, double *p, double *e);
+
+/* ------------------------------------------------------------------------- */
 /* Batched interface solves (host arrays in/out; for unit parity).            */
 /* Interface sides are 15 doubles: gamma, lambda_u, lambda_v, rho, u, v, p,   */
 /* d_rho, d_u, d_v, d_p, t_rho, t_u, t_v, t_p.                                */
