@@ -138,10 +138,10 @@ __global__ void __launch_bounds__(256, MINW) batch_y_kernel(BatchSweep b)
     y_sweep<NM, G>(a, BatchArgs{t.x}, (long)t.y + wv, t.z, stash[wv], mlp_lds[wv]);
 }
 
-// the one-grid launch variants (sweep2d.hip): X 0 = 8 rows at 4 waves/SIMD,
-// 1 = 8 rows at the compiler's budget, 2 = 16 rows; Y 0..4 = 16, 32, 64, 8
-// lanes per row at 4 waves/SIMD, 64 at the compiler's budget
-static int x_rows(int v) { return v == 2 ? 16 : 8; }
+// the one-grid launch variants (sweep2d.hip) X 0..2 and Y 0..4: X 0 = 8 rows
+// at 4 waves/SIMD, 1 = 8 rows at the compiler's budget, 2 = 16 rows; Y 0..4 =
+// 16, 32, 64, 8 lanes per row at 4 waves/SIMD, 64 at the compiler's budget
+static int x_cells_per_block(int v) { return v == 2 ? 15 : 7; }
 static int y_lanes_b(int v) { return v == 1 ? 32 : (v == 2 || v == 4) ? 64 : v == 3 ? 8 : 16; }
 
 template <int NM>
@@ -410,7 +410,7 @@ static int build_tiles(hgrp2db *c, int dir, int v)
     for (int r = 0; r < c->count; ++r) {
         const int i = c->order[r], m = c->m[i], n = c->n[i];
         if (dir == 0) {
-            const int rows = x_rows(v) - 1;
+            const int rows = x_cells_per_block(v);
             for (int by = 0; by < (m + rows - 1) / rows; ++by)
                 for (int bx = 0; bx < (n + 63) / 64; ++bx) t.push_back(make_int4(i, bx, by, 0));
         } else {

@@ -196,20 +196,25 @@ __device__ __forceinline__ double interface_guess(const SweepArgs &a, bool activ
 // lanes read consecutive doubles.  (For DIR 1 each lane would read its own row,
 // stride ld: on MI355X that form ran as fast as sweep_y_kernel at 1024^2 and
 // 3x slower at 4096^2, where the rows no longer stay in cache.)
-// (bx, by): the block's group of 64 lines and its run of W - 1 cells; src
-// gives the argument block again after the solve.
+// One chunk of a line-parallel sweep: the block's waves own cells
+// c0 - 1 + w (w = 0 .. W-1) of the 64 lines of group bx.  Wave w solves the
+// face right of its cell; waves 1 .. W-1 update their cells with the face left
+// of them, and wave 0 too when `carried` (with the face the previous chunk's
+// last wave left in xc[(k - 1) & 1]); with `carry` the last wave leaves its
+// face in xc[k & 1].  Returns false when the sweep is halted (only checked
+// with `first`).
+// lane, w: the thread's lane and wave (threadIdx.x & 63, threadIdx.x >> 6).
 template <int DIR, int W, int NM, class Src>
-__device__ __forceinline__ void line_sweep(const SweepArgs &a, const Src &src, int bx, int by, double (*xa)[4][64],
-                                           double (*xb)[4][64], double (*xf)[4][64], double (*xi)[4][64],
-                                           double (*xe)[64])
+__device__ __forceinline__ bool line_chunk(const SweepArgs &a, const Src &src, int lane, int w, int bx, int c0, int k,
+                                          bool first, bool carried, bool carry, double (*xa)[4][64],
+                                          double (*xb)[4][64], double (*xf)[4][64], double (*xi)[4][64],
+                                          double (*xe)[64], double (*xc)[8][64])
 {
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int line = bx * 64 + lane;
     const bool col = line < (DIR == 0 ? a.n : a.m);
     const Line<DIR> L(a, col ? line : 0);
     const int m = L.cnt;                          // cells along the sweep
-    const int j0 = by * (W - 1);
-    const int c = j0 - 1 + w;                     // owned cell; -1 and m are ghosts
+    const int c = c0 - 1 + w;                     // owned cell; -1 and m are ghosts
     const double hh = a.hh;
 
     // every global load of the owned cell up front: values, previous slopes, E
@@ -234,7 +239,7 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, const Src &src, i
     }
     // the predecessor's status is read while the loads above are in flight
     // (they stay in bounds either way); the whole block returns together
-    if (sweep_halted(a)) return;
+    if (first && sweep_halted(a)) return false;
     __syncthreads();
     if (w > 0)
 #pragma unroll
@@ -282,15 +287,67 @@ __device__ __forceinline__ void line_sweep(const SweepArgs &a, const Src &src, i
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) { xf[w][q][lane] = F[q]; xi[w][q][lane] = I[q]; }
+    if (carry && w == W - 1)                      // the next chunk's first face
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { xc[k & 1][q][lane] = F[q]; xc[k & 1][4 + q][lane] = I[q]; }
     __syncthreads();
-    if (own && w >= 1) {
+    if (own && (w >= 1 || carried)) {
         // the cell's values and E come back from LDS, so no register holds them across the solve
         const SweepArgs ar = src.get();
         const Line<DIR> Lr(ar, line);
         double Fl[4], Il[4], vc[4];
+        if (w >= 1) {
 #pragma unroll
-        for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; vc[q] = xa[w][q][lane]; }
+            for (int q = 0; q < 4; ++q) { Fl[q] = xf[w - 1][q][lane]; Il[q] = xi[w - 1][q][lane]; }
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { Fl[q] = xc[(k - 1) & 1][q][lane]; Il[q] = xc[(k - 1) & 1][4 + q][lane]; }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) vc[q] = xa[w][q][lane];
         Lr.update_e(c, vc, xe[w][lane], Fl, F, Il, I);
+    }
+    return true;
+}
+
+// Line-parallel sweep of one block: (bx, by) its group of 64 lines and its
+// run of W - 1 cells (by (W - 1) onwards, wave 0 holding the halo cell); src
+// gives the argument block again after the solve.
+template <int DIR, int W, int NM, class Src>
+__device__ __forceinline__ void line_sweep(const SweepArgs &a, const Src &src, int bx, int by, double (*xa)[4][64],
+                                           double (*xb)[4][64], double (*xf)[4][64], double (*xi)[4][64],
+                                           double (*xe)[64])
+{
+    (void)line_chunk<DIR, W, NM>(a, src, threadIdx.x & 63, threadIdx.x >> 6, bx, by * (W - 1), 0, true, false, false,
+                                 xa, xb, xf, xi, xe, nullptr);
+}
+
+// The same walking K chunks of W cells along the block's lines (by (K W - 1)
+// onwards): the first chunk's wave 0 is the halo cell as above, and in every
+// later chunk wave 0 updates its cell too, with the face that the previous
+// chunk's last wave solved (xc, double-buffered); a block then solves K W faces
+// for K W - 1 cells instead of W for W - 1.  Every LDS slot a chunk writes before
+// its first barrier is read by the previous chunk only before that chunk's last
+// barrier, so the chunks need no barrier between them.  The argument block is
+// read again for each chunk, and the lane and wave indices pass through an
+// empty asm each time, so that neither the arguments nor the LDS addresses
+// derived from them are held across the chunks' solves.
+template <int DIR, int W, int NM, int K, class Src>
+__device__ __forceinline__ void line_sweep_chunks(const Src &src, int bx, int by, double (*xa)[4][64],
+                                                  double (*xb)[4][64], double (*xf)[4][64], double (*xi)[4][64],
+                                                  double (*xe)[64], double (*xc)[8][64])
+{
+    const int s0 = by * (K * W - 1);
+    for (int k = 0; k < K; ++k) {
+        const SweepArgs a = src.get();
+        const int c0 = s0 + k * W;
+        if (c0 - 1 >= (DIR == 0 ? a.m : a.n)) break;   // uniform: the chunk lies past the last cell
+        int lane = threadIdx.x & 63;
+        int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        __asm__ volatile("" : "+v"(lane));
+        __asm__ volatile("" : "+s"(w));
+        if (!line_chunk<DIR, W, NM>(a, src, lane, w, bx, c0, k, k == 0, k > 0, k + 1 < K, xa, xb, xf, xi, xe, xc))
+            return;
     }
 }
 

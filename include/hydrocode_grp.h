@@ -140,9 +140,10 @@ int hgrp2d_download_fraction(hgrp2d *ctx, double *frac_a);
  * switches it off.  The other uses of eps are unchanged. */
 int hgrp2d_set_newton(hgrp2d *ctx, double tol, int rule, const float *params);
 
-/* Launch variants of the X sweep (0..2) and the Y sweep (0..4) (no counterpart
+/* Launch variants of the X sweep (0..4) and the Y sweep (0..4) (no counterpart
  * in the reference).  X: 0 = 8 cell rows per block at 4 waves/SIMD (the
- * default), 1 = 8 rows at the compiler's register budget, 2 = 16 rows.  Y: 0 =
+ * default), 1 = 8 rows at the compiler's register budget, 2 = 16 rows, 3 and 4
+ * = 8 rows walking 4 and 8 chunks along the lines (fewer repeated solves).  Y: 0 =
  * 4 rows of 16 lanes per wave (the default), 1 = 2 x 32, 2 = 1 x 64, 3 = 8 x 8,
  * 4 = 1 x 64 at the compiler's register budget.  Results do not depend on them. */
 int hgrp2d_set_sweeps(hgrp2d *ctx, int x_variant, int y_variant);
@@ -186,7 +187,7 @@ int hgrp2db_upload(hgrp2db *ctx, const double *rho, const double *u, const doubl
 /* As hgrp2d_set_newton, for every grid. */
 int hgrp2db_set_newton(hgrp2db *ctx, double tol, int rule, const float *params);
 
-/* Launch variants as hgrp2d_set_sweeps (the same numbering). */
+/* Launch variants as hgrp2d_set_sweeps (the same numbering; X 0..2 only). */
 int hgrp2db_set_sweeps(hgrp2db *ctx, int x_variant, int y_variant);
 
 /* Times every X variant, then every Y variant, over the first `steps` steps

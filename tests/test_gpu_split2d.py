@@ -231,6 +231,29 @@ def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by):
     assert parity(fg[:, npg - 1], fo[:, npo - 1]).max() <= TOL
 
 
+@pytest.mark.parametrize("xv", [2, 3, 4])
+def test_x_sweep_variants_every_boundary(hg, oracle, xv):
+    """The X sweep's other launch shapes (16 rows per block; 8 rows walking 4
+    and 8 chunks: 31 and 63 rows per block, so 70 rows end in a partial block
+    and a partial chunk, and 66 lines in a partial group) against the default
+    shape, bit for bit, for every boundary condition along x and each Newton
+    mode; and against the oracle."""
+    from hydrogrp import mlp
+    m, n = 70, 66
+    init = cases.smooth_random(m, n, 7)
+    for bx in (-1, -2, -4, -7, -24):
+        cfg = make_config(t_all=0.04, h_x=1 / m, h_y=1 / n, bc_x=bx, bc_y=-4, split=1)
+        for tol, params, conv in ((0.0, None, False), (1e-12, None, True), (1e-12, mlp.load(), True)):
+            a = hg.run_split2d(m, n, init, cfg, newton_tol=tol, guess_params=params, converge=conv)
+            b = hg.run_split2d(m, n, init, cfg, newton_tol=tol, guess_params=params, converge=conv, sweeps=(xv, 0))
+            assert (a[1], a[3][5], a[4].status) == (b[1], b[3][5], b[4].status) and a[4].status == 0, (bx, conv)
+            assert np.array_equal(a[0][:, a[1] - 1], b[0][:, b[1] - 1]), (bx, conv, params is not None)
+        fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg)
+        fg, npg, tpg, cg, res = hg.run_split2d(m, n, init, cfg, sweeps=(xv, 0))
+        assert (co[5], npo, tpo[npo - 1]) == (cg[5], npg, tpg[npg - 1])
+        assert parity(fg[:, npg - 1], fo[:, npo - 1]).max() <= TOL, bx
+
+
 def test_error_paths_vs_oracle(hg, oracle):
     m, n = 8, 6
     for mutate, kw in ((("p", (3, 2), 1e-12), {}), (("p", (3, 2), -1.0), {}), (None, {"CFL": 1e-12}),
@@ -242,7 +265,7 @@ def test_error_paths_vs_oracle(hg, oracle):
         base.update(kw)
         cfg = make_config(**base)
         fo, npo, tpo, co, st = oracle.split2d(m, n, init, cfg)
-        for xv in (0, 2):                              # blocks of 8 and of 16 cell rows
+        for xv in (0, 2, 3):                           # blocks of 8 and 16 cell rows, 4 chunks of 8
             fg, npg, tpg, cg, res = hg.run_split2d(m, n, init, cfg, sweeps=(xv, 0))
             assert (co[5], npo, tpo[npo - 1], st) == (cg[5], npg, tpg[npg - 1], res.status), (mutate, kw, xv)
             assert np.array_equal(np.isfinite(fo[:, npo - 1]), np.isfinite(fg[:, npg - 1]))
@@ -310,7 +333,7 @@ def test_launch_variants_and_tuner_change_no_result(hg):
         for settings in ((0.0, None, False), (1e-12, None, True), (1e-12, mlp.load(), True)):
             s.set_newton(*settings)
             ref = None
-            for xv, yv in ((0, 0), (1, 1), (2, 2), (0, 3), (1, 4)):
+            for xv, yv in ((0, 0), (1, 1), (2, 2), (0, 3), (1, 4), (3, 0), (4, 1)):
                 s.restore(blob)
                 s.set_sweeps(xv, yv)
                 s.bench(6)
