@@ -220,7 +220,7 @@ Sched schedule(const double *cfg, double h_S_max)
 {
     Sched S;
     const double t_all = cfg[1], eps = cfg[4], CFL = cfg[7];
-    const int N = (int)cfg[5];
+    const int N = cfg[5] >= 1.0 && cfg[5] < 2147483648.0 ? (int)cfg[5] : 2147483647;
     double tau = cfg[16], time_c = 0.0;
     int k = 1, DS = 1, status = 0, s = 0;
     bool stop_t = false;
@@ -322,6 +322,8 @@ extern "C" int hgrp2db_create(hgrp2db **out, int count, const int *m, const int 
     for (int i = 0; i < count; ++i)
         if (m[i] < 1 || n[i] < 1 || !(h_x[i] > 0.0) || !(h_y[i] > 0.0)) return ARG - 2;
     if (!valid_bc((int)config[17]) || !valid_bc((int)config[18])) return ARG - 4;
+    // the schedule is formed up front: the run must end by time or by step count
+    if (!isfinite(config[1]) && !(config[5] >= 1.0 && config[5] < 2147483648.0)) return ARG - 3;
     hgrp2db *c = new hgrp2db();
     c->count = count;
     memcpy(c->cfg, config, sizeof c->cfg);
@@ -661,7 +663,7 @@ extern "C" int hgrp2db_run(hgrp2db *c, double *ms, long long *cell_updates)
 
 extern "C" int hgrp2db_results(hgrp2db *c, int *k, int *status, double *time_c, int *steps)
 {
-    if (!c) return ARG - 1;
+    if (!c || c->res_k.size() != (size_t)c->count) return ARG - 1;     // nothing uploaded yet
     for (int i = 0; i < c->count; ++i) {
         if (k) k[i] = c->res_k[i];
         if (status) status[i] = c->res_status[i];
@@ -673,7 +675,7 @@ extern "C" int hgrp2db_results(hgrp2db *c, int *k, int *status, double *time_c, 
 
 extern "C" int hgrp2db_download(hgrp2db *c, double *rho, double *u, double *v, double *p, double *e)
 {
-    if (!c) return ARG - 1;
+    if (!c || c->res_cs.size() != (size_t)c->count) return ARG - 1;    // nothing uploaded yet
     double *dst[5] = {rho, u, v, p, e};
     size_t off = 0;
     for (int i = 0; i < c->count; ++i) {

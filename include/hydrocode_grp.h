@@ -175,7 +175,8 @@ typedef struct hgrp2db hgrp2db;
 /* count grids; grid g has m[g] x n[g] cells of size h_x[g] x h_y[g].  config:
  * the reference's config[] shared by all grids (indices read as for
  * hgrp2d_create; 10 and 11 are taken from h_x, h_y; 17 and 18 must be valid
- * boundary conditions, else ARG_ERROR - 4). */
+ * boundary conditions, else ARG_ERROR - 4; t_all (1) or the step cap (5) must
+ * be finite, else ARG_ERROR - 3). */
 int hgrp2db_create(hgrp2db **ctx, int count, const int *m, const int *n, const double *h_x,
                    const double *h_y, const double *config);
 int hgrp2db_destroy(hgrp2db *ctx);
@@ -205,7 +206,8 @@ int hgrp2db_run(hgrp2db *ctx, double *ms, long long *cell_updates);
  * may be NULL. */
 int hgrp2db_results(hgrp2db *ctx, int *k, int *status, double *time_c, int *steps);
 
-/* Final states, laid out as for hgrp2db_upload; e (total energy per mass) may be NULL. */
+/* Final states, laid out as for hgrp2db_upload; e (total energy per mass) may be NULL.
+ * hgrp2db_results and hgrp2db_download refuse a context never uploaded. */
 int hgrp2db_download(hgrp2db *ctx, double *rho, double *u, double *v, double *p, double *e);
 
 /* ------------------------------------------------------------------------- */

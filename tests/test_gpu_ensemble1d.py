@@ -195,3 +195,47 @@ def test_pauses_at_plot_times_leave_results_unchanged(hg):
     assert np.array_equal(got, want)
     for a, b in zip(got_res, want_res):
         assert np.array_equal(a, b)
+
+
+@pytest.mark.timeout(600)
+def test_c4_full_size(hg, oracle_devpow):
+    """The C4 ensemble at its full size: 512 instances, N uniform in
+    [256, 16384] (4.1 M cells, the largest about 31 600 steps), each to
+    t = 0.15 with its own tau.  Instances run alone (a one-instance ensemble,
+    the same engine) end bit for bit where they end in the batch, the largest
+    among them; the four smallest against the oracle with the device's pow
+    to 1e-12, as the 48-instance test holds the engine."""
+    cells, h, rho, u, p = cases.ensemble_1d(512, seed=2024)
+    cfg = _cfg(0.15)
+    ens = hg.Ensemble1D(cells, h, cfg)
+    ens.upload(rho, u, p)
+    ens.tune(32)
+    ms, cu = ens.run()
+    fin = ens.download()
+    steps, status, t = ens.results()
+    ens.close()
+    off = np.concatenate([[0], np.cumsum(cells)])
+    order = np.argsort(cells)
+    pick = [int(order[0]), int(order[len(order) // 2]), int(order[-1])]
+    for i in pick:
+        sl = slice(off[i], off[i + 1])
+        one = hg.Ensemble1D(cells[i:i + 1], h[i:i + 1], cfg)
+        one.upload(rho[sl], u[sl], p[sl])
+        one.run()
+        f1 = one.download()
+        s1, st1, t1 = one.results()
+        one.close()
+        assert (s1[0], st1[0], t1[0]) == (steps[i], status[i], t[i]), i
+        assert np.array_equal(f1, fin[:, sl], equal_nan=True), i
+    small = [int(i) for i in order[:4]]
+    sub_cells = cells[small]
+    sub = [np.concatenate([a[off[i]:off[i + 1]] for i in small]) for a in (rho, u, p)]
+    _, ref = _oracle_each(oracle_devpow, sub_cells, h[small], *sub, cfg)
+    worst = 0.0
+    for (f, k, st, te), i in zip(ref, small):
+        assert (steps[i], status[i], t[i]) == (k, st, te), i
+        worst = max(worst, _final_diff(fin[:, off[i]:off[i + 1]], f))
+    print(f"C4: 512 instances, {int(cells.sum())} cells, {cu} cell-updates in {ms:.0f} ms; instances "
+          f"{pick} (N = {list(cells[pick])}) alone: bitwise; the 4 smallest vs the device-pow oracle: "
+          f"{worst:.1e}; {int(np.sum(status != 0))} ended by a check; steps up to {int(steps.max())}", flush=True)
+    assert worst <= 1e-12
