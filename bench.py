@@ -67,11 +67,11 @@ def cpu_model():
     return {k: info.get(k) for k in ("Model name", "CPU(s)", "Thread(s) per core", "Core(s) per socket", "Socket(s)")}
 
 
-def measured_traffic(kernel, workload_tag):
+def measured_traffic(kernel, profile):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (profiles/*_hbm_traffic.md, a line
-    `traffic_bytes: {...}`), or None where no such profile exists."""
-    path = os.path.join(ROOT, "profiles", f"sweep_kernels_{workload_tag}_hbm_traffic.md")
+    WRITE_SIZE summary profiles/<profile>.md (a line `traffic_bytes: {...}`),
+    or None where no such profile or entry exists."""
+    path = os.path.join(ROOT, "profiles", f"{profile}.md")
     if not os.path.exists(path):
         return None
     for line in open(path):
@@ -240,6 +240,9 @@ def batch2d_leg(n_grids=128, seed=7, t_all=0.1):
             "speedup": (cu / wall) / (cu_one / secs_one)}
 
 
+TF_SWEEP_BYTES_PER_CELL = 160.0   # rho u v p, the volume fraction and their five slopes, read and written
+
+
 def two_fluid_leg(m=4096, n=1024, steps=300):
     """The two-fluid shock-bubble workload (4 x 1 domain, Mach 1.22 shock, gamma
     1.4 / 1.67 bubble, reflective top and bottom), Newton tolerance 1e-12, the
@@ -262,14 +265,20 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         s.set_fluids(1.67, frac)
         ms, ms_x, ms_y = s.bench(steps, 8)
+        kx, ky = s.sweep_kernel(0), s.sweep_kernel(1)
         state = s.download()
         s.close()
         assert np.all(np.isfinite(state)), "non-finite two-fluid state"
         out[name] = {"value": m * n * steps / (ms / 1000.0), "ms_per_step": ms / steps,
-                     "ms_per_launch": {"sweep_x_tf": ms_x, "sweep_y_tf": ms_y}}
+                     "ms_per_launch": {kx: ms_x, ky: ms_y}, "kernels": [kx, ky]}
+    kx = out["learned"]["kernels"][0]
+    achieved = TF_SWEEP_BYTES_PER_CELL * m * n / (out["learned"]["ms_per_launch"][kx] / 1000.0) / 1e9
     return {"workload": f"two-fluid shock-bubble {m}x{n} (4x1 domain), Newton tol 1e-12, {steps} steps from t=0",
             "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
-            "speedup": out["learned"]["value"] / out["plain"]["value"]}
+            "speedup": out["learned"]["value"] / out["plain"]["value"],
+            "roofline": {"bound": "hbm", "kernel": kx, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": TF_SWEEP_BYTES_PER_CELL,
+                         "traffic": measured_traffic(kx, f"sweep_tf_kernels_shockbubble{m}x{n}_trace")}}
 
 
 UNSPLIT_BYTES_PER_CELL = 208.0   # read and write rho u v p E and both slope sets (26 doubles)
@@ -320,7 +329,8 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
            "value": m * n * steps / (ms / 1000.0), "unit": "cell-updates/s", "ms_per_step": ms / steps,
            "roofline": {"bound": "hbm", "kernel": kernel, "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                        "traffic": measured_traffic(kernel, f"fourquadrant{m}"), "ms_per_launch": ms_step},
+                        "traffic": measured_traffic(kernel, f"unsplit_step_fourquadrant{m}_trace"),
+                        "ms_per_launch": ms_step},
            "learned_vs_plain": learned_vs_plain}
     if with_cpu:
         run2d, kind, label, threads = _cpu_solver(split=False)
@@ -452,7 +462,7 @@ def main():
     achieved = SWEEP_BYTES_PER_CELL * M * N / (ms_x / 1000.0) / 1e9
     nbytes = 1 << 30
     copy_gbs = 2.0 * nbytes / (hydrogrp.copy_probe(nbytes, 10) / 1000.0) / 1e9
-    tag = f"fourquadrant{M}_t015" if args.steps else None
+    tag = f"sweep_kernels_fourquadrant{M}_t015_hbm_traffic" if args.steps else None
     line = {
         "metric": METRIC,
         "value": learned["value"],

@@ -28,6 +28,7 @@ hipError_t launch_sweep_y(const SweepArgs &a, int variant, hipStream_t st);
 const char *sweep_kernel_name(int dir, int variant, int nm);
 int sweep_variants(int dir);
 hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st);
+const char *sweep_tf_kernel_name(int dir, int nm);
 hipError_t launch_sweep_y_tf(const SweepArgs &a, hipStream_t st);
 hipError_t launch_cfl_tf(const double *rho, const double *u, const double *v, const double *p, const double *fa,
                          int m, int n, int ld, const TwoGas &two, double hmin, unsigned long long *out, hipStream_t st);
@@ -239,7 +240,7 @@ extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
     if (!c || dir < 0 || dir > 2) return "";
     const int nm = c->use_mlp ? NM_LEARNED : c->newton_rule == HGRP_NEWTON_CONVERGE ? NM_CONV : NM_REF;
     if (dir == 2) return unsplit_kernel_name(c->m, c->n, nm);
-    if (c->two_fluid) return "";
+    if (c->two_fluid) return sweep_tf_kernel_name(dir, nm);
     return sweep_kernel_name(dir, dir == 0 ? c->xv : c->yv, nm);
 }
 

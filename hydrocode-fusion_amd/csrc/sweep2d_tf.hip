@@ -428,6 +428,15 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
     return hipGetLastError();
 }
 
+// the two-fluid sweep kernel's name as profilers print it
+const char *sweep_tf_kernel_name(int dir, int nm)
+{
+    static const char *xn[3] = {"sweep_x_tf_kernel<8, 0>", "sweep_x_tf_kernel<8, 1>", "sweep_x_tf_kernel<8, 2>"};
+    static const char *yn[3] = {"sweep_y_tf_kernel<0, 16>", "sweep_y_tf_kernel<1, 16>", "sweep_y_tf_kernel<2, 16>"};
+    if (nm < 0 || nm > 2 || dir < 0 || dir > 1) return "";
+    return dir == 0 ? xn[nm] : yn[nm];
+}
+
 hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st)
 {
     dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);

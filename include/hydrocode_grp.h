@@ -157,7 +157,8 @@ int hgrp2d_set_sweeps(hgrp2d *ctx, int x_variant, int y_variant);
 int hgrp2d_tune(hgrp2d *ctx, int steps, int *x_variant, int *y_variant, double *ms);
 
 /* Name of the kernel the context launches for an x (dir 0) or y (dir 1) sweep
- * (it depends on the grid size), or for an unsplit step (dir 2); "" for a bad
+ * (it depends on the launch variant, the Newton mode and, with two fluids,
+ * names the two-fluid sweeps), or for an unsplit step (dir 2); "" for a bad
  * argument.  For profiling tools. */
 const char *hgrp2d_sweep_kernel(hgrp2d *ctx, int dir);
 

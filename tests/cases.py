@@ -224,3 +224,19 @@ def write_ensemble_bin(path, n_inst=512, seed=2024):
         cells.astype("<i4").tofile(f)
         for a in (rho, u, p):
             np.asarray(a, "<f8").tofile(f)
+
+
+def write_grid_bin(path, config, init, frac=None, gamma_b=None):
+    """A 2D workload in the file format of grp2d_file_bench.out: int32 m, n;
+    float64 config[400]; float64 rho, u, v, p (m x n, row j at j*n); int32
+    two_fluid, then (if 1) float64 gamma_b and the volume fraction (m x n)."""
+    m, n = np.asarray(init["rho"]).shape
+    with open(path, "wb") as f:
+        np.array([m, n], "<i4").tofile(f)
+        np.asarray(config, "<f8").tofile(f)
+        for k in ("rho", "u", "v", "p"):
+            np.ascontiguousarray(init[k], "<f8").tofile(f)
+        np.array([0 if frac is None else 1], "<i4").tofile(f)
+        if frac is not None:
+            np.array([gamma_b], "<f8").tofile(f)
+            np.ascontiguousarray(frac, "<f8").tofile(f)
