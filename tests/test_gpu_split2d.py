@@ -64,7 +64,13 @@ def test_riemann_vectors(hg, golden_iface):
         assert d.max() <= 1e-9     # the Newton stop rule is an absolute 1e-9 gap
 
 
-def test_grp_flux_vectors(hg, golden_iface):
+def test_grp_flux_vectors(hg, golden_iface, oracle_devpow):
+    """GRP_2D_flux of the split sweeps on the seeded interface pairs, per
+    element relative to the value (floor: 1e-6 of the column's largest
+    magnitude): against the oracle with the device's pow to 1e-13, and
+    against the oracle with glibc's pow (the reference's arithmetic) to 1e-12
+    (the two oracles differ by at most 6.7e-16 on these pairs; one step's
+    interface values carry a last-bit pow difference no further)."""
     g = golden_iface
     L, R, tau = g["L"].copy(), g["R"].copy(), float(g["tau"])
     # the split path never has tangential derivatives
@@ -78,14 +84,20 @@ def test_grp_flux_vectors(hg, golden_iface):
             Ly[:, 11:15] = -0.0
             Ry[:, 11:15] = -0.0
         want = np.array([[rc, *o] for rc, o in (orc.flux2d(Ly[k], Ry[k], tau, EPS, *nxy) for k in range(len(L)))])
+        want_d = np.array([[rc, *o] for rc, o in (oracle_devpow.flux2d(Ly[k], Ry[k], tau, EPS, *nxy)
+                                                  for k in range(len(L)))])
         got = hg.flux_batch(Ly, Ry, tau, EPS, direction)
         ok = want[:, 0] == 0
         assert np.array_equal(got[:, 0] != 0, want[:, 0] != 0)
-        scale = np.maximum(np.abs(want[ok, 1:]), 1.0)
-        d = np.abs(got[ok, 1:] - want[ok, 1:]) / scale
-        print(f"flux dir {direction}: {np.mean(np.all(d == 0, axis=1)) * 100:.2f}% bit-identical rows, "
-              f"max rel {d.max():.3e}", flush=True)
-        assert d.max() <= 1e-8
+        assert np.array_equal(got[:, 0] != 0, want_d[:, 0] != 0)
+        floor = 1e-6 * np.abs(want[ok, 1:]).max(axis=0)
+        dd = np.abs(got[ok, 1:] - want_d[ok, 1:]) / np.maximum(np.abs(want_d[ok, 1:]), floor)
+        d = np.abs(got[ok, 1:] - want[ok, 1:]) / np.maximum(np.abs(want[ok, 1:]), floor)
+        print(f"flux dir {direction}: against the device-pow oracle {np.mean(np.all(dd == 0, axis=1)) * 100:.2f}% "
+              f"bit-identical rows, max rel {dd.max():.3e}; against the glibc-pow oracle "
+              f"{np.mean(np.all(d == 0, axis=1)) * 100:.2f}% bit-identical rows, max rel {d.max():.3e}", flush=True)
+        assert dd.max() <= 1e-13, direction
+        assert d.max() <= 1e-12, direction
 
 
 def _run(hg, m, n, init, cfg, time_plot=None):
