@@ -10,6 +10,7 @@
 #include "../../include/hydrocode_grp.h"
 #include "grp_device.hpp"
 #include "grp1d_device.hpp"
+#include "g2d_device.hpp"
 
 using namespace hgrp;
 
@@ -117,6 +118,21 @@ __global__ void q1d_batch_kernel(const double *L, const double *R, int count, do
     for (int q = 0; q < 6; ++q) o[10 + q] = us[q];
 }
 
+// linear_GRP_solver_Edir_G2D: sides of 21 doubles (g2_side_of), lambda_u of
+// the left side, lambda_v of the right one.  out: wave_speed[2], D[6], U[6],
+// U_star[6].
+__global__ void g2d_batch_kernel(const double *L, const double *R, int count, double eps, double atc, double *out)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const double *l = L + (size_t)k * 21, *r = R + (size_t)k * 21;
+    double *o = out + (size_t)k * 20;
+    double D[6], U[6], ws[2], us[6];
+    grp_g2d(D, U, ws, us, g2_side_of(l, -1.0), g2_side_of(r, 1.0), l[1], r[2], eps, atc, eps);
+    o[0] = ws[0]; o[1] = ws[1];
+    for (int q = 0; q < 6; ++q) { o[2 + q] = D[q]; o[8 + q] = U[q]; o[14 + q] = us[q]; }
+}
+
 // GRP_2D_flux for any unit normal, both sides' tangential derivatives and
 // gammas.  out: code, F_rho F_u F_v F_e, RHO_int U_int V_int P_int (zeros
 // when code != 0: the reference returns before it writes them).
@@ -204,12 +220,12 @@ struct RiemCtx { int count, N; };
 struct SideCtx { int count; double a, b, c, d; };   // per-entry-point scalars
 
 int side_batch(const double *L, const double *R, int count, int out_per, double *out, SideCtx sc,
-               void (*launch)(void *const *din, void *dout, void *ctx))
+               void (*launch)(void *const *din, void *dout, void *ctx), int width = 15)
 {
     if (count < 0 || (count && (!L || !R || !out))) return -1001;
     if (count == 0) return 0;
     const void *in[2] = {L, R};
-    const size_t nb[2] = {(size_t)count * 15 * 8, (size_t)count * 15 * 8};
+    const size_t nb[2] = {(size_t)count * width * 8, (size_t)count * width * 8};
     sc.count = count;
     return run_batch(in, nb, 2, out, (size_t)count * out_per * 8, launch, &sc);
 }
@@ -345,6 +361,18 @@ extern "C" int hgrp_q1d_batch(const double *L, const double *R, int count, doubl
         hipLaunchKernelGGL(q1d_batch_kernel, dim3((c->count + 127) / 128), dim3(128), 0, 0, (const double *)d[0],
                            (const double *)d[1], c->count, c->a, c->b, (double *)o);
     });
+}
+
+extern "C" int hgrp_g2d_batch(const double *L, const double *R, int count, double eps, double atc, double *out)
+{
+    return side_batch(
+        L, R, count, 20, out, SideCtx{0, eps, atc, 0.0, 0.0},
+        [](void *const *d, void *o, void *cx) {
+            const SideCtx *c = (const SideCtx *)cx;
+            hipLaunchKernelGGL(g2d_batch_kernel, dim3((c->count + 127) / 128), dim3(128), 0, 0, (const double *)d[0],
+                               (const double *)d[1], c->count, c->a, c->b, (double *)o);
+        },
+        21);
 }
 
 extern "C" int hgrp_grp2d_flux_general_batch(const double *L, const double *R, int count, double tau, double eps,

@@ -3,7 +3,7 @@
  *
  * Drop-ins with the reference's names and signatures for callers that use the
  * GRP solvers one interface at a time (src/include/riemann_solver.h:24-27, 60,
- * 66; src/include/flux_calc.h:28).  Each call runs one thread of the batched
+ * 66, 70; src/include/flux_calc.h:28).  Each call runs one thread of the batched
  * kernels of include/hydrocode_grp.h (the same device code the time step
  * uses), so it pays a device round trip per call: callers with many
  * interfaces call the hgrp_*_batch entry points instead.
@@ -75,6 +75,30 @@ void linear_GRP_solver_Edir_Q1D(double *wave_speed, double *D, double *U, double
     }
     for (int q = 0; q < 6; ++q)
         U_star[q] = out[10 + q];
+}
+
+/* src/riemann_solver/linear_grp_solver_Edir_G2D.c:48-637, the single-fluid
+ * build: z_a and phi are its constants (0, slopes -0), and D, U receive all
+ * six entries rho u v p z_a phi as the reference writes them (:160-166). */
+void linear_GRP_solver_Edir_G2D(double *wave_speed, double *D, double *U, double *U_star,
+                                const struct i_f_var *ifv_L, const struct i_f_var *ifv_R, const double eps,
+                                const double atc)
+{
+    double L[21], R[21], out[20];
+    const double zf[6] = {0.0, -0.0, -0.0, 0.0, -0.0, -0.0};
+    side_pack(L, ifv_L);
+    side_pack(R, ifv_R);
+    memcpy(L + 15, zf, sizeof zf);
+    memcpy(R + 15, zf, sizeof zf);
+    if (failed(hgrp_g2d_batch(L, R, 1, eps, atc, out), "linear_GRP_solver_Edir_G2D"))
+        fill_nan(out, 20);
+    wave_speed[0] = out[0];
+    wave_speed[1] = out[1];
+    for (int q = 0; q < 6; ++q) {
+        D[q] = out[2 + q];
+        U[q] = out[8 + q];
+        U_star[q] = out[14 + q];
+    }
 }
 
 /* src/riemann_solver/linear_grp_solver_Edir.c:33-337 */

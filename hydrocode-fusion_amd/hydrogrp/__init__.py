@@ -52,6 +52,7 @@ def lib():
     L.hgrp_grp2d_flux_tf_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _D, _D, _DP]
     L.hgrp_pow_batch.argtypes = [_DP, _DP, C.c_int, _DP]
     L.hgrp_q1d_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
+    L.hgrp_g2d_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _DP]
     L.hgrp_guessp_batch.argtypes = [_DP, C.c_int, _DP]
     L.hgrp_riemann_pvrs_batch.argtypes = [_DP, C.c_int, C.c_int, _DP]
     L.hgrp_grp2d_flux_general_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, _D, _D, _DP]
@@ -471,12 +472,33 @@ def riemann_pvrs_batch(rows, N=500):
     return out
 
 
-def _sides(L, R):
+def _sides(L, R, width=15):
     L = np.ascontiguousarray(L, dtype=np.float64)
     R = np.ascontiguousarray(R, dtype=np.float64)
-    if L.ndim != 2 or L.shape[1] != 15 or R.shape != L.shape:
-        raise ValueError("interface sides are (count, 15) arrays of equal shape")
+    if L.ndim != 2 or L.shape[1] != width or R.shape != L.shape:
+        raise ValueError(f"interface sides are (count, {width}) arrays of equal shape")
     return L, R
+
+
+# the single-fluid build's constants for z_a d_z_a t_z_a phi d_phi t_phi
+G2D_SINGLE_FLUID = (0.0, -0.0, -0.0, 0.0, -0.0, -0.0)
+
+
+def g2d_batch(L, R, eps, atc):
+    """linear_GRP_solver_Edir_G2D per pair (the genuinely-2D GRP solver, default
+    build).  Sides: (count, 21) = the 15 of q1d_batch, then z_a d_z_a t_z_a phi
+    d_phi t_phi; (count, 15) sides get the single-fluid constants.  Rows of
+    wave_speed[2], D[6], U[6] (rho u v p z_a phi), U_star[6]."""
+    L = np.asarray(L, dtype=np.float64)
+    R = np.asarray(R, dtype=np.float64)
+    if L.ndim == 2 and L.shape[1] == 15:
+        L = np.concatenate([L, np.tile(G2D_SINGLE_FLUID, (len(L), 1))], axis=1)
+    if R.ndim == 2 and R.shape[1] == 15:
+        R = np.concatenate([R, np.tile(G2D_SINGLE_FLUID, (len(R), 1))], axis=1)
+    L, R = _sides(L, R, 21)
+    out = np.zeros((len(L), 20))
+    _check(lib().hgrp_g2d_batch(_p(L), _p(R), len(L), eps, atc, _p(out)), "g2d batch")
+    return out
 
 
 def q1d_batch(L, R, eps, atc):

@@ -77,6 +77,13 @@ void linear_GRP_solver_Edir(double *D, double *U, const struct i_f_var *ifv_L, c
 void linear_GRP_solver_Edir_Q1D(double *wave_speed, double *D, double *U, double *U_star,
                                 const struct i_f_var *ifv_L, const struct i_f_var *ifv_R, const double eps,
                                 const double atc);
+/* replaces linear_GRP_solver_Edir_G2D, src/include/riemann_solver.h:70
+ * (src/riemann_solver/linear_grp_solver_Edir_G2D.c:48-637, default build
+ * without EXACT_TANGENT_DERIVATIVE); D and U receive six entries each
+ * (rho u v p z_a phi, the last two the single-fluid build's constants) */
+void linear_GRP_solver_Edir_G2D(double *wave_speed, double *D, double *U, double *U_star,
+                                const struct i_f_var *ifv_L, const struct i_f_var *ifv_R, const double eps,
+                                const double atc);
 /* replaces GRP_2D_flux, src/include/flux_calc.h:28 (src/flux_calc/flux_solver.c:158-237);
  * eps is config[4] */
 int GRP_2D_flux(struct i_f_var *ifv, struct i_f_var *ifv_R, const double tau);

@@ -274,6 +274,18 @@ int hgrp_riemann_pvrs_batch(const double *in, int count, int N, double *out);
  * (rho_star_L u_star rho_star_R p_star c_star_L c_star_R). */
 int hgrp_q1d_batch(const double *L, const double *R, int count, double eps, double atc, double *out);
 
+/* linear_GRP_solver_Edir_G2D (src/riemann_solver/linear_grp_solver_Edir_G2D.c:48-637,
+ * src/include/riemann_solver.h:70), the genuinely-2D direct Eulerian GRP
+ * solver as the reference builds it by default (EXACT_TANGENT_DERIVATIVE
+ * undefined), with all of its arguments: atc (eps: the GRP with its acoustic
+ * case; INFINITY: acoustic only; -0.0: nonlinear only), lambda_u of L and
+ * lambda_v of R (the reference reads them so), a gamma per side.  Sides are
+ * 21 doubles: the 15 of hgrp_q1d_batch, then z_a d_z_a t_z_a phi d_phi t_phi
+ * (the reference's multi-fluid members; a single-fluid caller passes
+ * 0 -0 -0 0 -0 -0, the constants its single-fluid build uses).
+ * out: count x 20 = wave_speed[2], D[6], U[6] (rho u v p z_a phi), U_star[6]. */
+int hgrp_g2d_batch(const double *L, const double *R, int count, double eps, double atc, double *out);
+
 /* GRP_2D_flux (src/flux_calc/flux_solver.c:158-237, src/include/flux_calc.h:28)
  * in full: unit normal (n_x, n_y), tangential derivatives and gamma of both
  * sides (lambda is set to 0 as the reference does; the energy flux uses L's

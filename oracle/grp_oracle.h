@@ -91,6 +91,17 @@ double orc_guessp(double DL, double DR, double UL, double UR, double PL, double 
 void orc_grp_q1d(double wave_speed[2], double D[4], double U[4], double U_star[6],
                  const orc_ifv *L, const orc_ifv *R, double eps, double atc);
 
+/* linear_GRP_solver_Edir_G2D (src/riemann_solver/linear_grp_solver_Edir_G2D.c:48-637), the
+ * genuinely-2D direct Eulerian GRP solver as the reference builds it by default
+ * (EXACT_TANGENT_DERIVATIVE undefined).  zf_L / zf_R: the multi-fluid members
+ * z_a d_z_a t_z_a phi d_phi t_phi of each side (-DMULTIFLUID_BASICS), or NULL
+ * for the single-fluid build's constants 0 -0 -0 0 -0 -0 (:70-77).  D[6], U[6] =
+ * (rho, u, v, p, z_a, phi); U_star[6].  lambda_u is read from L, lambda_v from R
+ * (:50).  The exact solve follows orc_set_newton like orc_grp_q1d's. */
+void orc_grp_g2d(double wave_speed[2], double D[6], double U[6], double U_star[6],
+                 const orc_ifv *L, const orc_ifv *R, const double *zf_L, const double *zf_R,
+                 double eps, double atc);
+
 /* ifvar_check / star_dire_check for dim == 2 (src/inter_process/fluid_var_check.c:21-51, 65-131). */
 int orc_ifvar_check2(const orc_ifv *L, const orc_ifv *R, double eps);
 int orc_star_dire_check2(const double mid[4], const double dire[4], double eps);

@@ -76,6 +76,15 @@ void ref_q1d(const double *Ls, const double *Rs, double eps, double atc, double 
     for (int q = 0; q < 6; ++q) out[10 + q] = Us[q];
 }
 
+/* linear_GRP_solver_Edir_G2D (single-fluid build: z_a and phi are the
+ * solver's constants).  out[20]: wave_speed[2], D[6], U[6], U_star[6]. */
+void ref_g2d(const double *Ls, const double *Rs, double eps, double atc, double *out)
+{
+    struct i_f_var L, R;
+    to_ifv(&L, Ls); to_ifv(&R, Rs);
+    linear_GRP_solver_Edir_G2D(out, out + 2, out + 8, out + 14, &L, &R, eps, atc);
+}
+
 /* GRP_2D_flux; config[4] (eps) must be set.  out: F_rho F_u F_v F_e RHO_int U_int V_int P_int */
 int ref_grp_2d_flux(const double *Ls, const double *Rs, double n_x, double n_y, double tau, double *out)
 {

@@ -50,6 +50,16 @@ void refmf_q1d(const double *L, const double *R, double eps, double atc, double 
     linear_GRP_solver_Edir_Q1D(out, out + 2, out + 8, out + 14, &l, &r, eps, atc);
 }
 
+/* linear_GRP_solver_Edir_G2D (src/riemann_solver/linear_grp_solver_Edir_G2D.c:48-637)
+ * with its multi-fluid members.  out[20]: wave_speed[2], D[6], U[6], U_star[6]. */
+void refmf_g2d(const double *L, const double *R, double eps, double atc, double *out)
+{
+    struct i_f_var l, r;
+    to_ifv(&l, L);
+    to_ifv(&r, R);
+    linear_GRP_solver_Edir_G2D(out, out + 2, out + 8, out + 14, &l, &r, eps, atc);
+}
+
 /* GRP_2D_flux (src/flux_calc/flux_solver.c:158-237) with its multi-fluid
  * branch (:203-235): eps from config[4], the energy flux with gamma at
  * t + tau/2 from z_a there.  out[13]: return code, F_rho F_u F_v F_e, RHO_int

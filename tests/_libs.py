@@ -78,6 +78,8 @@ class Oracle:
         L.orc_grp_q1d.argtypes = [DP, DP, DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), D, D]
         L.orc_grp_2d_flux.restype = C.c_int
         L.orc_grp_2d_flux.argtypes = [OrcIfv, OrcIfv, D, D, D, D, DP, DP]
+        L.orc_grp_g2d.restype = None
+        L.orc_grp_g2d.argtypes = [DP, DP, DP, DP, C.POINTER(OrcIfv), C.POINTER(OrcIfv), DP, DP, D, D]
         L.orc_guessp.restype = D
         L.orc_guessp.argtypes = [D] * 11
         L.orc_grp_edir.restype = None
@@ -129,6 +131,17 @@ class Oracle:
     def q1d(self, Lv, Rv, eps, atc):
         ws, Dd, U, Us = (D * 2)(), (D * 4)(), (D * 4)(), (D * 6)()
         self.lib.orc_grp_q1d(ws, Dd, U, Us, C.byref(ifv_from(Lv)), C.byref(ifv_from(Rv)), eps, atc)
+        return np.array(list(ws) + list(Dd) + list(U) + list(Us))
+
+    def g2d(self, Lv, Rv, eps, atc):
+        """linear_GRP_solver_Edir_G2D on sides of 15 doubles (the single-fluid
+        build) or 21 (the multi-fluid members z_a d_z_a t_z_a phi d_phi t_phi
+        appended).  Returns wave_speed[2], D[6], U[6], U_star[6]."""
+        ws, Dd, U, Us = (D * 2)(), (D * 6)(), (D * 6)(), (D * 6)()
+        zf = [None, None]
+        if len(Lv) == 21:
+            zf = [(D * 6)(*[float(x) for x in Lv[15:21]]), (D * 6)(*[float(x) for x in Rv[15:21]])]
+        self.lib.orc_grp_g2d(ws, Dd, U, Us, C.byref(ifv_from(Lv)), C.byref(ifv_from(Rv)), zf[0], zf[1], eps, atc)
         return np.array(list(ws) + list(Dd) + list(U) + list(Us))
 
     def flux2d(self, Lv, Rv, tau, eps, nx, ny):
@@ -238,6 +251,7 @@ class RefLib:
         L.ref_riemann_exact.argtypes = [DP, C.c_int, DP]
         L.ref_riemann_exact_ben.argtypes = [DP, C.c_int, DP]
         L.ref_q1d.argtypes = [DP, DP, D, D, DP]
+        L.ref_g2d.argtypes = [DP, DP, D, D, DP]
         L.ref_grp_2d_flux.argtypes = [DP, DP, D, D, D, DP]
         L.ref_grp_2d_flux.restype = C.c_int
         L.ref_grp_edir.argtypes = [DP, DP, D, D, DP]
@@ -275,6 +289,12 @@ class RefLib:
         Rv = np.ascontiguousarray(Rv, dtype=np.float64)
         out = np.zeros(16)
         self.lib.ref_q1d(dptr(Lv), dptr(Rv), eps, atc, dptr(out))
+        return out
+
+    def g2d(self, Lv, Rv, eps, atc):
+        out = np.zeros(20)
+        self.lib.ref_g2d(dptr(np.ascontiguousarray(Lv, dtype=np.float64)),
+                         dptr(np.ascontiguousarray(Rv, dtype=np.float64)), eps, atc, dptr(out))
         return out
 
     def flux2d(self, Lv, Rv, tau, eps, nx, ny):
@@ -383,6 +403,7 @@ class RefLibMF:
         L.refmf_config_set.argtypes = [C.c_int, D]
         L.refmf_q1d.argtypes = [DP, DP, D, D, DP]
         L.refmf_grp_2d_flux.argtypes = [DP, DP, D, D, D, DP]
+        L.refmf_g2d.argtypes = [DP, DP, D, D, DP]
 
     def set_config(self, values):
         self.lib.refmf_config_reset()
@@ -392,6 +413,12 @@ class RefLibMF:
     def q1d(self, Lv, Rv, eps, atc):
         out = np.zeros(20)
         self.lib.refmf_q1d(dptr(np.ascontiguousarray(Lv, dtype=np.float64)),
+                           dptr(np.ascontiguousarray(Rv, dtype=np.float64)), eps, atc, dptr(out))
+        return out
+
+    def g2d(self, Lv, Rv, eps, atc):
+        out = np.zeros(20)
+        self.lib.refmf_g2d(dptr(np.ascontiguousarray(Lv, dtype=np.float64)),
                            dptr(np.ascontiguousarray(Rv, dtype=np.float64)), eps, atc, dptr(out))
         return out
 

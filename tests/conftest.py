@@ -51,6 +51,11 @@ def golden_units():
 
 
 @pytest.fixture(scope="session")
+def golden_g2d():
+    return load_golden("g2d_vectors.npz")
+
+
+@pytest.fixture(scope="session")
 def golden_stages():
     return load_golden("stage_cases.npz")
 

@@ -325,6 +325,40 @@ def twofluid_vectors(_ref):
     np.savez_compressed(os.path.join(HERE, "twofluid_vectors.npz"), **out)
 
 
+def g2d_vectors(ref):
+    """linear_GRP_solver_Edir_G2D (src/riemann_solver/linear_grp_solver_Edir_G2D.c:48-637)
+    from the reference's single-fluid build (oracle/_ref/libhydroref.so) and
+    its -DMULTIFLUID_BASICS build (libhydroref_mf.so), at atc = eps, INFINITY
+    and -0.0.  Pairs: unit_pairs (grid velocity lambda_u on the left side,
+    lambda_v on the right one, a second gas on every other right side) with
+    near-equal pairs made near-equal in v too (the solver's distance includes
+    v), every seventh pair at gamma 3 (the rarefaction's log limit), and the
+    multi-fluid members z_a, phi in [0, 1] with normal and tangential slopes."""
+    from _libs import RefLibMF
+    mf = RefLibMF()
+    mf.set_config({4: EPS, 6: 1.4, 106: 1.67})
+    L, R = unit_pairs(800, seed=31)
+    n = len(L)
+    rng = np.random.default_rng(5)
+    R[:, 2] = rng.uniform(-1, 1, n) * np.sqrt(R[:, 0] * R[:, 6] / R[:, 3])
+    near = np.abs(L[:, 3] - R[:, 3]) < 1e-8
+    R[near, 5] = L[near, 5] + 1e-12
+    L[::7, 0] = 3.0
+    R[::7, 0] = 3.0
+
+    def members():
+        return np.concatenate([rng.uniform(0, 1, (n, 1)), rng.normal(0, 1, (n, 2)),
+                               rng.uniform(0, 1, (n, 1)), rng.normal(0, 1, (n, 2))], axis=1)
+
+    L = np.concatenate([L, members()], axis=1)
+    R = np.concatenate([R, members()], axis=1)
+    atcs = np.array([EPS, np.inf, -0.0])
+    single = np.array([[ref.g2d(L[k, :15], R[k, :15], EPS, a) for k in range(n)] for a in atcs])
+    multi = np.array([[mf.g2d(L[k], R[k], EPS, a) for k in range(n)] for a in atcs])
+    np.savez_compressed(os.path.join(HERE, "g2d_vectors.npz"), L=L, R=R, eps=EPS, atc=atcs,
+                        single=single, multi=multi)
+
+
 def unsplit2d_cases(ref):
     """GRP_solver_2D_EUL_source (the unsplit step, config[33] = 0)."""
     out = {}
@@ -431,6 +465,7 @@ def main():
     for name, fn in (("interface", interface_vectors), ("units", interface_units), ("stages", stage_cases),
                      ("split2d", split2d_cases), ("split2d_fq1024", split2d_fq1024),
                      ("split2d_fq1024_devpow", split2d_fq1024_devpow), ("twofluid", twofluid_vectors),
+                     ("g2d", g2d_vectors),
                      ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
         if not only or name in only:
             fn(ref)

@@ -1,9 +1,10 @@
 // Host build of the device GRP solvers (hydrocode-fusion_amd/csrc/grp_device.hpp,
-// grp1d_device.hpp) so that their arithmetic can be compared with the oracle
+// grp1d_device.hpp, g2d_device.hpp) so that their arithmetic can be compared with the oracle
 // on the CPU: same source, same expression order, no FMA contraction.
 #include <hip/hip_runtime.h>
 #include "grp_device.hpp"
 #include "grp1d_device.hpp"
+#include "g2d_device.hpp"
 
 using namespace hgrp;
 
@@ -65,6 +66,19 @@ extern "C" void hg_q1d_many(const double *L, const double *R, int n, double eps,
         o[0] = ws[0]; o[1] = ws[1];
         for (int q = 0; q < 4; ++q) { o[2 + q] = D[q]; o[6 + q] = U[q]; }
         for (int q = 0; q < 6; ++q) o[10 + q] = us[q];
+    }
+}
+
+// linear_GRP_solver_Edir_G2D on sides of 21 doubles; out: ws[2] D[6] U[6] U_star[6]
+extern "C" void hg_g2d_many(const double *L, const double *R, int n, double eps, double atc, double *out)
+{
+    for (int k = 0; k < n; ++k) {
+        const double *l = L + 21 * k, *r = R + 21 * k;
+        double D[6], U[6], ws[2], us[6];
+        grp_g2d(D, U, ws, us, g2_side_of(l, -1.0), g2_side_of(r, 1.0), l[1], r[2], eps, atc, eps);
+        double *o = out + 20 * k;
+        o[0] = ws[0]; o[1] = ws[1];
+        for (int q = 0; q < 6; ++q) { o[2 + q] = D[q]; o[8 + q] = U[q]; o[14 + q] = us[q]; }
     }
 }
 

@@ -262,3 +262,31 @@ def test_flux_check_codes_on_invalid_states(dev, golden_units):
             want = g["struct_out_L"][k, want_cols]
             assert np.max(np.abs(out[1:] - want) / np.maximum(np.abs(want), 1.0)) <= 1e-10, k
     assert bad > 0
+
+
+def test_g2d_against_reference_and_devpow_oracle(dev, golden_g2d, oracle_devpow):
+    """grp_g2d (csrc/g2d_device.hpp) built for the host: bit for bit the oracle
+    with the device's pow, and the reference's single-fluid and multi-fluid
+    builds to the last bits of glibc's pow."""
+    g = golden_g2d
+    n = len(g["L"])
+    single = np.array([0.0, -0.0, -0.0, 0.0, -0.0, -0.0])
+    for kind in ("single", "multi"):
+        L, R = g["L"].copy(), g["R"].copy()
+        if kind == "single":
+            L[:, 15:] = single
+            R[:, 15:] = single
+        L, R = np.ascontiguousarray(L), np.ascontiguousarray(R)
+        for a, atc in enumerate(g["atc"]):
+            out = np.zeros((n, 20))
+            dev.hg_g2d_many(_dp(L), _dp(R), n, C.c_double(EPS), C.c_double(atc), _dp(out))
+            w = 21 if kind == "multi" else 15
+            same = np.array([oracle_devpow.g2d(L[k, :w], R[k, :w], EPS, atc) for k in range(n)])
+            assert np.array_equal(out, same, equal_nan=True), (kind, atc)
+            ref = g[kind][a]
+            ok = np.isfinite(ref).all(axis=1)
+            assert np.array_equal(ok, np.isfinite(out).all(axis=1))
+            d = np.abs(out[ok] - ref[ok]) / np.maximum(np.abs(ref[ok]), 1e-6 * np.abs(ref[ok]).max(axis=0) + 1e-300)
+            print(f"G2D {kind} atc={atc}: {np.mean(np.all(out[ok] == ref[ok], axis=1)) * 100:.2f}% rows "
+                  f"bit-identical to the reference, max rel {d.max():.2e}")
+            assert d.max() <= 1e-14

@@ -295,6 +295,67 @@ def test_grp1d_plot_frames(oracle, golden_1d):
     assert not np.array_equal(g["frames_all"][:, 1], g["frames_all"][:, 2])
 
 
+def test_g2d_bitexact(oracle, golden_g2d):
+    """linear_GRP_solver_Edir_G2D of the oracle equals the reference's
+    single-fluid build and its -DMULTIFLUID_BASICS build bit for bit, at
+    atc = eps, INFINITY and -0.0."""
+    g = golden_g2d
+    L, R = g["L"], g["R"]
+    bad = []
+    for a, atc in enumerate(g["atc"]):
+        for k in range(len(L)):
+            if not _eq(oracle.g2d(L[k, :15], R[k, :15], EPS, atc), g["single"][a, k]):
+                bad.append(("single", a, k))
+            if not _eq(oracle.g2d(L[k], R[k], EPS, atc), g["multi"][a, k]):
+                bad.append(("multi", a, k))
+    assert not bad, bad[:10]
+
+
+def g2d_branches(L, R, out, atc):
+    """Which branch of linear_GRP_solver_Edir_G2D produced each row, read off
+    its inputs and outputs: 'upwind', 'acoustic', 'fan', or 'star-<side>-<wave>'
+    (the contact's side, L or R, and that side's wave, crw or shock)."""
+    lu = L[:, 1]
+    cL = np.sqrt(L[:, 0] * L[:, 6] / L[:, 3])
+    cR = np.sqrt(R[:, 0] * R[:, 6] / R[:, 3])
+    dist = np.sqrt(((L[:, 3:7] - R[:, 3:7]) ** 2).sum(axis=1))
+    ws, us = out[:, 0:2], out[:, 14:20]
+    crw_L = ws[:, 0] == L[:, 4] - cL
+    crw_R = ws[:, 1] == R[:, 4] + cR
+    u_star = us[:, 1]
+    fan = (crw_L & (u_star - us[:, 4] > lu)) | (crw_R & (u_star + us[:, 5] < lu))
+    names = []
+    for k in range(len(L)):
+        if ws[k, 0] > lu[k] or ws[k, 1] < lu[k]:
+            names.append("upwind")
+        elif dist[k] < atc:
+            names.append("acoustic")
+        elif fan[k]:
+            names.append("fan")
+        else:
+            left = not u_star[k] < lu[k]
+            crw = crw_L[k] if left else crw_R[k]
+            names.append(f"star-{'L' if left else 'R'}-{'crw' if crw else 'shock'}")
+    return np.array(names), crw_L, crw_R
+
+
+def test_g2d_fixture_covers_every_branch(golden_g2d):
+    """The G2D vectors reach every branch of the solver in the nonlinear mode
+    (atc = eps), the rarefaction's gamma = 3 limit, and the contact-side
+    rarefaction on the left with a rarefaction on the right (where the
+    reference's D[2] reads the right wave's c_frac)."""
+    g = golden_g2d
+    L, R = g["L"], g["R"]
+    names, crw_L, crw_R = g2d_branches(L, R, g["single"][0], EPS)
+    counts = {n: int((names == n).sum()) for n in np.unique(names)}
+    print(counts)
+    for n in ("upwind", "acoustic", "fan", "star-L-crw", "star-L-shock", "star-R-crw", "star-R-shock"):
+        assert counts.get(n, 0) >= 5, n
+    assert ((names == "star-L-crw") & crw_R).sum() >= 5
+    g3 = L[:, 0] == 3.0
+    assert (g3 & np.isin(names, ["fan", "star-L-crw", "star-R-crw"])).sum() >= 5
+
+
 def test_against_live_reference(oracle, reflib):
     """Fresh random vectors and runs (not in the fixtures) against the reference build."""
     L, R = cases.interface_pairs(3000, seed=99)
@@ -304,6 +365,7 @@ def test_against_live_reference(oracle, reflib):
         rr, r = reflib.flux2d(L[k], R[k], 1e-3, EPS, 0.0, 1.0)
         assert rc == rr and (rc != 0 or _eq(o, r)), k
         assert _eq(oracle.edir(L[k], R[k], EPS, EPS), reflib.edir(L[k], R[k], EPS, EPS)), k
+        assert _eq(oracle.g2d(L[k], R[k], EPS, EPS), reflib.g2d(L[k], R[k], EPS, EPS)), k
     for (m, n, seed) in ((7, 9, 1), (1, 13, 2), (13, 1, 3), (33, 31, 4)):
         cfg = make_config(t_all=0.03, h_x=1 / m, h_y=1 / n, bc_x=-7, bc_y=-2, split=1)
         init = cases.smooth_random(m, n, seed)

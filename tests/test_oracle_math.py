@@ -326,3 +326,81 @@ def test_converge_rule_reaches_the_root(oracle):
     # rule 0 is the reference's loop itself
     for args in _c3_inputs(states[:50]):
         assert oracle.riemann_rule(*args, 500, 0) == oracle.riemann(*args, 500)
+
+
+def _g2d_close(a, b, bar):
+    """max over finite rows of |a - b| relative to |b|, floored per column at
+    1e-3 of the column's largest magnitude (derivatives that cancel to near
+    zero carry the rounding of their terms)."""
+    ok = np.isfinite(a).all(axis=1) & np.isfinite(b).all(axis=1)
+    if not ok.any():
+        return True, 0.0
+    d = np.abs(a[ok] - b[ok]) / np.maximum(np.abs(b[ok]), 1e-3 * np.abs(b[ok]).max(axis=0) + 1e-300)
+    return d.max() <= bar, d.max()
+
+
+def test_g2d_without_tangential_terms_is_q1d(oracle, golden_g2d):
+    """With no tangential derivatives, v equal on both sides and one lambda_v,
+    the genuinely-2D solver's characteristic system collapses to the
+    quasi-1D one: wave speeds, D, U (rho u v p) and the star state equal Q1D's
+    (itself pinned to the reference) to rounding, in all three atc modes.  A
+    dropped or mis-signed term of the 2x2 system, the rarefaction relation or
+    the shock relations shows here."""
+    L, R = golden_g2d["L"][:, :15].copy(), golden_g2d["R"][:, :15].copy()
+    L[:, 11:15] = 0.0
+    R[:, 11:15] = 0.0
+    R[:, 5] = L[:, 5]
+    R[:, 2] = L[:, 2]
+    for atc in (1e-9, np.inf, -0.0):
+        a = np.array([oracle.g2d(L[k], R[k], 1e-9, atc) for k in range(len(L))])
+        q = np.array([oracle.q1d(L[k], R[k], 1e-9, atc) for k in range(len(L))])
+        a16 = np.concatenate([a[:, 0:2], a[:, 2:6], a[:, 8:12], a[:, 14:20]], axis=1)
+        ok, d = _g2d_close(a16, q, 1e-14)
+        print(f"G2D vs Q1D, atc {atc}: max rel {d:.2e}")
+        assert ok, (atc, d)
+
+
+def _reflect(S):
+    """x -> -x of interface sides (21 doubles): u, the normal derivatives of
+    rho, v, p, z_a, phi and the tangential one of u change sign."""
+    T = S.copy()
+    for col in (4, 7, 9, 10, 12, 16, 19):
+        T[:, col] = -T[:, col]
+    return T
+
+
+def test_g2d_mirror_symmetry(oracle, golden_g2d):
+    """Reflecting the problem (x -> -x, sides swapped, lambda_u negated)
+    reflects the solution: wave speeds swap and change sign, u and its time
+    derivative change sign, the star state swaps sides.  With the Newton solve
+    run to the root (tol 1e-13 with the converge rule) both problems agree to
+    1e-12 relative, in all three atc modes, except v_t where the contact's
+    side is a rarefaction and the other wave is one too: there the
+    reference's v_t takes c_frac from the right wave on either side (the
+    quirk the reference vectors pin), which no reflection preserves."""
+    from test_oracle_golden import g2d_branches
+    L, R = golden_g2d["L"].copy(), golden_g2d["R"].copy()
+    R[:, 2] = L[:, 2]
+    Lm, Rm = _reflect(R), _reflect(L)
+    Lm[:, 1] = -L[:, 1]
+    Rm[:, 2] = L[:, 2]
+    oracle.lib.orc_set_newton(1e-13, 1)
+    try:
+        for atc in (1e-9, np.inf, -0.0):
+            a = np.array([oracle.g2d(L[k], R[k], 1e-9, atc) for k in range(len(L))])
+            b = np.array([oracle.g2d(Lm[k], Rm[k], 1e-9, atc) for k in range(len(L))])
+            m = b.copy()
+            m[:, 0], m[:, 1] = -b[:, 1], -b[:, 0]
+            m[:, 3], m[:, 9] = -b[:, 3], -b[:, 9]
+            m[:, 14], m[:, 15], m[:, 16], m[:, 18], m[:, 19] = b[:, 16], -b[:, 15], b[:, 14], b[:, 19], b[:, 18]
+            names, crw_L, crw_R = g2d_branches(L, R, a, atc)
+            quirk = np.char.startswith(names, "star-") & np.char.endswith(names, "-crw") & crw_L & crw_R
+            keep = ~quirk
+            ok, d = _g2d_close(a[keep], m[keep], 1e-12)
+            cols = [c for c in range(20) if c != 4]
+            ok_q, d_q = _g2d_close(a[quirk][:, cols], m[quirk][:, cols], 1e-12)
+            print(f"G2D mirror, atc {atc}: max rel {d:.2e}; {quirk.sum()} rows with two rarefactions: "
+                  f"all but v_t {d_q:.2e}")
+            assert ok and ok_q, (atc, d, d_q)
+    finally:
+        oracle.lib.orc_set_newton(0.0, 0)
