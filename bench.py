@@ -291,8 +291,9 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
     at 1024^2), device time from HIP events; roofline of the step kernel
     against HBM peak with 208 B per cell (rho u v p E and the x- and y-slopes,
     read and written); the learned guess against the reference's, both at Newton
-    tolerance 1e-12 with the converge rule, over the same steps; and the
-    reference's unsplit loop on the host cores on a bounded number of steps."""
+    tolerance 1e-12 with the converge rule, over the same steps; the same steps
+    with GRP_2D_flux's alternative solver (G2D); and the reference's unsplit
+    loop on the host cores on a bounded number of steps."""
     import numpy as np
     import hydrogrp
     import cases
@@ -316,6 +317,14 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
         ms_f, ms_step_f = s.bench_unsplit(steps, 8)
         lvp[name] = {"value": m * n * steps / (ms_f / 1000.0), "ms_per_step": ms_f / steps,
                      "ms_per_launch": ms_step_f, "kernel": s.sweep_kernel(2), "state": s.download()}
+    # GRP_2D_flux with the reference's alternative solver (G2D), its own Newton settings
+    s.set_newton(0.0, None, converge=False)
+    s.set_interface_solver(True)
+    s.upload(init["rho"], init["u"], init["v"], init["p"])
+    ms_g, ms_step_g = s.bench_unsplit(steps, 8)
+    g2d = {"solver": "linear_GRP_solver_Edir_G2D (flux_solver.c:189), the reference's Newton setting",
+           "value": m * n * steps / (ms_g / 1000.0), "ms_per_step": ms_g / steps, "ms_per_launch": ms_step_g,
+           "kernel": s.sweep_kernel(2)}
     s.close()
     assert np.all(np.isfinite(state)), "non-finite state after the unsplit benchmark"
     a, b = lvp["learned"].pop("state"), lvp["plain"].pop("state")
@@ -331,7 +340,7 @@ def unsplit_leg(steps=2017, cpu_seconds=8.0, with_cpu=True, size=1024):
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "traffic": measured_traffic(kernel, f"unsplit_step_fourquadrant{m}_trace"),
                         "ms_per_launch": ms_step},
-           "learned_vs_plain": learned_vs_plain}
+           "learned_vs_plain": learned_vs_plain, "g2d": g2d}
     if with_cpu:
         run2d, kind, label, threads = _cpu_solver(split=False)
         h = 1.0 / m

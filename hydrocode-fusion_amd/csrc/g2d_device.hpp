@@ -317,4 +317,49 @@ __device__ __forceinline__ void grp_g2d(double D[6], double U[6], double ws[2], 
     D[3] = D[3] + lv * T_p;
 }
 
+// An interface side of the 2D step (single fluid: z_a and phi take the
+// reference's single-fluid constants 0, -0).  T: tangential derivatives
+// (rho, u, v, p) in the interface's frame.
+__device__ __forceinline__ G2Side g2_side(const Side &S, const double T[4], const Gas &G, double sg)
+{
+    G2Side s;
+    s.sg = sg;
+    s.G = G;
+    s.rho = S.rho; s.u = S.u; s.v = S.v; s.p = S.p;
+    s.d_rho = S.d_rho; s.d_u = S.d_u; s.d_v = S.d_v; s.d_p = S.d_p;
+    s.t_rho = T[0]; s.t_u = T[1]; s.t_v = T[2]; s.t_p = T[3];
+    s.z = 0.0; s.d_z = -0.0; s.t_z = -0.0; s.phi = 0.0; s.d_phi = -0.0; s.t_phi = -0.0;
+    s.c = sqrt(G.g * s.p / s.rho);
+    s.crw = 0;
+    return s;
+}
+
+// GRP_2D_flux with the reference's alternative interface solver, G2D at
+// atc = eps (src/flux_calc/flux_solver.c:189 in place of the Q1D call at
+// :191), in the unsplit step's form: one gas, normal (n_x, n_y), both sides'
+// tangential derivatives TL, TR in grid axes.  GRP_2D_flux zeroes the left
+// side's grid velocity and the flux generators leave the right side's at 0, so
+// the solver sees lambda_u = lambda_v = 0.  Results and codes as grp_2d_flux_g.
+template <bool CONV = false>
+__device__ __forceinline__ int grp_2d_flux_g2d(Side L, Side R, const double TL[4], const double TR[4], const Gas &G,
+                                               double n_x, double n_y, double tau, double eps, double F[4],
+                                               double I[4], double tol, double p_guess = -1.0)
+{
+    const double u = L.u * n_x + L.v * n_y;
+    const double u_R = R.u * n_x + R.v * n_y;
+    const double d_u = L.d_u * n_x + L.d_v * n_y;
+    const double d_u_R = R.d_u * n_x + R.d_v * n_y;
+    const double TLr[4] = {TL[0], TL[1] * n_x + TL[2] * n_y, -TL[1] * n_y + TL[2] * n_x, TL[3]};
+    const double TRr[4] = {TR[0], TR[1] * n_x + TR[2] * n_y, -TR[1] * n_y + TR[2] * n_x, TR[3]};
+    L.v = -L.u * n_y + L.v * n_x;
+    R.v = -R.u * n_y + R.v * n_x;
+    L.d_v = -L.d_u * n_y + L.d_v * n_x;
+    R.d_v = -R.d_u * n_y + R.d_v * n_x;
+    L.u = u; R.u = u_R; L.d_u = d_u; R.d_u = d_u_R;
+    double dire[6], mid[6], ws[2], us[6];
+    grp_g2d<CONV>(dire, mid, ws, us, g2_side(L, TLr, G, -1.0), g2_side(R, TRr, G, 1.0), 0.0, 0.0, eps, eps, tol,
+                  p_guess);
+    return flux_from_mid_n(mid, dire, G, n_x, n_y, tau, eps, F, I);
+}
+
 }  // namespace hgrp

@@ -37,7 +37,7 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st);
-const char *unsplit_kernel_name(int m, int n, int nm);
+const char *unsplit_kernel_name(int m, int n, int nm, int g2d);
 int unsplit_max_rows();
 }
 
@@ -82,6 +82,7 @@ struct hgrp2d {
     double newton_tol;          // <= 0: the reference's (eps)
     int newton_rule;            // HGRP_NEWTON_REFERENCE or HGRP_NEWTON_CONVERGE
     int xv, yv;                 // launch variants of the X and Y sweeps (hgrp2d_tune)
+    int g2d;                    // the unsplit step's interface solver: 0 Q1D, 1 G2D
     MlpPacked *mlp;             // device copy of the learned-guess weights
     bool use_mlp;
     bool two_fluid;             // volume fraction of fluid a carried (hgrp2d_set_fluids)
@@ -239,7 +240,7 @@ extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
 {
     if (!c || dir < 0 || dir > 2) return "";
     const int nm = c->use_mlp ? NM_LEARNED : c->newton_rule == HGRP_NEWTON_CONVERGE ? NM_CONV : NM_REF;
-    if (dir == 2) return unsplit_kernel_name(c->m, c->n, nm);
+    if (dir == 2) return unsplit_kernel_name(c->m, c->n, nm, c->g2d);
     if (c->two_fluid) return sweep_tf_kernel_name(dir, nm);
     return sweep_kernel_name(dir, dir == 0 ? c->xv : c->yv, nm);
 }
@@ -331,6 +332,21 @@ extern "C" int hgrp2d_set_newton(hgrp2d *c, double tol, int rule, const float *p
         HCHECK(hipMemcpy(c->mlp, &h, sizeof h, hipMemcpyHostToDevice));
         c->use_mlp = true;
     }
+    return 0;
+}
+
+extern "C" int hgrp2d_set_interface_solver(hgrp2d *c, int solver)
+{
+    if (!c || (solver != HGRP_SOLVER_Q1D && solver != HGRP_SOLVER_G2D)) return ARG_ERROR - 1;
+    // the split sweeps have no tangential derivatives, where G2D is Q1D but
+    // for its distance (which includes v) and the rounding of a few products;
+    // they keep the reference's solver
+    if (solver == HGRP_SOLVER_G2D && c->cfg[33] != 0.0) {
+        fprintf(stderr, "hydrogrp: the G2D interface solver runs in the unsplit scheme (config[33] = 0) only\n");
+        return ARG_ERROR - 2;
+    }
+    HCHECK(hipStreamSynchronize(c->stream));
+    c->g2d = solver == HGRP_SOLVER_G2D;
     return 0;
 }
 
@@ -840,6 +856,7 @@ static void fill_unsplit_args(hgrp2d *c, UnsplitArgs &a, double tau, double nu, 
     a.alpha = c->cfg[41];
     a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
     a.conv = c->newton_rule == HGRP_NEWTON_CONVERGE;
+    a.g2d = c->g2d;
     a.mlp = c->use_mlp ? c->mlp : nullptr;
     a.gas = make_gas(c->cfg[6]);
     a.status = c->status + 2 * slot;

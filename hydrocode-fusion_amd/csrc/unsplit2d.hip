@@ -22,6 +22,7 @@
 #include <hip/hip_runtime.h>
 #include <stdlib.h>
 #include "grp_device.hpp"
+#include "g2d_device.hpp"
 #include "unsplit2d.hpp"
 
 namespace hgrp {
@@ -170,15 +171,19 @@ struct Grid {
 // GRP solve of one interface with transversal derivatives, normal x (dir 0)
 // or y (dir 1); failures are recorded in status[dir] as
 // ((loop index << 2) | code) + 1, the reference's loop order
-template <int NM>
+// G2: GRP_2D_flux with its alternative solver, G2D (grp_2d_flux_g2d).
+template <int NM, bool G2>
 __device__ __forceinline__ void solve_t(const UnsplitArgs &a, int dir, const Side &SL, const Side &SR,
                                         const double TL[4], const double TR[4], unsigned long long loop_index,
                                         double F[4], double I[4], double p_guess)
 {
     int code = ifvar_bad_t(SL, SR, TL, TR, a.eps) ? 1 : 0;
-    if (grp_2d_flux_n<NM != NM_REF>(SL, SR, TL, TR, a.gas, dir ? 0.0 : 1.0, dir ? 1.0 : 0.0, a.tau, a.eps, F, I, a.tol,
-                                    p_guess))
-        code = 2;
+    const double n_x = dir ? 0.0 : 1.0, n_y = dir ? 1.0 : 0.0;
+    const int bad = G2 ? grp_2d_flux_g2d<NM != NM_REF>(SL, SR, TL, TR, a.gas, n_x, n_y, a.tau, a.eps, F, I, a.tol,
+                                                       p_guess)
+                       : grp_2d_flux_n<NM != NM_REF>(SL, SR, TL, TR, a.gas, n_x, n_y, a.tau, a.eps, F, I, a.tol,
+                                                     p_guess);
+    if (bad) code = 2;
     if (code) atomicMax(&a.status[dir].fail_key, ((loop_index << 2) | (unsigned long long)code) + 1ull);
 }
 
@@ -204,8 +209,8 @@ __device__ __forceinline__ double unsplit_guess(const UnsplitArgs &a, bool activ
 
 }  // namespace
 
-template <int W, int MINW, int NM>
-__global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs a)
+template <int W, int NM, bool G2>
+__device__ __forceinline__ void unsplit_step_body(const UnsplitArgs &a)
 {
     __shared__ double sd[W][12][64];     // side data of row c: value, x-slope, y-slope
     __shared__ double fx[W - 1][8][64];  // x-interface c+1: flux, values at t + tau
@@ -283,7 +288,7 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
         double F[4] = {0.0, 0.0, 0.0, 0.0}, I[4] = {0.0, 0.0, 0.0, 0.0};
         const double pg = unsplit_guess<NM>(a, active, pass, SL, SR, mlp_lds[NM == NM_LEARNED ? w : 0]);
         // the argument block is read again for the solve, so no scalar register holds it across
-        if (active) solve_t<NM>(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I, pg);
+        if (active) solve_t<NM, G2>(reload_args<UnsplitArgs>(), pass, SL, SR, TL, TR, idx, F, I, pg);
         if (pass == 0 && w <= W - 2) {
 #pragma unroll
             for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
@@ -329,6 +334,19 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
     }
 }
 
+template <int W, int MINW, int NM>
+__global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs a)
+{
+    unsplit_step_body<W, NM, false>(a);
+}
+
+// the same step with GRP_2D_flux's alternative interface solver, G2D
+template <int W, int MINW, int NM>
+__global__ void __launch_bounds__(64 * W, MINW) unsplit_step_g2d_kernel(UnsplitArgs a)
+{
+    unsplit_step_body<W, NM, true>(a);
+}
+
 // Strip form: one wave per block walks down R cell rows of a 62-cell strip
 // (lanes: cells i0-1 .. i0+62 of the row).  Iteration r (from r0-1) forms the
 // side data of row r+1, solves the y-interfaces of row r and the x-interface
@@ -336,8 +354,8 @@ __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs 
 // kept from the iteration before.  No wave waits for another: the block is
 // one wave, its LDS holds two rows of side data and two rows of interface
 // results (16 KiB).  One extra x-interface per R rows.
-template <int NM>
-__global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int R)
+template <int NM, bool G2>
+__device__ __forceinline__ void unsplit_strip_body(const UnsplitArgs &a, int R)
 {
     __shared__ double sd[2][12][64];   // side data of rows r and r + 1 (alternating)
     __shared__ double fl[8][64];       // x-interface r: flux, values at t + tau
@@ -409,7 +427,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
             }
             double Fo[4] = {0.0, 0.0, 0.0, 0.0}, Io[4] = {0.0, 0.0, 0.0, 0.0};
             const double pg = unsplit_guess<NM>(a, active, 1 - pass, SL, SR, mlp_lds);
-            if (active) solve_t<NM>(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io, pg);
+            if (active) solve_t<NM, G2>(reload_args<UnsplitArgs>(), 1 - pass, SL, SR, TL, TR, idx, Fo, Io, pg);
             if (pass == 0) {
 #pragma unroll
                 for (int q = 0; q < 4; ++q) { gb[q][lane] = Fo[q]; gb[4 + q][lane] = Io[q]; }
@@ -455,6 +473,19 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int
     }
 }
 
+template <int NM>
+__global__ void __launch_bounds__(64, 2) unsplit_strip_kernel(UnsplitArgs a, int R)
+{
+    unsplit_strip_body<NM, false>(a, R);
+}
+
+// the same with GRP_2D_flux's alternative interface solver, G2D
+template <int NM>
+__global__ void __launch_bounds__(64, 2) unsplit_strip_g2d_kernel(UnsplitArgs a, int R)
+{
+    unsplit_strip_body<NM, true>(a, R);
+}
+
 // ---- launchers -----------------------------------------------------------------
 // Two forms: the tile (8 waves, 6 rows per block, the compiler's register
 // budget: no spills, 2 waves/SIMD, which its 100 KiB of LDS per block sets
@@ -493,33 +524,45 @@ static int unsplit_variant(int m, int n)
 
 int unsplit_max_rows() { return 65535 * 4; }   // grid.y of the strip form at R = 4
 
-template <int NM>
+template <int NM, bool G2>
 static hipError_t launch_unsplit(const UnsplitArgs &a, hipStream_t st)
 {
     if (unsplit_variant(a.m, a.n) == 4) {
         const int R = strip_rows(a.m, a.n);
-        hipLaunchKernelGGL(unsplit_strip_kernel<NM>, dim3((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R)),
-                           dim3(64), 0, st, a, R);
+        const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R));
+        if (G2) hipLaunchKernelGGL(unsplit_strip_g2d_kernel<NM>, grid, dim3(64), 0, st, a, R);
+        else hipLaunchKernelGGL(unsplit_strip_kernel<NM>, grid, dim3(64), 0, st, a, R);
     } else {
         const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + 5) / 6));
-        hipLaunchKernelGGL((unsplit_step_kernel<8, 1, NM>), grid, dim3(512), 0, st, a);
+        if (G2) hipLaunchKernelGGL((unsplit_step_g2d_kernel<8, 1, NM>), grid, dim3(512), 0, st, a);
+        else hipLaunchKernelGGL((unsplit_step_kernel<8, 1, NM>), grid, dim3(512), 0, st, a);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
+template <bool G2>
+static hipError_t launch_unsplit_nm(const UnsplitArgs &a, hipStream_t st)
 {
-    if (a.mlp) return launch_unsplit<NM_LEARNED>(a, st);
-    return a.conv ? launch_unsplit<NM_CONV>(a, st) : launch_unsplit<NM_REF>(a, st);
+    if (a.mlp) return launch_unsplit<NM_LEARNED, G2>(a, st);
+    return a.conv ? launch_unsplit<NM_CONV, G2>(a, st) : launch_unsplit<NM_REF, G2>(a, st);
 }
 
-const char *unsplit_kernel_name(int m, int n, int nm)
+hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
 {
-    static const char *names[3][2] = {{"unsplit_step_kernel<8, 1, 0>", "unsplit_strip_kernel<0>"},
-                                      {"unsplit_step_kernel<8, 1, 1>", "unsplit_strip_kernel<1>"},
-                                      {"unsplit_step_kernel<8, 1, 2>", "unsplit_strip_kernel<2>"}};
+    return a.g2d ? launch_unsplit_nm<true>(a, st) : launch_unsplit_nm<false>(a, st);
+}
+
+const char *unsplit_kernel_name(int m, int n, int nm, int g2d)
+{
+    static const char *names[2][3][2] = {
+        {{"unsplit_step_kernel<8, 1, 0>", "unsplit_strip_kernel<0>"},
+         {"unsplit_step_kernel<8, 1, 1>", "unsplit_strip_kernel<1>"},
+         {"unsplit_step_kernel<8, 1, 2>", "unsplit_strip_kernel<2>"}},
+        {{"unsplit_step_g2d_kernel<8, 1, 0>", "unsplit_strip_g2d_kernel<0>"},
+         {"unsplit_step_g2d_kernel<8, 1, 1>", "unsplit_strip_g2d_kernel<1>"},
+         {"unsplit_step_g2d_kernel<8, 1, 2>", "unsplit_strip_g2d_kernel<2>"}}};
     if (nm < 0 || nm > 2) return "";
-    return names[nm][unsplit_variant(m, n) == 4 ? 1 : 0];
+    return names[g2d ? 1 : 0][nm][unsplit_variant(m, n) == 4 ? 1 : 0];
 }
 
 }  // namespace hgrp

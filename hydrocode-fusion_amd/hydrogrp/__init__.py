@@ -45,6 +45,7 @@ def lib():
     L.hgrp2d_bench.argtypes = [C.c_void_p, C.c_int, C.c_int, _DP, _DP, _DP]
     L.hgrp2d_bench_unsplit.argtypes = [C.c_void_p, C.c_int, C.c_int, _DP, _DP]
     L.hgrp2d_set_sweeps.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.hgrp2d_set_interface_solver.argtypes = [C.c_void_p, C.c_int]
     L.hgrp2d_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP, _DP]
     L.hgrp2d_h_over_smax.argtypes = [C.c_void_p, _DP]
     L.hgrp_grp2d_flux_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _DP]
@@ -180,6 +181,11 @@ class Solver2D:
         rule = NEWTON_CONVERGE if converge else NEWTON_REFERENCE
         _check(lib().hgrp2d_set_newton(self._h, tol, rule, fp), "hgrp2d_set_newton")
 
+    def set_interface_solver(self, g2d):
+        """GRP_2D_flux's solver in the unsplit step: g2d True for
+        linear_GRP_solver_Edir_G2D (the reference's alternative), False for Q1D."""
+        _check(lib().hgrp2d_set_interface_solver(self._h, 1 if g2d else 0), "hgrp2d_set_interface_solver")
+
     def set_fluids(self, gamma_b, frac_a):
         """Two fluids: volume fraction of fluid a (gamma = config[6]); fluid b has gamma_b."""
         fa = np.ascontiguousarray(frac_a, dtype=np.float64)
@@ -239,15 +245,16 @@ def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, gu
 
 
 def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None,
-                  converge=False):
+                  converge=False, g2d=False):
     """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0), like the
-    oracle's unsplit2d(); results as run_split2d."""
+    oracle's unsplit2d(); results as run_split2d.  g2d: GRP_2D_flux with
+    linear_GRP_solver_Edir_G2D (the reference's alternative solver)."""
     config = np.array(config, dtype=np.float64)
     config[33] = 0.0
-    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge)
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge, g2d=g2d)
 
 
-def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False, sweeps=None):
+def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False, sweeps=None, g2d=False):
     if time_plot is None:
         time_plot = np.array([0.0, config[1]])
     time_plot = np.array(time_plot, dtype=np.float64)
@@ -259,6 +266,8 @@ def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converg
             s.set_newton(newton_tol, guess_params, converge)
         if sweeps is not None:
             s.set_sweeps(*sweeps)
+        if g2d:
+            s.set_interface_solver(True)
         s.upload(init["rho"], init["u"], init["v"], init["p"], init.get("e"))
         frames = np.zeros((5, N_T, m, n))
         frames[:, 0] = s.download()

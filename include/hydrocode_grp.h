@@ -140,6 +140,17 @@ int hgrp2d_download_fraction(hgrp2d *ctx, double *frac_a);
  * switches it off.  The other uses of eps are unchanged. */
 int hgrp2d_set_newton(hgrp2d *ctx, double tol, int rule, const float *params);
 
+/* The interface solver of GRP_2D_flux in the unsplit step (config[33] = 0):
+ * HGRP_SOLVER_Q1D, linear_GRP_solver_Edir_Q1D, the reference's call
+ * (src/flux_calc/flux_solver.c:191, the default), or HGRP_SOLVER_G2D,
+ * linear_GRP_solver_Edir_G2D at atc = eps, the reference's commented-out
+ * alternative on the line above it (:189; the genuinely-2D solver, see
+ * hgrp_g2d_batch).  A split context (config[33] != 0) takes Q1D only: -1002
+ * and a message for G2D.  Works with every Newton setting of hgrp2d_set_newton. */
+#define HGRP_SOLVER_Q1D 0
+#define HGRP_SOLVER_G2D 1
+int hgrp2d_set_interface_solver(hgrp2d *ctx, int solver);
+
 /* Launch variants of the X sweep (0..4) and the Y sweep (0..4) (no counterpart
  * in the reference).  X: 0 = 8 cell rows per block at 4 waves/SIMD (the
  * default), 1 = 8 rows at the compiler's register budget, 2 = 16 rows, 3 and 4

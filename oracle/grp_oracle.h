@@ -113,6 +113,14 @@ int orc_star_dire_check2(const double mid[4], const double dire[4], double eps);
 int orc_grp_2d_flux(orc_ifv L, orc_ifv R, double tau, double eps,
                     double n_x, double n_y, double F[4], double I[4]);
 
+/* The interface solver of orc_grp_2d_flux, so of every 2D run of the oracle:
+ * 0 linear_GRP_solver_Edir_Q1D (the reference's call, flux_solver.c:191), 1
+ * linear_GRP_solver_Edir_G2D with atc = eps (its commented-out alternative,
+ * :189).  G2D reads lambda_v from the right side, which the flux generators
+ * leave at 0. */
+void orc_set_flux_solver(int solver);
+extern int orc_flux_solver;
+
 /* GRP_solver_2D_split_EUL_source (src/finite_volume/grp_solver_2D_split_EUL_source.c:75-379).
  * rho[k], u[k], v[k], p[k], e[k] point at frame k (m*n doubles each), k < N_T;
  * frame 0 holds the initial data.  config[] has the reference's index meaning

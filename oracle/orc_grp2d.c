@@ -7,11 +7,14 @@
  *   orc_ifvar_check2     <- ifvar_check (dim 2)         src/inter_process/fluid_var_check.c:21-51
  *   orc_star_dire_check2 <- star_dire_check (dim 2)     src/inter_process/fluid_var_check.c:65-131
  *   orc_grp_2d_flux      <- GRP_2D_flux                 src/flux_calc/flux_solver.c:158-237
+ *                           (its interface solver: Q1D, the reference's call at :191, or
+ *                           G2D, its commented-out alternative at :189; orc_set_flux_solver)
  *
  * Only the single-fluid members are carried (volume/mass fraction terms are
  * identically zero without MULTIFLUID_BASICS and never reach the flux).
  */
 #include <math.h>
+#include <stddef.h>
 #include "grp_oracle.h"
 
 /* Characteristic (upwind) derivative when the t-axis lies outside the fan:
@@ -318,6 +321,9 @@ int orc_star_dire_check2(const double mid[4], const double dire[4], double eps)
     return 0;
 }
 
+int orc_flux_solver = 0;
+void orc_set_flux_solver(int solver) { orc_flux_solver = solver; }
+
 int orc_grp_2d_flux(orc_ifv L, orc_ifv R, double tau, double eps,
                     double n_x, double n_y, double F[4], double I[4])
 {
@@ -339,8 +345,11 @@ int orc_grp_2d_flux(orc_ifv L, orc_ifv R, double tau, double eps,
     R.t_v = -R.t_u * n_y + R.t_v * n_x;
     L.u = u; R.u = u_R; L.d_u = d_u; R.d_u = d_u_R; L.t_u = t_u; R.t_u = t_u_R;
 
-    double wave_speed[2], dire[4], mid[4], star[6];
-    orc_grp_q1d(wave_speed, dire, mid, star, &L, &R, eps, eps);
+    double wave_speed[2], dire[6], mid[6], star[6];
+    if (orc_flux_solver == 1)   /* the reference's alternative call (:189) */
+        orc_grp_g2d(wave_speed, dire, mid, star, &L, &R, NULL, NULL, eps, eps);
+    else
+        orc_grp_q1d(wave_speed, dire, mid, star, &L, &R, eps, eps);
 
     const int retval = orc_star_dire_check2(mid, dire, eps);
     if (retval)

@@ -17,6 +17,7 @@ ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle.so")
 ORACLE_DEVPOW_SO = os.path.join(ROOT, "oracle", "liboracle_devpow.so")
 REF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref.so")
 REF_MF_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref_mf.so")
+REF_G2D_SO = os.path.join(ROOT, "oracle", "_ref", "libhydroref_g2d.so")
 PKG_DIR = os.path.join(ROOT, "hydrocode-fusion_amd")
 if PKG_DIR not in sys.path:
     sys.path.insert(0, PKG_DIR)
@@ -72,6 +73,7 @@ class Oracle:
         L.orc_riemann_exact_rule.restype = D
         L.orc_riemann_exact_rule.argtypes = [DP, DP, D, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP, C.c_int]
         L.orc_set_newton.argtypes = [D, C.c_int]
+        L.orc_set_flux_solver.argtypes = [C.c_int]
         L.orc_riemann_exact_ben.restype = D
         L.orc_riemann_exact_ben.argtypes = [DP, DP, D, D, D, D, D, D, D, IP, D, D, C.c_int, IP]
         L.orc_grp_q1d.restype = None
@@ -161,11 +163,20 @@ class Oracle:
         defaults are the reference's)."""
         return self._run2d(self.lib.orc_split2d, m, n, init, config, time_plot, N_T, newton_tol, converge)
 
-    def unsplit2d(self, m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, converge=False):
-        """GRP_solver_2D_EUL_source (the unsplit step); arguments and results as split2d."""
-        return self._run2d(self.lib.orc_unsplit2d, m, n, init, config, time_plot, N_T, newton_tol, converge)
+    def unsplit2d(self, m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, converge=False, g2d=False):
+        """GRP_solver_2D_EUL_source (the unsplit step); arguments and results as split2d.
+        g2d: GRP_2D_flux with linear_GRP_solver_Edir_G2D (its alternative solver)."""
+        return self._run2d(self.lib.orc_unsplit2d, m, n, init, config, time_plot, N_T, newton_tol, converge, g2d)
 
-    def _run2d(self, fn, m, n, init, config, time_plot, N_T, newton_tol=0.0, converge=False):
+    def flux2d_g2d(self, Lv, Rv, tau, eps, nx, ny):
+        """flux2d with GRP_2D_flux's alternative solver, linear_GRP_solver_Edir_G2D."""
+        self.lib.orc_set_flux_solver(1)
+        try:
+            return self.flux2d(Lv, Rv, tau, eps, nx, ny)
+        finally:
+            self.lib.orc_set_flux_solver(0)
+
+    def _run2d(self, fn, m, n, init, config, time_plot, N_T, newton_tol=0.0, converge=False, g2d=False):
         if time_plot is None:
             time_plot = np.array([0.0, config[1]])
         time_plot = np.ascontiguousarray(time_plot, dtype=np.float64)
@@ -182,10 +193,12 @@ class Oracle:
         npl = C.c_int(N_plot)
         cpu = np.zeros(N_T)
         self.lib.orc_set_newton(newton_tol, 1 if converge else 0)
+        self.lib.orc_set_flux_solver(1 if g2d else 0)
         try:
             st = fn(m, n, *ptrs, N_T, C.byref(npl), dptr(time_plot), dptr(cpu), dptr(cfg))
         finally:
             self.lib.orc_set_newton(0.0, 0)
+            self.lib.orc_set_flux_solver(0)
         return frames, npl.value, time_plot, cfg, st
 
     def grp1d(self, m, init, config, N_T=2, time_plot=None):

@@ -51,6 +51,11 @@ def golden_units():
 
 
 @pytest.fixture(scope="session")
+def golden_unsplit_g2d():
+    return load_golden("unsplit2d_g2d_cases.npz")
+
+
+@pytest.fixture(scope="session")
 def golden_g2d():
     return load_golden("g2d_vectors.npz")
 

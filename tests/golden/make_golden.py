@@ -427,6 +427,40 @@ def unsplit2d_cases(ref):
     np.savez_compressed(os.path.join(HERE, "unsplit2d_cases.npz"), **out)
 
 
+def unsplit2d_g2d_cases(_ref):
+    """The unsplit step with GRP_2D_flux's alternative interface solver,
+    linear_GRP_solver_Edir_G2D (flux_solver.c:189), from the reference build
+    oracle/_ref/libhydroref_g2d.so: four-quadrant 64^2 to t = 0.3, every
+    boundary-condition pair on a 12 x 10 smooth field (t = 0.2), a blast in a
+    reflective box (t = 0.15)."""
+    from _libs import REF_G2D_SO
+    ref = RefLib(REF_G2D_SO)
+    out = {}
+    m = n = 64
+    cfg = cases.four_quadrant_config(m, n, split=0)
+    fr, npl, tp, c = ref.unsplit2d(m, n, cases.four_quadrant(m, n), cfg)
+    out["fq64_final"] = fr[:, npl - 1]
+    out["fq64_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    bcs = (-1, -2, -4, -7, -24)
+    init = cases.smooth_random(12, 10, 5)
+    out["bc_init"] = np.stack([init[k] for k in ("rho", "u", "v", "p")])
+    finals, metas = [], []
+    for bx in bcs:
+        for by in bcs:
+            cfg = make_config(t_all=0.2, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=0)
+            fr, npl, tp, c = ref.unsplit2d(12, 10, init, cfg)
+            finals.append(fr[:, npl - 1])
+            metas.append([bx, by, c[5], npl, tp[npl - 1]])
+    out["bc_final"] = np.stack(finals)
+    out["bc_meta"] = np.array(metas, dtype=np.float64)
+    m, n = 40, 36
+    cfg = make_config(t_all=0.15, h_x=1 / m, h_y=1 / n, bc_x=-2, bc_y=-2, split=0)
+    fr, npl, tp, c = ref.unsplit2d(m, n, cases.blast_box(m, n), cfg)
+    out["blast_final"] = fr[:, npl - 1]
+    out["blast_meta"] = np.array([c[5], npl, tp[npl - 1]])
+    np.savez_compressed(os.path.join(HERE, "unsplit2d_g2d_cases.npz"), **out)
+
+
 def grp1d_cases(ref):
     out = {}
     cfg = make_config(t_all=0.2, h_x=1e-3, bc_x=-4, CFL=0.45)
@@ -465,7 +499,7 @@ def main():
     for name, fn in (("interface", interface_vectors), ("units", interface_units), ("stages", stage_cases),
                      ("split2d", split2d_cases), ("split2d_fq1024", split2d_fq1024),
                      ("split2d_fq1024_devpow", split2d_fq1024_devpow), ("twofluid", twofluid_vectors),
-                     ("g2d", g2d_vectors),
+                     ("g2d", g2d_vectors), ("unsplit2d_g2d", unsplit2d_g2d_cases),
                      ("unsplit2d", unsplit2d_cases), ("grp1d", grp1d_cases)):
         if not only or name in only:
             fn(ref)

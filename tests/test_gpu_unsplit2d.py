@@ -249,3 +249,84 @@ def test_unsplit_learned_guess_against_oracle(hg, oracle, oracle_devpow, m, n):
     d_dev = parity(fp[:, npp - 1], fd[:, npd - 1])
     print(f"unsplit {m}x{n}, learned (GPU) vs oracle: {d}; plain (GPU) vs device-pow oracle: {d_dev}", flush=True)
     assert d.max() <= 1e-10 and d_dev.max() <= 1e-13
+
+
+def test_unsplit_g2d_vs_reference(hg, golden_unsplit_g2d):
+    """The unsplit step with GRP_2D_flux's alternative interface solver
+    (linear_GRP_solver_Edir_G2D, flux_solver.c:189) against the reference
+    built that way: four-quadrant 64^2 to t = 0.3, every boundary-condition
+    pair, a blast in a reflective box."""
+    g = golden_unsplit_g2d
+    m = n = 64
+    fr, npl, tp, cfg, res = hg.run_unsplit2d(m, n, cases.four_quadrant(m, n), cases.four_quadrant_config(m, n, split=0),
+                                             g2d=True)
+    assert (cfg[5], npl, tp[npl - 1]) == tuple(g["fq64_meta"]) and res.status == 0
+    d = parity(fr[:, npl - 1], g["fq64_final"])
+    print("G2D fq64 rel Linf per field:", d, flush=True)
+    assert d.max() <= TOL
+    init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))
+    worst = 0.0
+    for k, (bx, by, steps, npl0, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.2, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=0)
+        fr, npl, tp, c, res = hg.run_unsplit2d(12, 10, init, cfg, g2d=True)
+        assert (c[5], npl, tp[npl - 1]) == (steps, npl0, t_end), (bx, by)
+        worst = max(worst, parity(fr[:, npl - 1], g["bc_final"][k]).max())
+    m, n = 40, 36
+    cfg = make_config(t_all=0.15, h_x=1 / m, h_y=1 / n, bc_x=-2, bc_y=-2, split=0)
+    fr, npl, tp, c, res = hg.run_unsplit2d(m, n, cases.blast_box(m, n), cfg, g2d=True)
+    assert (c[5], npl, tp[npl - 1]) == tuple(g["blast_meta"])
+    worst = max(worst, parity(fr[:, npl - 1], g["blast_final"]).max())
+    print("G2D boundary conditions and blast: worst rel Linf", worst, flush=True)
+    assert worst <= TOL
+
+
+@pytest.mark.parametrize("m,n", [(128, 128), (2048, 4096)])
+def test_unsplit_g2d_vs_devpow_oracle(hg, oracle_devpow, m, n):
+    """The G2D form against the oracle with the device's pow, per cell: 128^2
+    to t = 0.3 in the tile form, 2048 x 4096 for 3 steps in the strip form
+    (the kernel names checked), with the reference's Newton setting and, at
+    128^2, the learned guess under tol 1e-12 and the converge rule against
+    the oracle's plain solve with the same setting."""
+    from hydrogrp import mlp
+    init = cases.four_quadrant(m, n)
+    cfg = cases.four_quadrant_config(m, n, split=0)
+    if m > 512:
+        cfg = cases.four_quadrant_config(m, n, split=0, N=3)
+        cfg[1] = np.inf
+    tp = np.array([0.0, cfg[1]])
+    if not os.environ.get("HGRP_UNSPLIT"):
+        s = hg.Solver2D(m, n, cfg)
+        s.set_interface_solver(True)
+        want = "unsplit_strip_g2d_kernel<0>" if m > 512 else "unsplit_step_g2d_kernel<8, 1, 0>"
+        assert s.sweep_kernel(2) == want
+        s.close()
+    fg, npg, tpg, cg, rg = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True)
+    fd, npd, tpd, cd, sd = oracle_devpow.unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True)
+    assert rg.status == sd == 0 and (cg[5], npg, tpg[npg - 1]) == (cd[5], npd, tpd[npd - 1])
+    d = parity(fg[:, npg - 1], fd[:, npd - 1])
+    del fg
+    print(f"unsplit G2D {m}x{n} vs device-pow oracle: {d}", flush=True)
+    assert d.max() <= 1e-13
+    if m <= 512:
+        fl, npl, tpl, cl, rl = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12,
+                                                guess_params=mlp.load(), converge=True, g2d=True)
+        fo, npo, tpo, co, so = oracle_devpow.unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12,
+                                                       converge=True, g2d=True)
+        assert rl.status == so == 0 and (cl[5], npl) == (co[5], npo)
+        d = parity(fl[:, npl - 1], fo[:, npo - 1])
+        print(f"unsplit G2D {m}x{n}, learned guess vs device-pow oracle (tol 1e-12, converge rule): {d}", flush=True)
+        assert d.max() <= 1e-10
+
+
+def test_g2d_solver_refused_for_split(hg):
+    """The split sweeps keep the reference's solver: asking a split context
+    for G2D is an argument error."""
+    m, n = 16, 12
+    cfg = make_config(t_all=0.05, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-2, split=1)
+    s = hg.Solver2D(m, n, cfg)
+    try:
+        with pytest.raises(RuntimeError, match="hgrp2d_set_interface_solver"):
+            s.set_interface_solver(True)
+        s.set_interface_solver(False)
+    finally:
+        s.close()

@@ -311,6 +311,35 @@ def test_g2d_bitexact(oracle, golden_g2d):
     assert not bad, bad[:10]
 
 
+def test_unsplit2d_g2d_bitexact(oracle, golden_unsplit_g2d):
+    """The unsplit step with GRP_2D_flux's alternative solver (G2D, flux_solver.c:189)
+    equals the reference built that way (oracle/_ref/libhydroref_g2d.so) bit for
+    bit: four-quadrant 64^2 to t = 0.3, all 25 boundary-condition pairs, a blast
+    in a reflective box."""
+    g = golden_unsplit_g2d
+    m = n = 64
+    fo, npo, tpo, co, st = oracle.unsplit2d(m, n, cases.four_quadrant(m, n), cases.four_quadrant_config(m, n, split=0),
+                                            g2d=True)
+    assert (co[5], npo, tpo[npo - 1]) == tuple(g["fq64_meta"]) and st == 0
+    assert _eq(fo[:, npo - 1], g["fq64_final"])
+    init = dict(zip(("rho", "u", "v", "p"), g["bc_init"]))
+    for k, (bx, by, steps, npl, t_end) in enumerate(g["bc_meta"]):
+        cfg = make_config(t_all=0.2, h_x=1 / 12, h_y=1 / 10, bc_x=bx, bc_y=by, split=0)
+        fo, npo, tpo, co, st = oracle.unsplit2d(12, 10, init, cfg, g2d=True)
+        assert (co[5], npo, tpo[npo - 1]) == (steps, npl, t_end) and _eq(fo[:, npo - 1], g["bc_final"][k]), (bx, by)
+    m, n = 40, 36
+    cfg = make_config(t_all=0.15, h_x=1 / m, h_y=1 / n, bc_x=-2, bc_y=-2, split=0)
+    fo, npo, tpo, co, st = oracle.unsplit2d(m, n, cases.blast_box(m, n), cfg, g2d=True)
+    assert (co[5], npo, tpo[npo - 1]) == tuple(g["blast_meta"]) and _eq(fo[:, npo - 1], g["blast_final"])
+
+
+def test_unsplit2d_g2d_differs_from_q1d(oracle, golden_unsplit, golden_unsplit_g2d):
+    """The two interface solvers give two schemes: the G2D run of the
+    four-quadrant problem is not the Q1D run (the fixtures are not swapped)."""
+    d = np.abs(golden_unsplit_g2d["fq64_final"] - golden_unsplit["fq64_final"]).max()
+    assert d > 1e-3
+
+
 def g2d_branches(L, R, out, atc):
     """Which branch of linear_GRP_solver_Edir_G2D produced each row, read off
     its inputs and outputs: 'upwind', 'acoustic', 'fan', or 'star-<side>-<wave>'
