@@ -2,7 +2,7 @@
  * program so that profilers (rocprofv3 --kernel-trace / --pmc) see exactly
  * the launches of the benchmark's unsplit and two-fluid legs.
  *
- *   grp2d_file_bench.out FILE STEPS ref|conv|learned [X_VARIANT Y_VARIANT]
+ *   grp2d_file_bench.out FILE STEPS ref|conv|learned [X_VARIANT Y_VARIANT | g2d]
  *
  * FILE, little-endian (written by tests/cases.py: write_grid_bin):
  *   int32 m, n; float64 config[400] (the reference's config[]);
@@ -13,7 +13,9 @@
  * from the learned guess, weights from pstar_mlp.f32 next to the library) and
  * runs STEPS steps from t = 0 of the scheme config[33] selects (split: the
  * X(tau/2) Y(tau) X(tau/2) benchmark form, launch variants as given, default
- * 0 0; unsplit: one step launch per tau), then prints the device time.
+ * 0 0; unsplit: one step launch per tau, with GRP_2D_flux's alternative
+ * interface solver G2D when the last argument is "g2d"), then prints the device
+ * time.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <math.h>
@@ -48,13 +50,14 @@ static int read_all(FILE *f, void *p, size_t size, size_t n) { return fread(p, s
 int main(int argc, char **argv)
 {
     if (argc < 4) {
-        fprintf(stderr, "usage: %s FILE STEPS ref|conv|learned [X_VARIANT Y_VARIANT]\n", argv[0]);
+        fprintf(stderr, "usage: %s FILE STEPS ref|conv|learned [X_VARIANT Y_VARIANT | g2d]\n", argv[0]);
         return 2;
     }
     setvbuf(stdout, NULL, _IOLBF, 0);
     const int steps = atoi(argv[2]);
     const char *mode = argv[3];
     const int xv = argc > 5 ? atoi(argv[4]) : 0, yv = argc > 5 ? atoi(argv[5]) : 0;
+    const int g2d = argc == 5 && !strcmp(argv[4], "g2d");
     if (steps < 1 || (strcmp(mode, "ref") && strcmp(mode, "conv") && strcmp(mode, "learned"))) {
         fprintf(stderr, "grp2d_file_bench: bad arguments\n");
         return 2;
@@ -102,6 +105,7 @@ int main(int argc, char **argv)
     if (!rc && learned && load_weights(w)) rc = -1;
     if (!rc && strcmp(mode, "ref")) rc = hgrp2d_set_newton(ctx, 1e-12, HGRP_NEWTON_CONVERGE, learned ? w : NULL);
     if (!rc && !unsplit && !two) rc = hgrp2d_set_sweeps(ctx, xv, yv);
+    if (!rc && g2d) rc = hgrp2d_set_interface_solver(ctx, HGRP_SOLVER_G2D);
     if (rc) {
         fprintf(stderr, "grp2d_file_bench: setup failed (%d)\n", rc);
         return 1;
