@@ -28,9 +28,9 @@ def hg():
     return hydrogrp
 
 
-def _run(hg, m, n, init, cfg, time_plot=None, N_T=None):
-    print(f"unsplit run {m}x{n} bc=({cfg[17]:g},{cfg[18]:g})", flush=True)
-    return hg.run_unsplit2d(m, n, init, cfg, time_plot=time_plot, N_T=N_T)
+def _run(hg, m, n, init, cfg, time_plot=None, N_T=None, g2d=False):
+    print(f"unsplit run {m}x{n} bc=({cfg[17]:g},{cfg[18]:g}){' G2D' if g2d else ''}", flush=True)
+    return hg.run_unsplit2d(m, n, init, cfg, time_plot=time_plot, N_T=N_T, g2d=g2d)
 
 
 def test_four_quadrant_64_and_frames(hg, golden_unsplit):
@@ -105,18 +105,20 @@ def test_four_quadrant_256(hg, golden_unsplit):
 
 
 # tile edges: 62 cells per wave in y, 6 rows per block in x
+@pytest.mark.parametrize("g2d", [False, True], ids=["Q1D", "G2D"])
 @pytest.mark.parametrize("m,n,bx,by", [(1, 50, -4, -7), (50, 1, -7, -2), (6, 62, -2, -24), (7, 63, -24, -1),
                                        (13, 125, -7, -7), (127, 129, -1, -4), (65, 200, -2, -2)])
-def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by):
+def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by, g2d):
     init = cases.smooth_random(m, n, m * 1000 + n)
     cfg = make_config(t_all=0.04, h_x=1 / m, h_y=1 / n, bc_x=bx, bc_y=by, split=0)
-    fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg)
-    fg, npg, tpg, cg, res = _run(hg, m, n, init, cfg)
+    fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg, g2d=g2d)
+    fg, npg, tpg, cg, res = _run(hg, m, n, init, cfg, g2d=g2d)
     assert (co[5], npo, tpo[npo - 1]) == (cg[5], npg, tpg[npg - 1])
     assert parity(fg[:, npg - 1], fo[:, npo - 1]).max() <= TOL
 
 
-def test_error_paths_vs_oracle(hg, oracle):
+@pytest.mark.parametrize("g2d", [False, True], ids=["Q1D", "G2D"])
+def test_error_paths_vs_oracle(hg, oracle, g2d):
     m, n = 8, 6
     for mutate, kw in ((("p", (3, 2), 1e-12), {}), (("p", (3, 2), -1.0), {}), (None, {"CFL": 1e-12}),
                        (("rho", (5, 1), 1e-12), {}), (None, {"bc_y": -3}), (("u", (0, 0), np.nan), {})):
@@ -126,8 +128,8 @@ def test_error_paths_vs_oracle(hg, oracle):
         base = dict(t_all=0.1, h_x=1 / m, h_y=1 / n, bc_x=-4, bc_y=-4, split=0)
         base.update(kw)
         cfg = make_config(**base)
-        fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg)
-        fg, npg, tpg, cg, res = _run(hg, m, n, init, cfg)
+        fo, npo, tpo, co, st = oracle.unsplit2d(m, n, init, cfg, g2d=g2d)
+        fg, npg, tpg, cg, res = _run(hg, m, n, init, cfg, g2d=g2d)
         assert (co[5], npo, st) == (cg[5], npg, res.status), (mutate, kw)
         assert tpo[npo - 1] == tpg[npg - 1] or (np.isnan(tpo[npo - 1]) and np.isnan(tpg[npg - 1])), (mutate, kw)
         assert np.array_equal(np.isfinite(fo[:, npo - 1]), np.isfinite(fg[:, npg - 1]))
