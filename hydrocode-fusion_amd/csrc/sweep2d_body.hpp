@@ -204,6 +204,31 @@ __device__ __forceinline__ double interface_guess(const SweepArgs &a, bool activ
 // face in xc[k & 1].  Returns false when the sweep is halted (only checked
 // with `first`).
 // lane, w: the thread's lane and wave (threadIdx.x & 63, threadIdx.x >> 6).
+// The global loads of a chunk for the thread's own cell c = c0 - 1 + w: its
+// values, previous slopes and E.
+struct ChunkIn {
+    double v[4], sp[4], e_c;
+};
+
+template <int DIR, int W>
+__device__ __forceinline__ void chunk_loads(const SweepArgs &a, int lane, int w, int bx, int c0, ChunkIn &in)
+{
+    const int line = bx * 64 + lane;
+    const bool col = line < (DIR == 0 ? a.n : a.m);
+    const Line<DIR> L(a, col ? line : 0);
+    const int m = L.cnt;
+    const int c = c0 - 1 + w;
+    const bool own = col && c >= 0 && c < m;
+    in.v[0] = 1.0; in.v[1] = 0.0; in.v[2] = 0.0; in.v[3] = 1.0;
+    in.sp[0] = in.sp[1] = in.sp[2] = in.sp[3] = 0.0;
+    in.e_c = 0.0;
+    if (col && c <= m) L.value(c, in.v);
+    if (own) {
+        L.prev_slopes(c, in.sp);
+        in.e_c = a.st[L.at(c) + 4 * a.plane];
+    }
+}
+
 template <int DIR, int W, int NM, class Src>
 __device__ __forceinline__ bool line_chunk(const SweepArgs &a, const Src &src, int lane, int w, int bx, int c0, int k,
                                           bool first, bool carried, bool carry, double (*xa)[4][64],
@@ -217,26 +242,30 @@ __device__ __forceinline__ bool line_chunk(const SweepArgs &a, const Src &src, i
     const int c = c0 - 1 + w;                     // owned cell; -1 and m are ghosts
     const double hh = a.hh;
 
-    // every global load of the owned cell up front: values, previous slopes, E
+    // every global load of the chunk up front, before the first LDS store:
+    // the owned cell's values, previous slopes and E, then the end waves'
+    // extra cells
     const bool own = col && c >= 0 && c < m;
-    double v[4] = {1.0, 0.0, 0.0, 1.0}, vl[4], vr[4], sp[4] = {0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
-    if (col && c <= m) L.value(c, v);
-    if (own) {
-        L.prev_slopes(c, sp);
-        e_c = a.st[L.at(c) + 4 * a.plane];
+    double v[4], vl[4], vr[4], sp[4], e_c;
+    {
+        ChunkIn in;
+        chunk_loads<DIR, W>(a, lane, w, bx, c0, in);
+        e_c = in.e_c;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) { v[q] = in.v[q]; sp[q] = in.sp[q]; }
+    }
+    if (col && w == 0 && c >= 0) L.value(c - 1, vl);
+    // the last wave also limits cell c + 1 (the right side of its interface): its loads go out now too
+    double vrr[4], sp1[4];
+    if (col && w == W - 1 && c + 1 <= m) L.value(c + 1, vr);
+    if (col && w == W - 1 && c + 1 < m) {
+        L.value(c + 2, vrr);
+        L.prev_slopes(c + 1, sp1);
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) xa[w][q][lane] = v[q];
     xe[w][lane] = e_c;
-    if (col && w == 0 && c >= 0) L.value(c - 1, vl);
-    // the last wave also limits cell c + 1 (the right side of its interface): its loads go out now too
-    double vrr[4], sp1[4];
     const bool next_own = col && w == W - 1 && c + 1 < m;
-    if (col && w == W - 1 && c + 1 <= m) L.value(c + 1, vr);
-    if (next_own) {
-        L.value(c + 2, vrr);
-        L.prev_slopes(c + 1, sp1);
-    }
     // the predecessor's status is read while the loads above are in flight
     // (they stay in bounds either way); the whole block returns together
     if (first && sweep_halted(a)) return false;
