@@ -37,7 +37,7 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 hipError_t launch_cfl(const double *rho, const double *u, const double *v, const double *p, int m, int n,
                       int ld, double gamma, double hmin, unsigned long long *out, hipStream_t st);
 hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st);
-const char *unsplit_kernel_name(int m, int n, int nm, int g2d);
+const char *unsplit_kernel_name(int m, int n, int nm, int g2d, int form);
 int unsplit_max_rows();
 }
 
@@ -83,6 +83,7 @@ struct hgrp2d {
     int newton_rule;            // HGRP_NEWTON_REFERENCE or HGRP_NEWTON_CONVERGE
     int xv, yv;                 // launch variants of the X and Y sweeps (hgrp2d_tune)
     int g2d;                    // the unsplit step's interface solver: 0 Q1D, 1 G2D
+    int uform;                  // the unsplit step's launch form: 0 by grid size, 1 tile, 2 strip
     MlpPacked *mlp;             // device copy of the learned-guess weights
     bool use_mlp;
     bool two_fluid;             // volume fraction of fluid a carried (hgrp2d_set_fluids)
@@ -240,7 +241,7 @@ extern "C" const char *hgrp2d_sweep_kernel(hgrp2d *c, int dir)
 {
     if (!c || dir < 0 || dir > 2) return "";
     const int nm = c->use_mlp ? NM_LEARNED : c->newton_rule == HGRP_NEWTON_CONVERGE ? NM_CONV : NM_REF;
-    if (dir == 2) return unsplit_kernel_name(c->m, c->n, nm, c->g2d);
+    if (dir == 2) return unsplit_kernel_name(c->m, c->n, nm, c->g2d, c->uform);
     if (c->two_fluid) return sweep_tf_kernel_name(dir, nm);
     return sweep_kernel_name(dir, dir == 0 ? c->xv : c->yv, nm);
 }
@@ -459,6 +460,15 @@ extern "C" int hgrp2d_bench(hgrp2d *c, int steps, int sample_every, double *ms_t
     return 0;
 }
 
+extern "C" int hgrp2d_set_unsplit_form(hgrp2d *c, int form)
+{
+    if (!c || form < HGRP_UNSPLIT_AUTO || form > HGRP_UNSPLIT_STRIP) return ARG_ERROR - 1;
+    c->uform = form;
+    return 0;
+}
+
+static int tune_unsplit(hgrp2d *c, int steps, int *form_out, double *ms_out);
+
 extern "C" int hgrp2d_set_sweeps(hgrp2d *c, int xv, int yv)
 {
     if (!c || xv < 0 || xv >= sweep_variants(0) || yv < 0 || yv >= sweep_variants(1)) return ARG_ERROR - 1;
@@ -477,6 +487,11 @@ extern "C" int hgrp2d_tune(hgrp2d *c, int steps, int *xv_out, int *yv_out, doubl
 {
     if (!c || !c->uploaded || steps < 1) return ARG_ERROR - 1;
     if (c->two_fluid) return ARG_ERROR - 4;      // the two-fluid sweeps have one launch form
+    if (c->cfg[33] == 0.0) {                     // the unsplit scheme: its two launch forms
+        if (yv_out) *yv_out = -1;
+        if (ms_out) ms_out[1] = NAN;
+        return tune_unsplit(c, steps, xv_out, ms_out);
+    }
     if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
     const size_t cells = (size_t)c->m * c->ld;
     const size_t nb = (size_t)(2 * 5 + 2 * 4 + 2 * 4) * cells;     // state and slope planes
@@ -857,6 +872,7 @@ static void fill_unsplit_args(hgrp2d *c, UnsplitArgs &a, double tau, double nu, 
     a.tol = c->newton_tol > 0.0 ? c->newton_tol : a.eps;
     a.conv = c->newton_rule == HGRP_NEWTON_CONVERGE;
     a.g2d = c->g2d;
+    a.form = c->uform;
     a.mlp = c->use_mlp ? c->mlp : nullptr;
     a.gas = make_gas(c->cfg[6]);
     a.status = c->status + 2 * slot;
@@ -1032,6 +1048,78 @@ static int solve_unsplit(hgrp2d *c, const double *time_plot, int N_plot, int N_T
     res->time_c = time_c;
     res->tau = tau;
     res->seconds = secs;
+    return 0;
+}
+
+// The unsplit step's autotuner: times the tile and the strip form over
+// `steps` steps from the context's current state, two trials each (device
+// time of each launch from HIP events), keeps the faster and puts the state
+// back, as hgrp2d_tune does for the split sweeps.
+static int tune_unsplit(hgrp2d *c, int steps, int *form_out, double *ms_out)
+{
+    if (!valid_bc((int)c->cfg[17]) || !valid_bc((int)c->cfg[18])) return ARG_ERROR - 4;
+    if (!unsplit_supported(c)) return ARG_ERROR - 4;
+    const size_t cells = (size_t)c->m * c->ld;
+    const size_t nb = (size_t)(2 * 5 + 2 * 4 + 2 * 4) * cells;     // state and slope planes
+    double *save = nullptr;
+    HCHECK(hipMalloc((void **)&save, nb * sizeof(double)));
+    if (hipMemcpyAsync(save, c->block, nb * sizeof(double), hipMemcpyDeviceToDevice, c->stream) != hipSuccess) {
+        (void)hipFree(save);
+        return ARG_ERROR - 5;
+    }
+    const int cs = c->cs, cx = c->cx, cy = c->cy, fx = c->found_x, fy = c->found_y, form0 = c->uform;
+    auto put_back = [&]() -> int {
+        HCHECK(hipMemcpyAsync(c->block, save, nb * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+        c->cs = cs; c->cx = cx; c->cy = cy; c->found_x = fx; c->found_y = fy;
+        return 0;
+    };
+    const double tau = c->cfg[7] * c->h_S_max;
+    const double nu = tau / c->cfg[10], mu = tau / c->cfg[11];
+    std::vector<hipEvent_t> evs((size_t)2 * steps);
+    for (auto &e : evs)
+        if (hipEventCreate(&e) != hipSuccess) { (void)hipFree(save); return ARG_ERROR - 5; }
+    auto trial = [&](double *ms) -> int {
+        if (int r = put_back()) return r;
+        HCHECK(hipMemsetAsync(c->status, 0, STATUS_SLOTS * sizeof(SweepStatus), c->stream));
+        for (int s = 0; s < steps; ++s) {
+            HCHECK(hipEventRecord(evs[(size_t)2 * s], c->stream));
+            if (int r = enqueue_unsplit(c, tau, nu, mu, s % UNSPLIT_SLOTS, -1)) return r;
+            HCHECK(hipEventRecord(evs[(size_t)2 * s + 1], c->stream));
+        }
+        HCHECK(hipEventSynchronize(evs[(size_t)2 * steps - 1]));
+        double sum = 0.0;
+        for (int s = 0; s < steps; ++s) {
+            float t = 0.f;
+            HCHECK(hipEventElapsedTime(&t, evs[(size_t)2 * s], evs[(size_t)2 * s + 1]));
+            sum += t;
+        }
+        *ms = sum / steps;
+        return 0;
+    };
+    int rc = 0, pick = HGRP_UNSPLIT_TILE;
+    double best = 0.0;
+    for (int form = HGRP_UNSPLIT_TILE; form <= HGRP_UNSPLIT_STRIP && !rc; ++form) {
+        c->uform = form;
+        double t = 1e300;
+        for (int k = 0; k < 2 && !rc; ++k) {
+            double ms = 0.0;
+            rc = trial(&ms);
+            if (!rc && ms < t) t = ms;
+        }
+        if (!rc && (form == HGRP_UNSPLIT_TILE || t < best)) { best = t; pick = form; }
+    }
+    c->uform = rc ? form0 : pick;
+    const int rb = put_back();
+    if (!rc) rc = rb;
+    if (!rc) {
+        const hipError_t e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) rc = -(int)e - 1;
+    }
+    for (auto &e : evs) (void)hipEventDestroy(e);
+    (void)hipFree(save);
+    if (rc) return rc;
+    if (form_out) *form_out = c->uform;
+    if (ms_out) ms_out[0] = best;
     return 0;
 }
 

@@ -489,19 +489,9 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_g2d_kernel(UnsplitArgs a,
 // ---- launchers -----------------------------------------------------------------
 // Two forms: the tile (8 waves, 6 rows per block, the compiler's register
 // budget: no spills, 2 waves/SIMD, which its 100 KiB of LDS per block sets
-// anyway) and the strip (one wave per block walking down its rows).
-// HGRP_UNSPLIT = 1 (tile) or 4 (strip) forces one (read once, for tuning
-// runs); otherwise unsplit_variant picks by grid size.
-static int unsplit_env()
-{
-    static int v = -1;
-    if (v < 0) {
-        const char *e = getenv("HGRP_UNSPLIT");
-        v = e ? atoi(e) : 0;
-        if (v != 1 && v != 4) v = 0;
-    }
-    return v;
-}
+// anyway) and the strip (one wave per block walking down its rows).  The
+// context picks one (hgrp2d_set_unsplit_form, hgrp2d_tune) or leaves it to
+// the grid size (form 0).
 
 // rows per strip of the strip form: enough strips to give every CU several waves
 static int strip_rows(int m, int n)
@@ -512,14 +502,12 @@ static int strip_rows(int m, int n)
     return R;
 }
 
-// The strip form where it gets long strips (R >= 16: its extra x-interface
-// costs 1/R), else the tile form: on MI355X the strip form is 5 % faster at
-// 4096^2 (R = 32) and 1.5 % slower at 1024^2 (R = 4).
-static int unsplit_variant(int m, int n)
+// Form 0: the strip form where it gets long strips (R >= 16: its extra
+// x-interface costs 1/R), else the tile form: on MI355X the strip form is 5 %
+// faster at 4096^2 (R = 32) and 1.5 % slower at 1024^2 (R = 4).  1: tile, 2: strip.
+static bool use_strip(int m, int n, int form)
 {
-    const int v = unsplit_env();
-    if (v) return v;
-    return strip_rows(m, n) >= 16 ? 4 : 1;
+    return form == 2 || (form != 1 && strip_rows(m, n) >= 16);
 }
 
 int unsplit_max_rows() { return 65535 * 4; }   // grid.y of the strip form at R = 4
@@ -527,7 +515,7 @@ int unsplit_max_rows() { return 65535 * 4; }   // grid.y of the strip form at R 
 template <int NM, bool G2>
 static hipError_t launch_unsplit(const UnsplitArgs &a, hipStream_t st)
 {
-    if (unsplit_variant(a.m, a.n) == 4) {
+    if (use_strip(a.m, a.n, a.form)) {
         const int R = strip_rows(a.m, a.n);
         const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + R - 1) / R));
         if (G2) hipLaunchKernelGGL(unsplit_strip_g2d_kernel<NM>, grid, dim3(64), 0, st, a, R);
@@ -552,7 +540,7 @@ hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
     return a.g2d ? launch_unsplit_nm<true>(a, st) : launch_unsplit_nm<false>(a, st);
 }
 
-const char *unsplit_kernel_name(int m, int n, int nm, int g2d)
+const char *unsplit_kernel_name(int m, int n, int nm, int g2d, int form)
 {
     static const char *names[2][3][2] = {
         {{"unsplit_step_kernel<8, 1, 0>", "unsplit_strip_kernel<0>"},
@@ -562,7 +550,7 @@ const char *unsplit_kernel_name(int m, int n, int nm, int g2d)
          {"unsplit_step_g2d_kernel<8, 1, 1>", "unsplit_strip_g2d_kernel<1>"},
          {"unsplit_step_g2d_kernel<8, 1, 2>", "unsplit_strip_g2d_kernel<2>"}}};
     if (nm < 0 || nm > 2) return "";
-    return names[g2d ? 1 : 0][nm][unsplit_variant(m, n) == 4 ? 1 : 0];
+    return names[g2d ? 1 : 0][nm][use_strip(m, n, form) ? 1 : 0];
 }
 
 }  // namespace hgrp

@@ -31,6 +31,7 @@ struct UnsplitArgs {
     int lim3_x, lim3_y;                       // 0: minmod2 (first step), 1: minmod3 with the previous slope
     int conv;                                 // Newton stop: 0 the reference's (gap <= tol), 1 the converge rule
     int g2d;                                  // interface solver: 0 Q1D (the reference's), 1 G2D (its alternative)
+    int form;                                 // launch form: 0 by grid size, 1 tile, 2 strip
     double hh_x, hh_y;                        // h/2
     double tau, nu, mu, eps, alpha, tol;      // tau, tau/h_x, tau/h_y, ...
     RecipDiv div_hx, div_hy;                  // division by h_x, h_y

@@ -16,6 +16,7 @@ N_CONF = 400
 _D, _DP, _IP = C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_int)
 # stopping rules of the Newton solve for p* (HGRP_NEWTON_* in include/hydrocode_grp.h)
 NEWTON_REFERENCE, NEWTON_CONVERGE = 0, 1
+UNSPLIT_AUTO, UNSPLIT_TILE, UNSPLIT_STRIP = 0, 1, 2     # include/hydrocode_grp.h HGRP_UNSPLIT_*
 
 
 class Result(C.Structure):
@@ -46,6 +47,7 @@ def lib():
     L.hgrp2d_bench_unsplit.argtypes = [C.c_void_p, C.c_int, C.c_int, _DP, _DP]
     L.hgrp2d_set_sweeps.argtypes = [C.c_void_p, C.c_int, C.c_int]
     L.hgrp2d_set_interface_solver.argtypes = [C.c_void_p, C.c_int]
+    L.hgrp2d_set_unsplit_form.argtypes = [C.c_void_p, C.c_int]
     L.hgrp2d_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP, _DP]
     L.hgrp2d_h_over_smax.argtypes = [C.c_void_p, _DP]
     L.hgrp_grp2d_flux_batch.argtypes = [_DP, _DP, C.c_int, _D, _D, C.c_int, _DP]
@@ -186,6 +188,11 @@ class Solver2D:
         linear_GRP_solver_Edir_G2D (the reference's alternative), False for Q1D."""
         _check(lib().hgrp2d_set_interface_solver(self._h, 1 if g2d else 0), "hgrp2d_set_interface_solver")
 
+    def set_unsplit_form(self, form):
+        """Launch form of the unsplit step: UNSPLIT_AUTO (by grid size), UNSPLIT_TILE
+        or UNSPLIT_STRIP; tune() on an unsplit context picks the faster."""
+        _check(lib().hgrp2d_set_unsplit_form(self._h, form), "hgrp2d_set_unsplit_form")
+
     def set_fluids(self, gamma_b, frac_a):
         """Two fluids: volume fraction of fluid a (gamma = config[6]); fluid b has gamma_b."""
         fa = np.ascontiguousarray(frac_a, dtype=np.float64)
@@ -228,7 +235,8 @@ class Solver2D:
 
     def tune(self, steps=4):
         """Times the sweep launch variants on the current state (which it
-        restores); returns (x_variant, y_variant, ms per X launch, ms per Y launch)."""
+        restores); returns (x_variant, y_variant, ms per X launch, ms per Y launch).
+        On an unsplit context: (form, -1, ms per step, nan)."""
         xv, yv, ms = C.c_int(), C.c_int(), (C.c_double * 2)()
         _check(lib().hgrp2d_tune(self._h, steps, C.byref(xv), C.byref(yv), ms), "hgrp2d_tune")
         return xv.value, yv.value, ms[0], ms[1]

@@ -147,6 +147,16 @@ int hgrp2d_set_newton(hgrp2d *ctx, double tol, int rule, const float *params);
  * alternative on the line above it (:189; the genuinely-2D solver, see
  * hgrp_g2d_batch).  A split context (config[33] != 0) takes Q1D only: -1002
  * and a message for G2D.  Works with every Newton setting of hgrp2d_set_newton. */
+/* Launch form of the unsplit step (no counterpart in the reference): AUTO
+ * picks by grid size (the strip form where a grid gives long strips, about
+ * 2048^2 and up), TILE blocks of 8 waves over 6 x 62 cells, STRIP one wave per
+ * 62-cell strip walking down its rows.  hgrp2d_tune on an unsplit context times
+ * both and keeps the faster.  Results do not depend on the form. */
+#define HGRP_UNSPLIT_AUTO 0
+#define HGRP_UNSPLIT_TILE 1
+#define HGRP_UNSPLIT_STRIP 2
+int hgrp2d_set_unsplit_form(hgrp2d *ctx, int form);
+
 #define HGRP_SOLVER_Q1D 0
 #define HGRP_SOLVER_G2D 1
 int hgrp2d_set_interface_solver(hgrp2d *ctx, int solver);
@@ -164,7 +174,10 @@ int hgrp2d_set_sweeps(hgrp2d *ctx, int x_variant, int y_variant);
  * device time), sets the fastest with hgrp2d_set_sweeps and restores the state,
  * so that the run goes on exactly as without it.  x_variant, y_variant, ms[2]
  * (average ms per X and per Y launch of the winners) may be NULL.  Not for
- * two-fluid contexts. */
+ * two-fluid contexts.  On an unsplit context (config[33] = 0) it times the
+ * unsplit step's two launch forms instead and sets the faster with
+ * hgrp2d_set_unsplit_form: x_variant receives the form (HGRP_UNSPLIT_TILE or
+ * _STRIP), y_variant -1, ms[0] the winner's average ms per step, ms[1] NaN. */
 int hgrp2d_tune(hgrp2d *ctx, int steps, int *x_variant, int *y_variant, double *ms);
 
 /* Name of the kernel the context launches for an x (dir 0) or y (dir 1) sweep

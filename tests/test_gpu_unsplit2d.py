@@ -7,7 +7,6 @@ p and E (conftest.parity), as for the split path (the device's FP64 pow may
 differ from glibc's in the last bit; everything else follows the reference's
 expression order).
 """
-import os
 
 import numpy as np
 import pytest
@@ -149,8 +148,7 @@ def test_conservation_full_size(hg):
     init = cases.four_quadrant(m, n)
     s = hg.Solver2D(m, n, cfg)
     try:
-        if not os.environ.get("HGRP_UNSPLIT"):          # the launcher's own choice at this size
-            assert s.sweep_kernel(2) == "unsplit_strip_kernel<0>"
+        assert s.sweep_kernel(2) == "unsplit_strip_kernel<0>"   # the size's own choice
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         r0, u0, v0, p0, e0 = s.download()
         res = s.solve(np.array([0.0, np.inf]), 2)
@@ -234,11 +232,10 @@ def test_unsplit_learned_guess_against_oracle(hg, oracle, oracle_devpow, m, n):
     if m > 512:
         cfg = cases.four_quadrant_config(m, n, split=0, N=3)
         cfg[1] = np.inf
-        if not os.environ.get("HGRP_UNSPLIT"):
-            s = hg.Solver2D(m, n, cfg)
-            s.set_newton(1e-12, mlp.load(), converge=True)
-            assert s.sweep_kernel(2) == "unsplit_strip_kernel<2>"
-            s.close()
+        s = hg.Solver2D(m, n, cfg)
+        s.set_newton(1e-12, mlp.load(), converge=True)
+        assert s.sweep_kernel(2) == "unsplit_strip_kernel<2>"
+        s.close()
     tp = np.array([0.0, cfg[1]])
     fl, npl, tpl, cl, rl = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12, guess_params=mlp.load(),
                                             converge=True)
@@ -296,12 +293,11 @@ def test_unsplit_g2d_vs_devpow_oracle(hg, oracle_devpow, m, n):
         cfg = cases.four_quadrant_config(m, n, split=0, N=3)
         cfg[1] = np.inf
     tp = np.array([0.0, cfg[1]])
-    if not os.environ.get("HGRP_UNSPLIT"):
-        s = hg.Solver2D(m, n, cfg)
-        s.set_interface_solver(True)
-        want = "unsplit_strip_g2d_kernel<0>" if m > 512 else "unsplit_step_g2d_kernel<8, 1, 0>"
-        assert s.sweep_kernel(2) == want
-        s.close()
+    s = hg.Solver2D(m, n, cfg)
+    s.set_interface_solver(True)
+    want = "unsplit_strip_g2d_kernel<0>" if m > 512 else "unsplit_step_g2d_kernel<8, 1, 0>"
+    assert s.sweep_kernel(2) == want
+    s.close()
     fg, npg, tpg, cg, rg = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True)
     fd, npd, tpd, cd, sd = oracle_devpow.unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True)
     assert rg.status == sd == 0 and (cg[5], npg, tpg[npg - 1]) == (cd[5], npd, tpd[npd - 1])
@@ -332,3 +328,33 @@ def test_g2d_solver_refused_for_split(hg):
         s.set_interface_solver(False)
     finally:
         s.close()
+
+
+@pytest.mark.parametrize("g2d", [False, True], ids=["Q1D", "G2D"])
+def test_unsplit_forms_and_tuner_change_no_result(hg, g2d):
+    """The tile and strip forms of the unsplit step give the same state bit
+    for bit (a 200 x 150 grid: partial tiles and strips), and the tuner picks
+    one of them, puts the state back, and the run goes on as without it."""
+    m, n = 200, 150
+    cfg = cases.four_quadrant_config(m, n, split=0, t_all=0.1)
+    init = cases.four_quadrant(m, n)
+    tp = np.array([0.0, 0.1])
+    outs = []
+    for form, want in ((hg.UNSPLIT_TILE, "unsplit_step"), (hg.UNSPLIT_STRIP, "unsplit_strip"), (None, None)):
+        s = hg.Solver2D(m, n, cfg)
+        try:
+            s.set_interface_solver(g2d)
+            s.upload(init["rho"], init["u"], init["v"], init["p"])
+            if form is None:
+                picked, yv, ms, _ = s.tune(3)
+                print(f"unsplit tuner ({'G2D' if g2d else 'Q1D'}): form {picked}, {ms:.4f} ms per step", flush=True)
+                assert picked in (hg.UNSPLIT_TILE, hg.UNSPLIT_STRIP) and yv == -1 and ms > 0
+            else:
+                s.set_unsplit_form(form)
+                assert s.sweep_kernel(2).startswith(want)
+            res = s.solve(tp, 2)
+            outs.append((res.k, res.time_c, np.asarray(s.download())))
+        finally:
+            s.close()
+    for k, t, st in outs[1:]:
+        assert (k, t) == outs[0][:2] and np.array_equal(st, outs[0][2])
