@@ -496,11 +496,6 @@ extern "C" int hgrp1d_create(hgrp1d **out, int n_inst, const int *cells, const d
     c->mid = c->block + 7 * nc;
     c->dire = c->mid + 3 * ni;
     c->init = c->dire + 3 * ni;
-    const char *ev = getenv("HGRP_1D_BLOCKS");   // "<iface>,<update>" overrides the tuned sizes
-    if (ev) {
-        int a = 0, b = 0;
-        if (sscanf(ev, "%d,%d", &a, &b) == 2 && bs_index(a) >= 0 && bs_index(b) >= 0) { c->bs_if = a; c->bs_up = b; }
-    }
     *out = c;
     return 0;
 }

@@ -93,6 +93,7 @@ def lib():
     L.hgrp1d_results.argtypes = [C.c_void_p, _IP, _IP, _DP]
     L.hgrp1d_run_until.argtypes = [C.c_void_p, C.c_int, _DP, _IP]
     L.hgrp1d_tune.argtypes = [C.c_void_p, C.c_int, _IP, _IP]
+    L.hgrp1d_set_blocks.argtypes = [C.c_void_p, C.c_int, C.c_int]
     _lib = L
     return L
 
@@ -424,6 +425,10 @@ class Ensemble1D:
         a, b = C.c_int(), C.c_int()
         _check(lib().hgrp1d_tune(self._h, probe_steps, C.byref(a), C.byref(b)), "hgrp1d_tune")
         return a.value, b.value
+
+    def set_blocks(self, bs_iface, bs_update):
+        """Block sizes of the interface and update kernels (what tune() picks)."""
+        _check(lib().hgrp1d_set_blocks(self._h, bs_iface, bs_update), "hgrp1d_set_blocks")
 
 
 def run_grp1d(m, init, config):

@@ -2,7 +2,6 @@
 path: the golden Sod-1000 and boundary fixtures, ensembles of uneven problems
 against the oracle instance by instance, stop/abort paths, and the launch
 forms (captured graphs, block sizes, autotuner) giving identical results."""
-import os
 
 import numpy as np
 import pytest
@@ -111,13 +110,10 @@ def test_launch_forms_agree_bitwise(hg):
     cells, h, rho, u, p = cases.ensemble_1d(24, seed=8, n_lo=100, n_hi=3000)
     cfg = _cfg(0.05)
     outs = []
-    for graph, blocks in ((True, None), (False, None), (True, "64,512"), (True, "512,64")):
+    for graph, blocks in ((True, None), (False, None), (True, (64, 512)), (True, (512, 64))):
+        ens = hg.Ensemble1D(cells, h, cfg)
         if blocks:
-            os.environ["HGRP_1D_BLOCKS"] = blocks
-        try:
-            ens = hg.Ensemble1D(cells, h, cfg)
-        finally:
-            os.environ.pop("HGRP_1D_BLOCKS", None)
+            ens.set_blocks(*blocks)
         ens.upload(rho, u, p)
         ens.run(graph=graph)
         outs.append((ens.download(), ens.results()))
