@@ -214,14 +214,13 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
 
     const bool own = col && c >= 0 && c < m;
     double v[NQ] = {1.0, 0.0, 0.0, 1.0, 1.0}, vl[NQ], vr[NQ], sp[NQ] = {0.0, 0.0, 0.0, 0.0, 0.0}, e_c = 0.0;
+    // every global load before the first LDS store: the owned cell, then the
+    // end waves' extra cells
     if (col && c <= m) L.value(c, v);
     if (own) {
         L.prev_slopes(c, sp);
         e_c = a.st[L.at(c) + 4 * a.plane];
     }
-#pragma unroll
-    for (int q = 0; q < NQ; ++q) xa[w][q][lane] = v[q];
-    xe[w][lane] = e_c;
     if (col && w == 0 && c >= 0) L.value(c - 1, vl);
     double vrr[NQ], sp1[NQ];
     const bool next_own = col && w == W - 1 && c + 1 < m;
@@ -230,6 +229,9 @@ __global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
         L.value(c + 2, vrr);
         L.prev_slopes(c + 1, sp1);
     }
+#pragma unroll
+    for (int q = 0; q < NQ; ++q) xa[w][q][lane] = v[q];
+    xe[w][lane] = e_c;
     // the predecessor's status is read while the loads above are in flight
     // (they stay in bounds either way); the whole block returns together
     if (tf_halted(a)) return;
