@@ -504,7 +504,7 @@ static int strip_rows(int m, int n)
 
 // Form 0: the strip form where it gets long strips (R >= 16: its extra
 // x-interface costs 1/R), else the tile form: on MI355X the strip form is 5 %
-// faster at 4096^2 (R = 32) and 1.5 % slower at 1024^2 (R = 4).  1: tile, 2: strip.
+// faster at 4096^2 (R = 32) and 7 % slower at 1024^2 (R = 4).  1: tile, 2: strip.
 static bool use_strip(int m, int n, int form)
 {
     return form == 2 || (form != 1 && strip_rows(m, n) >= 16);
