@@ -1,7 +1,7 @@
 // unsplit2d.hip -- the genuinely 2D GRP step (no dimension splitting), one
 // launch per time step.
 //
-//   unsplit_step_kernel  a block of W waves covers 62 consecutive y-cells
+//   unsplit_step_kernel  a block of W waves (W = TILE_W) covers 62 consecutive y-cells
 //                        (lanes: cells i0-1 .. i0+62) of the cell rows
 //                        j0-1 .. j0+W-2 (wave w: row j0-1+w).  Every lane forms
 //                        the "side data" of its cell -- value, limited x-slope,
@@ -209,12 +209,15 @@ __device__ __forceinline__ double unsplit_guess(const UnsplitArgs &a, bool activ
 
 }  // namespace
 
-template <int W, int NM, bool G2>
+// TRIM: the y-interface results go to the slope slots of the side data (dead
+// after both solve passes; one more barrier) instead of an array of their own,
+// where the block's LDS would not fit otherwise.
+template <int W, int NM, bool G2, bool TRIM>
 __device__ __forceinline__ void unsplit_step_body(const UnsplitArgs &a)
 {
     __shared__ double sd[W][12][64];     // side data of row c: value, x-slope, y-slope
     __shared__ double fx[W - 1][8][64];  // x-interface c+1: flux, values at t + tau
-    __shared__ double gy[W - 2][8][64];  // y-interfaces of the updated rows: flux, values at t + tau
+    __shared__ double gy[TRIM ? 1 : W - 2][8][64];  // y-interfaces of the updated rows: flux, values at t + tau
     __shared__ uint32_t mlp_lds[NM == NM_LEARNED ? W : 1][NM == NM_LEARNED ? 64 * 8 : 1];   // network inputs
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const int m = a.m, n = a.n;
@@ -252,6 +255,7 @@ __device__ __forceinline__ void unsplit_step_body(const UnsplitArgs &a)
     const int yr = 1 + 2 * (task >> 1) + (lane >> 5);        // tile row of this lane's y-interface
     const int yt = ((task & 1) << 5) + (lane & 31);          // between tile lanes yt and yt + 1
     const int yc = j0 - 1 + yr, ycell = i0 - 1 + yt;
+    double Fy[4] = {0.0, 0.0, 0.0, 0.0}, Iyv[4] = {0.0, 0.0, 0.0, 0.0};   // TRIM: pass 1's results
 #pragma unroll 1
     for (int pass = 0; pass < 2; ++pass) {
         Side SL, SR;
@@ -294,11 +298,24 @@ __device__ __forceinline__ void unsplit_step_body(const UnsplitArgs &a)
             for (int q = 0; q < 4; ++q) { fx[w][q][lane] = F[q]; fx[w][4 + q][lane] = I[q]; }
         }
         if (pass == 1 && w >= 1 && w <= W - 2) {
+            if constexpr (TRIM) {
 #pragma unroll
-            for (int q = 0; q < 4; ++q) { gy[yr - 1][q][yt] = F[q]; gy[yr - 1][4 + q][yt] = I[q]; }
+                for (int q = 0; q < 4; ++q) { Fy[q] = F[q]; Iyv[q] = I[q]; }
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q) { gy[yr - 1][q][yt] = F[q]; gy[yr - 1][4 + q][yt] = I[q]; }
+            }
         }
     }
     __syncthreads();
+    if constexpr (TRIM) {
+        // every wave is past both passes: the slopes of the side data are dead
+        if (w >= 1 && w <= W - 2) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) { sd[yr][4 + q][yt] = Fy[q]; sd[yr][8 + q][yt] = Iyv[q]; }
+        }
+        __syncthreads();
+    }
 
     // combined update and slope refresh (grp_solver_2D_EUL_source.c:231-262)
     if (w >= 1 && w <= W - 2 && xin && yin && lane >= 1 && lane <= 62) {
@@ -311,8 +328,13 @@ __device__ __forceinline__ void unsplit_step_body(const UnsplitArgs &a)
             vc[q] = sd[w][q][lane];
             Fl[q] = fx[w - 1][q][lane]; Ixl[q] = fx[w - 1][4 + q][lane];
             Fr[q] = fx[w][q][lane];     Ixr[q] = fx[w][4 + q][lane];
-            Gl[q] = gy[w - 1][q][lane - 1]; Iyl[q] = gy[w - 1][4 + q][lane - 1];
-            Gy[q] = gy[w - 1][q][lane];     Iy[q] = gy[w - 1][4 + q][lane];
+            if constexpr (TRIM) {
+                Gl[q] = sd[w][4 + q][lane - 1]; Iyl[q] = sd[w][8 + q][lane - 1];
+                Gy[q] = sd[w][4 + q][lane];     Iy[q] = sd[w][8 + q][lane];
+            } else {
+                Gl[q] = gy[w - 1][q][lane - 1]; Iyl[q] = gy[w - 1][4 + q][lane - 1];
+                Gy[q] = gy[w - 1][q][lane];     Iy[q] = gy[w - 1][4 + q][lane];
+            }
         }
         const double nu = a.nu, mu = a.mu;
         const double mom_x = vc[0] * vc[1] - nu * (Fr[1] - Fl[1]) - mu * (Gy[1] - Gl[1]);
@@ -334,17 +356,27 @@ __device__ __forceinline__ void unsplit_step_body(const UnsplitArgs &a)
     }
 }
 
+// LDS of a tile of W rows: side data, x- and y-interface results, the
+// learned guess's network inputs; over 160 KiB the y results move into the
+// side data (TRIM)
+template <int W, int NM>
+constexpr bool tile_trim()
+{
+    return (size_t)W * 12 * 512 + (size_t)(W - 1) * 8 * 512 + (size_t)(W - 2) * 8 * 512 +
+               (NM == NM_LEARNED ? (size_t)W * 2048 : 0) > 160 * 1024;
+}
+
 template <int W, int MINW, int NM>
 __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_kernel(UnsplitArgs a)
 {
-    unsplit_step_body<W, NM, false>(a);
+    unsplit_step_body<W, NM, false, tile_trim<W, NM>()>(a);
 }
 
 // the same step with GRP_2D_flux's alternative interface solver, G2D
 template <int W, int MINW, int NM>
 __global__ void __launch_bounds__(64 * W, MINW) unsplit_step_g2d_kernel(UnsplitArgs a)
 {
-    unsplit_step_body<W, NM, true>(a);
+    unsplit_step_body<W, NM, true, tile_trim<W, NM>()>(a);
 }
 
 // Strip form: one wave per block walks down R cell rows of a 62-cell strip
@@ -487,11 +519,18 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_g2d_kernel(UnsplitArgs a,
 }
 
 // ---- launchers -----------------------------------------------------------------
-// Two forms: the tile (8 waves, 6 rows per block, the compiler's register
-// budget: no spills, 2 waves/SIMD, which its 100 KiB of LDS per block sets
-// anyway) and the strip (one wave per block walking down its rows).  The
-// context picks one (hgrp2d_set_unsplit_form, hgrp2d_tune) or leaves it to
-// the grid size (form 0).
+// Two forms: the tile (TILE_W waves, TILE_W - 2 updated rows per block) and the
+// strip (one wave per block walking down its rows).  The context picks one
+// (hgrp2d_set_unsplit_form, hgrp2d_tune) or leaves it to the grid size (form 0).
+// The tile is 12 waves tall: 156 KiB of LDS (140 with the learned guess, whose
+// y results then share the side data's slots), one block per CU, 168 VGPRs
+// (15-18 spilled; 39-77 in the G2D and learned forms) for 3 waves per SIMD.
+// Against 8 waves (6 rows, 100 KiB, 192 VGPRs, 2 waves per SIMD) its halo rows
+// are 1 in 6 instead of 1 in 4: at 1024^2 the whole run takes 602 instead of
+// 696 ms (the reference's Newton setting), 667 instead of 706 ms (learned),
+// 679 instead of 726 ms (G2D); 10 waves gave 626, 696 and 710 ms, 6 waves
+// (3 waves per SIMD, 4 rows) 854 ms.
+constexpr int TILE_W = 12;
 
 // rows per strip of the strip form: enough strips to give every CU several waves
 static int strip_rows(int m, int n)
@@ -502,13 +541,11 @@ static int strip_rows(int m, int n)
     return R;
 }
 
-// Form 0: the strip form where it gets long strips (R >= 16: its extra
-// x-interface costs 1/R), else the tile form: on MI355X the strip form is 5 %
-// faster at 4096^2 (R = 32) and 7 % slower at 1024^2 (R = 4).  1: tile, 2: strip.
-static bool use_strip(int m, int n, int form)
-{
-    return form == 2 || (form != 1 && strip_rows(m, n) >= 16);
-}
+// Form 0 and 1: the tile; 2: the strip.  On MI355X the 12-wave tile is the
+// faster at every size measured (four-quadrant, the reference's Newton
+// setting, per step from t = 0: 1024^2 0.207 against 0.313 ms, 2048^2 0.698
+// against 0.837 ms, 4096^2 2.467 against 2.88 ms).
+static bool use_strip(int, int, int form) { return form == 2; }
 
 int unsplit_max_rows() { return 65535 * 4; }   // grid.y of the strip form at R = 4
 
@@ -521,9 +558,9 @@ static hipError_t launch_unsplit(const UnsplitArgs &a, hipStream_t st)
         if (G2) hipLaunchKernelGGL(unsplit_strip_g2d_kernel<NM>, grid, dim3(64), 0, st, a, R);
         else hipLaunchKernelGGL(unsplit_strip_kernel<NM>, grid, dim3(64), 0, st, a, R);
     } else {
-        const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + 5) / 6));
-        if (G2) hipLaunchKernelGGL((unsplit_step_g2d_kernel<8, 1, NM>), grid, dim3(512), 0, st, a);
-        else hipLaunchKernelGGL((unsplit_step_kernel<8, 1, NM>), grid, dim3(512), 0, st, a);
+        const dim3 grid((unsigned)((a.n + 61) / 62), (unsigned)((a.m + TILE_W - 3) / (TILE_W - 2)));
+        if (G2) hipLaunchKernelGGL((unsplit_step_g2d_kernel<TILE_W, 1, NM>), grid, dim3(64 * TILE_W), 0, st, a);
+        else hipLaunchKernelGGL((unsplit_step_kernel<TILE_W, 1, NM>), grid, dim3(64 * TILE_W), 0, st, a);
     }
     return hipGetLastError();
 }
@@ -543,12 +580,12 @@ hipError_t launch_unsplit_step(const UnsplitArgs &a, hipStream_t st)
 const char *unsplit_kernel_name(int m, int n, int nm, int g2d, int form)
 {
     static const char *names[2][3][2] = {
-        {{"unsplit_step_kernel<8, 1, 0>", "unsplit_strip_kernel<0>"},
-         {"unsplit_step_kernel<8, 1, 1>", "unsplit_strip_kernel<1>"},
-         {"unsplit_step_kernel<8, 1, 2>", "unsplit_strip_kernel<2>"}},
-        {{"unsplit_step_g2d_kernel<8, 1, 0>", "unsplit_strip_g2d_kernel<0>"},
-         {"unsplit_step_g2d_kernel<8, 1, 1>", "unsplit_strip_g2d_kernel<1>"},
-         {"unsplit_step_g2d_kernel<8, 1, 2>", "unsplit_strip_g2d_kernel<2>"}}};
+        {{"unsplit_step_kernel<12, 1, 0>", "unsplit_strip_kernel<0>"},
+         {"unsplit_step_kernel<12, 1, 1>", "unsplit_strip_kernel<1>"},
+         {"unsplit_step_kernel<12, 1, 2>", "unsplit_strip_kernel<2>"}},
+        {{"unsplit_step_g2d_kernel<12, 1, 0>", "unsplit_strip_g2d_kernel<0>"},
+         {"unsplit_step_g2d_kernel<12, 1, 1>", "unsplit_strip_g2d_kernel<1>"},
+         {"unsplit_step_g2d_kernel<12, 1, 2>", "unsplit_strip_g2d_kernel<2>"}}};
     if (nm < 0 || nm > 2) return "";
     return names[g2d ? 1 : 0][nm][use_strip(m, n, form) ? 1 : 0];
 }

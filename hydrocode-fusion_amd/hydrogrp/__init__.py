@@ -254,16 +254,18 @@ def run_split2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, gu
 
 
 def run_unsplit2d(m, n, init, config, time_plot=None, N_T=None, newton_tol=0.0, guess_params=None,
-                  converge=False, g2d=False):
+                  converge=False, g2d=False, form=None):
     """The unsplit scheme (GRP_solver_2D_EUL_source, config[33] = 0), like the
     oracle's unsplit2d(); results as run_split2d.  g2d: GRP_2D_flux with
-    linear_GRP_solver_Edir_G2D (the reference's alternative solver)."""
+    linear_GRP_solver_Edir_G2D (the reference's alternative solver); form: the
+    launch form (UNSPLIT_TILE, UNSPLIT_STRIP; None: the default)."""
     config = np.array(config, dtype=np.float64)
     config[33] = 0.0
-    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge, g2d=g2d)
+    return _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge, g2d=g2d, form=form)
 
 
-def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False, sweeps=None, g2d=False):
+def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converge=False, sweeps=None, g2d=False,
+           form=None):
     if time_plot is None:
         time_plot = np.array([0.0, config[1]])
     time_plot = np.array(time_plot, dtype=np.float64)
@@ -277,6 +279,8 @@ def _run2d(m, n, init, config, time_plot, N_T, newton_tol, guess_params, converg
             s.set_sweeps(*sweeps)
         if g2d:
             s.set_interface_solver(True)
+        if form is not None:
+            s.set_unsplit_form(form)
         s.upload(init["rho"], init["u"], init["v"], init["p"], init.get("e"))
         frames = np.zeros((5, N_T, m, n))
         frames[:, 0] = s.download()

@@ -103,10 +103,10 @@ def test_four_quadrant_256(hg, golden_unsplit):
     assert np.allclose(fin.reshape(5, -1).sum(axis=1), g["fq256_sum"], rtol=1e-12, atol=1e-9)
 
 
-# tile edges: 62 cells per wave in y, 6 rows per block in x
+# tile edges: 62 cells per wave in y, 10 rows per block in x
 @pytest.mark.parametrize("g2d", [False, True], ids=["Q1D", "G2D"])
-@pytest.mark.parametrize("m,n,bx,by", [(1, 50, -4, -7), (50, 1, -7, -2), (6, 62, -2, -24), (7, 63, -24, -1),
-                                       (13, 125, -7, -7), (127, 129, -1, -4), (65, 200, -2, -2)])
+@pytest.mark.parametrize("m,n,bx,by", [(1, 50, -4, -7), (50, 1, -7, -2), (10, 62, -2, -24), (11, 63, -24, -1),
+                                       (21, 125, -7, -7), (127, 129, -1, -4), (65, 200, -2, -2)])
 def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by, g2d):
     init = cases.smooth_random(m, n, m * 1000 + n)
     cfg = make_config(t_all=0.04, h_x=1 / m, h_y=1 / n, bc_x=bx, bc_y=by, split=0)
@@ -138,8 +138,8 @@ def test_error_paths_vs_oracle(hg, oracle, g2d):
 
 def test_conservation_full_size(hg):
     """Periodic in both directions: mass, momentum and energy sums are conserved
-    (to rounding) over 24 steps of a 4096 x 4096 four-quadrant state (a size
-    the launcher runs in the strip form)."""
+    (to rounding) over 24 steps of a 4096 x 4096 four-quadrant state, in the
+    default (tile) form."""
     m = n = 4096
     cfg = cases.four_quadrant_config(m, n, split=0, N=24)
     cfg[17] = cfg[18] = -7
@@ -148,7 +148,7 @@ def test_conservation_full_size(hg):
     init = cases.four_quadrant(m, n)
     s = hg.Solver2D(m, n, cfg)
     try:
-        assert s.sweep_kernel(2) == "unsplit_strip_kernel<0>"   # the size's own choice
+        assert s.sweep_kernel(2) == "unsplit_step_kernel<12, 1, 0>"   # the default form
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         r0, u0, v0, p0, e0 = s.download()
         res = s.solve(np.array([0.0, np.inf]), 2)
@@ -223,27 +223,30 @@ def test_unsplit_learned_guess_against_oracle(hg, oracle, oracle_devpow, m, n):
     1e-12, the converge rule) against the oracle's unsplit step started from the
     reference's guess with the same tolerance and rule (per-cell relative
     L-infinity), and the GPU's plain form against the oracle with the device's
-    pow.  128^2 runs to t = 0.3 in the tile form; 2048 x 4096 takes 3 steps in
-    the strip form."""
+    pow.  128^2 runs to t = 0.3 in the tile form (the default); 2048 x 4096
+    takes 3 steps in the strip form."""
     from hydrogrp import mlp
     from conftest import parity
     init = cases.four_quadrant(m, n)
     cfg = cases.four_quadrant_config(m, n, split=0)
+    form = None
     if m > 512:
         cfg = cases.four_quadrant_config(m, n, split=0, N=3)
         cfg[1] = np.inf
+        form = hg.UNSPLIT_STRIP
         s = hg.Solver2D(m, n, cfg)
         s.set_newton(1e-12, mlp.load(), converge=True)
+        s.set_unsplit_form(form)
         assert s.sweep_kernel(2) == "unsplit_strip_kernel<2>"
         s.close()
     tp = np.array([0.0, cfg[1]])
     fl, npl, tpl, cl, rl = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12, guess_params=mlp.load(),
-                                            converge=True)
+                                            converge=True, form=form)
     fo, npo, tpo, co, so = oracle.unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12, converge=True)
     assert rl.status == so == 0 and (cl[5], npl, tpl[npl - 1]) == (co[5], npo, tpo[npo - 1])
     d = parity(fl[:, npl - 1], fo[:, npo - 1])
     del fl, fo
-    fp, npp, tpp, cp, rp = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12, converge=True)
+    fp, npp, tpp, cp, rp = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12, converge=True, form=form)
     fd, npd, tpd, cd, sd = oracle_devpow.unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12, converge=True)
     d_dev = parity(fp[:, npp - 1], fd[:, npd - 1])
     print(f"unsplit {m}x{n}, learned (GPU) vs oracle: {d}; plain (GPU) vs device-pow oracle: {d_dev}", flush=True)
@@ -282,10 +285,10 @@ def test_unsplit_g2d_vs_reference(hg, golden_unsplit_g2d):
 @pytest.mark.parametrize("m,n", [(128, 128), (2048, 4096)])
 def test_unsplit_g2d_vs_devpow_oracle(hg, oracle_devpow, m, n):
     """The G2D form against the oracle with the device's pow, per cell: 128^2
-    to t = 0.3 in the tile form, 2048 x 4096 for 3 steps in the strip form
-    (the kernel names checked), with the reference's Newton setting and, at
-    128^2, the learned guess under tol 1e-12 and the converge rule against
-    the oracle's plain solve with the same setting."""
+    to t = 0.3 in the tile form, 2048 x 4096 for 3 steps in the tile and the
+    strip form (the kernel names checked), with the reference's Newton setting
+    and, at 128^2, the learned guess under tol 1e-12 and the converge rule
+    against the oracle's plain solve with the same setting."""
     from hydrogrp import mlp
     init = cases.four_quadrant(m, n)
     cfg = cases.four_quadrant_config(m, n, split=0)
@@ -295,16 +298,18 @@ def test_unsplit_g2d_vs_devpow_oracle(hg, oracle_devpow, m, n):
     tp = np.array([0.0, cfg[1]])
     s = hg.Solver2D(m, n, cfg)
     s.set_interface_solver(True)
-    want = "unsplit_strip_g2d_kernel<0>" if m > 512 else "unsplit_step_g2d_kernel<8, 1, 0>"
-    assert s.sweep_kernel(2) == want
+    assert s.sweep_kernel(2) == "unsplit_step_g2d_kernel<12, 1, 0>"
+    s.set_unsplit_form(hg.UNSPLIT_STRIP)
+    assert s.sweep_kernel(2) == "unsplit_strip_g2d_kernel<0>"
     s.close()
-    fg, npg, tpg, cg, rg = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True)
     fd, npd, tpd, cd, sd = oracle_devpow.unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True)
-    assert rg.status == sd == 0 and (cg[5], npg, tpg[npg - 1]) == (cd[5], npd, tpd[npd - 1])
-    d = parity(fg[:, npg - 1], fd[:, npd - 1])
-    del fg
-    print(f"unsplit G2D {m}x{n} vs device-pow oracle: {d}", flush=True)
-    assert d.max() <= 1e-13
+    for form in ((None, hg.UNSPLIT_STRIP) if m > 512 else (None,)):
+        fg, npg, tpg, cg, rg = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, g2d=True, form=form)
+        assert rg.status == sd == 0 and (cg[5], npg, tpg[npg - 1]) == (cd[5], npd, tpd[npd - 1])
+        d = parity(fg[:, npg - 1], fd[:, npd - 1])
+        del fg
+        print(f"unsplit G2D {m}x{n} ({'strip' if form else 'tile'}) vs device-pow oracle: {d}", flush=True)
+        assert d.max() <= 1e-13
     if m <= 512:
         fl, npl, tpl, cl, rl = hg.run_unsplit2d(m, n, init, cfg, time_plot=tp, newton_tol=1e-12,
                                                 guess_params=mlp.load(), converge=True, g2d=True)
