@@ -11,9 +11,13 @@
 //   update_kernel  per cell: the two interface fluxes at t + tau/2, the
 //                  conservative update, positivity check, new slopes (:236-277)
 //   close_kernel   per instance: time, step counter, end and stop rules (:279-283)
-// Threads map to (instance, index) through per-entry tables built once per
-// active set, so a launch covers exactly the cells of the instances still
-// running.  Steps are captured in hipGraphs of STEPS_PER_GRAPH steps; the host
+// The two wide kernels map waves to (instance, first index) through per-wave
+// tables built once per active set: every instance starts on a wave boundary,
+// so a wave works on one instance only, reads that instance's fields as
+// uniform (scalar) loads and takes its h/S minimum with one atomic, and a
+// launch covers the cells of the instances still running (plus the unused
+// lanes of each instance's last wave).  Steps are captured in hipGraphs of
+// STEPS_PER_GRAPH steps; the host
 // reads the instances' done flags between graphs and rebuilds the tables (and
 // the graph) when enough work has finished.  Block sizes of the two wide
 // kernels are picked by a small autotuner that times candidates on the real
@@ -125,37 +129,34 @@ __device__ __forceinline__ void ghost_slope(const Inst &I, const Params &P, cons
 }
 
 struct Tables {
-    const int *if_slot, *c_slot;      // per entry: index into act[]
+    const int2 *w_if, *w_c;           // per wave of iface_kernel / update_kernel: {instance id, first index}
     const int *act;                   // active instance ids
-    const int *pre_if, *pre_c;        // prefix sums of interfaces / cells over act[]
 };
 
-__device__ __forceinline__ double hs_min_wave(double v, int slot)
+// The wave's index in a wide launch, as a uniform value.
+template <int BS>
+__device__ __forceinline__ int wave_index()
 {
-    // min over the lanes of the wave that share lane 0's slot; others return +inf
-    const int s0 = __shfl(slot, 0);
-    double r = slot == s0 ? v : INFINITY;
-    for (int off = 32; off > 0; off >>= 1) r = fmin(r, __shfl_xor(r, off));
-    return r;
+    return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * BS + threadIdx.x) >> 6));
 }
 
 // four waves per SIMD: 128 VGPRs and a 4-register spill (unbounded it takes 130, three waves);
 // measured 5 % faster on the 512-instance ensemble
 template <int BS>
-__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) iface_kernel(Tables T, int n_if, Inst *inst, Params P, const double *rho,
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) iface_kernel(Tables T, int n_w, Inst *inst, Params P, const double *rho,
                                                    const double *u, const double *p, const double *s_rho,
                                                    const double *s_u, const double *s_p, double *mid, double *dire,
                                                    size_t n_if_all)
 {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    const bool in = t < n_if;
-    const int slot = in ? T.if_slot[t] : -1;
-    Inst *Ip = in ? &inst[T.act[slot]] : nullptr;
-    const bool live = in && !Ip->done && !Ip->paused;
+    const int wv = wave_index<BS>();
+    if (wv >= n_w) return;
+    const int2 we = T.w_if[wv];
+    Inst *Ip = &inst[we.x];
+    if (Ip->done || Ip->paused) return;
+    const int j = we.y + (int)(threadIdx.x & 63);   // interface j: between cells j - 1 and j
     double hs = INFINITY;
-    if (live) {
+    if (j <= Ip->m) {
         const Inst &I = *Ip;
-        const int j = t - T.pre_if[slot];
         const int m = I.m;
         const double *const sp[3] = {s_rho, s_u, s_p};
         const double h = I.h, gamma = P.gamma;
@@ -193,16 +194,11 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) if
             for (int q = 0; q < 3; ++q) { mid[q * n_if_all + g] = U[q]; dire[q * n_if_all + g] = D[q]; }
         }
     }
-    // per-instance min of h/S: one atomic per wave where the wave is one instance
-    const double wmin = hs_min_wave(hs, slot);
-    const int s0 = __shfl(slot, 0);
-    const bool uniform = __all(slot == s0);
-    if (uniform) {
-        if ((threadIdx.x & 63) == 0 && s0 >= 0 && wmin < INFINITY)
-            atomicMin(&inst[T.act[s0]].hsm_bits, (unsigned long long)__double_as_longlong(wmin));
-    } else if (live && hs < INFINITY) {
-        atomicMin(&Ip->hsm_bits, (unsigned long long)__double_as_longlong(hs));
-    }
+    // per-instance min of h/S: one atomic per wave
+    double wmin = hs;
+    for (int off = 32; off > 0; off >>= 1) wmin = fmin(wmin, __shfl_xor(wmin, off));
+    if ((threadIdx.x & 63) == 0 && wmin < INFINITY)
+        atomicMin(&Ip->hsm_bits, (unsigned long long)__double_as_longlong(wmin));
 }
 
 // tau per instance (grp_solver_EUL_source.c:218-233)
@@ -252,16 +248,17 @@ __device__ __forceinline__ void iface_flux(const double *mid, const double *dire
 }
 
 template <int BS>
-__global__ void __launch_bounds__(BS) update_kernel(Tables T, int n_c, Inst *inst, Params P, double *rho, double *u,
+__global__ void __launch_bounds__(BS) update_kernel(Tables T, int n_w, Inst *inst, Params P, double *rho, double *u,
                                                     double *p, double *e, double *s_rho, double *s_u, double *s_p,
                                                     const double *mid, const double *dire, size_t n_if_all)
 {
-    const int t = blockIdx.x * BS + threadIdx.x;
-    if (t >= n_c) return;
-    const int slot = T.c_slot[t];
-    Inst &I = inst[T.act[slot]];
+    const int wv = wave_index<BS>();
+    if (wv >= n_w) return;
+    const int2 we = T.w_c[wv];
+    Inst &I = inst[we.x];
     if (I.done || I.paused) return;
-    const int j = t - T.pre_c[slot];
+    const int j = we.y + (int)(threadIdx.x & 63);   // cell j
+    if (j >= I.m) return;
     const double tau = I.tau, h = I.h, gamma = P.gamma;
     const double nu = tau / h;
     const size_t g = (size_t)I.ioff + j;
@@ -310,14 +307,6 @@ __global__ void close_kernel(const int *act, int n_act, Inst *inst, Params P)
     if (I.time_c >= I.t_pause) I.paused = 1;
 }
 
-__global__ void fill_tables_kernel(const int *pre_if, const int *pre_c, int n_act, int *if_slot, int *c_slot)
-{
-    const int s = blockIdx.x;
-    if (s >= n_act) return;
-    for (int q = pre_if[s] + threadIdx.x; q < pre_if[s + 1]; q += blockDim.x) if_slot[q] = s;
-    for (int q = pre_c[s] + threadIdx.x; q < pre_c[s + 1]; q += blockDim.x) c_slot[q] = s;
-}
-
 constexpr int BS_CHOICES[4] = {64, 128, 256, 512};
 
 template <int BS>
@@ -348,9 +337,10 @@ struct hgrp1d {
     double *f[7];
     double *mid, *dire, *init;         // init: rho u p e of the upload (for restarts)
     Inst *inst = nullptr;
-    int *tab = nullptr;                // if_slot[n_if], c_slot[n_cells], act, pre_if, pre_c
+    int *tab = nullptr;                // w_if[max_wi], w_c[max_wc] (int2 each), act[n_inst]
+    size_t max_wi = 0, max_wc = 0;     // waves of the two wide launches with every instance active
     Tables T{};
-    int n_act = 0, act_if = 0, act_c = 0;
+    int n_act = 0, n_wi = 0, n_wc = 0, act_c = 0;
     int bs_if = 128, bs_up = 256;      // block sizes (autotuned)
     bool uploaded = false;
     hipGraphExec_t gexec = nullptr;
@@ -367,20 +357,21 @@ static int bs_index(int bs)
 
 static void enqueue_step(hgrp1d *c)
 {
-    const dim3 gi((c->act_if + c->bs_if - 1) / c->bs_if), gu((c->act_c + c->bs_up - 1) / c->bs_up);
+    const dim3 gi((unsigned)(((size_t)c->n_wi * 64 + c->bs_if - 1) / c->bs_if)),
+        gu((unsigned)(((size_t)c->n_wc * 64 + c->bs_up - 1) / c->bs_up));
     const dim3 ga((c->n_act + 127) / 128);
     switch (c->bs_if) {
-    case 64: launch_iface<64>(gi, c->stream, c->T, c->act_if, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
-    case 256: launch_iface<256>(gi, c->stream, c->T, c->act_if, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
-    case 512: launch_iface<512>(gi, c->stream, c->T, c->act_if, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
-    default: launch_iface<128>(gi, c->stream, c->T, c->act_if, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    case 64: launch_iface<64>(gi, c->stream, c->T, c->n_wi, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    case 256: launch_iface<256>(gi, c->stream, c->T, c->n_wi, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    case 512: launch_iface<512>(gi, c->stream, c->T, c->n_wi, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    default: launch_iface<128>(gi, c->stream, c->T, c->n_wi, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
     }
     hipLaunchKernelGGL(tau_kernel, ga, dim3(128), 0, c->stream, c->T.act, c->n_act, c->inst, c->P);
     switch (c->bs_up) {
-    case 64: launch_update<64>(gu, c->stream, c->T, c->act_c, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
-    case 128: launch_update<128>(gu, c->stream, c->T, c->act_c, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
-    case 512: launch_update<512>(gu, c->stream, c->T, c->act_c, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
-    default: launch_update<256>(gu, c->stream, c->T, c->act_c, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    case 64: launch_update<64>(gu, c->stream, c->T, c->n_wc, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    case 128: launch_update<128>(gu, c->stream, c->T, c->n_wc, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    case 512: launch_update<512>(gu, c->stream, c->T, c->n_wc, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
+    default: launch_update<256>(gu, c->stream, c->T, c->n_wc, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
     }
     hipLaunchKernelGGL(close_kernel, ga, dim3(128), 0, c->stream, c->T.act, c->n_act, c->inst, c->P);
 }
@@ -396,32 +387,31 @@ static void drop_graph(hgrp1d *c)
 // rebuilds the active set from host_inst (done and paused flags) and the lookup tables
 static int rebuild_tables(hgrp1d *c)
 {
-    std::vector<int> act, pre_if(1, 0), pre_c(1, 0);
+    std::vector<int> act;
+    std::vector<int2> w_if, w_c;   // per wave: {instance, first interface / cell}
+    int act_c = 0;
     for (int i = 0; i < c->n_inst; ++i)
         if (!c->host_inst[i].done && !c->host_inst[i].paused) {
             act.push_back(i);
-            pre_if.push_back(pre_if.back() + c->m[i] + 1);
-            pre_c.push_back(pre_c.back() + c->m[i]);
+            act_c += c->m[i];
+            for (int j = 0; j <= c->m[i]; j += 64) w_if.push_back(make_int2(i, j));   // interfaces 0 .. m
+            for (int j = 0; j < c->m[i]; j += 64) w_c.push_back(make_int2(i, j));     // cells 0 .. m - 1
         }
     c->n_act = (int)act.size();
-    c->act_if = pre_if.back();
-    c->act_c = pre_c.back();
+    c->n_wi = (int)w_if.size();
+    c->n_wc = (int)w_c.size();
+    c->act_c = act_c;
     drop_graph(c);
     if (!c->n_act) return 0;
-    int *if_slot = c->tab, *c_slot = if_slot + c->n_if, *dact = c_slot + c->n_cells;
-    int *dpre_if = dact + c->n_inst, *dpre_c = dpre_if + c->n_inst + 1;
+    int2 *dw_if = (int2 *)c->tab, *dw_c = dw_if + c->max_wi;
+    int *dact = (int *)(dw_c + c->max_wc);
+    ECHECK(hipMemcpyAsync(dw_if, w_if.data(), w_if.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
+    ECHECK(hipMemcpyAsync(dw_c, w_c.data(), w_c.size() * sizeof(int2), hipMemcpyHostToDevice, c->stream));
     ECHECK(hipMemcpyAsync(dact, act.data(), act.size() * 4, hipMemcpyHostToDevice, c->stream));
-    ECHECK(hipMemcpyAsync(dpre_if, pre_if.data(), pre_if.size() * 4, hipMemcpyHostToDevice, c->stream));
-    ECHECK(hipMemcpyAsync(dpre_c, pre_c.data(), pre_c.size() * 4, hipMemcpyHostToDevice, c->stream));
-    hipLaunchKernelGGL(fill_tables_kernel, dim3(c->n_act), dim3(256), 0, c->stream, dpre_if, dpre_c, c->n_act, if_slot,
-                       c_slot);
-    ECHECK(hipGetLastError());
-    c->T.if_slot = if_slot;
-    c->T.c_slot = c_slot;
+    c->T.w_if = dw_if;
+    c->T.w_c = dw_c;
     c->T.act = dact;
-    c->T.pre_if = dpre_if;
-    c->T.pre_c = dpre_c;
-    ECHECK(hipStreamSynchronize(c->stream));
+    ECHECK(hipStreamSynchronize(c->stream));   // the host vectors go out of scope
     return 0;
 }
 
@@ -464,8 +454,10 @@ extern "C" int hgrp1d_create(hgrp1d **out, int n_inst, const int *cells, const d
         c->ioff.push_back((int)ni);
         nc += (size_t)cells[i];
         ni += (size_t)cells[i] + 1;
+        c->max_wi += ((size_t)cells[i] + 64) / 64;   // ceil((m + 1) / 64) waves of interfaces
+        c->max_wc += ((size_t)cells[i] + 63) / 64;
     }
-    if (ni > 0x7fffffff) { delete c; return -1001; }
+    if (ni > 0x7fffffff || c->max_wi > 0x7fffffff / 64) { delete c; return -1001; }   // thread and wave indices fit an int
     c->n_cells = nc;
     c->n_if = ni;
     c->P.t_all = config[1];
@@ -486,7 +478,7 @@ extern "C" int hgrp1d_create(hgrp1d **out, int n_inst, const int *cells, const d
     hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipMalloc((void **)&c->block, nb * 8);
     if (e == hipSuccess) e = hipMalloc((void **)&c->inst, n_inst * sizeof(Inst));
-    if (e == hipSuccess) e = hipMalloc((void **)&c->tab, (ni + nc + 3 * (size_t)n_inst + 2) * 4);
+    if (e == hipSuccess) e = hipMalloc((void **)&c->tab, (c->max_wi + c->max_wc) * sizeof(int2) + (size_t)n_inst * 4);
     if (e != hipSuccess) {
         fprintf(stderr, "hydrogrp: 1D ensemble allocation failed: %s\n", hipGetErrorString(e));
         hgrp1d_destroy(c);
