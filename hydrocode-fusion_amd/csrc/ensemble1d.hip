@@ -2,15 +2,18 @@
 // uneven sizes advanced together, each with its own time step.
 //
 // One step of GRP_solver_EUL_source (src/finite_volume/grp_solver_EUL_source.c:27-334)
-// for every active instance is four launches:
+// for every active instance is three launches:
 //   iface_kernel   ghosts + minmod slopes of the two cells, reconstruction,
 //                  checks, linear_GRP_solver_Edir; interface values and time
 //                  derivatives to HBM; per-instance min of h/S (:133-214)
-//   tau_kernel     per instance: tau from CFL * h/S_max with the reference's
-//                  clip and stop rules (:218-233)
-//   update_kernel  per cell: the two interface fluxes at t + tau/2, the
+//   update_kernel  per cell: tau from CFL * h/S_max with the reference's clip
+//                  and stop rules (:218-233, `step_tau`, the same in every wave
+//                  of the instance), the two interface fluxes at t + tau/2, the
 //                  conservative update, positivity check, new slopes (:236-277)
-//   close_kernel   per instance: time, step counter, end and stop rules (:279-283)
+//   close_kernel   per instance: the same tau, time, step counter, end and stop
+//                  rules (:279-283)
+// Waves of one launch share nothing but atomics into the instance's flags and
+// h/S minimum, which only the next launch reads.
 // The two wide kernels map waves to (instance, first index) through per-wave
 // tables built once per active set: every instance starts on a wave boundary,
 // so a wave works on one instance only, reads that instance's fields as
@@ -55,7 +58,8 @@ constexpr unsigned FLAG_ABORT = 1u, FLAG_STOP = 2u;
 struct Inst {
     int coff, ioff, m, bc;        // first cell, first interface, cells, boundary condition
     int k, done, status, found;   // step counter (the reference's k), finished, 0/1/2, limiter switch
-    unsigned flags;               // FLAG_* raised during the current step
+    unsigned flags;               // FLAG_* raised by the interfaces of the current step
+    unsigned uflags;              // FLAG_STOP raised by the update of the current step
     int paused;                   // time_c reached t_pause at the start of a step: waits for the host
     double h, time_c, tau;
     double t_pause;               // plot time the host waits for (INFINITY: none)
@@ -201,33 +205,21 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) if
         atomicMin(&Ip->hsm_bits, (unsigned long long)__double_as_longlong(wmin));
 }
 
-// tau per instance (grp_solver_EUL_source.c:218-233)
-__global__ void tau_kernel(const int *act, int n_act, Inst *inst, Params P)
+// tau of instance I for the step (grp_solver_EUL_source.c:218-233) from the
+// h/S minimum its interfaces left: `stop` where tau < eps (the step is taken,
+// then the run stops), `bad` where tau is not finite (no update, error).
+__device__ __forceinline__ double step_tau(const Inst &I, const Params &P, bool &stop, bool &bad)
 {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n_act) return;
-    Inst &I = inst[act[s]];
-    if (I.done || I.paused) return;
-    if (I.flags & FLAG_ABORT) {            // reconstruction error: no update, stop here
-        I.done = 1;
-        I.status = 1;
-        return;
-    }
+    stop = bad = false;
     double tau = I.tau;
     if (isfinite(P.t_all) || !isfinite(P.tau_fixed) || P.tau_fixed <= 0.0) {
         const double h_S_max = __longlong_as_double((long long)I.hsm_bits);
         tau = P.cfl * h_S_max;
-        if (tau < P.eps) {
-            I.flags |= FLAG_STOP;
-        } else if ((I.time_c + tau) > (P.t_all - P.eps)) {
-            tau = P.t_all - I.time_c;
-        } else if (!isfinite(tau)) {
-            I.done = 1;
-            I.status = 1;
-            return;
-        }
+        if (tau < P.eps) stop = true;
+        else if ((I.time_c + tau) > (P.t_all - P.eps)) tau = P.t_all - I.time_c;
+        else if (!isfinite(tau)) bad = true;
     }
-    I.tau = tau;
+    return tau;
 }
 
 __device__ __forceinline__ void iface_flux(const double *mid, const double *dire, size_t n_if_all, size_t g,
@@ -259,7 +251,11 @@ __global__ void __launch_bounds__(BS) update_kernel(Tables T, int n_w, Inst *ins
     if (I.done || I.paused) return;
     const int j = we.y + (int)(threadIdx.x & 63);   // cell j
     if (j >= I.m) return;
-    const double tau = I.tau, h = I.h, gamma = P.gamma;
+    if (I.flags & FLAG_ABORT) return;   // reconstruction error: no update (close_kernel ends the run)
+    bool stop, bad;
+    const double tau = step_tau(I, P, stop, bad);
+    if (bad) return;
+    const double h = I.h, gamma = P.gamma;
     const double nu = tau / h;
     const size_t g = (size_t)I.ioff + j;
     double Fl[3], Il[3], Fr[3], Ir[3];
@@ -274,14 +270,15 @@ __global__ void __launch_bounds__(BS) update_kernel(Tables T, int n_w, Inst *ins
     const double e1 = Ene / r1;
     const double p1 = (Ene - 0.5 * Mom * u1) * (gamma - 1.0);
     rho[o] = r1; u[o] = u1; e[o] = e1; p[o] = p1;
-    if (p1 < P.eps || r1 < P.eps) flag(&I, FLAG_STOP);
+    if (p1 < P.eps || r1 < P.eps) atomicOr(&I.uflags, FLAG_STOP);
     s_u[o] = (Ir[1] - Il[1]) / h;
     s_p[o] = (Ir[2] - Il[2]) / h;
     s_rho[o] = (Ir[0] - Il[0]) / h;
 }
 
-// end of a step (grp_solver_EUL_source.c:279-283): time, stop and end rules,
-// step counter; clears the per-step accumulators.  Then the plot-time test the
+// end of a step (grp_solver_EUL_source.c:218-233, 279-283): the step's tau
+// again (update_kernel used the same), time, stop and end rules, step counter;
+// clears the per-step accumulators.  Then the plot-time test the
 // reference makes at the start of the next step (:116-125): an instance whose
 // time has reached its t_pause waits for the host, which copies the frame.
 __global__ void close_kernel(const int *act, int n_act, Inst *inst, Params P)
@@ -290,10 +287,23 @@ __global__ void close_kernel(const int *act, int n_act, Inst *inst, Params P)
     if (s >= n_act) return;
     Inst &I = inst[act[s]];
     if (I.done || I.paused) return;
+    if (I.flags & FLAG_ABORT) {            // reconstruction error: no update, stop here
+        I.done = 1;
+        I.status = 1;
+        return;
+    }
+    bool stop, bad;
+    const double tau = step_tau(I, P, stop, bad);
+    if (bad) {
+        I.done = 1;
+        I.status = 1;
+        return;
+    }
+    I.tau = tau;
     I.found = 1;
-    I.time_c += I.tau;
-    const unsigned f = I.flags;
-    I.flags = 0u;
+    I.time_c += tau;
+    const unsigned f = I.flags | I.uflags | (stop ? FLAG_STOP : 0u);
+    I.flags = I.uflags = 0u;
     I.hsm_bits = 0x7ff0000000000000ull;
     if ((f & FLAG_STOP) || I.time_c > (P.t_all - P.eps) || !isfinite(I.time_c)) {
         I.done = 1;
@@ -366,7 +376,6 @@ static void enqueue_step(hgrp1d *c)
     case 512: launch_iface<512>(gi, c->stream, c->T, c->n_wi, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
     default: launch_iface<128>(gi, c->stream, c->T, c->n_wi, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
     }
-    hipLaunchKernelGGL(tau_kernel, ga, dim3(128), 0, c->stream, c->T.act, c->n_act, c->inst, c->P);
     switch (c->bs_up) {
     case 64: launch_update<64>(gu, c->stream, c->T, c->n_wc, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
     case 128: launch_update<128>(gu, c->stream, c->T, c->n_wc, c->inst, c->P, c->f, c->mid, c->dire, c->n_if); break;
@@ -419,7 +428,7 @@ static int reset_instances(hgrp1d *c)
 {
     for (int i = 0; i < c->n_inst; ++i) {
         Inst &I = c->host_inst[i];
-        I.k = 1; I.done = 0; I.status = 0; I.found = 0; I.flags = 0u; I.paused = 0;
+        I.k = 1; I.done = 0; I.status = 0; I.found = 0; I.flags = I.uflags = 0u; I.paused = 0;
         I.time_c = 0.0;
         I.t_pause = INFINITY;
         I.tau = c->P.tau_fixed;
