@@ -99,7 +99,20 @@ __device__ __forceinline__ void cell_value(const Inst &I, const double *rho, con
     if (I.bc == -2 || (I.bc == -24 && lo)) w[1] = -w[1];
 }
 
-// limited slopes (rho, u, p) of cell c (bound_cond_slope_limiter.c:83-92 with minmod_limiter)
+// limited slopes (rho, u, p) of cell c from the values of cells c - 1, c and
+// c + 1 (bound_cond_slope_limiter.c:83-92 with minmod_limiter)
+__device__ __forceinline__ void limit3(const Inst &I, const Params &P, const double *const s_prev[3], int c,
+                                       const double vm[3], const double v0[3], const double vp[3], double s[3])
+{
+    const double h = I.h;
+    for (int q = 0; q < 3; ++q) {
+        const double s_L = (v0[q] - vm[q]) / h;
+        const double s_R = (vp[q] - v0[q]) / h;
+        s[q] = I.found ? minmod3(P.alpha * s_L, P.alpha * s_R, s_prev[q][(size_t)I.coff + c]) : minmod2(s_L, s_R);
+    }
+}
+
+// the same, loading the three values
 __device__ __forceinline__ void limited(const Inst &I, const Params &P, const double *rho, const double *u,
                                         const double *p, const double *const s_prev[3], int c, double s[3])
 {
@@ -107,12 +120,7 @@ __device__ __forceinline__ void limited(const Inst &I, const Params &P, const do
     cell_value(I, rho, u, p, c - 1, vm);
     cell_value(I, rho, u, p, c, v0);
     cell_value(I, rho, u, p, c + 1, vp);
-    const double h = I.h;
-    for (int q = 0; q < 3; ++q) {
-        const double s_L = (v0[q] - vm[q]) / h;
-        const double s_R = (vp[q] - v0[q]) / h;
-        s[q] = I.found ? minmod3(P.alpha * s_L, P.alpha * s_R, s_prev[q][(size_t)I.coff + c]) : minmod2(s_L, s_R);
-    }
+    limit3(I, P, s_prev, c, vm, v0, vp, s);
 }
 
 // ghost slopes (bound_cond_slope_limiter.c:94-109): -2 u only, -7 all, -24 left u only
@@ -157,20 +165,32 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(4))) if
     const int2 we = T.w_if[wv];
     Inst *Ip = &inst[we.x];
     if (Ip->done || Ip->paused) return;
-    const int j = we.y + (int)(threadIdx.x & 63);   // interface j: between cells j - 1 and j
+    const Inst &I = *Ip;
+    const int m = I.m;
+    const double *const sp[3] = {s_rho, s_u, s_p};
+    const int t = (int)(threadIdx.x & 63);
+    const int j = we.y + t;                         // interface j: between cells j - 1 and j
+    // Each lane holds the left cell of its interface, c = j - 1 (-1 and m are
+    // the ghosts): its value, then its slope, limited with the neighbours'
+    // values from the previous and next lanes (the wave's end lanes load
+    // theirs).  The right cell's value and slope come from the next lane; the
+    // last lane forms its right cell's slope itself.
+    const int c = j - 1;
+    const bool interior = c >= 0 && c < m;
+    double wl[3] = {1.0, 0.0, 1.0}, wr[3], vm[3], sl[3] = {0.0, 0.0, 0.0}, sr[3];
+    if (c <= m) cell_value(I, rho, u, p, c, wl);
+    for (int q = 0; q < 3; ++q) { vm[q] = __shfl_up(wl[q], 1); wr[q] = __shfl_down(wl[q], 1); }
+    if (t == 0 && interior) cell_value(I, rho, u, p, c - 1, vm);
+    if (t == 63 && interior) cell_value(I, rho, u, p, c + 1, wr);
+    if (interior) limit3(I, P, sp, c, vm, wl, wr, sl);
+    else if (c == -1) ghost_slope(I, P, rho, u, p, sp, true, sl);
+    else if (c == m) ghost_slope(I, P, rho, u, p, sp, false, sl);
+    for (int q = 0; q < 3; ++q) sr[q] = __shfl_down(sl[q], 1);
+    if (t == 63 && j < m) limited(I, P, rho, u, p, sp, j, sr);
+    else if (t == 63 && j == m) ghost_slope(I, P, rho, u, p, sp, false, sr);
     double hs = INFINITY;
-    if (j <= Ip->m) {
-        const Inst &I = *Ip;
-        const int m = I.m;
-        const double *const sp[3] = {s_rho, s_u, s_p};
+    if (j <= m) {
         const double h = I.h, gamma = P.gamma;
-        double wl[3], wr[3], sl[3], sr[3];
-        cell_value(I, rho, u, p, j - 1, wl);
-        cell_value(I, rho, u, p, j, wr);
-        if (j > 0) limited(I, P, rho, u, p, sp, j - 1, sl);
-        else ghost_slope(I, P, rho, u, p, sp, true, sl);
-        if (j < m) limited(I, P, rho, u, p, sp, j, sr);
-        else ghost_slope(I, P, rho, u, p, sp, false, sr);
         Side1 L, R;
         L.rho = wl[0] + 0.5 * h * sl[0]; L.u = wl[1] + 0.5 * h * sl[1]; L.p = wl[2] + 0.5 * h * sl[2];
         if (j < m) {
