@@ -27,7 +27,7 @@ on a few steps of the same 4096^2 workload from the same t = 0.15 state.
 `ensemble_1d` runs 512 independent 1D GRP problems of uneven sizes through the
 batched engine; `batch_2d` 128 independent 2D problems of uneven sizes through
 the batched 2D engine, against the same grids solved one at a time; `two_fluid` runs the shock-bubble interaction (4096 x 1024)
-with and without the learned guess; `unsplit` runs the four-quadrant workload
+over the whole run to t = 1 with and without the learned guess; `unsplit` runs the four-quadrant workload
 at 1024^2 through the unsplit scheme (config[33] = 0) with its own roofline and
 CPU baseline, and its learned guess against the reference's at tol 1e-12.
 """
@@ -243,11 +243,12 @@ def batch2d_leg(n_grids=128, seed=7, t_all=0.1):
 TF_SWEEP_BYTES_PER_CELL = 160.0   # rho u v p, the volume fraction and their five slopes, read and written
 
 
-def two_fluid_leg(m=4096, n=1024, steps=300):
+def two_fluid_leg(m=4096, n=1024, steps=None):
     """The two-fluid shock-bubble workload (4 x 1 domain, Mach 1.22 shock, gamma
     1.4 / 1.67 bubble, reflective top and bottom), Newton tolerance 1e-12, the
     reference's starting guess against the learned one, `steps` steps from t = 0
-    each (the whole run to t = 1 is about 7900 steps)."""
+    each; by default the whole run to t = 1, ceil(1 / tau) steps of the frozen
+    tau (about 7900)."""
     import numpy as np
     import hydrogrp
     from hydrogrp import mlp
@@ -255,7 +256,7 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
     from _libs import make_config
     init, frac = cases.shock_bubble(m, n)
     cfg = make_config(t_all=1.0, h_x=4.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-2, split=1)
-    out = {}
+    out, finals = {}, {}
     for name, p in (("plain", None), ("learned", mlp.load())):
         s = hydrogrp.Solver2D(m, n, cfg)
         s.set_newton(1e-12, p, converge=True)
@@ -264,18 +265,27 @@ def two_fluid_leg(m=4096, n=1024, steps=300):
         s.bench(2)
         s.upload(init["rho"], init["u"], init["v"], init["p"])
         s.set_fluids(1.67, frac)
+        if steps is None:
+            steps = int(np.ceil(cfg[1] / (cfg[7] * s.h_over_smax())))
         ms, ms_x, ms_y = s.bench(steps, 8)
         kx, ky = s.sweep_kernel(0), s.sweep_kernel(1)
+        h_s = s.h_over_smax()
         state = s.download()
         s.close()
         assert np.all(np.isfinite(state)), "non-finite two-fluid state"
+        finals[name] = state
         out[name] = {"value": m * n * steps / (ms / 1000.0), "ms_per_step": ms / steps,
                      "ms_per_launch": {kx: ms_x, ky: ms_y}, "kernels": [kx, ky]}
+    a, b = finals["learned"], finals["plain"]
+    rel_l1 = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
+    del a, b, finals
     kx = out["learned"]["kernels"][0]
     achieved = TF_SWEEP_BYTES_PER_CELL * m * n / (out["learned"]["ms_per_launch"][kx] / 1000.0) / 1e9
-    return {"workload": f"two-fluid shock-bubble {m}x{n} (4x1 domain), Newton tol 1e-12, {steps} steps from t=0",
+    return {"workload": f"two-fluid shock-bubble {m}x{n} (4x1 domain), Newton tol 1e-12, {steps} steps from t=0 "
+                        f"(to t = {steps * cfg[7] * h_s:.3f} with the frozen tau)",
             "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
             "speedup": out["learned"]["value"] / out["plain"]["value"],
+            "rel_l1_difference": rel_l1,
             "roofline": {"bound": "hbm", "kernel": kx, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": TF_SWEEP_BYTES_PER_CELL,
                          "traffic": measured_traffic(kx, f"sweep_tf_kernels_shockbubble{m}x{n}_trace")}}
