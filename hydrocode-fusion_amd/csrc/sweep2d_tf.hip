@@ -193,9 +193,11 @@ __device__ __forceinline__ bool tf_halted(const SweepArgs &a)
     return false;
 }
 
-// X sweep, row-parallel (the layout of sweep_x_rows_kernel with five quantities)
+// X sweep, row-parallel (the layout of sweep_x_rows_kernel with five quantities):
+// W waves per block, W - 1 cell rows updated; one block fills a CU at W / 4
+// waves per SIMD
 template <int W, int NM>
-__global__ void __launch_bounds__(64 * W, 2) sweep_x_tf_kernel(SweepArgs a)
+__global__ void __launch_bounds__(64 * W, (W + 3) / 4) sweep_x_tf_kernel(SweepArgs a)
 {
     constexpr bool LEARNED = NM == NM_LEARNED;
     __shared__ double xa[W][NQ][64];   // values
@@ -433,19 +435,24 @@ hipError_t launch_energy_tf(const double *rho, const double *u, const double *v,
 // the two-fluid sweep kernel's name as profilers print it
 const char *sweep_tf_kernel_name(int dir, int nm)
 {
-    static const char *xn[3] = {"sweep_x_tf_kernel<8, 0>", "sweep_x_tf_kernel<8, 1>", "sweep_x_tf_kernel<8, 2>"};
+    static const char *xn[3] = {"sweep_x_tf_kernel<12, 0>", "sweep_x_tf_kernel<12, 1>", "sweep_x_tf_kernel<12, 2>"};
     static const char *yn[3] = {"sweep_y_tf_kernel<0, 16>", "sweep_y_tf_kernel<1, 16>", "sweep_y_tf_kernel<2, 16>"};
     if (nm < 0 || nm > 2 || dir < 0 || dir > 1) return "";
     return dir == 0 ? xn[nm] : yn[nm];
 }
 
+// Rows per block of the X sweep: 12 waves (11 rows updated) at 3 waves per SIMD,
+// 132 KiB of LDS
+constexpr int TF_X_W = 12;
+
 hipError_t launch_sweep_x_tf(const SweepArgs &a, hipStream_t st)
 {
-    dim3 grid((a.n + 63) / 64, (a.m + 6) / 7);
+    constexpr int W = TF_X_W;
+    dim3 grid((a.n + 63) / 64, (a.m + W - 2) / (W - 1));
     switch (newton_mode(a)) {
-    case NM_LEARNED: hipLaunchKernelGGL((sweep_x_tf_kernel<8, NM_LEARNED>), grid, dim3(64 * 8), 0, st, a); break;
-    case NM_CONV: hipLaunchKernelGGL((sweep_x_tf_kernel<8, NM_CONV>), grid, dim3(64 * 8), 0, st, a); break;
-    default: hipLaunchKernelGGL((sweep_x_tf_kernel<8, NM_REF>), grid, dim3(64 * 8), 0, st, a); break;
+    case NM_LEARNED: hipLaunchKernelGGL((sweep_x_tf_kernel<W, NM_LEARNED>), grid, dim3(64 * W), 0, st, a); break;
+    case NM_CONV: hipLaunchKernelGGL((sweep_x_tf_kernel<W, NM_CONV>), grid, dim3(64 * W), 0, st, a); break;
+    default: hipLaunchKernelGGL((sweep_x_tf_kernel<W, NM_REF>), grid, dim3(64 * W), 0, st, a); break;
     }
     return hipGetLastError();
 }
