@@ -248,7 +248,9 @@ def two_fluid_leg(m=4096, n=1024, steps=None):
     1.4 / 1.67 bubble, reflective top and bottom), Newton tolerance 1e-12, the
     reference's starting guess against the learned one, `steps` steps from t = 0
     each; by default the whole run to t = 1, ceil(1 / tau) steps of the frozen
-    tau (about 7900)."""
+    tau (about 7900).  The learned and plain final states are compared in
+    relative L1, beside the plain run from densities one ulp apart (how far
+    this flow carries a rounding difference over the same steps)."""
     import numpy as np
     import hydrogrp
     from hydrogrp import mlp
@@ -256,14 +258,20 @@ def two_fluid_leg(m=4096, n=1024, steps=None):
     from _libs import make_config
     init, frac = cases.shock_bubble(m, n)
     cfg = make_config(t_all=1.0, h_x=4.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-2, split=1)
+    init_ulp = dict(init, rho=np.nextafter(init["rho"], np.inf))
     out, finals = {}, {}
-    for name, p in (("plain", None), ("learned", mlp.load())):
+    for name, p, data in (("plain", None, init), ("learned", mlp.load(), init), ("plain_ulp", None, init_ulp)):
         s = hydrogrp.Solver2D(m, n, cfg)
         s.set_newton(1e-12, p, converge=True)
-        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.upload(data["rho"], data["u"], data["v"], data["p"])
         s.set_fluids(1.67, frac)
+        if name == "plain_ulp":                  # the state only, untimed
+            s.bench(steps)
+            finals[name] = s.download()
+            s.close()
+            continue
         s.bench(2)
-        s.upload(init["rho"], init["u"], init["v"], init["p"])
+        s.upload(data["rho"], data["u"], data["v"], data["p"])
         s.set_fluids(1.67, frac)
         if steps is None:
             steps = int(np.ceil(cfg[1] / (cfg[7] * s.h_over_smax())))
@@ -276,16 +284,17 @@ def two_fluid_leg(m=4096, n=1024, steps=None):
         finals[name] = state
         out[name] = {"value": m * n * steps / (ms / 1000.0), "ms_per_step": ms / steps,
                      "ms_per_launch": {kx: ms_x, ky: ms_y}, "kernels": [kx, ky]}
-    a, b = finals["learned"], finals["plain"]
-    rel_l1 = float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
-    del a, b, finals
+    def rel_l1(a, b):
+        return float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
+    d_learned, d_ulp = rel_l1(finals["learned"], finals["plain"]), rel_l1(finals["plain_ulp"], finals["plain"])
+    del finals
     kx = out["learned"]["kernels"][0]
     achieved = TF_SWEEP_BYTES_PER_CELL * m * n / (out["learned"]["ms_per_launch"][kx] / 1000.0) / 1e9
     return {"workload": f"two-fluid shock-bubble {m}x{n} (4x1 domain), Newton tol 1e-12, {steps} steps from t=0 "
                         f"(to t = {steps * cfg[7] * h_s:.3f} with the frozen tau)",
             "unit": "cell-updates/s", "plain": out["plain"], "learned": out["learned"],
             "speedup": out["learned"]["value"] / out["plain"]["value"],
-            "rel_l1_difference": rel_l1,
+            "rel_l1_difference": d_learned, "rel_l1_plain_from_data_one_ulp_apart": d_ulp,
             "roofline": {"bound": "hbm", "kernel": kx, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS, "bytes_per_cell": TF_SWEEP_BYTES_PER_CELL,
                          "traffic": measured_traffic(kx, f"sweep_tf_kernels_shockbubble{m}x{n}_trace")}}
