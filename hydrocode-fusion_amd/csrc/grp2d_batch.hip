@@ -125,6 +125,22 @@ __global__ void __launch_bounds__(64 * W, MINW) batch_x_kernel(BatchSweep b)
     line_sweep<0, W, NM>(a, BatchArgs{t.x}, t.y, t.z, xa, xb, xf, xi, xe);
 }
 
+// the X sweep walking K chunks of W rows per block (line_sweep_chunks): the
+// tile's row block is by (K W - 1) rows
+template <int W, int MINW, int NM, int K>
+__global__ void __launch_bounds__(64 * W, MINW) batch_x_chunks_kernel(BatchSweep b)
+{
+    __shared__ double xa[W][4][64];
+    __shared__ double xb[W][4][64];
+    __shared__ double xf[W][4][64];
+    __shared__ double xi[W][4][64];
+    __shared__ double xe[W][64];
+    __shared__ double xc[2][8][64];                     // the last face of the previous chunk: F, I
+    const int4 t = b.tiles[blockIdx.x];
+    if (b.grids[t.x].halted) return;                    // the whole block
+    line_sweep_chunks<0, W, NM, K>(BatchArgs{t.x}, t.y, t.z, xa, xb, xf, xi, xe, xc);
+}
+
 template <int MINW, int NM, int G>
 __global__ void __launch_bounds__(256, MINW) batch_y_kernel(BatchSweep b)
 {
@@ -138,10 +154,11 @@ __global__ void __launch_bounds__(256, MINW) batch_y_kernel(BatchSweep b)
     y_sweep<NM, G>(a, BatchArgs{t.x}, (long)t.y + wv, t.z, stash[wv], mlp_lds[wv]);
 }
 
-// the one-grid launch variants (sweep2d.hip) X 0..2 and Y 0..4: X 0 = 8 rows
-// at 4 waves/SIMD, 1 = 8 rows at the compiler's budget, 2 = 16 rows; Y 0..4 =
-// 16, 32, 64, 8 lanes per row at 4 waves/SIMD, 64 at the compiler's budget
-static int x_cells_per_block(int v) { return v == 2 ? 15 : 7; }
+// the one-grid launch variants (sweep2d.hip) X 0..4 and Y 0..4: X 0 = 8 rows
+// at 4 waves/SIMD, 1 = 8 rows at the compiler's budget, 2 = 16 rows, 3 and 4 =
+// 8 rows walking 4 and 8 chunks (31 and 63 rows per block); Y 0..4 = 16, 32,
+// 64, 8 lanes per row at 4 waves/SIMD, 64 at the compiler's budget
+static int x_cells_per_block(int v) { return v == 2 ? 15 : v == 3 ? 31 : v == 4 ? 63 : 7; }
 static int y_lanes_b(int v) { return v == 1 ? 32 : (v == 2 || v == 4) ? 64 : v == 3 ? 8 : 16; }
 
 template <int NM>
@@ -150,6 +167,8 @@ static void launch_bx(const BatchSweep &b, int v, int blocks, hipStream_t st)
     switch (v) {
     case 1: hipLaunchKernelGGL((batch_x_kernel<8, 1, NM>), dim3(blocks), dim3(512), 0, st, b); break;
     case 2: hipLaunchKernelGGL((batch_x_kernel<16, 4, NM>), dim3(blocks), dim3(1024), 0, st, b); break;
+    case 3: hipLaunchKernelGGL((batch_x_chunks_kernel<8, 4, NM, 4>), dim3(blocks), dim3(512), 0, st, b); break;
+    case 4: hipLaunchKernelGGL((batch_x_chunks_kernel<8, 4, NM, 8>), dim3(blocks), dim3(512), 0, st, b); break;
     default: hipLaunchKernelGGL((batch_x_kernel<8, 4, NM>), dim3(blocks), dim3(512), 0, st, b); break;
     }
 }
@@ -199,7 +218,7 @@ namespace {
 
 constexpr int RING = 96;             // sweeps between host checks
 constexpr int ARG = -1000;
-constexpr int NX_VAR = 3, NY_VAR = 5;
+constexpr int NX_VAR = 5, NY_VAR = 5;
 
 // One grid's run as the reference's loop takes it when nothing fails: its
 // sweeps, its two time steps, and how it ends.

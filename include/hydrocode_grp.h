@@ -213,7 +213,7 @@ int hgrp2db_upload(hgrp2db *ctx, const double *rho, const double *u, const doubl
 /* As hgrp2d_set_newton, for every grid. */
 int hgrp2db_set_newton(hgrp2db *ctx, double tol, int rule, const float *params);
 
-/* Launch variants as hgrp2d_set_sweeps (the same numbering; X 0..2 only). */
+/* Launch variants as hgrp2d_set_sweeps (the same numbering). */
 int hgrp2db_set_sweeps(hgrp2db *ctx, int x_variant, int y_variant);
 
 /* Times every X variant, then every Y variant, over the first `steps` steps

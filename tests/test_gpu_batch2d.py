@@ -107,7 +107,7 @@ def test_batch_variants_and_tuner_change_no_result(hg):
     inits = [_init(m, n, kind, 30 + g) for g, (m, n, kind) in enumerate(GRIDS)]
     newton = (1e-12, mlp.load(), True)
     ref = None
-    for sweeps in ((0, 0), (1, 1), (2, 2), (0, 3), (1, 4)):
+    for sweeps in ((0, 0), (1, 1), (2, 2), (0, 3), (1, 4), (3, 0), (4, 2)):
         out, k, st, t = _batch(hg, GRIDS, inits, -4, -4, newton, sweeps=sweeps)
         if ref is None:
             ref = (out, k, st, t)
