@@ -521,7 +521,7 @@ __global__ void __launch_bounds__(64, 2) unsplit_strip_g2d_kernel(UnsplitArgs a,
 // ---- launchers -----------------------------------------------------------------
 // Two forms: the tile (TILE_W waves, TILE_W - 2 updated rows per block) and the
 // strip (one wave per block walking down its rows).  The context picks one
-// (hgrp2d_set_unsplit_form, hgrp2d_tune) or leaves it to the grid size (form 0).
+// (hgrp2d_set_unsplit_form, hgrp2d_tune) or takes the default (form 0: the tile).
 // The tile is 12 waves tall: 156 KiB of LDS (140 with the learned guess, whose
 // y results then share the side data's slots), one block per CU, 168 VGPRs
 // (15-18 spilled; 39-77 in the G2D and learned forms) for 3 waves per SIMD.

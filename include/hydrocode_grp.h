@@ -147,11 +147,11 @@ int hgrp2d_set_newton(hgrp2d *ctx, double tol, int rule, const float *params);
  * alternative on the line above it (:189; the genuinely-2D solver, see
  * hgrp_g2d_batch).  A split context (config[33] != 0) takes Q1D only: -1002
  * and a message for G2D.  Works with every Newton setting of hgrp2d_set_newton. */
-/* Launch form of the unsplit step (no counterpart in the reference): AUTO
- * picks by grid size (the strip form where a grid gives long strips, about
- * 2048^2 and up), TILE blocks of 8 waves over 6 x 62 cells, STRIP one wave per
- * 62-cell strip walking down its rows.  hgrp2d_tune on an unsplit context times
- * both and keeps the faster.  Results do not depend on the form. */
+/* Launch form of the unsplit step (no counterpart in the reference): TILE
+ * blocks of 12 waves over 10 x 62 updated cells, STRIP one wave per 62-cell
+ * strip walking down its rows; AUTO takes the tile, the faster at every size
+ * measured on MI355X.  hgrp2d_tune on an unsplit context times both and keeps
+ * the faster.  Results do not depend on the form. */
 #define HGRP_UNSPLIT_AUTO 0
 #define HGRP_UNSPLIT_TILE 1
 #define HGRP_UNSPLIT_STRIP 2
