@@ -47,6 +47,22 @@ def test_single_fluid_limit_is_bitwise(hg):
     assert np.all(fa == 1.0)
 
 
+# the X sweep's blocks update 11 rows: heights 1, 11, 12, 22, 23 and 34 end
+# in a full, a one-row and a partial block
+@pytest.mark.parametrize("m,n,bx", [(1, 40, -4), (11, 20, -7), (12, 33, -2), (22, 65, -24), (23, 17, -2),
+                                    (34, 64, -4)])
+def test_single_fluid_limit_at_block_edges(hg, m, n, bx):
+    """a = 1 everywhere: the two-fluid sweeps equal the single-fluid ones bit for
+    bit on heights around the X sweep's block edges, for each x boundary the
+    two-fluid sweeps take (-2, -4, -7, -24)."""
+    init = cases.smooth_random(m, n, 100 * m + n)
+    cfg = make_config(t_all=0.03, h_x=1 / max(m, 2), h_y=1 / n, bc_x=bx, bc_y=-4, split=1)
+    one, _, r1 = _run(hg, m, n, init, None, cfg)
+    two, fa, r2 = _run(hg, m, n, init, np.ones((m, n)), cfg)
+    assert r1.status == 0 and (r1.k, r1.status, r1.time_c) == (r2.k, r2.status, r2.time_c)
+    assert np.array_equal(one.view(np.int64), two.view(np.int64))
+
+
 def test_abgrall_pressure_and_velocity_stay_uniform(hg):
     m = n = 64
     init, frac = cases.material_interface(m, n)
