@@ -243,11 +243,11 @@ def test_odd_shapes_vs_oracle(hg, oracle, m, n, bx, by):
     assert parity(fg[:, npg - 1], fo[:, npo - 1]).max() <= TOL
 
 
-@pytest.mark.parametrize("xv", [2, 3, 4])
+@pytest.mark.parametrize("xv", [2, 3, 4, 5])
 def test_x_sweep_variants_every_boundary(hg, oracle, xv):
-    """The X sweep's other launch shapes (16 rows per block; 8 rows walking 4
-    and 8 chunks: 31 and 63 rows per block, so 70 rows end in a partial block
-    and a partial chunk, and 66 lines in a partial group) against the default
+    """The X sweep's other launch shapes (16 rows per block; 8 rows walking 4,
+    8 and 6 chunks: 31, 63 and 47 rows per block, so 70 rows end in a partial
+    block and a partial chunk, and 66 lines in a partial group) against the default
     shape, bit for bit, for every boundary condition along x and each Newton
     mode; and against the oracle."""
     from hydrogrp import mlp
@@ -345,7 +345,7 @@ def test_launch_variants_and_tuner_change_no_result(hg):
         for settings in ((0.0, None, False), (1e-12, None, True), (1e-12, mlp.load(), True)):
             s.set_newton(*settings)
             ref = None
-            for xv, yv in ((0, 0), (1, 1), (2, 2), (0, 3), (1, 4), (3, 0), (4, 1)):
+            for xv, yv in ((0, 0), (1, 1), (2, 2), (0, 3), (1, 4), (3, 0), (4, 1), (5, 2)):
                 s.restore(blob)
                 s.set_sweeps(xv, yv)
                 s.bench(6)
