@@ -203,8 +203,8 @@ __device__ inline float mlp_eval_wave(const MlpPacked *__restrict__ W, const flo
 // has none (want == false, or no two-rarefaction estimate).  Wave-uniform call.
 // Weak interfaces (pressure, velocity and density jumps all within about 10 %)
 // start from p_TR itself, which is already within O(jump^3) of p* and where
-// the network's correction is nil; the network runs only in waves that hold a
-// strong interface.
+// the network's correction is nil, and so do two rarefactions of one gas; the
+// network runs only in waves that hold another (strong) interface.
 __device__ inline double learned_guess_wave(const MlpPacked *__restrict__ W, bool want, double rho_L, double u_L,
                                             double p_L, double rho_R, double u_R, double p_R, double g_L,
                                             double g_R, uint32_t *lds)
@@ -214,8 +214,12 @@ __device__ inline double learned_guess_wave(const MlpPacked *__restrict__ W, boo
     bool ok = false;
     if (want) ok = mlp_features(rho_L, u_L, p_L, rho_R, u_R, p_R, g_L, g_R, x, &p_tr);
     // |log(p_L/p_R)| < 0.1, |u_R-u_L| < 0.1 c_L, |log(rho_L/rho_R)| < 0.1, one gas
-    const bool strong = ok && (fabsf(x[1]) > 0.0109f || fabsf(x[2]) > 0.02f || fabsf(x[0]) > 0.0145f ||
-                               x[4] != x[5]);
+    // Two rarefactions of one gas (p_TR at or below both pressures): p_TR is then
+    // p* itself, to its single-precision rounding, and the network's correction
+    // would only add its fitting error
+    const bool two_rare = ok && x[4] == x[5] && p_tr <= fminf((float)p_L, (float)p_R);
+    const bool strong = ok && !two_rare &&
+                        (fabsf(x[1]) > 0.0109f || fabsf(x[2]) > 0.02f || fabsf(x[0]) > 0.0145f || x[4] != x[5]);
     if (!strong) {
 #pragma unroll
         for (int k = 0; k < MLP_IN; ++k) x[k] = 0.f;
