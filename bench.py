@@ -473,7 +473,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(the multi-GPU decomposition is described in DESIGN.md, not built)", file=sys.stderr)
+              "(the multi-GPU decomposition is a design, not built)", file=sys.stderr)
         sys.exit(2)
     if args.steps is not None and args.steps < 1:
         ap.error("--steps must be positive")

@@ -8,7 +8,7 @@
 // from the mean-exponent form for two gases), from six dimensionless inputs
 //   log(rho_L/rho_R)/ln 1e3, log(p_L/p_R)/ln 1e4, (u_R-u_L)/(5 c_L),
 //   log(c_R/c_L)/5, gamma_L - 1.4, gamma_R - 1.4,
-// and Newton starts from p_TR * exp(y).  The solver stays the judge: after at
+// and Newton starts from p_TR * exp(y).  The exact solver decides p*: after at
 // most GUESS_ITERS iterations from the learned guess without meeting the
 // tolerance it restarts from the reference's guess.
 //
