@@ -106,6 +106,44 @@ def test_ensemble_matches_oracle_per_instance(hg, oracle, oracle_devpow):
     assert len(amplified) <= 2
 
 
+# interface counts (m + 1) on and around the 64-lane waves of the interface
+# kernel, cell counts around those of the update kernel
+EDGE_SIZES = np.array([1, 2, 62, 63, 64, 65, 126, 127, 128, 129])
+
+
+@pytest.mark.parametrize("bc", [-4, -7, -2, -24, -1])
+def test_wave_edges_against_device_pow_oracle(hg, oracle_devpow, bc):
+    """Instances whose sizes put their first and last interfaces and cells on
+    the lanes where a wave's neighbours come from loads instead of the next
+    lane, for every boundary condition: each instance against the device-pow
+    oracle to 1e-12, with equal step counts, status and end time."""
+    rng = np.random.default_rng(11 + abs(bc))
+    rho, u, p = [], [], []
+    for m in EDGE_SIZES:
+        x = (np.arange(m) + 0.5) / m
+        left = x < 0.5 + 0.1 * rng.uniform(-1, 1)
+        rho.append(np.where(left, 1.0, 10 ** rng.uniform(-1, 0)))
+        u.append(np.where(left, 0.0, rng.uniform(-0.5, 0.5)))
+        p.append(np.where(left, 1.0, 10 ** rng.uniform(-1.5, 0)))
+    rho, u, p = (np.concatenate(a) for a in (rho, u, p))
+    h = 1.0 / EDGE_SIZES
+    cfg = _cfg(0.1, bc=bc)
+    ens = hg.Ensemble1D(EDGE_SIZES, h, cfg)
+    ens.upload(rho, u, p)
+    ens.run()
+    fin = ens.download()
+    steps, status, t = ens.results()
+    ens.close()
+    off, ref = _oracle_each(oracle_devpow, EDGE_SIZES, h, rho, u, p, cfg)
+    worst = 0.0
+    for i, (f, k, st, te) in enumerate(ref):
+        assert (steps[i], status[i], t[i]) == (k, st, te), (bc, EDGE_SIZES[i], steps[i], status[i], k, st)
+        d = _final_diff(fin[:, off[i]:off[i + 1]], f)
+        worst = max(worst, d)
+        assert d <= 1e-12, (bc, EDGE_SIZES[i], d)
+    print(f"BC {bc}: wave-edge instances vs device-pow oracle {worst:.2e}", flush=True)
+
+
 def test_launch_forms_agree_bitwise(hg):
     cells, h, rho, u, p = cases.ensemble_1d(24, seed=8, n_lo=100, n_hi=3000)
     cfg = _cfg(0.05)
