@@ -250,6 +250,39 @@ def test_c5_full_size_conservation(hg):
         assert abs(s1 - s0) <= 1e-11 * scale, name
 
 
+def test_c5_whole_run_learned_within_the_rounding_spread(hg):
+    """C5 to t = 1 (4096 x 1024, about 7900 steps), tol 1e-12 with the converge
+    rule: the learned and the plain run take the same steps and end no farther
+    apart in relative L1 than the plain run ends from the plain run on
+    densities one ulp apart (the shock-bubble's mixing layer carries any
+    rounding difference that far over the run)."""
+    from hydrogrp import mlp
+    m, n = 4096, 1024
+    init, frac = cases.shock_bubble(m, n)
+    cfg = make_config(t_all=1.0, h_x=4.0 / m, h_y=1.0 / n, bc_x=-4, bc_y=-2, split=1)
+    init_ulp = dict(init, rho=np.nextafter(init["rho"], np.inf))
+    out = {}
+    for name, data, params in (("plain", init, None), ("learned", init, mlp.load()), ("ulp", init_ulp, None)):
+        s = hg.Solver2D(m, n, cfg)
+        try:
+            s.set_newton(1e-12, params, converge=True)
+            s.upload(data["rho"], data["u"], data["v"], data["p"])
+            s.set_fluids(1.67, frac)
+            res = s.solve(np.array([0.0, 1.0]), 2)
+            out[name] = (s.download(), res)
+        finally:
+            s.close()
+        assert out[name][1].status == 0 and np.all(np.isfinite(out[name][0])), name
+
+    def rel_l1(a, b):
+        return float((np.abs(a - b).sum(axis=(1, 2)) / np.abs(b).sum(axis=(1, 2))).max())
+    assert out["learned"][1].k == out["plain"][1].k
+    d, d_ulp = rel_l1(out["learned"][0], out["plain"][0]), rel_l1(out["ulp"][0], out["plain"][0])
+    print(f"C5 to t = 1 ({out['plain'][1].k} steps): learned vs plain rel L1 {d:.2e}; plain from data "
+          f"one ulp apart vs plain {d_ulp:.2e}", flush=True)
+    assert d <= 2.0 * d_ulp and d <= 1e-5
+
+
 def test_two_fluid_flux_against_multifluid_reference(hg, golden_twofluid):
     """On the GPU: the two-fluid sweeps' interface solve (hgrp_grp2d_flux_tf_batch)
     against the reference's GRP_2D_flux built with -DMULTIFLUID_BASICS
